@@ -1,0 +1,235 @@
+#!/usr/bin/env python3
+"""Benchmark of the two hot paths on one MI355X.
+
+A "step" is one masked AO-DM-AO rho evaluation (the fused kernel path, FP64)
+over the resident workload: nao 2228 (700 s/p/d/f shells), ngrid 16384 (256
+blocks x 64 points), 3 PSD density matrices, 30 % per-(block, shell) mask.
+Inputs sit in HBM before the timed region; `value` is rho evaluations per
+second over exactly --steps steps, bracketed by device synchronization.
+
+The same JSON line carries:
+  * latency_ms: p50 / p99 of per-step GPU time (hipEvents on the launch stream);
+  * rel_err: relative Frobenius error of one GPU result vs the CPU oracle;
+  * roofline: the dominant kernel (rho_chunk_kernel), algorithmic masked flops
+    per launch / its average duration from hipEvents around each launch;
+  * cpu_baseline: the oracle (a C restatement of the reference's CPU loop nest)
+    timed on this host's cores on a bounded sample of the same workload;
+  * dgemm: FP64 DGEMM TFLOP/s and fraction of the FP64 MFMA peak for the
+    1031x977x1543 NT and 3527x9583x6581 NN shapes (dgemm_kernel hipEvent time).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-dgemm] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "cfgable-matmul_amd"))
+sys.path.insert(0, ROOT)
+
+import cfgmm  # noqa: E402
+from cfgmm import workloads as W  # noqa: E402
+
+FP64_MFMA_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0
+METRIC = "FP64 DGEMM TFLOP/s (% peak); rho evals/s + p99 latency at ≤1e-10 rel err"
+README_RHO_EVALS_PER_S = 1000.0 / 120.0  # README.md:51, 16-core Ryzen 7945HX
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def rel_fro(x, ref) -> float:
+    den = np.linalg.norm(ref)
+    return float(np.linalg.norm(x - ref) / den) if den > 0 else float(np.linalg.norm(x))
+
+
+def bench_rho(p: W.RhoProblem, steps: int, warmup: int, precision=cfgmm.PREC_FP64, nslices=0):
+    s = cfgmm.Stream()
+    d_r = cfgmm.DeviceBuffer(8 * p.nset * p.ngrid)
+    d_r.zero()
+    d_d = cfgmm.DeviceBuffer.from_array(p.dms)
+    d_phi = cfgmm.DeviceBuffer.from_array(p.phi)
+    d_tab = cfgmm.DeviceBuffer.from_array(np.ascontiguousarray(p.tab))
+
+    def step():
+        cfgmm.aodmao2rho_dev(s, d_r, d_d, d_phi, d_phi, p.nao, p.ngrid, p.nao, p.ngrid, p.nset, d_tab, p.nblk,
+                             p.blksize, precision, nslices)
+
+    # one checked evaluation (rho starts at zero)
+    step()
+    s.synchronize()
+    rho_once = d_r.to_array(np.float64, (p.nset, p.ngrid))
+    for _ in range(warmup):
+        step()
+    s.synchronize()
+
+    evs = [cfgmm.Event() for _ in range(steps + 1)]
+    cfgmm.trace_reset()
+    cfgmm.trace_enable(True)
+    cfgmm.check(cfgmm.lib().cfgmm_device_synchronize())
+    t0 = time.perf_counter()
+    evs[0].record(s)
+    for i in range(steps):
+        step()
+        evs[i + 1].record(s)
+    s.synchronize()
+    wall = time.perf_counter() - t0
+    cfgmm.trace_enable(False)
+    per_step = np.array([evs[i].elapsed_ms(evs[i + 1]) for i in range(steps)])
+    kern_ms, kern_n = cfgmm.trace_read("rho_chunk_kernel")
+    total_gpu_ms = evs[0].elapsed_ms(evs[-1])
+    cfgmm.trace_reset()
+    for e in evs:
+        e.destroy()
+    for b in (d_r, d_d, d_phi, d_tab):
+        b.free()
+    s.destroy()
+    return dict(rho_once=rho_once, wall_s=wall, gpu_ms=total_gpu_ms, per_step_ms=per_step,
+                kernel_avg_ms=kern_ms / max(kern_n, 1), kernel_launches=kern_n)
+
+
+def bench_dgemm(g: W.GemmProblem, reps: int, warmup: int = 2):
+    s = cfgmm.Stream()
+    dA = cfgmm.DeviceBuffer.from_array(g.a)
+    dB = cfgmm.DeviceBuffer.from_array(g.b)
+    dC = cfgmm.DeviceBuffer(8 * g.m * g.ldc)
+    dC.zero()
+    for _ in range(warmup):
+        cfgmm.dgemm_dev(s, dC, dA, dB, g.m, g.n, g.k, g.lda, g.ldb, g.ldc, g.transa, g.transb)
+    s.synchronize()
+    e0, e1 = cfgmm.Event(), cfgmm.Event()
+    cfgmm.trace_reset()
+    cfgmm.trace_enable(True)
+    e0.record(s)
+    for _ in range(reps):
+        cfgmm.dgemm_dev(s, dC, dA, dB, g.m, g.n, g.k, g.lda, g.ldb, g.ldc, g.transa, g.transb)
+    e1.record(s)
+    s.synchronize()
+    cfgmm.trace_enable(False)
+    step_ms = e0.elapsed_ms(e1) / reps
+    kern_ms, kern_n = cfgmm.trace_read("dgemm_kernel")
+    cfgmm.trace_reset()
+    for x in (dA, dB, dC):
+        x.free()
+    e0.destroy()
+    e1.destroy()
+    s.destroy()
+    kavg = kern_ms / max(kern_n, 1)
+    tf_step = g.flops / (step_ms * 1e-3) / 1e12
+    tf_kern = g.flops / (kavg * 1e-3) / 1e12
+    return dict(shape=f"{g.m}x{g.n}x{g.k}", trans=("T" if g.transa else "N") + ("T" if g.transb else "N"),
+                ms_per_call=round(step_ms, 4), tflops=round(tf_step, 2),
+                frac_fp64_peak=round(tf_step / FP64_MFMA_PEAK_TFLOPS, 4),
+                dgemm_kernel_avg_ms=round(kavg, 4), kernel_tflops=round(tf_kern, 2))
+
+
+def cpu_baseline_rho(p: W.RhoProblem, budget_s: float = 12.0):
+    import oracle  # the CPU oracle: only this leg of bench.py uses it
+    threads = min(16, os.cpu_count() or 1)
+    rhos = np.zeros(p.nset * p.ngrid)
+    args = (p.dms.ravel(), p.phi.ravel(), p.phi.ravel(), p.nao, p.ngrid, p.nao, p.ngrid, p.nset,
+            np.ascontiguousarray(p.tab), p.nblk, p.blksize)
+    t0 = time.perf_counter()
+    assert oracle.aodmao2rho(rhos, *args, nthreads=threads) == 0  # first eval doubles as the reference result
+    first = time.perf_counter() - t0
+    ref = rhos.reshape(p.nset, p.ngrid).copy()
+    n, t_tot = 1, first
+    while t_tot < budget_s and n < 50:
+        r2 = np.zeros_like(rhos)
+        t0 = time.perf_counter()
+        oracle.aodmao2rho(r2, *args, nthreads=threads)
+        t_tot += time.perf_counter() - t0
+        n += 1
+    return ref, dict(value=round(n / t_tot, 4), unit="rho evals/s", cores=threads, kind="port",
+                     sample=f"{n} full evaluations of the same workload ({t_tot:.1f} s of CPU time) with the C "
+                            "restatement of the reference's aodmao2rho_loop_parallel (MC=252, KC=512, MR=14, NR=16)")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-dgemm", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    log("[bench] generating the rho workload (seed 0)")
+    p = W.rho_problem(seed=0)
+    log(f"[bench] nao={p.nao} ngrid={p.ngrid} nset={p.nset} blocks={p.nblk} mean|L_b|={p.counts().mean():.1f}")
+    log(f"[bench] rho: {args.warmup} warmup + {args.steps} timed steps")
+    r = bench_rho(p, args.steps, args.warmup)
+    evals_per_s = args.steps / (r["gpu_ms"] * 1e-3)
+    ms_per_step = r["gpu_ms"] / args.steps
+    flops = p.flops
+    kern_tflops = flops / (r["kernel_avg_ms"] * 1e-3) / 1e12
+
+    out = {
+        "metric": METRIC,
+        "value": round(evals_per_s, 3),
+        "unit": "rho evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": round(evals_per_s / README_RHO_EVALS_PER_S, 2),
+        "vs_baseline_ref": "README.md:51 masked rho (nao 2228, ngrid 16384, 30 %, 3 DMs) 120 ms on a 16-core Ryzen "
+                           "7945HX = 8.33 evals/s",
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "masked AO-DM-AO rho, FP64: 700 shells (s/p/d/f 40/30/20/10 %) -> nao 2228, "
+                               "ngrid 16384 = 256 blocks x 64 points, nset 3 PSD DMs, 30 % (block, shell) mask",
+                   "nao": p.nao, "ngrid": p.ngrid, "nset": p.nset, "blksize": p.blksize,
+                   "mask_density": round(float(p.shell_tab.mean()), 4),
+                   "skipped_tile_fraction": round(p.skipped_fraction(), 4),
+                   "masked_gflop_per_eval": round(flops / 1e9, 3),
+                   "wall_s_timed_region": round(r["wall_s"], 4)},
+        "latency_ms": {"p50": round(float(np.percentile(r["per_step_ms"], 50)), 5),
+                       "p99": round(float(np.percentile(r["per_step_ms"], 99)), 5)},
+        "roofline": {"bound": "mfma", "kernel": "rho_chunk_kernel", "achieved": round(kern_tflops, 3),
+                     "peak": FP64_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(kern_tflops / FP64_MFMA_PEAK_TFLOPS, 4), "traffic": None,
+                     "kernel_avg_ms": round(r["kernel_avg_ms"], 5),
+                     "algorithmic_flop_per_launch": flops,
+                     "compulsory_bytes_per_eval": p.compulsory_bytes(),
+                     "achieved_compulsory_GBs": round(p.compulsory_bytes() / (ms_per_step * 1e-3) / 1e9, 1)},
+    }
+
+    if not args.no_cpu:
+        log("[bench] cpu baseline (oracle) on the host cores")
+        ref, cpu = cpu_baseline_rho(p)
+        out["cpu_baseline"] = cpu
+        out["rel_err"] = rel_fro(r["rho_once"], ref)
+        log(f"[bench] cpu baseline {cpu['value']} evals/s on {cpu['cores']} threads; rel_err {out['rel_err']:.3e}")
+    else:
+        out["cpu_baseline"] = None
+        out["rel_err"] = None
+
+    if not args.no_dgemm:
+        dg = {}
+        log("[bench] dgemm 1031x977x1543 NT")
+        dg["small_nt"] = bench_dgemm(W.dgemm_small_nt(seed=0), reps=50)
+        log(f"[bench]   {dg['small_nt']}")
+        log("[bench] dgemm 3527x9583x6581 NN")
+        dg["large_nn"] = bench_dgemm(W.dgemm_large_nn(seed=0), reps=10)
+        log(f"[bench]   {dg['large_nn']}")
+        out["dgemm"] = dg
+
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,235 @@
+"""Seeded synthetic workloads for the two hot paths (numpy PCG64 streams).
+
+* DGEMM: row-major f64 operands, i.i.d. uniform[-1, 1).
+* Masked AO-DM-AO rho: a random "molecule" -- shells of 1/3/5/7 AOs (s/p/d/f)
+  at uniform points in a cube, grid blocks of jittered points, Gaussian-damped
+  AO values phi[u][g] = N(0,1) * exp(-0.3 r^2), a per-(grid block, shell)
+  non0tab bit set where the block-to-shell minimum distance is below the
+  quantile cutoff that gives the requested density, phi zeroed under off bits,
+  and PSD density matrices D_s = C_s C_s^T / nocc.
+
+Shell-count rule: 700 shells at 40/30/20/10 % s/p/d/f give 2100 AOs on
+average, not the 2228 asked for; shell types are drawn with those
+probabilities and then s shells are promoted to p (+2 AOs each) in draw order
+(or p demoted to s when over) until the AO count hits the target exactly.
+
+Also: the reference tests' own closed-form generators (sin/cos operands, the
+periodic 30 % and 25 % masks), reproduced exactly
+(src/impl_matmul.rs:258-259, src/impl_matmul_non0tab.rs:343-357,
+src/impl_aodmao2rho.rs:269-296).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+SHELL_SIZES = np.array([1, 3, 5, 7])
+SHELL_PROBS = np.array([0.4, 0.3, 0.2, 0.1])
+
+
+# ---------------------------------------------------------------------------
+# DGEMM
+# ---------------------------------------------------------------------------
+@dataclass
+class GemmProblem:
+    m: int
+    n: int
+    k: int
+    transa: bool
+    transb: bool
+    a: np.ndarray
+    b: np.ndarray
+    lda: int
+    ldb: int
+    ldc: int
+
+    @property
+    def flops(self) -> float:
+        return 2.0 * self.m * self.n * self.k
+
+    def dense(self):
+        """(op(A), op(B)) as 2-D numpy views."""
+        A = self.a.reshape(-1, self.lda)
+        B = self.b.reshape(-1, self.ldb)
+        A = A[: self.k, : self.m].T if self.transa else A[: self.m, : self.k]
+        B = B[: self.n, : self.k].T if self.transb else B[: self.k, : self.n]
+        return A, B
+
+
+def gemm_problem(m, n, k, transa=False, transb=False, seed=0, lda=None, ldb=None, ldc=None) -> GemmProblem:
+    rng = np.random.default_rng(seed)
+    lda = lda or (m if transa else k)
+    ldb = ldb or (k if transb else n)
+    ldc = ldc or n
+    a_rows = k if transa else m
+    b_rows = n if transb else k
+    a = rng.uniform(-1.0, 1.0, size=a_rows * lda)
+    b = rng.uniform(-1.0, 1.0, size=b_rows * ldb)
+    return GemmProblem(m, n, k, transa, transb, a, b, lda, ldb, ldc)
+
+
+def dgemm_small_nt(seed=0) -> GemmProblem:
+    """C(1031x977) = A(1031x1543) . B(977x1543)^T, lda = ldb = 1543, ldc = 977."""
+    return gemm_problem(1031, 977, 1543, transa=False, transb=True, seed=seed)
+
+
+def dgemm_large_nn(seed=0) -> GemmProblem:
+    """(3527x6581) . (6581x9583), NN -- the README's benchmark shape (README.md:11-17)."""
+    return gemm_problem(3527, 9583, 6581, transa=False, transb=False, seed=seed)
+
+
+# ---------------------------------------------------------------------------
+# masked rho
+# ---------------------------------------------------------------------------
+@dataclass
+class RhoProblem:
+    nao: int
+    ngrid: int
+    nset: int
+    blksize: int
+    phi: np.ndarray          # (nao, ngrid) float64, zero under off bits
+    dms: np.ndarray          # (nset, nao, nao) float64
+    tab: np.ndarray          # (nao, nblk) uint8, per-AO expansion of the per-shell bits
+    shell_tab: np.ndarray    # (nshell, nblk) bool
+    ao_loc: np.ndarray       # (nshell + 1,) AO offsets of the shells
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def nblk(self) -> int:
+        return self.tab.shape[1]
+
+    def counts(self) -> np.ndarray:
+        """|L_b| per grid block."""
+        return self.tab.astype(bool).sum(axis=0)
+
+    @property
+    def flops(self) -> float:
+        """Masked algorithmic flops: sum_s sum_b npts_b * (2 n_b^2 + 2 n_b)."""
+        n_b = self.counts().astype(np.float64)
+        npts = np.full(self.nblk, float(self.blksize))
+        npts[-1] = self.ngrid - self.blksize * (self.nblk - 1)
+        return float(self.nset * np.sum(npts * (2.0 * n_b * n_b + 2.0 * n_b)))
+
+    @property
+    def dense_flops(self) -> float:
+        return float(self.nset) * self.ngrid * (2.0 * self.nao * self.nao + 2.0 * self.nao)
+
+    def skipped_fraction(self) -> float:
+        n_b = self.counts().astype(np.float64)
+        return float(1.0 - np.sum(n_b * n_b) / (self.nblk * float(self.nao) ** 2))
+
+    def compulsory_bytes(self) -> float:
+        """phi nonzero blocks read once + D + rho read/write + mask."""
+        n_b = self.counts().astype(np.float64)
+        phi = 8.0 * self.blksize * float(n_b.sum())
+        return phi + 8.0 * self.nset * self.nao ** 2 + 16.0 * self.nset * self.ngrid + float(self.tab.size)
+
+
+def _shell_types(rng, nshell, nao_target):
+    types = rng.choice(4, size=nshell, p=SHELL_PROBS)
+    sizes = SHELL_SIZES[types].copy()
+    if nao_target is not None:
+        total = int(sizes.sum())
+        if (nao_target - total) % 2 != 0:
+            raise ValueError("target AO count must have the parity of the shell count")
+        i = 0
+        while total != nao_target:
+            if i >= nshell:
+                raise ValueError("cannot reach the target AO count by promoting/demoting shells")
+            if total < nao_target and sizes[i] == 1:
+                sizes[i] = 3
+                total += 2
+            elif total > nao_target and sizes[i] == 3:
+                sizes[i] = 1
+                total -= 2
+            i += 1
+    return sizes
+
+
+def rho_problem(seed=0, nshell=700, nao=2228, ngrid=16384, nset=3, blksize=64, density=0.3, box=25.0,
+                nocc=200, dm_logspan=None) -> RhoProblem:
+    """The masked AO-DM-AO workload (defaults: 700 shells, 2228 AOs, 256 blocks x 64 points, 3 PSD DMs, 30 %).
+
+    ``dm_logspan=(lo, hi)`` replaces each D entry's magnitude by 10**U(lo, hi),
+    keeping its sign and the symmetry of D (entries drawn for i <= j, mirrored).
+    """
+    rng = np.random.default_rng(seed)
+    sizes = _shell_types(rng, nshell, nao)
+    nao = int(sizes.sum())
+    ao_loc = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    shell_of_ao = np.repeat(np.arange(nshell), sizes)
+    centers = rng.uniform(0.0, box, size=(nshell, 3))
+
+    nblk = (ngrid + blksize - 1) // blksize
+    blk_centers = rng.uniform(0.0, box, size=(nblk, 3))
+    blk_of_pt = np.minimum(np.arange(ngrid) // blksize, nblk - 1)
+    points = blk_centers[blk_of_pt] + rng.uniform(-0.5, 0.5, size=(ngrid, 3))
+
+    # per (block, shell) minimum distance
+    mind = np.empty((nshell, nblk))
+    for b in range(nblk):
+        pts = points[b * blksize: min((b + 1) * blksize, ngrid)]
+        d2 = ((pts[:, None, :] - centers[None, :, :]) ** 2).sum(-1)
+        mind[:, b] = np.sqrt(d2.min(axis=0))
+    cutoff = np.quantile(mind, density)
+    shell_tab = mind < cutoff
+    tab = shell_tab[shell_of_ao].astype(np.uint8)
+
+    # phi[u][g] = N(0,1) exp(-0.3 r^2), zero under off bits
+    r2 = ((centers[:, None, :] - points[None, :, :]) ** 2).sum(-1)  # (nshell, ngrid)
+    damp = np.exp(-0.3 * r2)
+    phi = rng.standard_normal((nao, ngrid))
+    phi *= damp[shell_of_ao]
+    on = np.repeat(tab.astype(bool), blksize, axis=1)[:, :ngrid]
+    phi[~on] = 0.0
+
+    dms = np.empty((nset, nao, nao))
+    for s in range(nset):
+        C = rng.standard_normal((nao, nocc))
+        dms[s] = C @ C.T / nocc
+        if dm_logspan is not None:
+            lo, hi = dm_logspan
+            mag = 10.0 ** rng.uniform(lo, hi, size=(nao, nao))
+            mag = np.triu(mag) + np.triu(mag, 1).T
+            dms[s] = np.sign(dms[s]) * mag
+    meta = dict(seed=seed, nshell=nshell, density=density, cutoff=float(cutoff), box=box, nocc=nocc,
+                dm_logspan=dm_logspan)
+    return RhoProblem(nao, ngrid, nset, blksize, np.ascontiguousarray(phi), np.ascontiguousarray(dms), tab, shell_tab,
+                      ao_loc, meta)
+
+
+def rho_reference_dense(p: RhoProblem, phi_lhs=None, phi_rhs=None) -> np.ndarray:
+    """rho[s][g] = sum_{u,v} lhs[u,g] D_s[u,v] rhs[v,g] with lhs/rhs zeroed under off bits (numpy float64)."""
+    on = np.repeat(p.tab.astype(bool), p.blksize, axis=1)[:, : p.ngrid]
+    L = np.where(on, p.phi if phi_lhs is None else phi_lhs, 0.0)
+    R = np.where(on, p.phi if phi_rhs is None else phi_rhs, 0.0)
+    out = np.empty((p.nset, p.ngrid))
+    for s in range(p.nset):
+        out[s] = np.einsum("ug,ug->g", L, p.dms[s] @ R)
+    return out
+
+
+# ---------------------------------------------------------------------------
+# the reference tests' closed-form generators
+# ---------------------------------------------------------------------------
+MASK_30 = np.array([True, False, True, False, False, True, False, False, False, False])  # src/impl_matmul_non0tab.rs:343
+MASK_25 = np.array([True, False, True, False, False, False, False, False, False, False,
+                    False, True, True, False, False, False, False, False, True, False])  # src/impl_aodmao2rho.rs:277-280
+
+
+def sin_cos_operands(na: int, nb: int):
+    """a[x] = sin(x), b[x] = cos(x) over flat indices (src/impl_matmul.rs:258-259)."""
+    return np.sin(np.arange(na, dtype=np.float64)), np.cos(np.arange(nb, dtype=np.float64))
+
+
+def periodic_mask(nrows: int, nblk: int, pattern: np.ndarray) -> np.ndarray:
+    """tab[i] = pattern[i % len(pattern)] over the flat (row, blk) index."""
+    idx = np.arange(nrows * nblk)
+    return pattern[idx % len(pattern)].reshape(nrows, nblk).astype(np.uint8)
+
+
+def zero_under_mask(x: np.ndarray, tab: np.ndarray, blksize: int, ncols: int) -> None:
+    """x[p][j] = 0 where !tab[p][j / blksize] (src/impl_matmul_non0tab.rs:346-357)."""
+    on = np.repeat(tab.astype(bool), blksize, axis=1)[:, :ncols]
+    x.reshape(tab.shape[0], -1)[:, :ncols][~on] = 0.0

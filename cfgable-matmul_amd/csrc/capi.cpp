@@ -1,0 +1,626 @@
+// capi.cpp -- the extern "C" boundary of libcfgmm (declared in include/cfgmm.h).
+//
+// Host-pointer entry points mirror the reference's whole-problem functions
+// (src/impl_matmul.rs:155-248, src/impl_matmul_non0tab.rs:210-326,
+// src/impl_aodmao2rho.rs:171-261): same argument meaning, accumulate into the
+// output, synchronous return.  Where the reference panics (slice index out of
+// bounds, `unimplemented!` for transb with a mask, `assert!` on BLKSIZE) the
+// C ABI returns a status code and records a message for cfgmm_last_error().
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "internal.hpp"
+
+namespace cfgmm {
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+static thread_local std::string g_last_error;
+
+void set_error(const std::string &msg) { g_last_error = msg; }
+void clear_error() { g_last_error.clear(); }
+int fail(int status, const std::string &msg) {
+    set_error(msg);
+    return status;
+}
+
+int hip_status(hipError_t e, const char *what) {
+    if (e == hipSuccess) return CFGMM_OK;
+    std::string msg = std::string(what) + ": " + hipGetErrorString(e);
+    if (e == hipErrorOutOfMemory) return fail(CFGMM_EOOM, msg);
+    if (e == hipErrorNoDevice || e == hipErrorNoBinaryForGpu || e == hipErrorInvalidDevice)
+        return fail(CFGMM_ENODEV, msg);
+    return fail(CFGMM_EHIP, msg);
+}
+
+// ---------------------------------------------------------------------------
+// tracing
+// ---------------------------------------------------------------------------
+struct TraceRec {
+    std::string name;
+    hipEvent_t start, end;
+};
+static std::atomic<bool> g_trace_on{false};
+static std::mutex g_trace_mutex;
+static std::vector<TraceRec> g_trace;
+
+hipEvent_t trace_start(hipStream_t stream) {
+    if (!g_trace_on.load(std::memory_order_relaxed)) return nullptr;
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    if (hipEventRecord(e, stream) != hipSuccess) {
+        (void)hipEventDestroy(e);
+        return nullptr;
+    }
+    return e;
+}
+
+void trace_stop(hipStream_t stream, const char *name, hipEvent_t start) {
+    if (!start) return;
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess || hipEventRecord(e, stream) != hipSuccess) {
+        (void)hipEventDestroy(start);
+        if (e) (void)hipEventDestroy(e);
+        return;
+    }
+    std::lock_guard<std::mutex> lock(g_trace_mutex);
+    g_trace.push_back(TraceRec{name, start, e});
+}
+
+// ---------------------------------------------------------------------------
+// device checks and properties
+// ---------------------------------------------------------------------------
+static std::mutex g_dev_mutex;
+static std::map<int, std::pair<int, int>> g_dev_info;  // device -> (ok, cu count)
+
+static int check_device(int *cus_out = nullptr) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_status(e, "hipGetDevice");
+    std::lock_guard<std::mutex> lock(g_dev_mutex);
+    auto it = g_dev_info.find(dev);
+    if (it == g_dev_info.end()) {
+        hipDeviceProp_t prop;
+        e = hipGetDeviceProperties(&prop, dev);
+        if (e != hipSuccess) return hip_status(e, "hipGetDeviceProperties");
+        const bool ok = std::strncmp(prop.gcnArchName, "gfx950", 6) == 0;
+        it = g_dev_info.emplace(dev, std::make_pair(ok ? 1 : 0, prop.multiProcessorCount)).first;
+        if (!ok) set_error(std::string("device is ") + prop.gcnArchName + ", libcfgmm is built for gfx950 only");
+    }
+    if (!it->second.first)
+        return fail(CFGMM_ENODEV, "libcfgmm is built for gfx950 only; the current device is not gfx950");
+    if (cus_out) *cus_out = it->second.second;
+    return CFGMM_OK;
+}
+
+int device_cu_count() {
+    int cus = 256;
+    check_device(&cus);
+    return cus;
+}
+
+// ---------------------------------------------------------------------------
+// per-stream workspace
+// ---------------------------------------------------------------------------
+static std::mutex g_ws_mutex;
+static std::map<std::pair<int, hipStream_t>, Workspace> g_ws;
+
+int workspace_get(hipStream_t stream, size_t bytes, void **out) {
+    int dev = 0;
+    CFGMM_HIP_TRY(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(g_ws_mutex);
+    Workspace &ws = g_ws[std::make_pair(dev, stream)];
+    if (ws.bytes < bytes) {
+        if (ws.ptr) {
+            // kernels already enqueued on this stream may still read the old buffer
+            CFGMM_HIP_TRY(hipStreamSynchronize(stream));
+            CFGMM_HIP_TRY(hipFree(ws.ptr));
+            ws.ptr = nullptr;
+            ws.bytes = 0;
+        }
+        const size_t want = std::max(bytes + bytes / 4, (size_t)1 << 20);
+        void *p = nullptr;
+        CFGMM_HIP_TRY(hipMalloc(&p, want));
+        ws.ptr = p;
+        ws.bytes = want;
+    }
+    *out = ws.ptr;
+    return CFGMM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// argument validation (extents the reference would index)
+// ---------------------------------------------------------------------------
+static bool mul_add_overflows(size_t a, size_t b, size_t c, size_t *out) {
+    size_t t;
+    if (__builtin_mul_overflow(a, b, &t)) return true;
+    return __builtin_add_overflow(t, c, out);
+}
+
+// Elements spanned by `rows` rows of `cols` entries at leading dimension `ld`.
+static int extent(size_t rows, size_t cols, size_t ld, size_t *out) {
+    if (rows == 0 || cols == 0) {
+        *out = 0;
+        return CFGMM_OK;
+    }
+    if (mul_add_overflows(rows - 1, ld, cols, out)) return fail(CFGMM_EINVAL_DIMS, "extent overflows size_t");
+    return CFGMM_OK;
+}
+
+static int check_len(const char *name, const void *ptr, size_t len, size_t need) {
+    if (need == 0) return CFGMM_OK;
+    if (!ptr) return fail(CFGMM_EINVAL, std::string(name) + " is null");
+    if (len < need)
+        return fail(CFGMM_EBOUNDS, std::string(name) + ": slice length " + std::to_string(len) + " < required " +
+                                       std::to_string(need) + " (index out of bounds)");
+    return CFGMM_OK;
+}
+
+static int check_ld(const char *name, size_t ld, size_t need, bool used) {
+    if (used && ld < need)
+        return fail(CFGMM_EINVAL_DIMS, std::string(name) + " = " + std::to_string(ld) + " < row length " +
+                                           std::to_string(need));
+    return CFGMM_OK;
+}
+
+#define CFGMM_TRY(expr)                 \
+    do {                                \
+        int _st = (expr);               \
+        if (_st != CFGMM_OK) return _st; \
+    } while (0)
+
+struct GemmExtents {
+    size_t a, b, c;
+};
+
+static int gemm_extents(size_t m, size_t n, size_t k, size_t lda, size_t ldb, size_t ldc, bool ta, bool tb,
+                        GemmExtents *ext) {
+    const bool any = m && n && k;
+    CFGMM_TRY(check_ld("lda", lda, ta ? m : k, any));
+    CFGMM_TRY(check_ld("ldb", ldb, tb ? k : n, any));
+    CFGMM_TRY(check_ld("ldc", ldc, n, m && n));
+    ext->a = ext->b = 0;
+    if (any) {
+        CFGMM_TRY(ta ? extent(k, m, lda, &ext->a) : extent(m, k, lda, &ext->a));
+        CFGMM_TRY(tb ? extent(n, k, ldb, &ext->b) : extent(k, n, ldb, &ext->b));
+    }
+    CFGMM_TRY(extent(m, n, ldc, &ext->c));
+    return CFGMM_OK;
+}
+
+static int check_blksize(size_t blksize) {
+    if (blksize == 0 || blksize % 16 != 0)
+        return fail(CFGMM_EINVAL, "blksize must be a positive multiple of 16 (BLKSIZE % (NR_LANE*LANE) == 0), got " +
+                                      std::to_string(blksize));
+    return CFGMM_OK;
+}
+
+struct RhoExtents {
+    size_t rhos, dms, ao, tab;
+};
+
+static int rho_extents(size_t stride_ao, size_t stride_grid, size_t nao, size_t ngrid, size_t nset, size_t ldtab,
+                       size_t blksize, RhoExtents *ext) {
+    CFGMM_TRY(check_blksize(blksize));
+    const bool any = nao && ngrid && nset;
+    const size_t nblk = ngrid ? div_up(ngrid, blksize) : 0;
+    CFGMM_TRY(check_ld("stride_ao", stride_ao, nao, any));
+    CFGMM_TRY(check_ld("stride_grid", stride_grid, ngrid, any));
+    CFGMM_TRY(check_ld("ldtab", ldtab, nblk, any));
+    *ext = RhoExtents{0, 0, 0, 0};
+    if (!any) return CFGMM_OK;
+    size_t sa2;
+    if (__builtin_mul_overflow(stride_ao, stride_ao, &sa2)) return fail(CFGMM_EINVAL_DIMS, "stride_ao^2 overflows");
+    size_t dm_one;
+    CFGMM_TRY(extent(nao, nao, stride_ao, &dm_one));
+    if (mul_add_overflows(nset - 1, sa2, dm_one, &ext->dms)) return fail(CFGMM_EINVAL_DIMS, "dms extent overflows");
+    CFGMM_TRY(extent(nset, ngrid, stride_grid, &ext->rhos));
+    CFGMM_TRY(extent(nao, ngrid, stride_grid, &ext->ao));
+    CFGMM_TRY(extent(nao, nblk, ldtab, &ext->tab));
+    return CFGMM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// host-pointer staging: one stream and a set of grow-only device buffers per
+// calling thread (kept until the process ends).
+// ---------------------------------------------------------------------------
+struct HostStage {
+    int device = -1;
+    hipStream_t stream = nullptr;
+    void *buf[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    size_t cap[5] = {0, 0, 0, 0, 0};
+};
+static thread_local HostStage t_stage;
+
+static int stage_begin() {
+    CFGMM_TRY(check_device());
+    int dev = 0;
+    CFGMM_HIP_TRY(hipGetDevice(&dev));
+    if (t_stage.device != dev) {
+        // buffers of another device are abandoned, not freed (their device is not current)
+        t_stage = HostStage();
+        t_stage.device = dev;
+        CFGMM_HIP_TRY(hipStreamCreateWithFlags(&t_stage.stream, hipStreamNonBlocking));
+    }
+    return CFGMM_OK;
+}
+
+static int stage_buf(int slot, size_t bytes, void **out) {
+    if (t_stage.cap[slot] < bytes) {
+        if (t_stage.buf[slot]) {
+            CFGMM_HIP_TRY(hipStreamSynchronize(t_stage.stream));
+            CFGMM_HIP_TRY(hipFree(t_stage.buf[slot]));
+            t_stage.buf[slot] = nullptr;
+            t_stage.cap[slot] = 0;
+        }
+        CFGMM_HIP_TRY(hipMalloc(&t_stage.buf[slot], bytes));
+        t_stage.cap[slot] = bytes;
+    }
+    *out = t_stage.buf[slot];
+    return CFGMM_OK;
+}
+
+template <typename T>
+static int stage_in(int slot, const T *host, size_t count, T **dev) {
+    void *p = nullptr;
+    CFGMM_TRY(stage_buf(slot, std::max<size_t>(count * sizeof(T), 256), &p));
+    if (count) CFGMM_HIP_TRY(hipMemcpyAsync(p, host, count * sizeof(T), hipMemcpyHostToDevice, t_stage.stream));
+    *dev = static_cast<T *>(p);
+    return CFGMM_OK;
+}
+
+static int stage_out(double *host, const double *dev, size_t count) {
+    if (count)
+        CFGMM_HIP_TRY(hipMemcpyAsync(host, dev, count * sizeof(double), hipMemcpyDeviceToHost, t_stage.stream));
+    CFGMM_HIP_TRY(hipStreamSynchronize(t_stage.stream));
+    return CFGMM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// shared bodies
+// ---------------------------------------------------------------------------
+static int gemm_host(double *c, size_t c_len, const double *a, size_t a_len, const double *b, size_t b_len, size_t m,
+                     size_t n, size_t k, size_t lda, size_t ldb, size_t ldc, int transa, int transb,
+                     const uint8_t *tab, size_t tab_len, size_t ldtab, size_t blksize) {
+    clear_error();
+    const bool masked = tab != nullptr || blksize != 0;
+    if (masked && transb)
+        return fail(CFGMM_EINVAL_TRANSB_MASK,
+                    "transb=true is not implemented in matmul_loop_parallel_mnk_pack_a_non0tab");
+    if (masked) CFGMM_TRY(check_blksize(blksize));
+    GemmExtents ext;
+    CFGMM_TRY(gemm_extents(m, n, k, lda, ldb, ldc, transa, transb, &ext));
+    size_t tab_ext = 0;
+    if (masked && m && n && k) {
+        const size_t nblk = div_up(n, blksize);
+        CFGMM_TRY(check_ld("ldtab", ldtab, nblk, true));
+        CFGMM_TRY(extent(k, nblk, ldtab, &tab_ext));
+        CFGMM_TRY(check_len("tab", tab, tab_len, tab_ext));
+    }
+    CFGMM_TRY(check_len("c", c, c_len, ext.c));
+    CFGMM_TRY(check_len("a", a, a_len, ext.a));
+    CFGMM_TRY(check_len("b", b, b_len, ext.b));
+    if (!m || !n || !k) return CFGMM_OK;
+    CFGMM_TRY(stage_begin());
+    double *dc, *da, *db;
+    CFGMM_TRY(stage_in(0, c, ext.c, &dc));
+    CFGMM_TRY(stage_in(1, a, ext.a, &da));
+    CFGMM_TRY(stage_in(2, b, ext.b, &db));
+    if (masked) {
+        uint8_t *dt;
+        CFGMM_TRY(stage_in(3, tab, tab_ext, &dt));
+        CFGMM_TRY(launch_dgemm_non0tab(t_stage.stream, dc, da, db, m, n, k, lda, ldb, ldc, transa != 0, dt, ldtab,
+                                       blksize));
+    } else {
+        CFGMM_TRY(launch_dgemm(t_stage.stream, dc, da, db, m, n, k, lda, ldb, ldc, transa != 0, transb != 0));
+    }
+    return stage_out(c, dc, ext.c);
+}
+
+static int rho_host(double *rhos, size_t rhos_len, const double *dms, size_t dms_len, const double *ao_lhs,
+                    size_t ao_lhs_len, const double *ao_rhs, size_t ao_rhs_len, size_t stride_ao, size_t stride_grid,
+                    size_t nao, size_t ngrid, size_t nset, const uint8_t *tab, size_t tab_len, size_t ldtab,
+                    size_t blksize) {
+    clear_error();
+    RhoExtents ext;
+    CFGMM_TRY(rho_extents(stride_ao, stride_grid, nao, ngrid, nset, ldtab, blksize, &ext));
+    CFGMM_TRY(check_len("rhos", rhos, rhos_len, ext.rhos));
+    CFGMM_TRY(check_len("dms", dms, dms_len, ext.dms));
+    CFGMM_TRY(check_len("ao_lhs", ao_lhs, ao_lhs_len, ext.ao));
+    CFGMM_TRY(check_len("ao_rhs", ao_rhs, ao_rhs_len, ext.ao));
+    CFGMM_TRY(check_len("tab", tab, tab_len, ext.tab));
+    if (!nao || !ngrid || !nset) return CFGMM_OK;
+    CFGMM_TRY(stage_begin());
+    double *dr, *dd, *dl, *dright;
+    uint8_t *dt;
+    CFGMM_TRY(stage_in(0, rhos, ext.rhos, &dr));
+    CFGMM_TRY(stage_in(1, dms, ext.dms, &dd));
+    CFGMM_TRY(stage_in(2, ao_lhs, ext.ao, &dl));
+    if (ao_rhs == ao_lhs) {
+        dright = dl;
+    } else {
+        CFGMM_TRY(stage_in(4, ao_rhs, ext.ao, &dright));
+    }
+    CFGMM_TRY(stage_in(3, tab, ext.tab, &dt));
+    CFGMM_TRY(launch_rho(t_stage.stream, dr, dd, dl, dright, stride_ao, stride_grid, nao, ngrid, nset, dt, ldtab,
+                         blksize));
+    return stage_out(rhos, dr, ext.rhos);
+}
+
+static int check_dev_gemm(const double *c, const double *a, const double *b, size_t m, size_t n, size_t k,
+                          size_t lda, size_t ldb, size_t ldc, int transa, int transb) {
+    GemmExtents ext;
+    CFGMM_TRY(gemm_extents(m, n, k, lda, ldb, ldc, transa, transb, &ext));
+    if (m && n && k && (!c || !a || !b)) return fail(CFGMM_EINVAL, "null device pointer");
+    return check_device();
+}
+
+}  // namespace cfgmm
+
+using namespace cfgmm;
+
+extern "C" {
+
+const char *cfgmm_last_error(void) { return g_last_error.c_str(); }
+int cfgmm_version(void) { return 10000; }
+
+// ---- GEMM ------------------------------------------------------------------
+int cfgmm_matmul_loop_parallel_mnk_pack_a(double *c, size_t c_len, const double *a, size_t a_len, const double *b,
+                                          size_t b_len, size_t m, size_t n, size_t k, size_t lda, size_t ldb,
+                                          size_t ldc, int transa, int transb) {
+    return gemm_host(c, c_len, a, a_len, b, b_len, m, n, k, lda, ldb, ldc, transa, transb, nullptr, 0, 0, 0);
+}
+
+int cfgmm_matmul_loop_macro_mb(double *c, size_t c_len, const double *a, size_t a_len, const double *b, size_t b_len,
+                               size_t m, size_t n, size_t k, size_t lda, size_t ldb, size_t ldc, int transa,
+                               int transb) {
+    return gemm_host(c, c_len, a, a_len, b, b_len, m, n, k, lda, ldb, ldc, transa, transb, nullptr, 0, 0, 0);
+}
+
+int cfgmm_matmul_anyway_full(double *c, size_t c_len, const double *a, size_t a_len, const double *b, size_t b_len,
+                             size_t m, size_t n, size_t k, size_t lda, size_t ldb, size_t ldc, int transa,
+                             int transb) {
+    return gemm_host(c, c_len, a, a_len, b, b_len, m, n, k, lda, ldb, ldc, transa, transb, nullptr, 0, 0, 0);
+}
+
+int cfgmm_dgemm_dev(void *stream, double *c, const double *a, const double *b, size_t m, size_t n, size_t k,
+                    size_t lda, size_t ldb, size_t ldc, int transa, int transb) {
+    clear_error();
+    CFGMM_TRY(check_dev_gemm(c, a, b, m, n, k, lda, ldb, ldc, transa, transb));
+    return launch_dgemm(static_cast<hipStream_t>(stream), c, a, b, m, n, k, lda, ldb, ldc, transa != 0, transb != 0);
+}
+
+int cfgmm_dgemm_ex_dev(void *stream, double *c, const double *a, const double *b, size_t m, size_t n, size_t k,
+                       size_t lda, size_t ldb, size_t ldc, int transa, int transb, int precision, int nslices) {
+    clear_error();
+    CFGMM_TRY(check_dev_gemm(c, a, b, m, n, k, lda, ldb, ldc, transa, transb));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (precision == CFGMM_PREC_FP64)
+        return launch_dgemm(s, c, a, b, m, n, k, lda, ldb, ldc, transa != 0, transb != 0);
+    if (precision == CFGMM_PREC_OZAKI_INT8) {
+        if (nslices < 2 || nslices > 8) return fail(CFGMM_EINVAL, "nslices must be in [2, 8]");
+        return launch_dgemm_ozaki(s, c, a, b, m, n, k, lda, ldb, ldc, transa != 0, transb != 0, nslices);
+    }
+    return fail(CFGMM_EINVAL, "unknown precision mode " + std::to_string(precision));
+}
+
+// ---- masked GEMM -------------------------------------------------------------
+int cfgmm_matmul_loop_parallel_mnk_pack_a_non0tab(double *c, size_t c_len, const double *a, size_t a_len,
+                                                  const double *b, size_t b_len, size_t m, size_t n, size_t k,
+                                                  size_t lda, size_t ldb, size_t ldc, int transa, int transb,
+                                                  const uint8_t *tab, size_t tab_len, size_t ldtab, size_t blksize) {
+    if (blksize == 0) {
+        clear_error();
+        return fail(CFGMM_EINVAL, "blksize must be a positive multiple of 16, got 0");
+    }
+    return gemm_host(c, c_len, a, a_len, b, b_len, m, n, k, lda, ldb, ldc, transa, transb, tab, tab_len, ldtab,
+                     blksize);
+}
+
+int cfgmm_matmul_anyway_full_non0tab(double *c, size_t c_len, const double *a, size_t a_len, const double *b,
+                                     size_t b_len, size_t m, size_t n, size_t k, size_t lda, size_t ldb, size_t ldc,
+                                     int transa, int transb, const uint8_t *tab, size_t tab_len, size_t ldtab) {
+    return gemm_host(c, c_len, a, a_len, b, b_len, m, n, k, lda, ldb, ldc, transa, transb, tab, tab_len, ldtab, 48);
+}
+
+int cfgmm_dgemm_non0tab_dev(void *stream, double *c, const double *a, const double *b, size_t m, size_t n, size_t k,
+                            size_t lda, size_t ldb, size_t ldc, int transa, int transb, const uint8_t *tab,
+                            size_t ldtab, size_t blksize) {
+    clear_error();
+    if (transb)
+        return fail(CFGMM_EINVAL_TRANSB_MASK,
+                    "transb=true is not implemented in matmul_loop_parallel_mnk_pack_a_non0tab");
+    CFGMM_TRY(check_blksize(blksize));
+    CFGMM_TRY(check_dev_gemm(c, a, b, m, n, k, lda, ldb, ldc, transa, 0));
+    if (m && n && k) {
+        CFGMM_TRY(check_ld("ldtab", ldtab, div_up(n, blksize), true));
+        if (!tab) return fail(CFGMM_EINVAL, "tab is null");
+    }
+    return launch_dgemm_non0tab(static_cast<hipStream_t>(stream), c, a, b, m, n, k, lda, ldb, ldc, transa != 0, tab,
+                                ldtab, blksize);
+}
+
+// ---- rho ------------------------------------------------------------------------
+int cfgmm_aodmao2rho_loop_parallel(double *rhos, size_t rhos_len, const double *dms, size_t dms_len,
+                                   const double *ao_lhs, size_t ao_lhs_len, const double *ao_rhs, size_t ao_rhs_len,
+                                   size_t stride_ao, size_t stride_grid, size_t nao, size_t ngrid, size_t nset,
+                                   const uint8_t *tab, size_t tab_len, size_t ldtab, size_t blksize) {
+    return rho_host(rhos, rhos_len, dms, dms_len, ao_lhs, ao_lhs_len, ao_rhs, ao_rhs_len, stride_ao, stride_grid, nao,
+                    ngrid, nset, tab, tab_len, ldtab, blksize);
+}
+
+int cfgmm_aodmao2rho_anyway(double *rhos, size_t rhos_len, const double *dms, size_t dms_len, const double *ao_lhs,
+                            size_t ao_lhs_len, const double *ao_rhs, size_t ao_rhs_len, size_t stride_ao,
+                            size_t stride_grid, size_t nao, size_t ngrid, size_t nset, const uint8_t *tab,
+                            size_t tab_len, size_t ldtab) {
+    return rho_host(rhos, rhos_len, dms, dms_len, ao_lhs, ao_lhs_len, ao_rhs, ao_rhs_len, stride_ao, stride_grid, nao,
+                    ngrid, nset, tab, tab_len, ldtab, 48);
+}
+
+static int rho_dev_common(void *stream, double *rhos, const double *dms, const double *ao_lhs, const double *ao_rhs,
+                          size_t stride_ao, size_t stride_grid, size_t nao, size_t ngrid, size_t nset,
+                          const uint8_t *tab, size_t ldtab, size_t blksize, int precision, int nslices) {
+    clear_error();
+    RhoExtents ext;
+    CFGMM_TRY(rho_extents(stride_ao, stride_grid, nao, ngrid, nset, ldtab, blksize, &ext));
+    if (nao && ngrid && nset && (!rhos || !dms || !ao_lhs || !ao_rhs || !tab))
+        return fail(CFGMM_EINVAL, "null device pointer");
+    CFGMM_TRY(check_device());
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (precision == CFGMM_PREC_FP64)
+        return launch_rho(s, rhos, dms, ao_lhs, ao_rhs, stride_ao, stride_grid, nao, ngrid, nset, tab, ldtab, blksize);
+    if (precision == CFGMM_PREC_OZAKI_INT8) {
+        if (nslices < 2 || nslices > 8) return fail(CFGMM_EINVAL, "nslices must be in [2, 8]");
+        return launch_rho_ozaki(s, rhos, dms, ao_lhs, ao_rhs, stride_ao, stride_grid, nao, ngrid, nset, tab, ldtab,
+                                blksize, nslices);
+    }
+    return fail(CFGMM_EINVAL, "unknown precision mode " + std::to_string(precision));
+}
+
+int cfgmm_aodmao2rho_dev(void *stream, double *rhos, const double *dms, const double *ao_lhs, const double *ao_rhs,
+                         size_t stride_ao, size_t stride_grid, size_t nao, size_t ngrid, size_t nset,
+                         const uint8_t *tab, size_t ldtab, size_t blksize) {
+    return rho_dev_common(stream, rhos, dms, ao_lhs, ao_rhs, stride_ao, stride_grid, nao, ngrid, nset, tab, ldtab,
+                          blksize, CFGMM_PREC_FP64, 0);
+}
+
+int cfgmm_aodmao2rho_ex_dev(void *stream, double *rhos, const double *dms, const double *ao_lhs,
+                            const double *ao_rhs, size_t stride_ao, size_t stride_grid, size_t nao, size_t ngrid,
+                            size_t nset, const uint8_t *tab, size_t ldtab, size_t blksize, int precision,
+                            int nslices) {
+    return rho_dev_common(stream, rhos, dms, ao_lhs, ao_rhs, stride_ao, stride_grid, nao, ngrid, nset, tab, ldtab,
+                          blksize, precision, nslices);
+}
+
+// ---- plumbing -------------------------------------------------------------------
+int cfgmm_device_count(int *count) {
+    clear_error();
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e == hipErrorNoDevice) n = 0, e = hipSuccess;
+    CFGMM_HIP_TRY(e);
+    *count = n;
+    return CFGMM_OK;
+}
+int cfgmm_device_malloc(void **ptr, size_t bytes) {
+    clear_error();
+    CFGMM_TRY(check_device());
+    return hip_status(hipMalloc(ptr, std::max<size_t>(bytes, 1)), "hipMalloc");
+}
+int cfgmm_device_free(void *ptr) { return hip_status(hipFree(ptr), "hipFree"); }
+int cfgmm_memcpy_htod(void *dst, const void *src, size_t bytes) {
+    return hip_status(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), "hipMemcpy H2D");
+}
+int cfgmm_memcpy_dtoh(void *dst, const void *src, size_t bytes) {
+    return hip_status(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+}
+int cfgmm_memcpy_htod_async(void *dst, const void *src, size_t bytes, void *stream) {
+    return hip_status(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, static_cast<hipStream_t>(stream)),
+                      "hipMemcpyAsync H2D");
+}
+int cfgmm_memcpy_dtoh_async(void *dst, const void *src, size_t bytes, void *stream) {
+    return hip_status(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, static_cast<hipStream_t>(stream)),
+                      "hipMemcpyAsync D2H");
+}
+int cfgmm_memset_dev(void *dst, int value, size_t bytes) {
+    return hip_status(hipMemset(dst, value, bytes), "hipMemset");
+}
+int cfgmm_mem_get_info(size_t *free_bytes, size_t *total_bytes) {
+    return hip_status(hipMemGetInfo(free_bytes, total_bytes), "hipMemGetInfo");
+}
+int cfgmm_stream_create(void **stream) {
+    hipStream_t s = nullptr;
+    CFGMM_HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *stream = s;
+    return CFGMM_OK;
+}
+int cfgmm_stream_destroy(void *stream) {
+    {
+        int dev = 0;
+        if (hipGetDevice(&dev) == hipSuccess) {
+            std::lock_guard<std::mutex> lock(g_ws_mutex);
+            auto it = g_ws.find(std::make_pair(dev, static_cast<hipStream_t>(stream)));
+            if (it != g_ws.end()) {
+                (void)hipStreamSynchronize(static_cast<hipStream_t>(stream));
+                if (it->second.ptr) (void)hipFree(it->second.ptr);
+                g_ws.erase(it);
+            }
+        }
+    }
+    return hip_status(hipStreamDestroy(static_cast<hipStream_t>(stream)), "hipStreamDestroy");
+}
+int cfgmm_stream_synchronize(void *stream) {
+    return hip_status(hipStreamSynchronize(static_cast<hipStream_t>(stream)), "hipStreamSynchronize");
+}
+int cfgmm_device_synchronize(void) { return hip_status(hipDeviceSynchronize(), "hipDeviceSynchronize"); }
+int cfgmm_event_create(void **event) {
+    hipEvent_t e = nullptr;
+    CFGMM_HIP_TRY(hipEventCreate(&e));
+    *event = e;
+    return CFGMM_OK;
+}
+int cfgmm_event_destroy(void *event) {
+    return hip_status(hipEventDestroy(static_cast<hipEvent_t>(event)), "hipEventDestroy");
+}
+int cfgmm_event_record(void *event, void *stream) {
+    return hip_status(hipEventRecord(static_cast<hipEvent_t>(event), static_cast<hipStream_t>(stream)),
+                      "hipEventRecord");
+}
+int cfgmm_event_synchronize(void *event) {
+    return hip_status(hipEventSynchronize(static_cast<hipEvent_t>(event)), "hipEventSynchronize");
+}
+int cfgmm_event_query(void *event) {
+    hipError_t e = hipEventQuery(static_cast<hipEvent_t>(event));
+    if (e == hipErrorNotReady) return 1;
+    return hip_status(e, "hipEventQuery");
+}
+int cfgmm_event_elapsed_ms(float *ms, void *start, void *end) {
+    return hip_status(
+        hipEventElapsedTime(ms, static_cast<hipEvent_t>(start), static_cast<hipEvent_t>(end)),
+        "hipEventElapsedTime");
+}
+
+int cfgmm_trace_enable(int on) {
+    g_trace_on.store(on != 0);
+    return CFGMM_OK;
+}
+
+int cfgmm_trace_reset(void) {
+    std::lock_guard<std::mutex> lock(g_trace_mutex);
+    for (auto &r : g_trace) {
+        (void)hipEventSynchronize(r.end);
+        (void)hipEventDestroy(r.start);
+        (void)hipEventDestroy(r.end);
+    }
+    g_trace.clear();
+    return CFGMM_OK;
+}
+
+int cfgmm_trace_read(const char *name, double *total_ms, long long *count) {
+    clear_error();
+    std::lock_guard<std::mutex> lock(g_trace_mutex);
+    double tot = 0.0;
+    long long n = 0;
+    for (auto &r : g_trace) {
+        if (r.name != name) continue;
+        CFGMM_HIP_TRY(hipEventSynchronize(r.end));
+        float ms = 0.f;
+        CFGMM_HIP_TRY(hipEventElapsedTime(&ms, r.start, r.end));
+        tot += ms;
+        ++n;
+    }
+    *total_ms = tot;
+    *count = n;
+    return CFGMM_OK;
+}
+
+}  // extern "C"

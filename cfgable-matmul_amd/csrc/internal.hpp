@@ -1,0 +1,73 @@
+// internal.hpp -- shared host/device helpers of libcfgmm (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "../../include/cfgmm.h"
+
+namespace cfgmm {
+
+// ---- error reporting ------------------------------------------------------
+// Thread-local last-error string returned by cfgmm_last_error().
+void set_error(const std::string &msg);
+void clear_error();
+int fail(int status, const std::string &msg);
+
+// Returns CFGMM_OK or records the HIP error and returns CFGMM_EHIP / CFGMM_EOOM.
+int hip_status(hipError_t e, const char *what);
+
+#define CFGMM_HIP_TRY(expr)                                      \
+    do {                                                         \
+        int _st = ::cfgmm::hip_status((expr), #expr);            \
+        if (_st != CFGMM_OK) return _st;                         \
+    } while (0)
+
+// ---- device workspace -------------------------------------------------------
+// A growable device buffer owned by one stream.  Entry points that need
+// scratch memory (split-K partials, compacted AO lists, rho partials) take it
+// from the workspace bound to their stream; growing it synchronizes that
+// stream first so that no in-flight kernel still reads the old buffer.
+struct Workspace {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+};
+int workspace_get(hipStream_t stream, size_t bytes, void **out);
+
+// ---- tracing -----------------------------------------------------------------
+// trace_start returns nullptr unless tracing is on; trace_stop(nullptr) is a no-op.
+hipEvent_t trace_start(hipStream_t stream);
+void trace_stop(hipStream_t stream, const char *name, hipEvent_t start);
+
+// ---- device properties ------------------------------------------------------
+int device_cu_count();  // compute units of the current device (cached per device)
+
+// ---- small math --------------------------------------------------------------
+__host__ __device__ inline size_t div_up(size_t a, size_t b) { return (a + b - 1) / b; }
+
+// ---- kernel launchers (one per .hip file) -----------------------------------
+// Compacts, for each column block b < nblk, the rows x < nrows with
+// tab[x*ldtab + b] != 0 into lists[b*nrows + *] (ascending) and counts[b].
+int launch_build_lists(hipStream_t stream, const uint8_t *tab, size_t ldtab, size_t nrows, size_t nblk, int *lists,
+                       int *counts);
+int launch_dgemm(hipStream_t stream, double *c, const double *a, const double *b, size_t m, size_t n, size_t k,
+                 size_t lda, size_t ldb, size_t ldc, bool transa, bool transb);
+int launch_dgemm_non0tab(hipStream_t stream, double *c, const double *a, const double *b, size_t m, size_t n,
+                         size_t k, size_t lda, size_t ldb, size_t ldc, bool transa, const uint8_t *tab, size_t ldtab,
+                         size_t blksize);
+int launch_rho(hipStream_t stream, double *rhos, const double *dms, const double *ao_lhs, const double *ao_rhs,
+               size_t stride_ao, size_t stride_grid, size_t nao, size_t ngrid, size_t nset, const uint8_t *tab,
+               size_t ldtab, size_t blksize);
+int launch_dgemm_ozaki(hipStream_t stream, double *c, const double *a, const double *b, size_t m, size_t n, size_t k,
+                       size_t lda, size_t ldb, size_t ldc, bool transa, bool transb, int nslices);
+int launch_rho_ozaki(hipStream_t stream, double *rhos, const double *dms, const double *ao_lhs, const double *ao_rhs,
+                     size_t stride_ao, size_t stride_grid, size_t nao, size_t ngrid, size_t nset, const uint8_t *tab,
+                     size_t ldtab, size_t blksize, int nslices);
+
+// ---- MFMA fragment types -------------------------------------------------------
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+}  // namespace cfgmm

@@ -1,0 +1,194 @@
+/*
+ * cfgmm.h -- C ABI of the MI355X (gfx950) implementation of the two hot paths
+ * of ajz34/cfgable-matmul:
+ *
+ *   1. row-major f64 GEMM   C += op(A) op(B)
+ *   2. non0tab-masked AO-DM-AO density contraction
+ *        rho[s][g] += sum_{u,v : tab[u][g/B] && tab[v][g/B]} lhs[u][g] D_s[u][v] rhs[v][g]
+ *      and the masked GEMM that is its unfused first half.
+ *
+ * Conventions (identical to the reference):
+ *   - row-major; op(A)(i,p) = transa ? a[p*lda+i] : a[i*lda+p],
+ *                op(B)(p,j) = transb ? b[j*ldb+p] : b[p*ldb+j]
+ *     (src/impl_matmul.rs:198-201, 83-85, 107, 217-220);
+ *   - every entry point ACCUMULATES into its output (C += ..., rho += ...);
+ *     there is no alpha/beta (src/impl_matmul.rs:33-46, src/impl_aodmao2rho.rs:77-92);
+ *   - masks are one byte per entry, 0 = false, anything else = true (Rust
+ *     `bool` is one byte); tab[p*ldtab + j/blksize] for the masked GEMM and
+ *     tab[u*ldtab + g/blksize] for rho; the mask is authoritative: values under
+ *     a false bit are never read;
+ *   - only i<m, j<n (resp. g<ngrid) are written; padding columns of C / rho
+ *     beyond the logical extent are untouched.
+ *
+ * The `*_len` arguments are the element counts of the caller's slices (Rust
+ * `&[T]::len()`); an extent that the arguments would index past returns
+ * CFGMM_EBOUNDS, where the reference would panic with an index-out-of-bounds.
+ *
+ * Host-pointer entry points are synchronous drop-ins for the reference
+ * functions: they copy inputs to HBM, run the HIP kernels on a per-thread
+ * stream, and return after the result is back in the caller's buffer.
+ * `_dev` entry points take device pointers and a hipStream_t (passed as
+ * void*), enqueue the work and return without synchronizing.
+ *
+ * All functions return a cfgmm status; cfgmm_last_error() gives a message for
+ * the last failing call on the calling thread.
+ */
+#ifndef CFGMM_H
+#define CFGMM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+enum cfgmm_status {
+    CFGMM_OK = 0,
+    CFGMM_EINVAL = 1,             /* bad argument (blksize, nslices, null pointer) */
+    CFGMM_EINVAL_TRANSB_MASK = 2, /* transb with a mask: unimplemented!() at src/impl_matmul_non0tab.rs:181-183,228-230 */
+    CFGMM_EINVAL_DIMS = 3,        /* leading dimension / stride smaller than the row length */
+    CFGMM_EOOM = 4,               /* device or pinned-host allocation failed */
+    CFGMM_EHIP = 5,               /* HIP runtime error */
+    CFGMM_ENODEV = 6,             /* no gfx950 device visible */
+    CFGMM_EBOUNDS = 7             /* a slice is shorter than the extent the arguments index */
+};
+
+/* Message for the last failing call on this thread ("" if none). */
+const char *cfgmm_last_error(void);
+/* Library version as 10000*major + 100*minor + patch. */
+int cfgmm_version(void);
+
+/* ---- precision modes (split-precision variants) -------------------------- */
+enum cfgmm_precision {
+    CFGMM_PREC_FP64 = 0,        /* plain FP64 MFMA */
+    CFGMM_PREC_OZAKI_INT8 = 1   /* Ozaki scheme II-style slicing of f64 into int8 slices on the int8 MFMA,
+                                   exact int32 accumulation, FP64 recombination; `nslices` in [2, 8] */
+};
+
+/* ======================================================================== */
+/* 1. row-major f64 GEMM                                                    */
+/* ======================================================================== */
+
+/* Replaces MatmulLoops::<f64, MC, KC, NC, MR, NR_LANE, LANE>::matmul_loop_parallel_mnk_pack_a
+ * (src/impl_matmul.rs:155-231).  The CPU blocking const generics have no GPU
+ * meaning and are not arguments. */
+int cfgmm_matmul_loop_parallel_mnk_pack_a(double *c, size_t c_len, const double *a, size_t a_len, const double *b,
+                                          size_t b_len, size_t m, size_t n, size_t k, size_t lda, size_t ldb,
+                                          size_t ldc, int transa, int transb);
+
+/* The README's name for the same function, with an extra MB const generic
+ * (README.md:22-31); identical semantics to the above. */
+int cfgmm_matmul_loop_macro_mb(double *c, size_t c_len, const double *a, size_t a_len, const double *b, size_t b_len,
+                               size_t m, size_t n, size_t k, size_t lda, size_t ldb, size_t ldc, int transa,
+                               int transb);
+
+/* Replaces matmul_anyway_full (src/impl_matmul.rs:234-248). */
+int cfgmm_matmul_anyway_full(double *c, size_t c_len, const double *a, size_t a_len, const double *b, size_t b_len,
+                             size_t m, size_t n, size_t k, size_t lda, size_t ldb, size_t ldc, int transa,
+                             int transb);
+
+/* Device-pointer GEMM, enqueued on `stream` (hipStream_t; NULL = null stream). */
+int cfgmm_dgemm_dev(void *stream, double *c, const double *a, const double *b, size_t m, size_t n, size_t k,
+                    size_t lda, size_t ldb, size_t ldc, int transa, int transb);
+
+/* Split-precision GEMM (no counterpart in the reference): same contract as
+ * cfgmm_dgemm_dev, computed with `precision` / `nslices`. */
+int cfgmm_dgemm_ex_dev(void *stream, double *c, const double *a, const double *b, size_t m, size_t n, size_t k,
+                       size_t lda, size_t ldb, size_t ldc, int transa, int transb, int precision, int nslices);
+
+/* ======================================================================== */
+/* 2a. non0tab-masked GEMM                                                  */
+/* ======================================================================== */
+
+/* Replaces matmul_loop_parallel_mnk_pack_a_non0tab::<BLKSIZE>
+ * (src/impl_matmul_non0tab.rs:210-305):
+ *   C(i,j) += sum_{p<k : tab[p*ldtab + j/blksize]} op(A)(i,p) B(p,j).
+ * transb != 0 returns CFGMM_EINVAL_TRANSB_MASK; blksize must be a positive
+ * multiple of 16 (the reference's BLKSIZE % (NR_LANE*LANE) == 0). */
+int cfgmm_matmul_loop_parallel_mnk_pack_a_non0tab(double *c, size_t c_len, const double *a, size_t a_len,
+                                                  const double *b, size_t b_len, size_t m, size_t n, size_t k,
+                                                  size_t lda, size_t ldb, size_t ldc, int transa, int transb,
+                                                  const uint8_t *tab, size_t tab_len, size_t ldtab, size_t blksize);
+
+/* Replaces matmul_anyway_full_non0tab (src/impl_matmul_non0tab.rs:308-326): blksize = 48. */
+int cfgmm_matmul_anyway_full_non0tab(double *c, size_t c_len, const double *a, size_t a_len, const double *b,
+                                     size_t b_len, size_t m, size_t n, size_t k, size_t lda, size_t ldb, size_t ldc,
+                                     int transa, int transb, const uint8_t *tab, size_t tab_len, size_t ldtab);
+
+/* Device-pointer masked GEMM. */
+int cfgmm_dgemm_non0tab_dev(void *stream, double *c, const double *a, const double *b, size_t m, size_t n, size_t k,
+                            size_t lda, size_t ldb, size_t ldc, int transa, int transb, const uint8_t *tab,
+                            size_t ldtab, size_t blksize);
+
+/* ======================================================================== */
+/* 2b. fused masked AO-DM-AO -> rho                                         */
+/* ======================================================================== */
+
+/* Replaces aodmao2rho_loop_parallel::<BLKSIZE> (src/impl_aodmao2rho.rs:171-244):
+ *   rhos[s*stride_grid + g] += sum_{u,v on} ao_lhs[u*stride_grid+g]
+ *                              * dms[s*stride_ao*stride_ao + u*stride_ao + v]
+ *                              * ao_rhs[v*stride_grid+g]
+ * for s < nset, g < ngrid, with u,v < nao "on" when tab[x*ldtab + g/blksize].
+ * `strides = [stride_ao, stride_grid]`, `shape = [nao, ngrid]` in the reference. */
+int cfgmm_aodmao2rho_loop_parallel(double *rhos, size_t rhos_len, const double *dms, size_t dms_len,
+                                   const double *ao_lhs, size_t ao_lhs_len, const double *ao_rhs, size_t ao_rhs_len,
+                                   size_t stride_ao, size_t stride_grid, size_t nao, size_t ngrid, size_t nset,
+                                   const uint8_t *tab, size_t tab_len, size_t ldtab, size_t blksize);
+
+/* Replaces aodmao2rho_anyway (src/impl_aodmao2rho.rs:247-261): blksize = 48. */
+int cfgmm_aodmao2rho_anyway(double *rhos, size_t rhos_len, const double *dms, size_t dms_len, const double *ao_lhs,
+                            size_t ao_lhs_len, const double *ao_rhs, size_t ao_rhs_len, size_t stride_ao,
+                            size_t stride_grid, size_t nao, size_t ngrid, size_t nset, const uint8_t *tab,
+                            size_t tab_len, size_t ldtab);
+
+/* Device-pointer rho. */
+int cfgmm_aodmao2rho_dev(void *stream, double *rhos, const double *dms, const double *ao_lhs, const double *ao_rhs,
+                         size_t stride_ao, size_t stride_grid, size_t nao, size_t ngrid, size_t nset,
+                         const uint8_t *tab, size_t ldtab, size_t blksize);
+
+/* Split-precision rho (no counterpart in the reference). */
+int cfgmm_aodmao2rho_ex_dev(void *stream, double *rhos, const double *dms, const double *ao_lhs,
+                            const double *ao_rhs, size_t stride_ao, size_t stride_grid, size_t nao, size_t ngrid,
+                            size_t nset, const uint8_t *tab, size_t ldtab, size_t blksize, int precision,
+                            int nslices);
+
+/* ======================================================================== */
+/* device plumbing for callers without their own HIP binding                */
+/* ======================================================================== */
+int cfgmm_device_count(int *count);
+int cfgmm_device_malloc(void **ptr, size_t bytes);
+int cfgmm_device_free(void *ptr);
+int cfgmm_memcpy_htod(void *dst, const void *src, size_t bytes);
+int cfgmm_memcpy_dtoh(void *dst, const void *src, size_t bytes);
+int cfgmm_memcpy_htod_async(void *dst, const void *src, size_t bytes, void *stream);
+int cfgmm_memcpy_dtoh_async(void *dst, const void *src, size_t bytes, void *stream);
+int cfgmm_memset_dev(void *dst, int value, size_t bytes);
+int cfgmm_mem_get_info(size_t *free_bytes, size_t *total_bytes);
+int cfgmm_stream_create(void **stream);
+int cfgmm_stream_destroy(void *stream);
+int cfgmm_stream_synchronize(void *stream);
+int cfgmm_device_synchronize(void);
+int cfgmm_event_create(void **event);
+int cfgmm_event_destroy(void *event);
+int cfgmm_event_record(void *event, void *stream);
+int cfgmm_event_synchronize(void *event);
+int cfgmm_event_query(void *event); /* CFGMM_OK when complete, 1 when pending */
+int cfgmm_event_elapsed_ms(float *ms, void *start, void *end);
+
+/* ---- tracing: hipEvent timing of the library's own kernels --------------- */
+/* When enabled, every kernel launch of the library is bracketed by two
+ * hipEvents recorded on the stream it is launched on.  cfgmm_trace_read waits
+ * for the recorded events of kernel `name` (e.g. "dgemm_kernel",
+ * "rho_chunk_kernel") and returns their summed duration and launch count
+ * since the last reset.  Tracing adds two event records per launch. */
+int cfgmm_trace_enable(int on);
+int cfgmm_trace_reset(void);
+int cfgmm_trace_read(const char *name, double *total_ms, long long *count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CFGMM_H */

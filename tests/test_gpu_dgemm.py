@@ -1,0 +1,104 @@
+"""GPU parity of the FP64-MFMA GEMM (through the C ABI) against the CPU oracle.
+
+Tolerance: relative Frobenius error <= 1e-12 (plain FP64)."""
+import itertools
+
+import numpy as np
+import pytest
+
+import cfgmm
+import oracle
+from cfgmm import workloads as W
+from helpers import op_view, rel_fro
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def _run_both(m, n, k, ta, tb, seed, pad=(0, 0, 0), c_init=True, api="host"):
+    rng = np.random.default_rng(seed)
+    lda = (m if ta else k) + pad[0]
+    ldb = (k if tb else n) + pad[1]
+    ldc = n + pad[2]
+    a = rng.uniform(-1, 1, size=(k if ta else m) * lda)
+    b = rng.uniform(-1, 1, size=(n if tb else k) * ldb)
+    c = rng.uniform(-1, 1, size=m * ldc) if c_init else np.zeros(m * ldc)
+    c_ref = c.copy()
+    oracle.matmul(c_ref, a, b, m, n, k, lda, ldb, ldc, ta, tb)
+    c_gpu = c.copy()
+    if api == "host":
+        cfgmm.MatmulLoops(252, 512, 240, 14, 2, 8).matmul_loop_parallel_mnk_pack_a(c_gpu, a, b, m, n, k, lda, ldb,
+                                                                                   ldc, ta, tb)
+    else:
+        s = cfgmm.Stream()
+        dA, dB, dC = (cfgmm.DeviceBuffer.from_array(x) for x in (a, b, c_gpu))
+        cfgmm.dgemm_dev(s, dC, dA, dB, m, n, k, lda, ldb, ldc, ta, tb)
+        s.synchronize()
+        dC.download(c_gpu)
+        s.destroy()
+    return c, c_ref, c_gpu, ldc
+
+
+def test_dgemm_small_nt_workload():
+    """C(1031x977) += A(1031x1543) . B(977x1543)^T, uniform[-1,1) seed 0, C = 0."""
+    g = W.dgemm_small_nt(seed=0)
+    c_ref = np.zeros(g.m * g.ldc)
+    oracle.matmul(c_ref, g.a, g.b, g.m, g.n, g.k, g.lda, g.ldb, g.ldc, g.transa, g.transb)
+    c = np.zeros(g.m * g.ldc)
+    cfgmm.MatmulLoops().matmul_loop_macro_mb(c, g.a, g.b, g.m, g.n, g.k, g.lda, g.ldb, g.ldc, g.transa, g.transb)
+    err = rel_fro(c, c_ref)
+    print(f"dgemm 1031x977x1543 NT rel_fro = {err:.3e}")
+    assert err <= TOL
+    # numpy (OpenBLAS) as a second, independent anchor
+    A, B = g.dense()
+    assert rel_fro(c.reshape(g.m, g.n), A @ B) <= TOL
+
+
+@pytest.mark.parametrize("ta,tb", list(itertools.product([False, True], repeat=2)))
+@pytest.mark.parametrize("m,n,k", [(1, 1, 1), (17, 33, 7), (130, 129, 100), (300, 520, 700), (1100, 1200, 300)])
+def test_dgemm_trans_ragged(m, n, k, ta, tb):
+    c0, c_ref, c_gpu, ldc = _run_both(m, n, k, ta, tb, seed=m + n + k, pad=(1, 2, 3))
+    assert rel_fro(c_gpu.reshape(m, ldc)[:, :n], c_ref.reshape(m, ldc)[:, :n]) <= TOL
+    # columns of C beyond n are never written
+    assert np.array_equal(c_gpu.reshape(m, ldc)[:, n:], c0.reshape(m, ldc)[:, n:])
+
+
+@pytest.mark.parametrize("api", ["host", "dev"])
+def test_dgemm_large_tile_path(api):
+    """Enough 128x128 tiles to take the large-tile path (no split-K)."""
+    m, n, k = 2100, 2300, 333
+    _, c_ref, c_gpu, _ = _run_both(m, n, k, False, False, seed=1, api=api)
+    assert rel_fro(c_gpu, c_ref) <= TOL
+
+
+def test_dgemm_split_k_path_deterministic():
+    """Few tiles + deep K takes the split-K path; the fixed-order reduction is bitwise repeatable."""
+    m, n, k = 200, 190, 4100
+    _, c_ref, c1, _ = _run_both(m, n, k, True, True, seed=2)
+    assert rel_fro(c1, c_ref) <= TOL
+    _, _, c2, _ = _run_both(m, n, k, True, True, seed=2)
+    assert np.array_equal(c1, c2)
+
+
+def test_dgemm_integer_exact():
+    """Small-integer operands: every partial sum is an exact f64 integer -> bitwise exact result."""
+    rng = np.random.default_rng(7)
+    m, n, k = 700, 650, 900
+    a = rng.integers(-3, 4, size=m * k).astype(np.float64)
+    b = rng.integers(-3, 4, size=n * k).astype(np.float64)
+    c = rng.integers(-5, 6, size=m * n).astype(np.float64)
+    ref = c.reshape(m, n).astype(np.int64) + a.reshape(m, k).astype(np.int64) @ b.reshape(n, k).astype(np.int64).T
+    cfgmm.matmul_anyway_full(c, a, b, m, n, k, k, k, n, False, True)
+    assert np.array_equal(c.reshape(m, n).astype(np.int64), ref)
+
+
+def test_dgemm_accumulates_twice():
+    """Two calls add twice (C += op(A) op(B) has no beta)."""
+    rng = np.random.default_rng(8)
+    m, n, k = 64, 80, 96
+    a = rng.uniform(-1, 1, m * k)
+    b = rng.uniform(-1, 1, k * n)
+    c = np.zeros(m * n)
+    cfgmm.matmul_anyway_full(c, a, b, m, n, k, k, n, n, False, False)
+    cfgmm.matmul_anyway_full(c, a, b, m, n, k, k, n, n, False, False)
+    assert rel_fro(c.reshape(m, n), 2 * (a.reshape(m, k) @ b.reshape(k, n))) <= TOL

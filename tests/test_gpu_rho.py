@@ -1,0 +1,141 @@
+"""GPU parity of the fused masked AO-DM-AO rho kernel against the CPU oracle.
+
+Tolerance: relative Frobenius error <= 1e-12 (plain FP64), plus size-independent
+properties at the full workload size (PSD => rho >= 0, linearity in D)."""
+import numpy as np
+import pytest
+
+import cfgmm
+import oracle
+from cfgmm import workloads as W
+from helpers import rel_fro
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def _oracle_rho(p, rhos0=None, blksize=None):
+    blksize = blksize or p.blksize
+    rhos = np.zeros(p.nset * p.ngrid) if rhos0 is None else rhos0.copy()
+    assert oracle.aodmao2rho(rhos, p.dms.ravel(), p.phi.ravel(), p.phi.ravel(), p.nao, p.ngrid, p.nao, p.ngrid,
+                             p.nset, np.ascontiguousarray(p.tab), p.nblk, blksize) == 0
+    return rhos
+
+
+@pytest.fixture(scope="module")
+def workload():
+    return W.rho_problem(seed=0)  # 700 shells, nao 2228, ngrid 16384 (256 x 64), 3 PSD DMs, 30 %
+
+
+def test_rho_full_workload_vs_oracle(workload):
+    p = workload
+    ref = _oracle_rho(p)
+    rhos = np.zeros(p.nset * p.ngrid)
+    cfgmm.MatmulLoops().aodmao2rho_loop_parallel(64, rhos, p.dms.ravel(), p.phi.ravel(), p.phi.ravel(),
+                                                 [p.nao, p.ngrid], [p.nao, p.ngrid], p.nset,
+                                                 np.ascontiguousarray(p.tab), p.nblk)
+    err = rel_fro(rhos, ref)
+    print(f"rho nao={p.nao} ngrid={p.ngrid} nset={p.nset} rel_fro = {err:.3e}")
+    assert err <= TOL
+    assert np.all(rhos >= -1e-12 * np.max(np.abs(rhos)))  # PSD D with lhs == rhs
+
+
+def test_rho_full_workload_dev_api_linear(workload):
+    """Device-pointer API; rho(D1 + D2) == rho(D1) + rho(D2) at full size."""
+    p = workload
+    dms = np.concatenate([p.dms[0].ravel(), p.dms[1].ravel(), (p.dms[0] + p.dms[1]).ravel()])
+    s = cfgmm.Stream()
+    d_r = cfgmm.DeviceBuffer(8 * 3 * p.ngrid)
+    d_r.zero()
+    d_d = cfgmm.DeviceBuffer.from_array(dms)
+    d_phi = cfgmm.DeviceBuffer.from_array(p.phi)
+    d_tab = cfgmm.DeviceBuffer.from_array(np.ascontiguousarray(p.tab))
+    cfgmm.aodmao2rho_dev(s, d_r, d_d, d_phi, d_phi, p.nao, p.ngrid, p.nao, p.ngrid, 3, d_tab, p.nblk, p.blksize)
+    s.synchronize()
+    r = d_r.to_array(np.float64, (3, p.ngrid))
+    assert rel_fro(r[2], r[0] + r[1]) <= TOL
+    s.destroy()
+
+
+@pytest.mark.parametrize("blksize", [16, 48, 64, 128])
+@pytest.mark.parametrize("nset", [1, 2, 5])
+def test_rho_blksize_nset_ragged(blksize, nset):
+    """Odd ngrid (partial last block), nset > 4 (two set groups), rho accumulated into nonzero input,
+    lhs != rhs and both nonzero under off bits (mask authority)."""
+    rng = np.random.default_rng(blksize * 10 + nset)
+    nao, ngrid = 301, 1001
+    nblk = (ngrid + blksize - 1) // blksize
+    lhs = rng.standard_normal(nao * ngrid)
+    rhs = rng.standard_normal(nao * ngrid)
+    dms = rng.standard_normal(nset * nao * nao)
+    tab = W.periodic_mask(nao, nblk, W.MASK_25)
+    rhos = rng.standard_normal(nset * ngrid)
+    ref = rhos.copy()
+    assert oracle.aodmao2rho(ref, dms, lhs, rhs, nao, ngrid, nao, ngrid, nset, tab, nblk, blksize) == 0
+    cfgmm.MatmulLoops().aodmao2rho_loop_parallel(blksize, rhos, dms, lhs, rhs, [nao, ngrid], [nao, ngrid], nset,
+                                                 tab, nblk)
+    assert rel_fro(rhos, ref) <= TOL
+
+
+def test_rho_strided_padding_untouched():
+    rng = np.random.default_rng(3)
+    nao, ngrid, nset, blksize = 140, 300, 2, 48
+    sa, sg = nao + 3, ngrid + 5
+    nblk = (ngrid + blksize - 1) // blksize
+    ldtab = nblk + 2
+    phi = rng.standard_normal(nao * sg)
+    dms = rng.standard_normal(nset * sa * sa)
+    tab = W.periodic_mask(nao, ldtab, W.MASK_30)
+    rhos = rng.standard_normal(nset * sg)
+    ref = rhos.copy()
+    assert oracle.aodmao2rho(ref, dms, phi, phi, sa, sg, nao, ngrid, nset, tab, ldtab, blksize) == 0
+    got = rhos.copy()
+    cfgmm.MatmulLoops().aodmao2rho_loop_parallel(blksize, got, dms, phi, phi, [sa, sg], [nao, ngrid], nset, tab, ldtab)
+    assert rel_fro(got.reshape(nset, sg)[:, :ngrid], ref.reshape(nset, sg)[:, :ngrid]) <= TOL
+    assert np.array_equal(got.reshape(nset, sg)[:, ngrid:], rhos.reshape(nset, sg)[:, ngrid:])
+
+
+def test_rho_empty_and_full_mask():
+    rng = np.random.default_rng(4)
+    nao, ngrid, blksize = 90, 250, 64
+    nblk = (ngrid + blksize - 1) // blksize
+    phi = rng.standard_normal(nao * ngrid)
+    dms = rng.standard_normal(nao * nao)
+    rhos = rng.standard_normal(ngrid)
+    r0 = rhos.copy()
+    cfgmm.MatmulLoops().aodmao2rho_loop_parallel(blksize, rhos, dms, phi, phi, [nao, ngrid], [nao, ngrid], 1,
+                                                 np.zeros((nao, nblk), np.uint8), nblk)
+    assert np.array_equal(rhos, r0)
+    cfgmm.MatmulLoops().aodmao2rho_loop_parallel(blksize, rhos, dms, phi, phi, [nao, ngrid], [nao, ngrid], 1,
+                                                 np.ones((nao, nblk), np.uint8), nblk)
+    L = phi.reshape(nao, ngrid)
+    assert rel_fro(rhos, r0 + np.einsum("ug,ug->g", L, dms.reshape(nao, nao) @ L)) <= TOL
+
+
+def test_rho_reference_test_generators():
+    """src/impl_aodmao2rho.rs:264-345 at reduced ngrid: sin/cos phi, D = sin(0.38x+1.2), 25 % mask,
+    BLKSIZE 48, nao 2228, odd ngrid."""
+    nao, ngrid, nset = 2228, 8193, 1
+    lhs, rhs = W.sin_cos_operands(nao * ngrid, nao * ngrid)
+    dms = np.sin(np.arange(nset * nao * nao, dtype=np.float64) * 0.38 + 1.2)
+    nblk = (ngrid + 47) // 48
+    tab = W.periodic_mask(nao, nblk, W.MASK_25)
+    W.zero_under_mask(lhs, tab, 48, ngrid)
+    W.zero_under_mask(rhs, tab, 48, ngrid)
+    ref = np.zeros(ngrid)
+    assert oracle.aodmao2rho(ref, dms, lhs, rhs, nao, ngrid, nao, ngrid, nset, tab, nblk, 48) == 0
+    rhos = np.zeros(ngrid)
+    cfgmm.aodmao2rho_anyway(rhos, dms, lhs, rhs, [nao, ngrid], [nao, ngrid], nset, tab, nblk)
+    assert rel_fro(rhos, ref) <= TOL
+
+
+def test_rho_deterministic(workload):
+    p = workload
+    outs = []
+    for _ in range(2):
+        rhos = np.zeros(p.nset * p.ngrid)
+        cfgmm.MatmulLoops().aodmao2rho_loop_parallel(64, rhos, p.dms.ravel(), p.phi.ravel(), p.phi.ravel(),
+                                                     [p.nao, p.ngrid], [p.nao, p.ngrid], p.nset,
+                                                     np.ascontiguousarray(p.tab), p.nblk)
+        outs.append(rhos)
+    assert np.array_equal(outs[0], outs[1])
