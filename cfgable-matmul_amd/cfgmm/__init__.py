@@ -67,6 +67,7 @@ _SIGNATURES = {
     "cfgmm_memcpy_dtoh": ([_VP, _VP, _SZ], _I),
     "cfgmm_memcpy_htod_async": ([_VP, _VP, _SZ, _VP], _I),
     "cfgmm_memcpy_dtoh_async": ([_VP, _VP, _SZ, _VP], _I),
+    "cfgmm_memcpy_dtod_async": ([_VP, _VP, _SZ, _VP], _I),
     "cfgmm_memset_dev": ([_VP, _I, _SZ], _I),
     "cfgmm_mem_get_info": ([ctypes.POINTER(_SZ), ctypes.POINTER(_SZ)], _I),
     "cfgmm_stream_create": ([ctypes.POINTER(_VP)], _I),

@@ -531,6 +531,10 @@ int cfgmm_memcpy_dtoh_async(void *dst, const void *src, size_t bytes, void *stre
     return hip_status(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, static_cast<hipStream_t>(stream)),
                       "hipMemcpyAsync D2H");
 }
+int cfgmm_memcpy_dtod_async(void *dst, const void *src, size_t bytes, void *stream) {
+    return hip_status(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, static_cast<hipStream_t>(stream)),
+                      "hipMemcpyAsync D2D");
+}
 int cfgmm_memset_dev(void *dst, int value, size_t bytes) {
     return hip_status(hipMemset(dst, value, bytes), "hipMemset");
 }

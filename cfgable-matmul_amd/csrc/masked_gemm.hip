@@ -17,11 +17,16 @@
 #include "internal.hpp"
 
 namespace cfgmm {
+
+// Zero source for loads outside the problem or the compacted K list.
+__device__ const double kZerosM[16] = {};
+
 namespace {
 
 constexpr int kThreads = 256;
 constexpr int kBM = 128;  // rows per workgroup (4 waves x 32)
 constexpr int kBK = 16;   // compacted-K slice depth
+constexpr int kSeg = 2048;  // compacted-K list entries staged in LDS at a time
 
 template <int NT, bool TA>
 __global__ __launch_bounds__(kThreads, 2) void masked_gemm_kernel(double *__restrict__ c, const double *__restrict__ a,
@@ -43,82 +48,92 @@ __global__ __launch_bounds__(kThreads, 2) void masked_gemm_kernel(double *__rest
     const int *__restrict__ list = lists + (long)cb * kmax;
     const int m0 = blockIdx.y * kBM;
     const int n0 = cb * blksize + sub * W;
+    if (n0 >= n) return;  // subtile past the end of a partial last column block
 
     __shared__ double As[2][kBK][LDA_S];  // op(A)[i][list[pp]] stored [pp][i]
     __shared__ double Bs[2][kBK][W];      // B[list[pp]][j]
+    __shared__ int vls[kSeg];             // a segment of the compacted K list
 
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int lrow = lane & 15, lk = lane >> 4;
 
-    double ra[A_PER_T], rb[B_PER_T];
-    auto load_slice = [&](int k0) {
-#pragma unroll
-        for (int e = 0; e < A_PER_T; ++e) {
-            const int idx = tid + kThreads * e;
-            const int pp = TA ? idx / kBM : idx % kBK;
-            const int i = TA ? idx % kBM : idx / kBK;
-            const int gi = m0 + i;
-            double v = 0.0;
-            if (gi < m && k0 + pp < cnt) {
-                const long p = list[k0 + pp];
-                v = TA ? a[p * lda + gi] : a[(long)gi * lda + p];
-            }
-            ra[e] = v;
-        }
-#pragma unroll
-        for (int e = 0; e < B_PER_T; ++e) {
-            const int idx = tid + kThreads * e;
-            const int pp = idx / W, j = idx % W;
-            const int gj = n0 + j;
-            double v = 0.0;
-            if (gj < n && k0 + pp < cnt) v = b[(long)list[k0 + pp] * ldb + gj];
-            rb[e] = v;
-        }
-    };
-    auto store_slice = [&](int buf) {
-#pragma unroll
-        for (int e = 0; e < A_PER_T; ++e) {
-            const int idx = tid + kThreads * e;
-            const int pp = TA ? idx / kBM : idx % kBK;
-            const int i = TA ? idx % kBM : idx / kBK;
-            As[buf][pp][i] = ra[e];
-        }
-#pragma unroll
-        for (int e = 0; e < B_PER_T; ++e) {
-            const int idx = tid + kThreads * e;
-            Bs[buf][idx / W][idx % W] = rb[e];
-        }
-    };
+    // Every load of a slice is issued unconditionally: indices come from LDS,
+    // and an entry outside the problem reads kZerosM (the address is
+    // selected, not the value, so no load hides behind a branch).
 
+    double ra[A_PER_T], rb[B_PER_T];
     f64x4 acc[FM][NT];
 #pragma unroll
     for (int mi = 0; mi < FM; ++mi)
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) acc[mi][nt] = f64x4{0.0, 0.0, 0.0, 0.0};
 
-    const int nslices = (cnt + kBK - 1) / kBK;
-    load_slice(0);
-    store_slice(0);
-    __syncthreads();
-    for (int t = 0; t < nslices; ++t) {
-        const int cur = t & 1;
-        if (t + 1 < nslices) load_slice((t + 1) * kBK);
-#pragma unroll
-        for (int kk = 0; kk < kBK / 4; ++kk) {
-            double fa[FM], fb[NT];
-#pragma unroll
-            for (int mi = 0; mi < FM; ++mi) fa[mi] = As[cur][kk * 4 + lk][wave * 32 + mi * 16 + lrow];
-#pragma unroll
-            for (int nt = 0; nt < NT; ++nt) fb[nt] = Bs[cur][kk * 4 + lk][nt * 16 + lrow];
-#pragma unroll
-            for (int mi = 0; mi < FM; ++mi)
-#pragma unroll
-                for (int nt = 0; nt < NT; ++nt)
-                    acc[mi][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[mi], fb[nt], acc[mi][nt], 0, 0, 0);
-        }
-        if (t + 1 < nslices) store_slice(cur ^ 1);
+    for (int seg0 = 0; seg0 < cnt; seg0 += kSeg) {
+        const int segn = min(kSeg, cnt - seg0);
         __syncthreads();
+        for (int j = tid; j < segn; j += kThreads) vls[j] = list[seg0 + j];
+        __syncthreads();
+
+        auto load_slice = [&](int k0) {
+#pragma unroll
+            for (int e = 0; e < A_PER_T; ++e) {
+                const int idx = tid + kThreads * e;
+                const int pp = TA ? idx / kBM : idx % kBK;
+                const int i = TA ? idx % kBM : idx / kBK;
+                const int gi = m0 + i;
+                const bool ok = gi < m && k0 + pp < segn;
+                const long p = vls[ok ? k0 + pp : 0];
+                const double *src = ok ? (TA ? a + p * lda + gi : a + (long)gi * lda + p) : kZerosM;
+                ra[e] = *src;
+            }
+#pragma unroll
+            for (int e = 0; e < B_PER_T; ++e) {
+                const int idx = tid + kThreads * e;
+                const int pp = idx / W, gj = n0 + idx % W;
+                const bool ok = gj < n && k0 + pp < segn;
+                const double *src = ok ? b + (long)vls[ok ? k0 + pp : 0] * ldb + gj : kZerosM;
+                rb[e] = *src;
+            }
+        };
+        auto store_slice = [&](int buf) {
+#pragma unroll
+            for (int e = 0; e < A_PER_T; ++e) {
+                const int idx = tid + kThreads * e;
+                const int pp = TA ? idx / kBM : idx % kBK;
+                const int i = TA ? idx % kBM : idx / kBK;
+                As[buf][pp][i] = ra[e];
+            }
+#pragma unroll
+            for (int e = 0; e < B_PER_T; ++e) {
+                const int idx = tid + kThreads * e;
+                Bs[buf][idx / W][idx % W] = rb[e];
+            }
+        };
+
+        const int nslices = (segn + kBK - 1) / kBK;
+        load_slice(0);
+        store_slice(0);
+        __syncthreads();
+        for (int t = 0; t < nslices; ++t) {
+            const int cur = t & 1;
+            if (t + 1 < nslices) load_slice((t + 1) * kBK);
+#pragma unroll
+            for (int kk = 0; kk < kBK / 4; ++kk) {
+                double fa[FM], fb[NT];
+#pragma unroll
+                for (int mi = 0; mi < FM; ++mi) fa[mi] = As[cur][kk * 4 + lk][wave * 32 + mi * 16 + lrow];
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt) fb[nt] = Bs[cur][kk * 4 + lk][nt * 16 + lrow];
+#pragma unroll
+                for (int mi = 0; mi < FM; ++mi)
+#pragma unroll
+                    for (int nt = 0; nt < NT; ++nt)
+                        acc[mi][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[mi], fb[nt], acc[mi][nt], 0, 0, 0);
+            }
+            if (t + 1 < nslices) store_slice(cur ^ 1);
+            __syncthreads();
+        }
     }
 
 #pragma unroll

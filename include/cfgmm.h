@@ -164,6 +164,7 @@ int cfgmm_memcpy_htod(void *dst, const void *src, size_t bytes);
 int cfgmm_memcpy_dtoh(void *dst, const void *src, size_t bytes);
 int cfgmm_memcpy_htod_async(void *dst, const void *src, size_t bytes, void *stream);
 int cfgmm_memcpy_dtoh_async(void *dst, const void *src, size_t bytes, void *stream);
+int cfgmm_memcpy_dtod_async(void *dst, const void *src, size_t bytes, void *stream);
 int cfgmm_memset_dev(void *dst, int value, size_t bytes);
 int cfgmm_mem_get_info(size_t *free_bytes, size_t *total_bytes);
 int cfgmm_stream_create(void **stream);
