@@ -64,11 +64,20 @@ def test_dgemm_trans_ragged(m, n, k, ta, tb):
 
 
 @pytest.mark.parametrize("api", ["host", "dev"])
-def test_dgemm_large_tile_path(api):
-    """Enough 128x128 tiles to take the large-tile path (no split-K)."""
+def test_dgemm_many_small_tiles(api):
+    """Enough 64x64 tiles to fill the chip without splitting K."""
     m, n, k = 2100, 2300, 333
     _, c_ref, c_gpu, _ = _run_both(m, n, k, False, False, seed=1, api=api)
     assert rel_fro(c_gpu, c_ref) <= TOL
+
+
+@pytest.mark.parametrize("ta,tb", [(False, False), (True, True)])
+def test_dgemm_whole_tiles_plus_split_tail(ta, tb):
+    """529 tiles of 128x128: 512 whole tiles, the last 17 split along K and reduced in a fixed order."""
+    m, n, k = 2900, 2900, 1000
+    c0, c_ref, c_gpu, ldc = _run_both(m, n, k, ta, tb, seed=4, pad=(0, 0, 5))
+    assert rel_fro(c_gpu.reshape(m, ldc)[:, :n], c_ref.reshape(m, ldc)[:, :n]) <= TOL
+    assert np.array_equal(c_gpu.reshape(m, ldc)[:, n:], c0.reshape(m, ldc)[:, n:])
 
 
 def test_dgemm_split_k_path_deterministic():
