@@ -407,7 +407,7 @@ int cfgmm_dgemm_ex_dev(void *stream, double *c, const double *a, const double *b
     if (precision == CFGMM_PREC_FP64)
         return launch_dgemm(s, c, a, b, m, n, k, lda, ldb, ldc, transa != 0, transb != 0);
     if (precision == CFGMM_PREC_OZAKI_INT8) {
-        if (nslices < 2 || nslices > 8) return fail(CFGMM_EINVAL, "nslices must be in [2, 8]");
+        if (nslices < 2 || nslices > 6) return fail(CFGMM_EINVAL, "nslices must be in [2, 6]");
         return launch_dgemm_ozaki(s, c, a, b, m, n, k, lda, ldb, ldc, transa != 0, transb != 0, nslices);
     }
     return fail(CFGMM_EINVAL, "unknown precision mode " + std::to_string(precision));
@@ -479,7 +479,7 @@ static int rho_dev_common(void *stream, double *rhos, const double *dms, const d
     if (precision == CFGMM_PREC_FP64)
         return launch_rho(s, rhos, dms, ao_lhs, ao_rhs, stride_ao, stride_grid, nao, ngrid, nset, tab, ldtab, blksize);
     if (precision == CFGMM_PREC_OZAKI_INT8) {
-        if (nslices < 2 || nslices > 8) return fail(CFGMM_EINVAL, "nslices must be in [2, 8]");
+        if (nslices < 2 || nslices > 6) return fail(CFGMM_EINVAL, "nslices must be in [2, 6]");
         return launch_rho_ozaki(s, rhos, dms, ao_lhs, ao_rhs, stride_ao, stride_grid, nao, ngrid, nset, tab, ldtab,
                                 blksize, nslices);
     }

@@ -53,6 +53,10 @@ __host__ __device__ inline size_t div_up(size_t a, size_t b) { return (a + b - 1
 // tab[x*ldtab + b] != 0 into lists[b*nrows + *] (ascending) and counts[b].
 int launch_build_lists(hipStream_t stream, const uint8_t *tab, size_t ldtab, size_t nrows, size_t nblk, int *lists,
                        int *counts);
+// rho[s*stride_grid + g] += sum over the 64-row chunks of g's block of partial[chunk][s][g]
+// (partial laid out [chunk][nset][ngrid]; chunk order fixed).
+int launch_rho_reduce(hipStream_t stream, double *rhos, const double *partial, const int *counts, size_t stride_grid,
+                      size_t ngrid, size_t nset, size_t blksize);
 int launch_dgemm(hipStream_t stream, double *c, const double *a, const double *b, size_t m, size_t n, size_t k,
                  size_t lda, size_t ldb, size_t ldc, bool transa, bool transb);
 int launch_dgemm_non0tab(hipStream_t stream, double *c, const double *a, const double *b, size_t m, size_t n,

@@ -62,6 +62,30 @@ int launch_build_lists(hipStream_t stream, const uint8_t *tab, size_t ldtab, siz
     return hip_status(hipGetLastError(), "build_ao_lists launch");
 }
 
+// rho[s*stride_grid + g] += sum_{chunk < ceil(|L_b|/64)} partial[chunk][s][g], b = g / blksize,
+// in increasing chunk order (deterministic).
+__global__ void rho_reduce_kernel(double *__restrict__ rhos, const double *__restrict__ partial,
+                                  const int *__restrict__ counts, long stride_grid, int ngrid, int nset,
+                                  int blksize) {
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)nset * ngrid) return;
+    const int s = (int)(idx / ngrid), g = (int)(idx % ngrid);
+    const int nch = (counts[g / blksize] + 63) / 64;
+    double v = 0.0;
+    for (int ch = 0; ch < nch; ++ch) v += partial[((long)ch * nset + s) * ngrid + g];
+    rhos[(long)s * stride_grid + g] += v;
+}
+
+int launch_rho_reduce(hipStream_t stream, double *rhos, const double *partial, const int *counts, size_t stride_grid,
+                      size_t ngrid, size_t nset, size_t blksize) {
+    const size_t total = nset * ngrid;
+    hipEvent_t t0 = trace_start(stream);
+    hipLaunchKernelGGL(rho_reduce_kernel, dim3((unsigned)div_up(total, 256)), dim3(256), 0, stream, rhos, partial,
+                       counts, (long)stride_grid, (int)ngrid, (int)nset, (int)blksize);
+    trace_stop(stream, "rho_reduce_kernel", t0);
+    return hip_status(hipGetLastError(), "rho_reduce_kernel launch");
+}
+
 namespace {
 
 constexpr int kThreads = 256;
@@ -254,19 +278,6 @@ __global__ __launch_bounds__(kThreads, 2) void rho_chunk_kernel(
     }
 }
 
-// rho[s*stride_grid + g] += sum_{chunk < ceil(|L_b|/64)} partial[chunk][s][g], b = g / blksize.
-__global__ void rho_reduce_kernel(double *__restrict__ rhos, const double *__restrict__ partial,
-                                  const int *__restrict__ counts, long stride_grid, int ngrid, int nset,
-                                  int blksize) {
-    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (long)nset * ngrid) return;
-    const int s = (int)(idx / ngrid), g = (int)(idx % ngrid);
-    const int nch = (counts[g / blksize] + kUC - 1) / kUC;
-    double v = 0.0;
-    for (int ch = 0; ch < nch; ++ch) v += partial[((long)ch * nset + s) * ngrid + g];
-    rhos[(long)s * stride_grid + g] += v;
-}
-
 template <int NS>
 int launch_ns(int nt, dim3 grid, hipStream_t stream, double *partial, const double *dms, const double *ao_lhs,
               const double *ao_rhs, long stride_ao, long stride_grid, int nao, int ngrid, int nset, int set0,
@@ -335,12 +346,7 @@ int launch_rho(hipStream_t stream, double *rhos, const double *dms, const double
         }
         if (st != CFGMM_OK) return st;
     }
-    const size_t total = nset * ngrid;
-    hipEvent_t t0 = trace_start(stream);
-    hipLaunchKernelGGL(rho_reduce_kernel, dim3((unsigned)div_up(total, 256)), dim3(256), 0, stream, rhos, partial,
-                       counts, (long)stride_grid, (int)ngrid, (int)nset, (int)blksize);
-    trace_stop(stream, "rho_reduce_kernel", t0);
-    return hip_status(hipGetLastError(), "rho_reduce_kernel launch");
+    return launch_rho_reduce(stream, rhos, partial, counts, stride_grid, ngrid, nset, blksize);
 }
 
 }  // namespace cfgmm

@@ -63,8 +63,9 @@ int cfgmm_version(void);
 /* ---- precision modes (split-precision variants) -------------------------- */
 enum cfgmm_precision {
     CFGMM_PREC_FP64 = 0,        /* plain FP64 MFMA */
-    CFGMM_PREC_OZAKI_INT8 = 1   /* Ozaki scheme II-style slicing of f64 into int8 slices on the int8 MFMA,
-                                   exact int32 accumulation, FP64 recombination; `nslices` in [2, 8] */
+    CFGMM_PREC_OZAKI_INT8 = 1   /* Ozaki-scheme splitting of f64 into `nslices` int8 slices (7 bits each,
+                                   per-row / per-column power-of-two scaling), slice products on the int8
+                                   MFMA with exact int32 accumulation, FP64 recombination; nslices in [2, 6] */
 };
 
 /* ======================================================================== */
