@@ -34,6 +34,10 @@ EBOUNDS = 7
 PREC_FP64 = 0
 PREC_OZAKI_INT8 = 1
 
+TUNE_BATCH_WGS_PER_CU = 0
+TUNE_BATCH_MAX_SETS = 1
+TUNE_DGEMM_GROUP_M = 2
+
 _lib = None
 
 _SZ = ctypes.c_size_t
@@ -60,6 +64,9 @@ _SIGNATURES = {
     "cfgmm_aodmao2rho_anyway": ([_VP, _SZ, _VP, _SZ, _VP, _SZ, _VP, _SZ] + [_SZ] * 5 + [_VP, _SZ, _SZ], _I),
     "cfgmm_aodmao2rho_dev": ([_VP, _VP, _VP, _VP, _VP] + [_SZ] * 5 + [_VP, _SZ, _SZ], _I),
     "cfgmm_aodmao2rho_ex_dev": ([_VP, _VP, _VP, _VP, _VP] + [_SZ] * 5 + [_VP, _SZ, _SZ, _I, _I], _I),
+    "cfgmm_aodmao2rho_batched_dev": ([_VP, _VP, _SZ, _I, _I], _I),
+    "cfgmm_tune_set": ([_I, _I], _I),
+    "cfgmm_tune_get": ([_I, ctypes.POINTER(_I)], _I),
     "cfgmm_device_count": ([ctypes.POINTER(_I)], _I),
     "cfgmm_device_malloc": ([ctypes.POINTER(_VP), _SZ], _I),
     "cfgmm_device_free": ([_VP], _I),
@@ -336,6 +343,29 @@ def trace_read(name: str) -> tuple[float, int]:
     ms, n = ctypes.c_double(0.0), ctypes.c_longlong(0)
     check(lib().cfgmm_trace_read(name.encode(), ctypes.byref(ms), ctypes.byref(n)))
     return ms.value, n.value
+
+
+class RhoProblem(ctypes.Structure):
+    """cfgmm_rho_problem: one problem of a batched rho call (device pointers)."""
+    _fields_ = [("rhos", _VP), ("dms", _VP), ("ao_lhs", _VP), ("ao_rhs", _VP), ("stride_ao", _SZ),
+                ("stride_grid", _SZ), ("nao", _SZ), ("ngrid", _SZ), ("nset", _SZ), ("tab", _VP), ("ldtab", _SZ),
+                ("blksize", _SZ)]
+
+
+def aodmao2rho_batched_dev(stream, problems, precision=PREC_FP64, nslices=0):
+    """problems: a sequence of RhoProblem (or a ctypes array of them)."""
+    arr = (RhoProblem * len(problems))(*problems) if not isinstance(problems, ctypes.Array) else problems
+    check(lib().cfgmm_aodmao2rho_batched_dev(_h(stream), ctypes.cast(arr, _VP), len(arr), precision, nslices))
+
+
+def tune_set(param: int, value: int) -> None:
+    check(lib().cfgmm_tune_set(param, value))
+
+
+def tune_get(param: int) -> int:
+    v = ctypes.c_int(0)
+    check(lib().cfgmm_tune_get(param, ctypes.byref(v)))
+    return v.value
 
 
 def _h(stream):

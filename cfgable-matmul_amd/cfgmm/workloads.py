@@ -133,10 +133,10 @@ def _shell_types(rng, nshell, nao_target):
         total = int(sizes.sum())
         if (nao_target - total) % 2 != 0:
             raise ValueError("target AO count must have the parity of the shell count")
+        if not nshell <= nao_target <= 7 * nshell:
+            raise ValueError("target AO count out of reach of the shell count")
         i = 0
-        while total != nao_target:
-            if i >= nshell:
-                raise ValueError("cannot reach the target AO count by promoting/demoting shells")
+        while total != nao_target and i < nshell:  # first pass: s <-> p in draw order
             if total < nao_target and sizes[i] == 1:
                 sizes[i] = 3
                 total += 2
@@ -144,6 +144,15 @@ def _shell_types(rng, nshell, nao_target):
                 sizes[i] = 1
                 total -= 2
             i += 1
+        i = 0
+        while total != nao_target:  # only if still off: any one-step promotion / demotion, cyclically
+            if total < nao_target and sizes[i] < 7:
+                sizes[i] += 2
+                total += 2
+            elif total > nao_target and sizes[i] > 1:
+                sizes[i] -= 2
+                total -= 2
+            i = (i + 1) % nshell
     return sizes
 
 

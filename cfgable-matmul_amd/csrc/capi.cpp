@@ -77,6 +77,15 @@ void trace_stop(hipStream_t stream, const char *name, hipEvent_t start) {
 }
 
 // ---------------------------------------------------------------------------
+// tunables
+// ---------------------------------------------------------------------------
+static std::atomic<int> g_tune[CFGMM_TUNE_COUNT] = {{2}, {2}, {4}};
+static const int g_tune_lo[CFGMM_TUNE_COUNT] = {1, 1, 1};
+static const int g_tune_hi[CFGMM_TUNE_COUNT] = {8, 4, 64};
+
+int tune_get(int param) { return g_tune[param].load(std::memory_order_relaxed); }
+
+// ---------------------------------------------------------------------------
 // device checks and properties
 // ---------------------------------------------------------------------------
 static std::mutex g_dev_mutex;
@@ -491,6 +500,49 @@ int cfgmm_aodmao2rho_dev(void *stream, double *rhos, const double *dms, const do
                          const uint8_t *tab, size_t ldtab, size_t blksize) {
     return rho_dev_common(stream, rhos, dms, ao_lhs, ao_rhs, stride_ao, stride_grid, nao, ngrid, nset, tab, ldtab,
                           blksize, CFGMM_PREC_FP64, 0);
+}
+
+int cfgmm_aodmao2rho_batched_dev(void *stream, const cfgmm_rho_problem *problems, size_t nproblems, int precision,
+                                 int nslices) {
+    clear_error();
+    if (nproblems && !problems) return fail(CFGMM_EINVAL, "problems is null");
+    for (size_t i = 0; i < nproblems; ++i) {
+        const cfgmm_rho_problem &q = problems[i];
+        RhoExtents ext;
+        int st = rho_extents(q.stride_ao, q.stride_grid, q.nao, q.ngrid, q.nset, q.ldtab, q.blksize, &ext);
+        if (st != CFGMM_OK) return fail(st, "problem " + std::to_string(i) + ": " + g_last_error);
+        if (q.nao && q.ngrid && q.nset && (!q.rhos || !q.dms || !q.ao_lhs || !q.ao_rhs || !q.tab))
+            return fail(CFGMM_EINVAL, "problem " + std::to_string(i) + ": null device pointer");
+    }
+    CFGMM_TRY(check_device());
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (precision == CFGMM_PREC_FP64) return launch_rho_batched(s, problems, nproblems);
+    if (precision == CFGMM_PREC_OZAKI_INT8) {
+        if (nslices < 2 || nslices > 6) return fail(CFGMM_EINVAL, "nslices must be in [2, 6]");
+        for (size_t i = 0; i < nproblems; ++i) {
+            const cfgmm_rho_problem &q = problems[i];
+            CFGMM_TRY(launch_rho_ozaki(s, q.rhos, q.dms, q.ao_lhs, q.ao_rhs, q.stride_ao, q.stride_grid, q.nao,
+                                       q.ngrid, q.nset, q.tab, q.ldtab, q.blksize, nslices));
+        }
+        return CFGMM_OK;
+    }
+    return fail(CFGMM_EINVAL, "unknown precision mode " + std::to_string(precision));
+}
+
+int cfgmm_tune_set(int param, int value) {
+    clear_error();
+    if (param < 0 || param >= CFGMM_TUNE_COUNT) return fail(CFGMM_EINVAL, "unknown tunable " + std::to_string(param));
+    if (value < g_tune_lo[param] || value > g_tune_hi[param])
+        return fail(CFGMM_EINVAL, "tunable " + std::to_string(param) + " out of range: " + std::to_string(value));
+    g_tune[param].store(value);
+    return CFGMM_OK;
+}
+
+int cfgmm_tune_get(int param, int *value) {
+    clear_error();
+    if (param < 0 || param >= CFGMM_TUNE_COUNT) return fail(CFGMM_EINVAL, "unknown tunable " + std::to_string(param));
+    *value = tune_get(param);
+    return CFGMM_OK;
 }
 
 int cfgmm_aodmao2rho_ex_dev(void *stream, double *rhos, const double *dms, const double *ao_lhs,

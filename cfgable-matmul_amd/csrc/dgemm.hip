@@ -282,7 +282,7 @@ template <int BM, int BN, int WM, int WN>
 int launch_plan(hipStream_t stream, const DgemmPlan &pl, double *c, const double *a, const double *b, int m, int n,
                 int k, long lda, long ldb, long ldc, bool ta, bool tb) {
     constexpr int BK = 16;
-    const int group_m = 4;
+    const int group_m = tune_get(CFGMM_TUNE_DGEMM_GROUP_M);
     const int tail_tiles = pl.tiles - pl.dp_tiles;
     double *partial = nullptr;
     if (tail_tiles > 0) {

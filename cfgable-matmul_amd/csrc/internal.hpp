@@ -42,6 +42,9 @@ int workspace_get(hipStream_t stream, size_t bytes, void **out);
 hipEvent_t trace_start(hipStream_t stream);
 void trace_stop(hipStream_t stream, const char *name, hipEvent_t start);
 
+// ---- tunables (cfgmm_tune_set / cfgmm_tune_get) ----------------------------------
+int tune_get(int param);
+
 // ---- device properties ------------------------------------------------------
 int device_cu_count();  // compute units of the current device (cached per device)
 
@@ -67,6 +70,7 @@ int launch_rho(hipStream_t stream, double *rhos, const double *dms, const double
                size_t ldtab, size_t blksize);
 int launch_dgemm_ozaki(hipStream_t stream, double *c, const double *a, const double *b, size_t m, size_t n, size_t k,
                        size_t lda, size_t ldb, size_t ldc, bool transa, bool transb, int nslices);
+int launch_rho_batched(hipStream_t stream, const cfgmm_rho_problem *problems, size_t nprob);
 int launch_rho_ozaki(hipStream_t stream, double *rhos, const double *dms, const double *ao_lhs, const double *ao_rhs,
                      size_t stride_ao, size_t stride_grid, size_t nao, size_t ngrid, size_t nset, const uint8_t *tab,
                      size_t ldtab, size_t blksize, int nslices);

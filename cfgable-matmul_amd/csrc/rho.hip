@@ -24,6 +24,7 @@
 //   3. rho_reduce_kernel: rho[s][g] += sum over the chunks of g's block, in
 //      increasing chunk order -- deterministic, no float atomics.
 #include "internal.hpp"
+#include "rho_body.hpp"
 
 namespace cfgmm {
 
@@ -88,194 +89,33 @@ int launch_rho_reduce(hipStream_t stream, double *rhos, const double *partial, c
 
 namespace {
 
-constexpr int kThreads = 256;
-constexpr int kUC = 64;  // rows of L_b per workgroup (4 waves x 16)
-constexpr int kKV = 16;  // v slice depth per LDS stage
-constexpr int kSeg = 2048;  // compacted-list entries staged in LDS at a time
+using rho::kThreads;
+using rho::kUC;
 
 template <int NS, int NT>
 __global__ __launch_bounds__(kThreads, 2) void rho_chunk_kernel(
     double *__restrict__ partial, const double *__restrict__ dms, const double *__restrict__ ao_lhs,
     const double *__restrict__ ao_rhs, long stride_ao, long stride_grid, int nao, int ngrid, int nset, int set0,
     const int *__restrict__ lists, const int *__restrict__ counts, int blksize, int nsub) {
-    constexpr int W = NT * 16;                          // grid points per workgroup
-    constexpr int PHI_PER_T = kKV * W / kThreads;       // phi_rhs elements per thread per slice
-    constexpr int D_ROWS_PER_T = kUC * kKV / kThreads;  // D rows per thread per slice and set (4)
-    static_assert(PHI_PER_T * kThreads == kKV * W, "phi tile split");
-    static_assert(D_ROWS_PER_T * kThreads == kUC * kKV, "D tile split");
-
-    const int chunk = blockIdx.y;
+    __shared__ __attribute__((aligned(16))) unsigned char smem[rho::chunk_lds_bytes<NS, NT>()];
     const int blk = blockIdx.x / nsub, sub = blockIdx.x % nsub;
-    const int cnt = counts[blk];
-    const int u_begin = chunk * kUC;
-    if (u_begin >= cnt) return;
-    const int *__restrict__ list = lists + (long)blk * nao;
-    const int g0 = blk * blksize + sub * W;
-    if (g0 >= ngrid) return;  // subtile past the end of a partial last block
-
-    // LDS: double-buffered A (D_s[u][v] stored [v][u]) and B (phi_rhs[v][g]) slices, a
-    // segment of the compacted v list, this chunk's u rows.  The cross-wave
-    // reduction buffer of the epilogue reuses the A slices.
-    constexpr int AS_ELEMS = 2 * NS * kKV * (kUC + 1);
-    static_assert(AS_ELEMS >= (kThreads / 64) * NS * W, "epilogue buffer fits in the A slices");
-    __shared__ double As_raw[AS_ELEMS];
-    __shared__ double Bs[2][kKV][W];
-    __shared__ int vls[kSeg];
-    __shared__ int ulist[kUC];
-    auto As = [&](int buf, int s, int p, int i) -> double & {
-        return As_raw[((buf * NS + s) * kKV + p) * (kUC + 1) + i];
-    };
-
-    const int tid = threadIdx.x;
-    const int lane = tid & 63, wave = tid >> 6;
-    const int lrow = lane & 15, lk = lane >> 4;
-    const int nrows = min(kUC, cnt - u_begin);
-    // rows past the end of the list read a valid row (the chunk's first) and are zeroed
-    if (tid < kUC) ulist[tid] = list[u_begin + (tid < nrows ? tid : 0)];
-
-    const long dstride = stride_ao * stride_ao;
-    // Fixed per-thread D coordinates: column slot p_d, rows tid/16 + 16e.
-    const int p_d = tid % kKV;
-
-    double rphi[PHI_PER_T], rd[NS][D_ROWS_PER_T];
-    f64x4 acc[NS][NT];
-#pragma unroll
-    for (int s = 0; s < NS; ++s)
-#pragma unroll
-        for (int nt = 0; nt < NT; ++nt) acc[s][nt] = f64x4{0.0, 0.0, 0.0, 0.0};
-
-    for (int seg0 = 0; seg0 < cnt; seg0 += kSeg) {
-        const int segn = min(kSeg, cnt - seg0);
-        __syncthreads();  // previous segment's readers of vls / slices are done
-        for (int j = tid; j < segn; j += kThreads) vls[j] = list[seg0 + j];
-        __syncthreads();
-        const double *__restrict__ drow[NS][D_ROWS_PER_T];
-#pragma unroll
-        for (int e = 0; e < D_ROWS_PER_T; ++e) {
-            const int i = tid / kKV + (kThreads / kKV) * e;
-#pragma unroll
-            for (int s = 0; s < NS; ++s) drow[s][e] = dms + (long)(set0 + s) * dstride + (long)ulist[i] * stride_ao;
-        }
-
-        // Loads of a full 16-deep slice are unconditional: every list entry is
-        // valid, rows past the chunk end read a valid fallback row (their
-        // products are dropped in the epilogue) and columns past the grid end
-        // are clamped (their outputs are never written).  The last, partial
-        // slice of a segment is loaded by load_tail, after the main loop, so
-        // that no select on loaded values sits in the pipelined path.
-        auto load_full = [&](int k0) {
-            const long vcol = vls[k0 + p_d];
-#pragma unroll
-            for (int e = 0; e < D_ROWS_PER_T; ++e)
-#pragma unroll
-                for (int s = 0; s < NS; ++s) rd[s][e] = drow[s][e][vcol];
-#pragma unroll
-            for (int e = 0; e < PHI_PER_T; ++e) {
-                const int idx = tid + kThreads * e;
-                const int g = min(g0 + idx % W, ngrid - 1);
-                rphi[e] = ao_rhs[(long)vls[k0 + idx / W] * stride_grid + g];
-            }
-        };
-        auto load_tail = [&](int k0) {
-            const int pv = k0 + p_d;
-            const long vcol = vls[pv < segn ? pv : 0];
-#pragma unroll
-            for (int e = 0; e < D_ROWS_PER_T; ++e)
-#pragma unroll
-                for (int s = 0; s < NS; ++s) rd[s][e] = pv < segn ? drow[s][e][vcol] : 0.0;
-#pragma unroll
-            for (int e = 0; e < PHI_PER_T; ++e) {
-                const int idx = tid + kThreads * e;
-                const int pr = k0 + idx / W;
-                const int g = min(g0 + idx % W, ngrid - 1);
-                rphi[e] = pr < segn ? ao_rhs[(long)vls[pr] * stride_grid + g] : 0.0;
-            }
-        };
-        auto store_slice = [&](int buf) {
-#pragma unroll
-            for (int e = 0; e < PHI_PER_T; ++e) {
-                const int idx = tid + kThreads * e;
-                Bs[buf][idx / W][idx % W] = rphi[e];
-            }
-#pragma unroll
-            for (int e = 0; e < D_ROWS_PER_T; ++e)
-#pragma unroll
-                for (int s = 0; s < NS; ++s) As(buf, s, p_d, tid / kKV + (kThreads / kKV) * e) = rd[s][e];
-        };
-        auto compute_slice = [&](int cur) {
-#pragma unroll
-            for (int kk = 0; kk < kKV / 4; ++kk) {
-                double fa[NS], fb[NT];
-#pragma unroll
-                for (int s = 0; s < NS; ++s) fa[s] = As(cur, s, kk * 4 + lk, wave * 16 + lrow);
-#pragma unroll
-                for (int nt = 0; nt < NT; ++nt) fb[nt] = Bs[cur][kk * 4 + lk][nt * 16 + lrow];
-#pragma unroll
-                for (int s = 0; s < NS; ++s)
-#pragma unroll
-                    for (int nt = 0; nt < NT; ++nt)
-                        acc[s][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[s], fb[nt], acc[s][nt], 0, 0, 0);
-            }
-        };
-
-        const int nfull = segn / kKV;
-        if (nfull > 0) {
-            load_full(0);
-            store_slice(0);
-        }
-        __syncthreads();
-        for (int t = 0; t < nfull; ++t) {
-            const int cur = t & 1;
-            if (t + 1 < nfull) load_full((t + 1) * kKV);
-            compute_slice(cur);
-            if (t + 1 < nfull) store_slice(cur ^ 1);
-            __syncthreads();
-        }
-        if (nfull * kKV < segn) {
-            load_tail(nfull * kKV);
-            store_slice(0);
-            __syncthreads();
-            compute_slice(0);
-            __syncthreads();
-        }
-    }
-
-    // Epilogue: rho_partial[g] = sum_u lhs[u][g] * T[u][g] over this chunk's rows.
-    double *red = As_raw;  // [4 waves][NS][W]; all slice reads finished at the last barrier
-#pragma unroll
-    for (int nt = 0; nt < NT; ++nt) {
-        const int g = g0 + nt * 16 + lrow;
-        const bool g_ok = g < ngrid;
-        double lhs[4];
-        bool row_ok[4];
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int i = wave * 16 + lk + 4 * r;  // D[l/16 + 4r][l%16]
-            row_ok[r] = i < nrows;
-            lhs[r] = ao_lhs[(long)ulist[i] * stride_grid + (g_ok ? g : ngrid - 1)];
-        }
-#pragma unroll
-        for (int s = 0; s < NS; ++s) {
-            // rows past the chunk end (fallback rows) contribute nothing
-            double part = 0.0;
-#pragma unroll
-            for (int r = 0; r < 4; ++r) part = row_ok[r] ? fma(acc[s][nt][r], lhs[r], part) : part;
-            part += __shfl_xor(part, 16);
-            part += __shfl_xor(part, 32);
-            if (lk == 0) red[(wave * NS + s) * W + nt * 16 + lrow] = part;
-        }
-    }
-    __syncthreads();
-    for (int idx = tid; idx < NS * W; idx += kThreads) {
-        const int s = idx / W, col = idx % W;
-        const int g = g0 + col;
-        if (g < ngrid) {
-            double v = red[s * W + col];
-#pragma unroll
-            for (int w = 1; w < kThreads / 64; ++w) v += red[(w * NS + s) * W + col];
-            partial[((long)chunk * nset + set0 + s) * ngrid + g] = v;
-        }
-    }
+    rho::RhoChunk t;
+    t.cnt = counts[blk];
+    t.chunk = blockIdx.y;
+    if (t.chunk * kUC >= t.cnt) return;
+    t.g0 = blk * blksize + sub * NT * 16;
+    if (t.g0 >= ngrid) return;  // subtile past the end of a partial last block
+    t.partial = partial;
+    t.dms = dms;
+    t.ao_lhs = ao_lhs;
+    t.ao_rhs = ao_rhs;
+    t.stride_ao = stride_ao;
+    t.stride_grid = stride_grid;
+    t.ngrid = ngrid;
+    t.nset = nset;
+    t.set0 = set0;
+    t.list = lists + (long)blk * nao;
+    rho::rho_chunk_body<NS, NT>(t, smem);
 }
 
 template <int NS>

@@ -1,0 +1,269 @@
+// rho_batched.hip -- many independent masked rho problems in one persistent launch.
+//
+// No counterpart in the reference (aodmao2rho_loop_parallel handles one problem,
+// src/impl_aodmao2rho.rs:171-244); each problem keeps exactly that contract.
+//
+//   1. batch_build_lists: the compacted AO lists of every (problem, grid block).
+//   2. batch_make_tasks:  one task per (problem, block, 64-point subtile,
+//      64-row chunk of the block's list, group of density matrices), appended
+//      to a device task array; problems are numbered largest-first by the host.
+//   3. batch_rho_kernel:  a persistent grid (CUs x workgroups-per-CU, both
+//      tunable) pulls tasks from an atomic counter until the array is drained
+//      and runs the same chunk body as the per-problem kernel (rho_body.hpp).
+//   4. batch_reduce:      rho += sum of the chunk partials, in a fixed order.
+// Every task writes its own partial slot, so the result does not depend on
+// which workgroup ran which task: it is bitwise the same as a per-problem call.
+#include "internal.hpp"
+#include "rho_body.hpp"
+
+#include <algorithm>
+#include <numeric>
+#include <vector>
+
+namespace cfgmm {
+namespace {
+
+using rho::kThreads;
+using rho::kUC;
+
+struct BatchProblem {
+    double *rhos;
+    const double *dms, *ao_lhs, *ao_rhs;
+    const uint8_t *tab;
+    long stride_ao, stride_grid, ldtab;
+    int nao, ngrid, nset, blksize, nblk, nsub, nt;
+    long lists_off, counts_off, partial_off;
+};
+
+__global__ __launch_bounds__(256) void batch_build_lists(const BatchProblem *__restrict__ probs, int *__restrict__ lists,
+                                                         int *__restrict__ counts) {
+    const BatchProblem p = probs[blockIdx.y];
+    const int b = blockIdx.x;
+    if (b >= p.nblk) return;
+    __shared__ int wave_cnt[4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int *__restrict__ out = lists + p.lists_off + (long)b * p.nao;
+    int base = 0;
+    for (int x0 = 0; x0 < p.nao; x0 += 256) {
+        const int x = x0 + threadIdx.x;
+        const bool on = x < p.nao && p.tab[(long)x * p.ldtab + b] != 0;
+        const unsigned long long bal = __ballot(on);
+        const int prefix = __popcll(bal & ((1ull << lane) - 1ull));
+        if (lane == 0) wave_cnt[wave] = __popcll(bal);
+        __syncthreads();
+        int off = base;
+        for (int w = 0; w < wave; ++w) off += wave_cnt[w];
+        if (on) out[off + prefix] = x;
+        base += wave_cnt[0] + wave_cnt[1] + wave_cnt[2] + wave_cnt[3];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) counts[p.counts_off + b] = base;
+}
+
+// Task word: x = problem, y = block, z = subtile << 16 | chunk, w = first density matrix.
+__global__ void batch_make_tasks(const BatchProblem *__restrict__ probs, const int *__restrict__ counts, int nsmax,
+                                 int4 *__restrict__ tasks, int *__restrict__ ntasks) {
+    const int pi = blockIdx.y;
+    const BatchProblem p = probs[pi];
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= p.nblk) return;
+    const int cnt = counts[p.counts_off + b];
+    const int nch = (cnt + kUC - 1) / kUC;
+    for (int sub = 0; sub < p.nsub; ++sub) {
+        if (b * p.blksize + sub * p.nt * 16 >= p.ngrid) break;
+        for (int ch = 0; ch < nch; ++ch)
+            for (int s0 = 0; s0 < p.nset; s0 += nsmax) {
+                const int t = atomicAdd(ntasks, 1);
+                tasks[t] = int4{pi, b, (sub << 16) | ch, s0};
+            }
+    }
+}
+
+__global__ void zero_int_kernel(int *p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+template <int NS>
+__device__ inline void run_chunk_ns(int nt, const rho::RhoChunk &t, unsigned char *smem) {
+    switch (nt) {
+        case 1: rho::rho_chunk_body<NS, 1>(t, smem); break;
+        case 2: rho::rho_chunk_body<NS, 2>(t, smem); break;
+        case 3: rho::rho_chunk_body<NS, 3>(t, smem); break;
+        default: rho::rho_chunk_body<NS, 4>(t, smem); break;
+    }
+}
+
+template <int NSMAX>
+__global__ __launch_bounds__(kThreads) void batch_rho_kernel(const BatchProblem *__restrict__ probs,
+                                                             const int4 *__restrict__ tasks,
+                                                             const int *__restrict__ ntasks, int *__restrict__ head,
+                                                             const int *__restrict__ lists,
+                                                             const int *__restrict__ counts, double *__restrict__ partial) {
+    __shared__ __attribute__((aligned(16))) unsigned char smem[rho::chunk_lds_bytes<NSMAX, 4>()];
+    __shared__ int cur;
+    const int n = *ntasks;
+    for (;;) {
+        if (threadIdx.x == 0) cur = atomicAdd(head, 1);
+        __syncthreads();
+        const int ti = cur;
+        __syncthreads();  // every thread has read `cur` before thread 0 overwrites it
+        if (ti >= n) break;
+        const int4 w = tasks[ti];
+        const BatchProblem &p = probs[w.x];
+        rho::RhoChunk t;
+        t.partial = partial + p.partial_off;
+        t.dms = p.dms;
+        t.ao_lhs = p.ao_lhs;
+        t.ao_rhs = p.ao_rhs;
+        t.stride_ao = p.stride_ao;
+        t.stride_grid = p.stride_grid;
+        t.ngrid = p.ngrid;
+        t.nset = p.nset;
+        t.set0 = w.w;
+        t.list = lists + p.lists_off + (long)w.y * p.nao;
+        t.cnt = counts[p.counts_off + w.y];
+        t.chunk = w.z & 0xffff;
+        t.g0 = w.y * p.blksize + (w.z >> 16) * p.nt * 16;
+        const int ns = min(NSMAX, p.nset - w.w);
+        if (NSMAX >= 4 && ns == 4)
+            run_chunk_ns<(NSMAX >= 4 ? 4 : 1)>(p.nt, t, smem);
+        else if (NSMAX >= 3 && ns == 3)
+            run_chunk_ns<(NSMAX >= 3 ? 3 : 1)>(p.nt, t, smem);
+        else if (NSMAX >= 2 && ns == 2)
+            run_chunk_ns<(NSMAX >= 2 ? 2 : 1)>(p.nt, t, smem);
+        else
+            run_chunk_ns<1>(p.nt, t, smem);
+    }
+}
+
+__global__ void batch_reduce(const BatchProblem *__restrict__ probs, const int *__restrict__ counts,
+                             const double *__restrict__ partial) {
+    const BatchProblem p = probs[blockIdx.y];
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)p.nset * p.ngrid) return;
+    const int s = (int)(idx / p.ngrid), g = (int)(idx % p.ngrid);
+    const int nch = (counts[p.counts_off + g / p.blksize] + kUC - 1) / kUC;
+    const double *__restrict__ part = partial + p.partial_off;
+    double v = 0.0;
+    for (int ch = 0; ch < nch; ++ch) v += part[((long)ch * p.nset + s) * p.ngrid + g];
+    p.rhos[(long)s * p.stride_grid + g] += v;
+}
+
+int tile_nt(size_t blksize) {
+    const size_t q = blksize / 16;
+    for (int nt = 4; nt >= 1; --nt)
+        if (q % nt == 0) return nt;
+    return 1;
+}
+
+}  // namespace
+
+int launch_rho_batched(hipStream_t stream, const cfgmm_rho_problem *problems, size_t nprob) {
+    if (nprob == 0) return CFGMM_OK;
+    if (nprob > 65535) return fail(CFGMM_EINVAL, "batched rho: at most 65535 problems per call");
+    // sets per task: 1, 2 or 4 -- the task builder and the kernel must agree
+    const int nsmax_req = tune_get(CFGMM_TUNE_BATCH_MAX_SETS);
+    const int nsmax = nsmax_req >= 4 ? 4 : (nsmax_req >= 2 ? 2 : 1);
+    const int wgs_per_cu = tune_get(CFGMM_TUNE_BATCH_WGS_PER_CU);
+    // largest problems first, so the persistent grid drains big tasks early
+    std::vector<size_t> order(nprob);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) {
+        const auto &a = problems[x], &b = problems[y];
+        return (double)a.nao * a.nao * a.ngrid * a.nset > (double)b.nao * b.nao * b.ngrid * b.nset;
+    });
+    std::vector<BatchProblem> bp;
+    bp.reserve(nprob);
+    long lists_n = 0, counts_n = 0, partial_n = 0, tasks_max = 0;
+    int max_nblk = 0;
+    long max_elems = 0;
+    for (size_t oi : order) {
+        const cfgmm_rho_problem &q = problems[oi];
+        if (q.nao == 0 || q.ngrid == 0 || q.nset == 0) continue;
+        if (q.nao > INT32_MAX || q.ngrid > INT32_MAX || q.nset > 65535)
+            return fail(CFGMM_EINVAL, "batched rho: problem size out of range");
+        BatchProblem p{};
+        p.rhos = q.rhos;
+        p.dms = q.dms;
+        p.ao_lhs = q.ao_lhs;
+        p.ao_rhs = q.ao_rhs;
+        p.tab = q.tab;
+        p.stride_ao = (long)q.stride_ao;
+        p.stride_grid = (long)q.stride_grid;
+        p.ldtab = (long)q.ldtab;
+        p.nao = (int)q.nao;
+        p.ngrid = (int)q.ngrid;
+        p.nset = (int)q.nset;
+        p.blksize = (int)q.blksize;
+        p.nblk = (int)div_up(q.ngrid, q.blksize);
+        p.nt = tile_nt(q.blksize);
+        p.nsub = (int)(q.blksize / (16 * p.nt));
+        p.lists_off = lists_n;
+        p.counts_off = counts_n;
+        p.partial_off = partial_n;
+        const long maxchunks = (long)div_up(q.nao, kUC);
+        if (maxchunks > 0xffff || p.nsub > 0x7fff) return fail(CFGMM_EINVAL, "batched rho: problem too large");
+        lists_n += (long)p.nblk * p.nao;
+        counts_n += p.nblk;
+        partial_n += maxchunks * p.nset * p.ngrid;
+        tasks_max += (long)p.nblk * p.nsub * maxchunks * (long)div_up(q.nset, nsmax);
+        max_nblk = std::max(max_nblk, p.nblk);
+        max_elems = std::max(max_elems, (long)p.nset * p.ngrid);
+        bp.push_back(p);
+    }
+    if (bp.empty()) return CFGMM_OK;
+    if (tasks_max > INT32_MAX) return fail(CFGMM_EINVAL, "batched rho: too many tasks for one launch");
+    const int np = (int)bp.size();
+
+    auto al = [](size_t x) { return div_up(x, 256) * 256; };
+    const size_t o_lists = al(sizeof(BatchProblem) * np);
+    const size_t o_counts = o_lists + al(sizeof(int) * lists_n);
+    const size_t o_part = o_counts + al(sizeof(int) * counts_n);
+    const size_t o_tasks = o_part + al(sizeof(double) * partial_n);
+    const size_t o_ctr = o_tasks + al(sizeof(int4) * tasks_max);
+    void *ws = nullptr;
+    int st = workspace_get(stream, o_ctr + 2 * sizeof(int), &ws);
+    if (st != CFGMM_OK) return st;
+    char *base = static_cast<char *>(ws);
+    BatchProblem *d_probs = reinterpret_cast<BatchProblem *>(base);
+    int *lists = reinterpret_cast<int *>(base + o_lists);
+    int *counts = reinterpret_cast<int *>(base + o_counts);
+    double *partial = reinterpret_cast<double *>(base + o_part);
+    int4 *tasks = reinterpret_cast<int4 *>(base + o_tasks);
+    int *ctr = reinterpret_cast<int *>(base + o_ctr);  // [0] = task count, [1] = queue head
+
+    // Pageable source: the copy is complete when hipMemcpyAsync returns, so `bp` may go out of scope.
+    CFGMM_HIP_TRY(hipMemcpyAsync(d_probs, bp.data(), sizeof(BatchProblem) * np, hipMemcpyHostToDevice, stream));
+    hipEvent_t t0 = trace_start(stream);
+    hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(64), 0, stream, ctr, 2);
+    hipLaunchKernelGGL(batch_build_lists, dim3((unsigned)max_nblk, (unsigned)np), dim3(256), 0, stream, d_probs,
+                       lists, counts);
+    hipLaunchKernelGGL(batch_make_tasks, dim3((unsigned)div_up(max_nblk, 64), (unsigned)np), dim3(64), 0, stream,
+                       d_probs, counts, nsmax, tasks, ctr);
+    trace_stop(stream, "batch_prepare", t0);
+    st = hip_status(hipGetLastError(), "batched rho prepare launch");
+    if (st != CFGMM_OK) return st;
+
+    const int grid = std::max(1, device_cu_count() * wgs_per_cu);
+    t0 = trace_start(stream);
+    if (nsmax >= 4)
+        hipLaunchKernelGGL((batch_rho_kernel<4>), dim3(grid), dim3(kThreads), 0, stream, d_probs, tasks, ctr, ctr + 1,
+                           lists, counts, partial);
+    else if (nsmax == 2)
+        hipLaunchKernelGGL((batch_rho_kernel<2>), dim3(grid), dim3(kThreads), 0, stream, d_probs, tasks, ctr, ctr + 1,
+                           lists, counts, partial);
+    else
+        hipLaunchKernelGGL((batch_rho_kernel<1>), dim3(grid), dim3(kThreads), 0, stream, d_probs, tasks, ctr, ctr + 1,
+                           lists, counts, partial);
+    trace_stop(stream, "batch_rho_kernel", t0);
+    st = hip_status(hipGetLastError(), "batch_rho_kernel launch");
+    if (st != CFGMM_OK) return st;
+    t0 = trace_start(stream);
+    hipLaunchKernelGGL(batch_reduce, dim3((unsigned)div_up(max_elems, 256), (unsigned)np), dim3(256), 0, stream,
+                       d_probs, counts, partial);
+    trace_stop(stream, "batch_reduce", t0);
+    return hip_status(hipGetLastError(), "batch_reduce launch");
+}
+
+}  // namespace cfgmm

@@ -156,6 +156,39 @@ int cfgmm_aodmao2rho_ex_dev(void *stream, double *rhos, const double *dms, const
                             int nslices);
 
 /* ======================================================================== */
+/* 2c. batched rho (no counterpart in the reference)                        */
+/* ======================================================================== */
+
+/* One problem of a batch: the arguments of cfgmm_aodmao2rho_dev, device pointers. */
+typedef struct cfgmm_rho_problem {
+    double *rhos;
+    const double *dms;
+    const double *ao_lhs;
+    const double *ao_rhs;
+    size_t stride_ao, stride_grid, nao, ngrid, nset;
+    const uint8_t *tab;
+    size_t ldtab, blksize;
+} cfgmm_rho_problem;
+
+/* Evaluates every problem of the batch, each with exactly the contract of
+ * cfgmm_aodmao2rho_dev.  FP64: one persistent launch drains a device-built
+ * task queue of all (problem, grid block, 64-row chunk, density-matrix group)
+ * tasks; split precision: the problems are enqueued one after another.
+ * Results are bitwise those of separate cfgmm_aodmao2rho_ex_dev calls. */
+int cfgmm_aodmao2rho_batched_dev(void *stream, const cfgmm_rho_problem *problems, size_t nproblems, int precision,
+                                 int nslices);
+
+/* ---- tunables (set by the autotuner; process-wide) ------------------------ */
+enum cfgmm_tunable {
+    CFGMM_TUNE_BATCH_WGS_PER_CU = 0, /* batched rho: persistent workgroups per CU, 1..8 (default 2) */
+    CFGMM_TUNE_BATCH_MAX_SETS = 1,   /* batched rho: density matrices per task, 1, 2 or 4 (default 2) */
+    CFGMM_TUNE_DGEMM_GROUP_M = 2,    /* DGEMM: tile rows per raster group, 1..64 (default 4) */
+    CFGMM_TUNE_COUNT = 3
+};
+int cfgmm_tune_set(int param, int value);
+int cfgmm_tune_get(int param, int *value);
+
+/* ======================================================================== */
 /* device plumbing for callers without their own HIP binding                */
 /* ======================================================================== */
 int cfgmm_device_count(int *count);
