@@ -242,3 +242,55 @@ def zero_under_mask(x: np.ndarray, tab: np.ndarray, blksize: int, ncols: int) ->
     """x[p][j] = 0 where !tab[p][j / blksize] (src/impl_matmul_non0tab.rs:346-357)."""
     on = np.repeat(tab.astype(bool), blksize, axis=1)[:, :ncols]
     x.reshape(tab.shape[0], -1)[:, :ncols][~on] = 0.0
+
+
+# ---------------------------------------------------------------------------
+# many independent rho problems (batched launch, serving mix)
+# ---------------------------------------------------------------------------
+@dataclass
+class Molecule:
+    """A generated molecule on a 64-block grid, sliced by the problems that use it."""
+    seed: int
+    prob: RhoProblem  # nset = 4 density matrices, ngrid = 64 * nblk_max
+
+
+@dataclass
+class PoolProblem:
+    mol: int      # index into the molecule pool
+    ngrid: int    # leading grid points of the molecule used (multiple of 64)
+    nset: int     # leading density matrices used
+
+    def flops(self, pool) -> float:
+        p = pool[self.mol].prob
+        nblk = self.ngrid // p.blksize
+        n_b = p.counts()[:nblk].astype(np.float64)
+        return float(self.nset * np.sum(p.blksize * (2.0 * n_b * n_b + 2.0 * n_b)))
+
+
+def molecule_pool(n=32, seed=100, nao_range=(300, 2228), max_blocks=64, blksize=64, density_range=(0.1, 0.6),
+                  nset=4):
+    """n molecules, nao ~ U{nao_range} (shells ~ nao/3), density ~ U(density_range), max_blocks 64-point blocks."""
+    rng = np.random.default_rng(seed)
+    pool = []
+    for i in range(n):
+        nao = int(rng.integers(nao_range[0], nao_range[1] + 1))
+        nshell = max(1, nao // 3)
+        if (nao - nshell) % 2:
+            nao += 1
+        dens = float(rng.uniform(*density_range))
+        p = rho_problem(seed=seed * 1000 + i, nshell=nshell, nao=nao, ngrid=max_blocks * blksize, nset=nset,
+                        blksize=blksize, density=dens, nocc=min(200, nao))
+        pool.append(Molecule(seed * 1000 + i, p))
+    return pool
+
+
+def batched_problems(pool, n=512, seed=7, max_blocks=64, blksize=64, nset_range=(1, 4)):
+    """n problems: a molecule from the pool, ngrid = blksize * U{1..max_blocks}, nset ~ U{nset_range}."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        m = int(rng.integers(0, len(pool)))
+        ngrid = blksize * int(rng.integers(1, max_blocks + 1))
+        nset = int(rng.integers(nset_range[0], nset_range[1] + 1))
+        out.append(PoolProblem(m, ngrid, nset))
+    return out

@@ -81,6 +81,13 @@ def _u8ptr(x: np.ndarray):
     return x.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
 
 
+def _need(name: str, arr: np.ndarray, rows: int, cols: int, ld: int) -> None:
+    """The C restatement indexes like the reference; refuse buffers shorter than that extent."""
+    need = 0 if rows == 0 or cols == 0 else (rows - 1) * ld + cols
+    if arr.size < need:
+        raise ValueError(f"oracle: {name} has {arr.size} elements, the arguments index {need}")
+
+
 def _check(st: int, what: str) -> None:
     if st != OK:
         raise OracleError(f"{what} failed with status {st}")
@@ -88,6 +95,10 @@ def _check(st: int, what: str) -> None:
 
 def matmul(c, a, b, m, n, k, lda, ldb, ldc, transa, transb, nthreads=0):
     """C += op(A) op(B) in place on flat float64 arrays (src/impl_matmul.rs:155-231)."""
+    if m and n and k:
+        _need("a", a, k if transa else m, m if transa else k, lda)
+        _need("b", b, n if transb else k, k if transb else n, ldb)
+    _need("c", c, m, n, ldc)
     _check(lib().oracle_matmul_loop_parallel_mnk_pack_a(_dptr(c), _dptr(a), _dptr(b), m, n, k, lda, ldb, ldc,
                                                         int(transa), int(transb), nthreads), "oracle matmul")
     return c
@@ -95,6 +106,11 @@ def matmul(c, a, b, m, n, k, lda, ldb, ldc, transa, transb, nthreads=0):
 
 def matmul_non0tab(c, a, b, m, n, k, lda, ldb, ldc, transa, transb, tab, ldtab, blksize=48, nthreads=0):
     """Masked C += op(A)(B masked by tab) (src/impl_matmul_non0tab.rs:210-305); returns the status."""
+    if m and n and k and not transb and blksize:
+        _need("a", a, k if transa else m, m if transa else k, lda)
+        _need("b", b, k, n, ldb)
+        _need("tab", tab.reshape(-1), k, -(-n // blksize), ldtab)
+    _need("c", c, m, n, ldc)
     return lib().oracle_matmul_loop_parallel_mnk_pack_a_non0tab(_dptr(c), _dptr(a), _dptr(b), m, n, k, lda, ldb, ldc,
                                                                 int(transa), int(transb), _u8ptr(tab), ldtab,
                                                                 blksize, nthreads)
@@ -102,6 +118,14 @@ def matmul_non0tab(c, a, b, m, n, k, lda, ldb, ldc, transa, transb, tab, ldtab, 
 
 def aodmao2rho(rhos, dms, ao_lhs, ao_rhs, stride_ao, stride_grid, nao, ngrid, nset, tab, ldtab, blksize=48,
                nthreads=0):
-    """rho += sum phi_lhs D phi_rhs under the mask (src/impl_aodmao2rho.rs:171-244); returns the status."""
+    """rho += sum phi_lhs D phi_rhs under the mask (src/impl_aodmao2rho.rs:171-244); returns the status.
+
+    rho rows are strided by stride_grid, like the phi rows (src/impl_aodmao2rho.rs:80,88)."""
+    if nao and ngrid and nset and blksize:
+        _need("rhos", rhos, nset, ngrid, stride_grid)
+        _need("ao_lhs", ao_lhs, nao, ngrid, stride_grid)
+        _need("ao_rhs", ao_rhs, nao, ngrid, stride_grid)
+        _need("dms", dms, (nset - 1) * stride_ao + nao, nao, stride_ao)
+        _need("tab", tab.reshape(-1), nao, -(-ngrid // blksize), ldtab)
     return lib().oracle_aodmao2rho_loop_parallel(_dptr(rhos), _dptr(dms), _dptr(ao_lhs), _dptr(ao_rhs), stride_ao,
                                                  stride_grid, nao, ngrid, nset, _u8ptr(tab), ldtab, blksize, nthreads)
