@@ -46,6 +46,10 @@ FP64_MFMA_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
 METRIC = "FP64 DGEMM TFLOP/s (% peak); rho evals/s + p99 latency at ≤1e-10 rel err"
 README_RHO_EVALS_PER_S = 1000.0 / 120.0  # README.md:51, 16-core Ryzen 7945HX
+# Bytes moved past L2 by one rho_chunk_kernel launch on the default workload, from two rocprofv3
+# --pmc passes (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, factors from a 256 MiB device copy):
+# profiles/rho_chunk_kernel_fp64_nao2228_ngrid16384_nset3.txt.  Only valid for that workload.
+RHO_DEFAULT_TRAFFIC_BYTES = 4249269 * 1024 * 2.000 + 4167 * 1024 * 1.000
 
 
 def log(msg: str) -> None:
@@ -137,7 +141,7 @@ def bench_dgemm(g: W.GemmProblem, reps: int, warmup: int = 2):
                 dgemm_kernel_avg_ms=round(kavg, 4), kernel_tflops=round(tf_kern, 2))
 
 
-def cpu_baseline_rho(p: W.RhoProblem, budget_s: float = 12.0):
+def cpu_baseline_rho(p: W.RhoProblem, budget_s: float = 15.0):
     import oracle  # the CPU oracle: only this leg of bench.py uses it
     threads = min(16, os.cpu_count() or 1)
     rhos = np.zeros(p.nset * p.ngrid)
@@ -148,7 +152,7 @@ def cpu_baseline_rho(p: W.RhoProblem, budget_s: float = 12.0):
     first = time.perf_counter() - t0
     ref = rhos.reshape(p.nset, p.ngrid).copy()
     n, t_tot = 1, first
-    while t_tot < budget_s and n < 50:
+    while t_tot < budget_s and n < 400:
         r2 = np.zeros_like(rhos)
         t0 = time.perf_counter()
         oracle.aodmao2rho(r2, *args, nthreads=threads)
@@ -287,6 +291,7 @@ def main():
 
     log("[bench] generating the rho workload (seed 0)")
     p = W.rho_problem(seed=0)
+    default_workload = True
     log(f"[bench] nao={p.nao} ngrid={p.ngrid} nset={p.nset} blocks={p.nblk} mean|L_b|={p.counts().mean():.1f}")
     log(f"[bench] rho: {args.warmup} warmup + {args.steps} timed steps")
     r = bench_rho(p, args.steps, args.warmup)
@@ -321,7 +326,9 @@ def main():
                        "p99": round(float(np.percentile(r["per_step_ms"], 99)), 5)},
         "roofline": {"bound": "mfma", "kernel": "rho_chunk_kernel", "achieved": round(kern_tflops, 3),
                      "peak": FP64_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(kern_tflops / FP64_MFMA_PEAK_TFLOPS, 4), "traffic": None,
+                     "frac": round(kern_tflops / FP64_MFMA_PEAK_TFLOPS, 4),
+                     "traffic": RHO_DEFAULT_TRAFFIC_BYTES if default_workload else None,
+                     "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, copy-calibrated",
                      "kernel_avg_ms": round(r["kernel_avg_ms"], 5),
                      "algorithmic_flop_per_launch": flops,
                      "compulsory_bytes_per_eval": p.compulsory_bytes(),

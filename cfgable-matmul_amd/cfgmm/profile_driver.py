@@ -1,0 +1,68 @@
+"""Runs one hot-path operation a few times on resident inputs, for profilers.
+
+    python -m cfgmm.profile_driver <mode> [reps]
+
+Modes:
+  rho            masked FP64 rho, nao 2228, ngrid 16384, nset 3, 30 % mask (rho_chunk_kernel)
+  rho_split<S>   the same rho in split precision with S int8 slices (rho_ozaki_kernel)
+  dgemm_large    FP64 DGEMM 3527x9583x6581 NN (dgemm_kernel)
+  dgemm_small    FP64 DGEMM 1031x977x1543 NT
+  gemm_split<S>  the large GEMM in split precision (ozaki_gemm_kernel)
+  copy           a 256 MiB device-to-device copy (byte-counter calibration)
+
+Each pass of a hardware-counter profiler is a separate process running this
+module; the inputs are generated, uploaded once and then the operation runs
+`reps` times.
+"""
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+
+import cfgmm
+from cfgmm import workloads as W
+
+
+def main(argv: list[str]) -> int:
+    mode = argv[1] if len(argv) > 1 else "rho"
+    reps = int(argv[2]) if len(argv) > 2 else 3
+    s = cfgmm.Stream()
+    if mode == "rho" or mode.startswith("rho_split"):
+        prec, ns = (cfgmm.PREC_FP64, 0) if mode == "rho" else (cfgmm.PREC_OZAKI_INT8, int(mode[len("rho_split"):]))
+        p = W.rho_problem(seed=0)
+        d_r = cfgmm.DeviceBuffer(8 * p.nset * p.ngrid)
+        d_r.zero()
+        d_d = cfgmm.DeviceBuffer.from_array(p.dms)
+        d_phi = cfgmm.DeviceBuffer.from_array(p.phi)
+        d_tab = cfgmm.DeviceBuffer.from_array(np.ascontiguousarray(p.tab))
+        for _ in range(reps):
+            cfgmm.aodmao2rho_dev(s, d_r, d_d, d_phi, d_phi, p.nao, p.ngrid, p.nao, p.ngrid, p.nset, d_tab, p.nblk,
+                                 p.blksize, prec, ns)
+    elif mode in ("dgemm_large", "dgemm_small") or mode.startswith("gemm_split"):
+        g = W.dgemm_small_nt() if mode == "dgemm_small" else W.dgemm_large_nn()
+        prec, ns = (cfgmm.PREC_FP64, 0) if mode.startswith("dgemm") else (cfgmm.PREC_OZAKI_INT8,
+                                                                           int(mode[len("gemm_split"):]))
+        d_a = cfgmm.DeviceBuffer.from_array(g.a)
+        d_b = cfgmm.DeviceBuffer.from_array(g.b)
+        d_c = cfgmm.DeviceBuffer(8 * g.m * g.ldc)
+        d_c.zero()
+        for _ in range(reps):
+            cfgmm.dgemm_dev(s, d_c, d_a, d_b, g.m, g.n, g.k, g.lda, g.ldb, g.ldc, g.transa, g.transb, prec, ns)
+    elif mode == "copy":
+        nbytes = 256 << 20
+        a = cfgmm.DeviceBuffer(nbytes)
+        b = cfgmm.DeviceBuffer(nbytes)
+        a.zero()
+        for _ in range(reps):
+            cfgmm.check(cfgmm.lib().cfgmm_memcpy_dtod_async(b.ptr, a.ptr, nbytes, s.handle))
+    else:
+        print(f"unknown mode {mode!r}", file=sys.stderr)
+        return 2
+    s.synchronize()
+    print(f"profile_driver: {mode} x{reps} done", flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main(sys.argv))
