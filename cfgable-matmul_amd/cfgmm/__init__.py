@@ -65,6 +65,10 @@ _SIGNATURES = {
     "cfgmm_aodmao2rho_dev": ([_VP, _VP, _VP, _VP, _VP] + [_SZ] * 5 + [_VP, _SZ, _SZ], _I),
     "cfgmm_aodmao2rho_ex_dev": ([_VP, _VP, _VP, _VP, _VP] + [_SZ] * 5 + [_VP, _SZ, _SZ, _I, _I], _I),
     "cfgmm_aodmao2rho_batched_dev": ([_VP, _VP, _SZ, _I, _I], _I),
+    "cfgmm_rho_resident_create": ([_VP, _VP, _VP, _VP] + [_SZ] * 5 + [_VP, _SZ, _SZ, _I, _I, ctypes.POINTER(_VP)], _I),
+    "cfgmm_rho_resident_eval": ([_VP, _VP, _VP, _SZ, _SZ, _SZ], _I),
+    "cfgmm_rho_resident_bytes": ([_VP, ctypes.POINTER(_SZ)], _I),
+    "cfgmm_rho_resident_destroy": ([_VP], _I),
     "cfgmm_tune_set": ([_I, _I], _I),
     "cfgmm_tune_get": ([_I, ctypes.POINTER(_I)], _I),
     "cfgmm_device_count": ([ctypes.POINTER(_I)], _I),
@@ -356,6 +360,44 @@ def aodmao2rho_batched_dev(stream, problems, precision=PREC_FP64, nslices=0):
     """problems: a sequence of RhoProblem (or a ctypes array of them)."""
     arr = (RhoProblem * len(problems))(*problems) if not isinstance(problems, ctypes.Array) else problems
     check(lib().cfgmm_aodmao2rho_batched_dev(_h(stream), ctypes.cast(arr, _VP), len(arr), precision, nslices))
+
+
+class RhoResident:
+    """Resident operands of one masked rho problem (cfgmm_rho_resident_*), split precision.
+
+    Built once from device buffers (dms, phi, mask); every ``eval`` adds
+    rho_s(g) for the leading ``ngrid`` points and ``nset`` matrices into ``rhos``.
+    """
+
+    def __init__(self, stream, dms, ao_lhs, ao_rhs, stride_ao, stride_grid, nao, ngrid, nset, tab, ldtab, blksize,
+                 precision=PREC_OZAKI_INT8, nslices=6):
+        h = _VP()
+        check(lib().cfgmm_rho_resident_create(_h(stream), _dev(dms), _dev(ao_lhs), _dev(ao_rhs), stride_ao,
+                                              stride_grid, nao, ngrid, nset, _dev(tab), ldtab, blksize, precision,
+                                              nslices, ctypes.byref(h)))
+        self.handle = h.value
+        self.ngrid, self.nset, self.blksize = ngrid, nset, blksize
+
+    @property
+    def bytes(self) -> int:
+        v = _SZ(0)
+        check(lib().cfgmm_rho_resident_bytes(self.handle, ctypes.byref(v)))
+        return v.value
+
+    def eval(self, stream, rhos, stride_rho, ngrid=None, nset=None) -> None:
+        check(lib().cfgmm_rho_resident_eval(_h(stream), self.handle, _dev(rhos), stride_rho,
+                                            self.ngrid if ngrid is None else ngrid,
+                                            self.nset if nset is None else nset))
+
+    def destroy(self) -> None:
+        if self.handle:
+            check(lib().cfgmm_rho_resident_destroy(self.handle))
+            self.handle = None
+
+
+def _dev(x):
+    """Device address of a DeviceBuffer or a raw integer pointer."""
+    return x.ptr if hasattr(x, "ptr") else x
 
 
 def tune_set(param: int, value: int) -> None:

@@ -5,6 +5,8 @@
 Modes:
   rho            masked FP64 rho, nao 2228, ngrid 16384, nset 3, 30 % mask (rho_chunk_kernel)
   rho_split<S>   the same rho in split precision with S int8 slices (rho_ozaki_kernel)
+  rho_resident<S> the same rho over resident operands, S int8 slices (rho_resident_kernel; the
+                 one-time preparation runs before the repetitions)
   dgemm_large    FP64 DGEMM 3527x9583x6581 NN (dgemm_kernel)
   dgemm_small    FP64 DGEMM 1031x977x1543 NT
   gemm_split<S>  the large GEMM in split precision (ozaki_gemm_kernel)
@@ -39,6 +41,17 @@ def main(argv: list[str]) -> int:
         for _ in range(reps):
             cfgmm.aodmao2rho_dev(s, d_r, d_d, d_phi, d_phi, p.nao, p.ngrid, p.nao, p.ngrid, p.nset, d_tab, p.nblk,
                                  p.blksize, prec, ns)
+    elif mode.startswith("rho_resident"):
+        p = W.rho_problem(seed=0)
+        d_r = cfgmm.DeviceBuffer(8 * p.nset * p.ngrid)
+        d_r.zero()
+        d_d = cfgmm.DeviceBuffer.from_array(p.dms)
+        d_phi = cfgmm.DeviceBuffer.from_array(p.phi)
+        d_tab = cfgmm.DeviceBuffer.from_array(np.ascontiguousarray(p.tab))
+        h = cfgmm.RhoResident(s, d_d, d_phi, d_phi, p.nao, p.ngrid, p.nao, p.ngrid, p.nset, d_tab, p.nblk,
+                              p.blksize, cfgmm.PREC_OZAKI_INT8, int(mode[len("rho_resident"):]))
+        for _ in range(reps):
+            h.eval(s, d_r, p.ngrid)
     elif mode in ("dgemm_large", "dgemm_small") or mode.startswith("gemm_split"):
         g = W.dgemm_small_nt() if mode == "dgemm_small" else W.dgemm_large_nn()
         prec, ns = (cfgmm.PREC_FP64, 0) if mode.startswith("dgemm") else (cfgmm.PREC_OZAKI_INT8,

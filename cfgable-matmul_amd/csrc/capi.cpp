@@ -529,6 +529,62 @@ int cfgmm_aodmao2rho_batched_dev(void *stream, const cfgmm_rho_problem *problems
     return fail(CFGMM_EINVAL, "unknown precision mode " + std::to_string(precision));
 }
 
+// ---- resident rho operands -------------------------------------------------------
+static ResidentRho *impl(const cfgmm_rho_resident *h) {
+    return reinterpret_cast<ResidentRho *>(const_cast<cfgmm_rho_resident *>(h));
+}
+
+int cfgmm_rho_resident_create(void *stream, const double *dms, const double *ao_lhs, const double *ao_rhs,
+                              size_t stride_ao, size_t stride_grid, size_t nao, size_t ngrid, size_t nset,
+                              const uint8_t *tab, size_t ldtab, size_t blksize, int precision, int nslices,
+                              cfgmm_rho_resident **out) {
+    clear_error();
+    if (!out) return fail(CFGMM_EINVAL, "out is null");
+    *out = nullptr;
+    RhoExtents ext;
+    CFGMM_TRY(rho_extents(stride_ao, stride_grid, nao, ngrid, nset, ldtab, blksize, &ext));
+    if (nao && ngrid && nset && (!dms || !ao_lhs || !ao_rhs || !tab))
+        return fail(CFGMM_EINVAL, "null device pointer");
+    if (precision != CFGMM_PREC_OZAKI_INT8)
+        return fail(CFGMM_EINVAL, "resident rho operands exist in split precision (CFGMM_PREC_OZAKI_INT8) only");
+    CFGMM_TRY(check_device());
+    ResidentRho *h = nullptr;
+    CFGMM_TRY(resident_create(static_cast<hipStream_t>(stream), dms, ao_lhs, ao_rhs, stride_ao, stride_grid, nao,
+                              ngrid, nset, tab, ldtab, blksize, nslices, &h));
+    *out = reinterpret_cast<cfgmm_rho_resident *>(h);
+    return CFGMM_OK;
+}
+
+int cfgmm_rho_resident_eval(void *stream, const cfgmm_rho_resident *h, double *rhos, size_t stride_rho,
+                            size_t ngrid_used, size_t nset_used) {
+    clear_error();
+    if (!h) return fail(CFGMM_EINVAL, "resident handle is null");
+    const ResidentRho *r = impl(h);
+    if (ngrid_used > resident_ngrid(r) || nset_used > resident_nset(r))
+        return fail(CFGMM_EINVAL_DIMS, "resident rho: ngrid_used / nset_used exceed the resident problem");
+    if (ngrid_used != resident_ngrid(r) && ngrid_used % resident_blksize(r) != 0)
+        return fail(CFGMM_EINVAL_DIMS, "resident rho: ngrid_used must be the full grid or a multiple of blksize");
+    if (ngrid_used && nset_used > 1 && stride_rho < ngrid_used)
+        return fail(CFGMM_EINVAL_DIMS, "resident rho: stride_rho < ngrid_used");
+    if (ngrid_used && nset_used && !rhos) return fail(CFGMM_EINVAL, "rhos is null");
+    int dev = 0;
+    CFGMM_HIP_TRY(hipGetDevice(&dev));
+    if (dev != resident_device(r)) return fail(CFGMM_EINVAL, "resident rho: handle belongs to another device");
+    return resident_eval(static_cast<hipStream_t>(stream), r, rhos, stride_rho, ngrid_used, nset_used);
+}
+
+int cfgmm_rho_resident_bytes(const cfgmm_rho_resident *h, size_t *bytes) {
+    clear_error();
+    if (!h || !bytes) return fail(CFGMM_EINVAL, "null argument");
+    *bytes = resident_bytes(impl(h));
+    return CFGMM_OK;
+}
+
+int cfgmm_rho_resident_destroy(cfgmm_rho_resident *h) {
+    clear_error();
+    return resident_destroy(impl(h));
+}
+
 int cfgmm_tune_set(int param, int value) {
     clear_error();
     if (param < 0 || param >= CFGMM_TUNE_COUNT) return fail(CFGMM_EINVAL, "unknown tunable " + std::to_string(param));

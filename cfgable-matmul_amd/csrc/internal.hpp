@@ -75,6 +75,20 @@ int launch_rho_ozaki(hipStream_t stream, double *rhos, const double *dms, const 
                      size_t stride_ao, size_t stride_grid, size_t nao, size_t ngrid, size_t nset, const uint8_t *tab,
                      size_t ldtab, size_t blksize, int nslices);
 
+// Resident (per-grid-block compacted, int8-sliced) rho operands: rho_resident.hip.
+struct ResidentRho;
+int resident_create(hipStream_t stream, const double *dms, const double *ao_lhs, const double *ao_rhs,
+                    size_t stride_ao, size_t stride_grid, size_t nao, size_t ngrid, size_t nset, const uint8_t *tab,
+                    size_t ldtab, size_t blksize, int nslices, ResidentRho **out);
+int resident_eval(hipStream_t stream, const ResidentRho *h, double *rhos, size_t stride_rho, size_t ngrid_used,
+                  size_t nset_used);
+int resident_destroy(ResidentRho *h);
+size_t resident_bytes(const ResidentRho *h);
+size_t resident_ngrid(const ResidentRho *h);
+size_t resident_nset(const ResidentRho *h);
+size_t resident_blksize(const ResidentRho *h);
+int resident_device(const ResidentRho *h);
+
 // ---- MFMA fragment types -------------------------------------------------------
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 

@@ -1,0 +1,562 @@
+// rho_resident.hip -- masked AO-DM-AO rho over operands held resident on the device
+// in per-grid-block compacted, int8-sliced form (split precision, Ozaki scheme).
+//
+// Same contract as launch_rho (src/impl_aodmao2rho.rs:171-244):
+//   rho[s][g] += sum_{u,v in L_b} lhs[u][g] D_s[u][v] rhs[v][g],  b = g / blksize.
+//
+// The per-call kernels gather D_s[L_b, L_b] from the full matrix for every
+// block; those gathers use a third of each cache line and dominate their run
+// time.  When the same (D, phi, mask) are evaluated many times -- the serving
+// case, where a molecule stays resident across requests -- the gather is done
+// once, at creation, into one contiguous tile per block:
+//
+//   Dq[b][s][p](i, j)  p-th 7-bit slice of D_s[L_b[i]][L_b[j]] * 2^-ed[b][s][i],
+//                      i, j < kpad_b = round_up(|L_b|, 64) (zero padded),
+//                      ed = exponent of the row maximum over the block's own v;
+//                      stored as 1 KiB tiles of 16 rows x 64 k, [i/16][j/64], each
+//                      tile ordered [k/16][i%16][k%16] -- one MFMA A fragment per
+//                      lane, 64 lanes reading 1 KiB contiguously;
+//   Rq[b][p](c, i)     p-th slice of rhs[L_b[i]][b*blksize + c] * 2^-fg[g],
+//                      fg = exponent of the column maximum over the block's AOs;
+//                      stored [i/64][c][i%64] -- one 64-k step of all columns of
+//                      the block is blksize*64 contiguous bytes;
+//   Lc[b][i][c]        lhs[L_b[i]][b*blksize + c] in f64 (zero padded).
+//
+// An evaluation then streams Dq once: each workgroup owns 64 compacted rows of
+// one block (four waves of 16) and 16..64 of its grid points, loads its A
+// fragments straight from global memory (16 contiguous bytes per lane and
+// slice), shares the B tile of the block through double-buffered LDS, and
+// accumulates every slice product (sa + sb < S) exactly in int32 with
+// v_mfma_i32_16x16x64_i8.  The epilogue recombines the levels in f64 with the
+// power-of-two scales, multiplies by lhs and reduces the 64 rows; per-tile
+// partial rows are summed in a fixed order by launch_rho_reduce.
+//
+// A row of D_s or a block column of rhs holding Inf/NaN makes the rho values
+// it touches NaN.
+#include "internal.hpp"
+
+#include <algorithm>
+#include <vector>
+
+namespace cfgmm {
+namespace {
+
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned long long u64;
+
+constexpr int kThreads = 256;
+constexpr int kRows = 64;   // compacted rows per workgroup (4 waves x 16)
+constexpr int kK = 64;      // k per v_mfma_i32_16x16x64_i8
+constexpr int kExpNonFinite = 0x7fffffff;
+constexpr int kMaxKpad = 16384;  // keeps every int32 level sum below 2^31: 6 * 16384 * 127^2 < 2^31
+
+struct BlockDesc {
+    long d_off;  // bytes: Dq of set 0 ([nset][S] planes of kpad*kpad int8, 1 KiB tiles)
+    long r_off;  // bytes: Rq ([S][kpad/64][blksize][64] int8)
+    long l_off;  // bytes: Lc ([kpad][blksize] double)
+    long e_off;  // bytes: ed ([nset][kpad] int)
+    int n, kpad;
+};
+
+__device__ inline int exponent_of(u64 maxbits) {
+    const double m = __longlong_as_double((long long)maxbits);
+    if (!(m <= 1.79769313486231570815e308)) return kExpNonFinite;
+    if (m == 0.0) return 0;
+    int e;
+    frexp(m, &e);
+    return e;
+}
+
+// byte `q` of the result is slice `p` of x * 2^-e, for four values
+template <int S>
+__device__ inline void slice4(const double x[4], const int e[4], unsigned w[S]) {
+#pragma unroll
+    for (int p = 0; p < S; ++p) w[p] = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        double rest = e[q] == kExpNonFinite ? 0.0 : ldexp(x[q], -e[q]);
+#pragma unroll
+        for (int p = 0; p < S; ++p) {
+            const double v = rest * 128.0;
+            const double t = trunc(v);
+            rest = v - t;
+            w[p] |= ((unsigned)(int)t & 0xffu) << (8 * q);
+        }
+    }
+}
+
+// One wave per compacted row i of block b and set s: ed and the S slice planes of the row.
+template <int S>
+__global__ __launch_bounds__(kThreads) void prep_dq_kernel(unsigned char *__restrict__ base,
+                                                           const BlockDesc *__restrict__ blocks,
+                                                           const double *__restrict__ dms, long stride_ao, int nao,
+                                                           const int *__restrict__ lists) {
+    const int b = blockIdx.y, s = blockIdx.z;
+    const BlockDesc bd = blocks[b];
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+    if (i >= bd.kpad) return;
+    const long plane = (long)bd.kpad * bd.kpad;
+    const int nks = bd.kpad / kK;
+    // byte of (i, j) inside a plane: 1 KiB tile (i/16, j/64), chunk (j%64)/16 * 16 + i%16, byte j%16
+    unsigned char *__restrict__ out = base + bd.d_off + (long)s * S * plane + (long)(i >> 4) * nks * 1024 + (i & 15) * 16;
+    auto at = [&](int j) { return (long)(j >> 6) * 1024 + ((j & 63) >> 4) * 256 + (j & 15); };
+    int *__restrict__ ed = reinterpret_cast<int *>(base + bd.e_off) + (long)s * bd.kpad;
+    const int *__restrict__ list = lists + (long)b * nao;
+    if (i >= bd.n) {
+        for (int j = 4 * lane; j < bd.kpad; j += 256)
+#pragma unroll
+            for (int p = 0; p < S; ++p) *reinterpret_cast<unsigned *>(out + p * plane + at(j)) = 0u;
+        if (lane == 0) ed[i] = 0;
+        return;
+    }
+    const double *__restrict__ row = dms + (long)s * stride_ao * stride_ao + (long)list[i] * stride_ao;
+    u64 m = 0;
+    for (int j = lane; j < bd.n; j += 64) {
+        const u64 bits = (u64)__double_as_longlong(fabs(row[list[j]]));
+        m = bits > m ? bits : m;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const u64 o = __shfl_xor(m, off);
+        m = o > m ? o : m;
+    }
+    const int e = exponent_of(m);
+    if (lane == 0) ed[i] = e;
+    for (int j = 4 * lane; j < bd.kpad; j += 256) {
+        double x[4];
+        int ee[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            x[q] = j + q < bd.n ? row[list[j + q]] : 0.0;
+            ee[q] = e;
+        }
+        unsigned w[S];
+        slice4<S>(x, ee, w);
+#pragma unroll
+        for (int p = 0; p < S; ++p) *reinterpret_cast<unsigned *>(out + p * plane + at(j)) = w[p];
+    }
+}
+
+// One workgroup per grid block: fg of its points, Rq and Lc.
+template <int S>
+__global__ __launch_bounds__(kThreads) void prep_phi_kernel(unsigned char *__restrict__ base,
+                                                            const BlockDesc *__restrict__ blocks,
+                                                            const double *__restrict__ ao_lhs,
+                                                            const double *__restrict__ ao_rhs, long stride_grid,
+                                                            int ngrid, int nao, int blksize,
+                                                            const int *__restrict__ lists, int *__restrict__ fg) {
+    extern __shared__ u64 colmax[];  // [blksize]
+    const int b = blockIdx.x;
+    const BlockDesc bd = blocks[b];
+    const int g0 = b * blksize;
+    const int ncol = min(blksize, ngrid - g0);
+    const int *__restrict__ list = lists + (long)b * nao;
+    for (int c = threadIdx.x; c < blksize; c += kThreads) colmax[c] = 0;
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < bd.n * ncol; idx += kThreads) {
+        const int i = idx / ncol, c = idx % ncol;
+        const u64 bits = (u64)__double_as_longlong(fabs(ao_rhs[(long)list[i] * stride_grid + g0 + c]));
+        atomicMax(&colmax[c], bits);
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < ncol; c += kThreads) fg[g0 + c] = exponent_of(colmax[c]);
+    // Rq[p][i/64][c][i%64], four consecutive i per thread
+    const int quads = bd.kpad / 4;
+    const long plane = (long)blksize * bd.kpad;
+    unsigned char *__restrict__ rq = base + bd.r_off;
+    for (int idx = threadIdx.x; idx < blksize * quads; idx += kThreads) {
+        const int c = idx / quads, i = 4 * (idx % quads);
+        double x[4];
+        int ee[4];
+        const int e = c < ncol ? exponent_of(colmax[c]) : 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            x[q] = (i + q < bd.n && c < ncol) ? ao_rhs[(long)list[i + q] * stride_grid + g0 + c] : 0.0;
+            ee[q] = e;
+        }
+        unsigned w[S];
+        slice4<S>(x, ee, w);
+#pragma unroll
+        for (int p = 0; p < S; ++p)
+            *reinterpret_cast<unsigned *>(rq + p * plane + ((long)(i >> 6) * blksize + c) * kK + (i & 63)) = w[p];
+    }
+    // Lc[i][c]
+    double *__restrict__ lc = reinterpret_cast<double *>(base + bd.l_off);
+    for (int idx = threadIdx.x; idx < bd.kpad * blksize; idx += kThreads) {
+        const int i = idx / blksize, c = idx % blksize;
+        lc[idx] = (i < bd.n && c < ncol) ? ao_lhs[(long)list[i] * stride_grid + g0 + c] : 0.0;
+    }
+}
+
+// byte offset of 16-byte chunk `ch` of a 64-byte LDS row (XOR swizzle against ds_read_b128 conflicts)
+__device__ inline int swz(int row, int ch) { return row * kK + ((ch ^ ((row >> 2) & 3)) << 4); }
+
+// Task t of an evaluation: tasks[t] = {block, row tile, column subtile}.  Workgroup i takes task
+// start_x + i/8 with x = i%8, so consecutive tasks (the tiles of one block, which share its Rq)
+// run on one XCD and its L2.
+__device__ inline int xcd_task(int i, int ntasks) {
+    const int q = ntasks / 8, r = ntasks % 8, x = i % 8;
+    return x * q + min(x, r) + i / 8;
+}
+
+template <int S, int NT>
+__global__ __launch_bounds__(kThreads, 2) void rho_resident_kernel(
+    double *__restrict__ partial, const unsigned char *__restrict__ base, const BlockDesc *__restrict__ blocks,
+    const int4 *__restrict__ tasks, int ntasks, const int *__restrict__ fg, int blksize, int ngrid, int nset) {
+    constexpr int W = NT * 16;
+    constexpr int kBChunks = S * W * (kK / 16);  // 16-byte chunks of one B tile
+    constexpr int kBPer = (kBChunks + kThreads - 1) / kThreads;
+    __shared__ __attribute__((aligned(16))) int8_t Bs[2][S][W * kK];
+    __shared__ double red[kThreads / 64][W];
+
+    const int4 task = tasks[xcd_task(blockIdx.x, ntasks)];
+    const int b = task.x, tile = task.y, sub = task.z;
+    const int s = blockIdx.y;
+    const BlockDesc bd = blocks[b];
+    const int kpad = bd.kpad;
+    const int c0 = sub * W;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int lrow = lane & 15, lk = lane >> 4;
+
+    const long dplane = (long)kpad * kpad;
+    const int nks = kpad / kK;
+    // this wave's 16 rows: 1 KiB A tiles at (row tile, k step); lane l reads bytes 16l..16l+15
+    const int8_t *__restrict__ arow = reinterpret_cast<const int8_t *>(base + bd.d_off) + (long)s * S * dplane +
+                                      (long)(tile * (kRows / 16) + wave) * nks * 1024 + lane * 16;
+    const long rplane = (long)blksize * kpad;
+    const int8_t *__restrict__ rq = reinterpret_cast<const int8_t *>(base + bd.r_off) + (long)c0 * kK;
+
+    i32x4 acc[S][NT];
+#pragma unroll
+    for (int l = 0; l < S; ++l)
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) acc[l][nt] = i32x4{0, 0, 0, 0};
+
+    i32x4 fa0[S], fa1[S], breg[kBPer];
+    auto load_a = [&](int ks, i32x4 *dst) {
+#pragma unroll
+        for (int p = 0; p < S; ++p) dst[p] = *reinterpret_cast<const i32x4 *>(arow + p * dplane + (long)ks * 1024);
+    };
+    // B tile of step ks: S planes x W columns x 64 k, contiguous per plane
+    auto load_b = [&](int ks) {
+#pragma unroll
+        for (int e = 0; e < kBPer; ++e) {
+            const int q = tid + e * kThreads;
+            if (kBChunks % kThreads == 0 || q < kBChunks) {
+                const int p = q / (W * 4), rem = q % (W * 4);
+                breg[e] = *reinterpret_cast<const i32x4 *>(rq + p * rplane + (long)ks * blksize * kK + rem * 16);
+            }
+        }
+    };
+    auto store_b = [&](int buf) {
+#pragma unroll
+        for (int e = 0; e < kBPer; ++e) {
+            const int q = tid + e * kThreads;
+            if (kBChunks % kThreads == 0 || q < kBChunks) {
+                const int p = q / (W * 4), rem = q % (W * 4), c = rem >> 2, ch = rem & 3;
+                *reinterpret_cast<i32x4 *>(&Bs[buf][p][swz(c, ch)]) = breg[e];
+            }
+        }
+    };
+    auto compute = [&](int buf, const i32x4 *fa) {
+#pragma unroll
+        for (int sb = 0; sb < S; ++sb) {
+            i32x4 fb[NT];
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt)
+                fb[nt] = *reinterpret_cast<const i32x4 *>(&Bs[buf][sb][swz(nt * 16 + lrow, lk)]);
+#pragma unroll
+            for (int sa = 0; sa + sb < S; ++sa)
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt)
+                    acc[sa + sb][nt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[sa], fb[nt], acc[sa + sb][nt], 0, 0, 0);
+        }
+    };
+
+    // The loads of step ks+1 are issued before step ks computes and stay in flight
+    // meanwhile.  The prefetch index is clamped instead of guarded (a guard makes the
+    // compiler wait for every load at its join): past the last step it re-reads that
+    // step, and the values are not used.
+    const int last = nks - 1;
+    load_b(0);
+    load_a(0, fa0);
+    for (int ks = 0; ks < nks; ++ks) {
+        const int buf = ks & 1;
+        store_b(buf);
+        __syncthreads();
+        load_b(min(ks + 1, last));
+        load_a(min(ks + 1, last), fa1);
+        __builtin_amdgcn_sched_barrier(0);  // keep the prefetch ahead of the MFMAs (not sunk to its use)
+        compute(buf, fa0);
+#pragma unroll
+        for (int p = 0; p < S; ++p) fa0[p] = fa1[p];
+    }
+
+    // Epilogue.  i32 16x16 layout: lane l, element r holds row 4*(l/16)+r, column l%16.
+    const double nan = __builtin_nan("");
+    const int *__restrict__ ed = reinterpret_cast<const int *>(base + bd.e_off) + (long)s * kpad;
+    const double *__restrict__ lc = reinterpret_cast<const double *>(base + bd.l_off);
+    const int gblk = b * blksize;
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) {
+        const int c = c0 + nt * 16 + lrow;
+        const int g = gblk + c;
+        const bool g_ok = g < ngrid;
+        const int eg = g_ok ? fg[g] : 0;
+        double part = 0.0;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int i = tile * kRows + wave * 16 + 4 * lk + r;
+            if (i < bd.n && g_ok) {
+                const int eu = ed[i];
+                double t = 0.0;
+#pragma unroll
+                for (int l = S - 1; l >= 0; --l) t += ldexp((double)acc[l][nt][r], -7 * (l + 2));
+                t = (eu == kExpNonFinite || eg == kExpNonFinite) ? nan : ldexp(t, eu + eg);
+                part = fma(lc[(long)i * blksize + c], t, part);
+            }
+        }
+        part += __shfl_xor(part, 16);
+        part += __shfl_xor(part, 32);
+        if (lk == 0) red[wave][nt * 16 + lrow] = part;
+    }
+    __syncthreads();
+    for (int c = tid; c < W; c += kThreads) {
+        const int g = gblk + c0 + c;
+        if (g < ngrid) {
+            double v = red[0][c];
+#pragma unroll
+            for (int w = 1; w < kThreads / 64; ++w) v += red[w][c];
+            partial[((long)tile * nset + s) * ngrid + g] = v;
+        }
+    }
+}
+
+int tile_nt(size_t blksize) {
+    const size_t q = blksize / 16;
+    for (int nt = 4; nt >= 1; --nt)
+        if (q % nt == 0) return nt;
+    return 1;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+struct ResidentRho {
+    int device = 0;
+    int nslices = 0;
+    size_t nao = 0, ngrid = 0, nset = 0, blksize = 0, nblk = 0;
+    int nt = 4, nsub = 1;
+    int max_tiles = 0;
+    unsigned char *base = nullptr;  // one allocation: tiles, tables, exponents
+    size_t bytes = 0;
+    BlockDesc *d_blocks = nullptr;
+    int *d_counts = nullptr;
+    int *d_fg = nullptr;
+    int4 *d_tasks = nullptr;
+    std::vector<int> task_prefix;  // tasks of blocks < b, for b = 0..nblk
+};
+
+template <int S>
+static int prep_launch(hipStream_t stream, ResidentRho *h, const double *dms, const double *ao_lhs,
+                       const double *ao_rhs, size_t stride_ao, size_t stride_grid, const int *lists, int max_kpad) {
+    hipEvent_t t0 = trace_start(stream);
+    hipLaunchKernelGGL((prep_dq_kernel<S>), dim3((unsigned)div_up(max_kpad, kThreads / 64), (unsigned)h->nblk,
+                                                 (unsigned)h->nset),
+                       dim3(kThreads), 0, stream, h->base, h->d_blocks, dms, (long)stride_ao, (int)h->nao, lists);
+    hipLaunchKernelGGL((prep_phi_kernel<S>), dim3((unsigned)h->nblk), dim3(kThreads), sizeof(u64) * h->blksize, stream,
+                       h->base, h->d_blocks, ao_lhs, ao_rhs, (long)stride_grid, (int)h->ngrid, (int)h->nao,
+                       (int)h->blksize, lists, h->d_fg);
+    trace_stop(stream, "rho_resident_prep", t0);
+    return hip_status(hipGetLastError(), "rho resident prep launch");
+}
+
+int resident_create(hipStream_t stream, const double *dms, const double *ao_lhs, const double *ao_rhs,
+                    size_t stride_ao, size_t stride_grid, size_t nao, size_t ngrid, size_t nset, const uint8_t *tab,
+                    size_t ldtab, size_t blksize, int nslices, ResidentRho **out) {
+    *out = nullptr;
+    if (nslices < 2 || nslices > 6) return fail(CFGMM_EINVAL, "nslices must be in [2, 6]");
+    if (nao > INT32_MAX || ngrid > INT32_MAX || nset > 65535 || blksize > 4096)
+        return fail(CFGMM_EINVAL, "resident rho: size out of range");
+    auto h = new ResidentRho();
+    CFGMM_HIP_TRY(hipGetDevice(&h->device));
+    h->nslices = nslices;
+    h->nao = nao;
+    h->ngrid = ngrid;
+    h->nset = nset;
+    h->blksize = blksize;
+    h->nblk = ngrid ? div_up(ngrid, blksize) : 0;
+    h->nt = tile_nt(blksize);
+    h->nsub = (int)(blksize / (16 * h->nt));
+    const size_t nblk = h->nblk;
+    if (nao == 0 || ngrid == 0 || nset == 0) {
+        h->task_prefix.assign(nblk + 1, 0);
+        *out = h;
+        return CFGMM_OK;
+    }
+
+    // 1. compacted AO lists of every block (device), counts to the host
+    void *ws = nullptr;
+    const size_t lists_bytes = sizeof(int) * nblk * nao;
+    int st = workspace_get(stream, lists_bytes + sizeof(int) * nblk, &ws);
+    if (st != CFGMM_OK) {
+        delete h;
+        return st;
+    }
+    int *lists = static_cast<int *>(ws);
+    int *counts = lists + nblk * nao;
+    st = launch_build_lists(stream, tab, ldtab, nao, nblk, lists, counts);
+    std::vector<int> hcounts(nblk);
+    if (st == CFGMM_OK)
+        st = hip_status(hipMemcpyAsync(hcounts.data(), counts, sizeof(int) * nblk, hipMemcpyDeviceToHost, stream),
+                        "resident rho: counts D2H");
+    if (st == CFGMM_OK) st = hip_status(hipStreamSynchronize(stream), "resident rho: sync");
+    if (st != CFGMM_OK) {
+        delete h;
+        return st;
+    }
+
+    // 2. layout
+    auto al = [](size_t x) { return div_up(x, 256) * 256; };
+    const int S = nslices;
+    std::vector<BlockDesc> blocks(nblk);
+    std::vector<int4> tasks;
+    h->task_prefix.assign(nblk + 1, 0);
+    size_t off = 0;
+    int max_kpad = 0;
+    for (size_t b = 0; b < nblk; ++b) {
+        const int n = hcounts[b];
+        const int kpad = (int)div_up((size_t)n, kK) * kK;
+        if (kpad > kMaxKpad) {
+            delete h;
+            return fail(CFGMM_EINVAL, "resident rho: a grid block has more than 16384 active AOs");
+        }
+        max_kpad = std::max(max_kpad, kpad);
+        BlockDesc &bd = blocks[b];
+        bd.n = n;
+        bd.kpad = kpad;
+        const size_t k2 = (size_t)kpad * kpad;
+        bd.d_off = (long)off;
+        off += al(nset * S * k2);
+        bd.r_off = (long)off;
+        off += al((size_t)S * blksize * kpad);
+        bd.l_off = (long)off;
+        off += al(sizeof(double) * kpad * blksize);
+        bd.e_off = (long)off;
+        off += al(sizeof(int) * nset * kpad);
+        const int ntiles = kpad / kRows;
+        h->max_tiles = std::max(h->max_tiles, ntiles);
+        for (int t = 0; t < ntiles; ++t)
+            for (int sub = 0; sub < h->nsub; ++sub) tasks.push_back(int4{(int)b, t, sub, 0});
+        h->task_prefix[b + 1] = (int)tasks.size();
+    }
+    const size_t o_blocks = off;
+    off += al(sizeof(BlockDesc) * nblk);
+    const size_t o_counts = off;
+    off += al(sizeof(int) * nblk);
+    const size_t o_fg = off;
+    off += al(sizeof(int) * nblk * blksize);
+    const size_t o_tasks = off;
+    off += al(sizeof(int4) * std::max<size_t>(tasks.size(), 1));
+    h->bytes = off;
+
+    // 3. allocate, upload the tables, slice
+    st = hip_status(hipMalloc(&h->base, h->bytes), "resident rho: hipMalloc");
+    if (st != CFGMM_OK) {
+        h->base = nullptr;
+        delete h;
+        return st;
+    }
+    h->d_blocks = reinterpret_cast<BlockDesc *>(h->base + o_blocks);
+    h->d_counts = reinterpret_cast<int *>(h->base + o_counts);
+    h->d_fg = reinterpret_cast<int *>(h->base + o_fg);
+    h->d_tasks = reinterpret_cast<int4 *>(h->base + o_tasks);
+    st = hip_status(hipMemcpyAsync(h->d_blocks, blocks.data(), sizeof(BlockDesc) * nblk, hipMemcpyHostToDevice, stream),
+                    "resident rho: table H2D");
+    if (st == CFGMM_OK)
+        st = hip_status(hipMemcpyAsync(h->d_counts, hcounts.data(), sizeof(int) * nblk, hipMemcpyHostToDevice, stream),
+                        "resident rho: counts H2D");
+    if (st == CFGMM_OK && !tasks.empty())
+        st = hip_status(hipMemcpyAsync(h->d_tasks, tasks.data(), sizeof(int4) * tasks.size(), hipMemcpyHostToDevice,
+                                       stream),
+                        "resident rho: tasks H2D");
+    if (st == CFGMM_OK) {
+        switch (S) {
+            case 2: st = prep_launch<2>(stream, h, dms, ao_lhs, ao_rhs, stride_ao, stride_grid, lists, max_kpad); break;
+            case 3: st = prep_launch<3>(stream, h, dms, ao_lhs, ao_rhs, stride_ao, stride_grid, lists, max_kpad); break;
+            case 4: st = prep_launch<4>(stream, h, dms, ao_lhs, ao_rhs, stride_ao, stride_grid, lists, max_kpad); break;
+            case 5: st = prep_launch<5>(stream, h, dms, ao_lhs, ao_rhs, stride_ao, stride_grid, lists, max_kpad); break;
+            default: st = prep_launch<6>(stream, h, dms, ao_lhs, ao_rhs, stride_ao, stride_grid, lists, max_kpad); break;
+        }
+    }
+    // the host tables above must outlive their copies
+    if (st == CFGMM_OK) st = hip_status(hipStreamSynchronize(stream), "resident rho: sync");
+    if (st != CFGMM_OK) {
+        (void)hipStreamSynchronize(stream);
+        (void)hipFree(h->base);
+        delete h;
+        return st;
+    }
+    *out = h;
+    return CFGMM_OK;
+}
+
+int resident_destroy(ResidentRho *h) {
+    if (!h) return CFGMM_OK;
+    int st = CFGMM_OK;
+    if (h->base) st = hip_status(hipFree(h->base), "resident rho: hipFree");
+    delete h;
+    return st;
+}
+
+size_t resident_bytes(const ResidentRho *h) { return h ? h->bytes : 0; }
+size_t resident_ngrid(const ResidentRho *h) { return h->ngrid; }
+size_t resident_nset(const ResidentRho *h) { return h->nset; }
+size_t resident_blksize(const ResidentRho *h) { return h->blksize; }
+int resident_device(const ResidentRho *h) { return h->device; }
+
+template <int S>
+static int eval_launch(hipStream_t stream, const ResidentRho *h, double *partial, dim3 grid, int ntasks, int ngrid,
+                       int nset) {
+    hipEvent_t t0 = trace_start(stream);
+#define CFGMM_RES_LAUNCH(NT_)                                                                                      \
+    hipLaunchKernelGGL((rho_resident_kernel<S, NT_>), grid, dim3(kThreads), 0, stream, partial, h->base, h->d_blocks, \
+                       h->d_tasks, ntasks, h->d_fg, (int)h->blksize, ngrid, nset)
+    switch (h->nt) {
+        case 1: CFGMM_RES_LAUNCH(1); break;
+        case 2: CFGMM_RES_LAUNCH(2); break;
+        case 3: CFGMM_RES_LAUNCH(3); break;
+        default: CFGMM_RES_LAUNCH(4); break;
+    }
+#undef CFGMM_RES_LAUNCH
+    trace_stop(stream, "rho_resident_kernel", t0);
+    return hip_status(hipGetLastError(), "rho_resident_kernel launch");
+}
+
+// rho[s*stride_rho + g] += ... for g < ngrid_used, s < nset_used (leading grid points and density matrices).
+int resident_eval(hipStream_t stream, const ResidentRho *h, double *rhos, size_t stride_rho, size_t ngrid_used,
+                  size_t nset_used) {
+    if (ngrid_used == 0 || nset_used == 0 || h->nao == 0) return CFGMM_OK;
+    const size_t nblk_used = div_up(ngrid_used, h->blksize);
+    const int ntasks = h->task_prefix[nblk_used];
+    if (ntasks == 0) return CFGMM_OK;
+    if (nset_used > 65535) return fail(CFGMM_EINVAL, "resident rho: too many density matrices");
+    void *ws = nullptr;
+    int st = workspace_get(stream, sizeof(double) * h->max_tiles * nset_used * ngrid_used, &ws);
+    if (st != CFGMM_OK) return st;
+    double *partial = static_cast<double *>(ws);
+    const dim3 grid((unsigned)ntasks, (unsigned)nset_used, 1);
+    switch (h->nslices) {
+        case 2: st = eval_launch<2>(stream, h, partial, grid, ntasks, (int)ngrid_used, (int)nset_used); break;
+        case 3: st = eval_launch<3>(stream, h, partial, grid, ntasks, (int)ngrid_used, (int)nset_used); break;
+        case 4: st = eval_launch<4>(stream, h, partial, grid, ntasks, (int)ngrid_used, (int)nset_used); break;
+        case 5: st = eval_launch<5>(stream, h, partial, grid, ntasks, (int)ngrid_used, (int)nset_used); break;
+        default: st = eval_launch<6>(stream, h, partial, grid, ntasks, (int)ngrid_used, (int)nset_used); break;
+    }
+    if (st != CFGMM_OK) return st;
+    return launch_rho_reduce(stream, rhos, partial, h->d_counts, stride_rho, ngrid_used, nset_used, h->blksize);
+}
+
+}  // namespace cfgmm

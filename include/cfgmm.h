@@ -178,6 +178,37 @@ typedef struct cfgmm_rho_problem {
 int cfgmm_aodmao2rho_batched_dev(void *stream, const cfgmm_rho_problem *problems, size_t nproblems, int precision,
                                  int nslices);
 
+/* ======================================================================== */
+/* 2d. resident rho operands (no counterpart in the reference)              */
+/* ======================================================================== */
+
+/* Operands of one masked rho problem kept on the device in the form its
+ * evaluations read: for every grid block b, the density-matrix sub-blocks
+ * D_s[L_b, L_b] and the block's AO values compacted to contiguous tiles and
+ * cut into int8 slices (split precision, `nslices` in [2, 6]), with
+ * power-of-two scales per block row and grid point.  Creation gathers and
+ * slices once (arguments as cfgmm_aodmao2rho_ex_dev, device pointers; the
+ * inputs may be freed afterwards); each evaluation then streams the tiles
+ * without gathers.  For callers that evaluate the same (D, phi, mask) many
+ * times -- a serving loop that keeps a molecule resident.  Memory is
+ * sum_b (nset*nslices*kpad_b^2 + blksize*kpad_b*(nslices + 8)) bytes plus small
+ * tables, kpad_b = |L_b| rounded up to 64 (cfgmm_rho_resident_bytes).
+ * precision must be CFGMM_PREC_OZAKI_INT8. */
+typedef struct cfgmm_rho_resident cfgmm_rho_resident;
+int cfgmm_rho_resident_create(void *stream, const double *dms, const double *ao_lhs, const double *ao_rhs,
+                              size_t stride_ao, size_t stride_grid, size_t nao, size_t ngrid, size_t nset,
+                              const uint8_t *tab, size_t ldtab, size_t blksize, int precision, int nslices,
+                              cfgmm_rho_resident **out);
+/* rhos[s*stride_rho + g] += rho_s(g) for the leading ngrid_used grid points
+ * (ngrid_used == ngrid, or a multiple of blksize below it) and the leading
+ * nset_used density matrices.  Evaluations of one handle on different
+ * streams may run concurrently (each uses its stream's workspace). */
+int cfgmm_rho_resident_eval(void *stream, const cfgmm_rho_resident *h, double *rhos, size_t stride_rho,
+                            size_t ngrid_used, size_t nset_used);
+int cfgmm_rho_resident_bytes(const cfgmm_rho_resident *h, size_t *bytes);
+/* Frees the device memory; no evaluation of the handle may still be running. */
+int cfgmm_rho_resident_destroy(cfgmm_rho_resident *h);
+
 /* ---- tunables (set by the autotuner; process-wide) ------------------------ */
 enum cfgmm_tunable {
     CFGMM_TUNE_BATCH_WGS_PER_CU = 0, /* batched rho: persistent workgroups per CU, 1..8 (default 2) */

@@ -1,0 +1,172 @@
+"""Resident rho operands (cfgmm_rho_resident_*): split precision over per-block compacted tiles.
+
+Held to the CPU oracle normwise (relative Frobenius over rho) at 1e-10 with 6
+int8 slices, as the per-call split-precision path; evaluations of leading grid
+blocks / density matrices must equal the per-call FP64 result on that range
+within the same tolerance and leave everything else untouched; repeated and
+concurrent evaluations are bitwise identical (fixed reduction order)."""
+import numpy as np
+import pytest
+
+import cfgmm
+import oracle
+from cfgmm import workloads as W
+from helpers import rel_fro
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle(p, rhos0=None, lhs=None, rhs=None, dms=None):
+    lhs = p.phi if lhs is None else lhs
+    rhs = p.phi if rhs is None else rhs
+    dms = p.dms if dms is None else dms
+    out = np.zeros(p.nset * p.ngrid) if rhos0 is None else rhos0.copy()
+    assert oracle.aodmao2rho(out, np.ascontiguousarray(dms).ravel(), np.ascontiguousarray(lhs).ravel(),
+                             np.ascontiguousarray(rhs).ravel(), p.nao, p.ngrid, p.nao, p.ngrid, p.nset,
+                             np.ascontiguousarray(p.tab), p.nblk, p.blksize) == 0
+    return out.reshape(p.nset, p.ngrid)
+
+
+class Resident:
+    def __init__(self, p, ns=6, lhs=None, rhs=None, dms=None):
+        self.s = cfgmm.Stream()
+        self.bufs = [cfgmm.DeviceBuffer.from_array(np.ascontiguousarray(x)) for x in
+                     (p.dms if dms is None else dms, p.phi if lhs is None else lhs, p.phi if rhs is None else rhs,
+                      p.tab)]
+        d, l, r, t = self.bufs
+        self.h = cfgmm.RhoResident(self.s, d, l, r, p.nao, p.ngrid, p.nao, p.ngrid, p.nset, t, p.nblk, p.blksize,
+                                   cfgmm.PREC_OZAKI_INT8, ns)
+        self.p = p
+
+    def eval(self, rhos0=None, stride=None, ngrid=None, nset=None, stream=None):
+        p = self.p
+        stride = p.ngrid if stride is None else stride
+        r0 = np.zeros(p.nset * stride) if rhos0 is None else rhos0
+        d_r = cfgmm.DeviceBuffer.from_array(r0)
+        s = self.s if stream is None else stream
+        self.h.eval(s, d_r, stride, ngrid, nset)
+        s.synchronize()
+        out = d_r.to_array(np.float64, r0.shape)
+        d_r.free()
+        return out
+
+    def close(self):
+        self.h.destroy()
+        for b in self.bufs:
+            b.free()
+        self.s.destroy()
+
+
+def test_resident_full_workload():
+    p = W.rho_problem(seed=0)
+    ref = _oracle(p)
+    r = Resident(p)
+    got = r.eval().reshape(p.nset, p.ngrid)
+    again = r.eval().reshape(p.nset, p.ngrid)
+    nbytes = r.h.bytes
+    r.close()
+    err = rel_fro(got, ref)
+    print(f"resident ns=6 rel_fro {err:.3e}, {nbytes / 2**30:.2f} GiB")
+    assert err <= 1e-10
+    assert np.array_equal(got, again)
+
+
+@pytest.mark.parametrize("ns,tol", [(4, 1e-6), (5, 1e-8)])
+def test_resident_fewer_slices(ns, tol):
+    p = W.rho_problem(seed=3, nshell=200, nao=640, ngrid=2048, nset=2, density=0.3)
+    r = Resident(p, ns=ns)
+    got = r.eval().reshape(p.nset, p.ngrid)
+    r.close()
+    assert rel_fro(got, _oracle(p)) <= tol
+
+
+def test_resident_wide_range_dm_and_accumulate():
+    """D magnitudes 10^U(-8, 2); rho accumulates into a nonzero start value."""
+    p = W.rho_problem(seed=7, nshell=200, nao=640, ngrid=2048, nset=2, density=0.3, dm_logspan=(-8, 2))
+    rng = np.random.default_rng(1)
+    r0 = rng.standard_normal(p.nset * p.ngrid)
+    ref = _oracle(p, rhos0=r0)
+    r = Resident(p)
+    got = r.eval(rhos0=r0).reshape(p.nset, p.ngrid)
+    r.close()
+    assert rel_fro(got - r0.reshape(p.nset, p.ngrid), ref - r0.reshape(p.nset, p.ngrid)) <= 1e-10
+
+
+def test_resident_leading_range_and_stride():
+    """Leading 12 of 20 blocks and 2 of 3 sets, rho rows strided by 1400 > ngrid: the rest is untouched."""
+    p = W.rho_problem(seed=5, nshell=160, nao=520, ngrid=1280, nset=3, density=0.4)
+    stride = 1400
+    r0 = np.full(p.nset * stride, 7.0)
+    r = Resident(p)
+    got = r.eval(rhos0=r0.copy(), stride=stride, ngrid=12 * 64, nset=2).reshape(p.nset, stride)
+    r.close()
+    ref = _oracle(p).reshape(p.nset, p.ngrid)
+    assert rel_fro(got[:2, :768] - 7.0, ref[:2, :768]) <= 1e-10
+    assert np.all(got[:2, 768:] == 7.0) and np.all(got[2] == 7.0)
+
+
+def test_resident_blksize48_partial_block_lhs_ne_rhs():
+    """blksize 48 (three 16-column tiles), ngrid not a multiple of it, lhs != rhs, mask authority."""
+    p = W.rho_problem(seed=9, nshell=120, nao=400, ngrid=1000, nset=2, blksize=48, density=0.35)
+    rng = np.random.default_rng(4)
+    lhs = p.phi * rng.uniform(0.5, 1.5, size=p.phi.shape)
+    on = np.repeat(p.tab.astype(bool), p.blksize, axis=1)[:, : p.ngrid]
+    lhs[~on] = 1e30  # under off bits: must be ignored
+    rhs = p.phi.copy()
+    rhs[~on] = -1e30
+    ref = _oracle(p, lhs=np.where(on, lhs, 0.0), rhs=np.where(on, rhs, 0.0))
+    r = Resident(p, lhs=lhs, rhs=rhs)
+    got = r.eval().reshape(p.nset, p.ngrid)
+    r.close()
+    assert rel_fro(got, ref) <= 1e-10
+
+
+def test_resident_concurrent_streams_bitwise():
+    p = W.rho_problem(seed=6, nshell=200, nao=640, ngrid=4096, nset=2, density=0.3)
+    r = Resident(p)
+    s2 = cfgmm.Stream()
+    a = r.eval()
+    b = r.eval(stream=s2)
+    r.close()
+    s2.destroy()
+    assert np.array_equal(a, b)
+
+
+def test_resident_golden_fixture():
+    import os
+    f = np.load(os.path.join(os.path.dirname(__file__), "golden", "rho_sincos.npz"), allow_pickle=False)
+    sa, sg, nao, ngrid, nset, ldtab, blksize = (int(x) for x in f["dims"])
+    s = cfgmm.Stream()
+    bufs = {k: cfgmm.DeviceBuffer.from_array(np.ascontiguousarray(f[k])) for k in ("rhos0", "dms", "lhs", "rhs", "tab")}
+    h = cfgmm.RhoResident(s, bufs["dms"], bufs["lhs"], bufs["rhs"], sa, sg, nao, ngrid, nset, bufs["tab"], ldtab,
+                          blksize)
+    h.eval(s, bufs["rhos0"], sg)
+    s.synchronize()
+    got = bufs["rhos0"].to_array(np.float64, f["rhos0"].shape)
+    h.destroy()
+    for b in bufs.values():
+        b.free()
+    s.destroy()
+    assert rel_fro(got, f["rhos"]) <= 1e-10
+
+
+def test_resident_errors():
+    p = W.rho_problem(seed=1, nshell=40, nao=128, ngrid=256, nset=1, density=0.5)
+    s = cfgmm.Stream()
+    d, l, t = (cfgmm.DeviceBuffer.from_array(np.ascontiguousarray(x)) for x in (p.dms, p.phi, p.tab))
+    with pytest.raises(cfgmm.CfgmmError) as e:
+        cfgmm.RhoResident(s, d, l, l, p.nao, p.ngrid, p.nao, p.ngrid, p.nset, t, p.nblk, p.blksize,
+                          cfgmm.PREC_FP64, 0)
+    assert e.value.status == cfgmm.EINVAL
+    h = cfgmm.RhoResident(s, d, l, l, p.nao, p.ngrid, p.nao, p.ngrid, p.nset, t, p.nblk, p.blksize)
+    out = cfgmm.DeviceBuffer(8 * p.ngrid)
+    with pytest.raises(cfgmm.CfgmmError) as e:
+        h.eval(s, out, p.ngrid, 100, 1)  # not a multiple of blksize
+    assert e.value.status == cfgmm.EINVAL_DIMS
+    with pytest.raises(cfgmm.CfgmmError) as e:
+        h.eval(s, out, p.ngrid, p.ngrid, 2)  # more sets than resident
+    assert e.value.status == cfgmm.EINVAL_DIMS
+    h.destroy()
+    for b in (d, l, t, out):
+        b.free()
+    s.destroy()
