@@ -69,6 +69,12 @@ _SIGNATURES = {
     "cfgmm_rho_resident_eval": ([_VP, _VP, _VP, _SZ, _SZ, _SZ], _I),
     "cfgmm_rho_resident_bytes": ([_VP, ctypes.POINTER(_SZ)], _I),
     "cfgmm_rho_resident_destroy": ([_VP], _I),
+    "cfgmm_server_create": ([_VP, ctypes.POINTER(_VP)], _I),
+    "cfgmm_server_add_molecule": ([_VP, _I, _VP, _VP, _VP, _SZ, _SZ, _SZ, _SZ], _I),
+    "cfgmm_server_warm": ([_VP, _VP, _SZ], _I),
+    "cfgmm_server_run": ([_VP, _VP, _SZ, _VP], _I),
+    "cfgmm_server_get_stats": ([_VP, _VP], _I),
+    "cfgmm_server_destroy": ([_VP], _I),
     "cfgmm_tune_set": ([_I, _I], _I),
     "cfgmm_tune_get": ([_I, ctypes.POINTER(_I)], _I),
     "cfgmm_device_count": ([ctypes.POINTER(_I)], _I),
@@ -392,6 +398,76 @@ class RhoResident:
     def destroy(self) -> None:
         if self.handle:
             check(lib().cfgmm_rho_resident_destroy(self.handle))
+            self.handle = None
+
+
+REQ_RHO = 0
+REQ_GEMM = 1
+
+
+class ServerConfig(ctypes.Structure):
+    """cfgmm_server_config."""
+    _fields_ = [("hbm_budget_bytes", _SZ), ("slots", _I), ("rho_precision", _I), ("rho_nslices", _I),
+                ("max_batch", _I)]
+
+
+class Request(ctypes.Structure):
+    """cfgmm_request."""
+    _fields_ = [("kind", _I), ("molecule", _I), ("ngrid", _SZ), ("nset", _SZ), ("m", _SZ), ("n", _SZ), ("k", _SZ),
+                ("seed", ctypes.c_ulonglong), ("arrival_s", ctypes.c_double), ("out", _VP)]
+
+
+class RequestResult(ctypes.Structure):
+    """cfgmm_request_result."""
+    _fields_ = [("admit_s", ctypes.c_double), ("done_s", ctypes.c_double), ("checksum", ctypes.c_ulonglong),
+                ("status", _I), ("batch", _I), ("batch_size", _I)]
+
+
+class ServerStats(ctypes.Structure):
+    """cfgmm_server_stats."""
+    _fields_ = [(k, ctypes.c_ulonglong) for k in ("rho_requests", "gemm_requests", "batches", "hits", "misses",
+                                                   "uploads", "evictions")] + [("peak_bytes", _SZ),
+                                                                               ("held_bytes", _SZ)]
+
+
+class Server:
+    """The serving loop (cfgmm_server_*): molecules are registered as host numpy arrays, which
+    this object keeps alive; ``run`` replays a request list in real time."""
+
+    def __init__(self, hbm_budget_bytes, slots=4, rho_precision=PREC_FP64, rho_nslices=6, max_batch=16):
+        cfg = ServerConfig(hbm_budget_bytes, slots, rho_precision, rho_nslices, max_batch)
+        h = _VP()
+        check(lib().cfgmm_server_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self.handle = h.value
+        self._arrays = []
+
+    def add_molecule(self, mid, dms, phi, tab, blksize):
+        dms, phi, tab = _f64(dms), _f64(phi), _u8(tab)
+        nset, nao = dms.shape[0], dms.shape[1]
+        ngrid = phi.shape[1]
+        self._arrays += [dms, phi, tab]
+        check(lib().cfgmm_server_add_molecule(self.handle, mid, _ptr(dms), _ptr(phi), _ptr(tab), nao, ngrid, nset,
+                                              blksize))
+
+    def warm(self, ids):
+        arr = (ctypes.c_int * len(ids))(*ids)
+        check(lib().cfgmm_server_warm(self.handle, ctypes.cast(arr, _VP), len(ids)))
+
+    def run(self, requests):
+        """requests: sequence of Request; returns a ctypes array of RequestResult."""
+        reqs = (Request * len(requests))(*requests)
+        res = (RequestResult * len(requests))()
+        check(lib().cfgmm_server_run(self.handle, ctypes.cast(reqs, _VP), len(reqs), ctypes.cast(res, _VP)))
+        return res
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        check(lib().cfgmm_server_get_stats(self.handle, ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}
+
+    def destroy(self):
+        if self.handle:
+            check(lib().cfgmm_server_destroy(self.handle))
             self.handle = None
 
 

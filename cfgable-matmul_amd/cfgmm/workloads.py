@@ -294,3 +294,108 @@ def batched_problems(pool, n=512, seed=7, max_blocks=64, blksize=64, nset_range=
         nset = int(rng.integers(nset_range[0], nset_range[1] + 1))
         out.append(PoolProblem(m, ngrid, nset))
     return out
+
+
+# ---------------------------------------------------------------------------
+# serving mix
+# ---------------------------------------------------------------------------
+def splitmix64_uniform(seed: int, n: int) -> np.ndarray:
+    """The serving loop's device generator: x[i] = (splitmix64((seed << 32) + i) >> 11) * 2^-52 - 1."""
+    with np.errstate(over="ignore"):
+        z = (np.uint64(seed) << np.uint64(32)) + np.arange(n, dtype=np.uint64)
+        z = z + np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return (z >> np.uint64(11)).astype(np.float64) * 2.0 ** -52 - 1.0
+
+
+def serving_gemm_operands(m: int, n: int, k: int, seed: int):
+    """(A, B) of a served GEMM request, as the device generates them (row-major, NN)."""
+    return (splitmix64_uniform(2 * seed, m * k).reshape(m, k), splitmix64_uniform(2 * seed + 1, k * n).reshape(k, n))
+
+
+def result_checksum(x: np.ndarray) -> int:
+    """The serving loop's result hash: sum_i splitmix64(bits(x[i]) ^ splitmix64(i)) mod 2^64."""
+    def mix(z):
+        with np.errstate(over="ignore"):
+            z = z + np.uint64(0x9E3779B97F4A7C15)
+            z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+            z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+            return z ^ (z >> np.uint64(31))
+    bits = np.ascontiguousarray(x, dtype=np.float64).ravel().view(np.uint64)
+    with np.errstate(over="ignore"):
+        return int(np.sum(mix(bits ^ mix(np.arange(bits.size, dtype=np.uint64))), dtype=np.uint64))
+
+
+def tile_bytes_estimate(p: "RhoProblem", nslices: int = 6) -> float:
+    """Bytes of the split-precision resident tiles of a problem (kpad_b = |L_b| rounded up to 64)."""
+    kp = np.ceil(p.counts() / 64.0) * 64.0
+    return float(np.sum(p.nset * nslices * kp * kp + p.blksize * kp * (nslices + 8)))
+
+
+def serving_pool(n=8, seed=500, nao_range=(500, 4000), blocks_range=(64, 1024), density_range=(0.1, 0.6), nset=4,
+                 blksize=64, max_tile_bytes=40e9):
+    """Molecules for the serving mix: nao ~ U{nao_range} (shells ~ nao/3), ngrid = blksize * U{blocks_range},
+    mask density ~ U(density_range), nset PSD DMs.  A molecule whose 6-slice resident tiles would
+    exceed max_tile_bytes is redrawn (the serving loop's HBM budget must hold it)."""
+    rng = np.random.default_rng(seed)
+    pool = []
+    i = 0
+    while len(pool) < n:
+        nao = int(rng.integers(nao_range[0], nao_range[1] + 1))
+        nshell = max(1, nao // 3)
+        if (nao - nshell) % 2:
+            nao += 1
+        nblk = int(rng.integers(blocks_range[0], blocks_range[1] + 1))
+        dens = float(rng.uniform(*density_range))
+        # cheap pre-check before generating: blocks of about dens * nao AOs
+        kp = np.ceil(dens * nao / 64.0) * 64.0
+        i += 1
+        if nblk * (nset * 6 * kp * kp + blksize * kp * 14) > max_tile_bytes:
+            continue
+        p = rho_problem(seed=seed * 1000 + i, nshell=nshell, nao=nao, ngrid=nblk * blksize, nset=nset,
+                        blksize=blksize, density=dens, nocc=min(200, nao))
+        if tile_bytes_estimate(p) > max_tile_bytes:
+            continue
+        pool.append(Molecule(seed * 1000 + i, p))
+    return pool
+
+
+@dataclass
+class ServedRequest:
+    kind: int            # 0 rho, 1 GEMM
+    arrival_s: float
+    mol: int = -1        # rho: pool index
+    ngrid: int = 0
+    nset: int = 0
+    m: int = 0
+    n: int = 0
+    k: int = 0
+    seed: int = 0
+
+    def flops(self, pool) -> float:
+        if self.kind == 1:
+            return 2.0 * self.m * self.n * self.k
+        return PoolProblem(self.mol, self.ngrid, self.nset).flops(pool)
+
+
+def serving_requests(pool, n, rate, seed=11, rho_fraction=0.7, gemm_range=(256, 4096)):
+    """n requests with Poisson arrivals at `rate` per second: rho (a pool molecule, leading
+    64*U{1..nblk} points, U{1..nset} DMs) with probability rho_fraction, else a DGEMM with
+    m, n, k = round(exp U(ln lo, ln hi))."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, size=n))
+    out = []
+    lo, hi = np.log(gemm_range[0]), np.log(gemm_range[1])
+    for i in range(n):
+        if rng.uniform() < rho_fraction:
+            mi = int(rng.integers(0, len(pool)))
+            p = pool[mi].prob
+            nb = int(rng.integers(1, p.nblk + 1))
+            out.append(ServedRequest(0, float(t[i]), mol=mi, ngrid=nb * p.blksize,
+                                     nset=int(rng.integers(1, p.nset + 1))))
+        else:
+            m, nn, k = (int(round(np.exp(rng.uniform(lo, hi)))) for _ in range(3))
+            out.append(ServedRequest(1, float(t[i]), m=m, n=nn, k=k, seed=int(rng.integers(1, 2 ** 30))))
+    return out

@@ -585,6 +585,51 @@ int cfgmm_rho_resident_destroy(cfgmm_rho_resident *h) {
     return resident_destroy(impl(h));
 }
 
+// ---- serving loop -------------------------------------------------------------------
+static Server *srv(const cfgmm_server *s) { return reinterpret_cast<Server *>(const_cast<cfgmm_server *>(s)); }
+
+int cfgmm_server_create(const cfgmm_server_config *cfg, cfgmm_server **out) {
+    clear_error();
+    if (!cfg || !out) return fail(CFGMM_EINVAL, "null argument");
+    *out = nullptr;
+    CFGMM_TRY(check_device());
+    Server *s = nullptr;
+    CFGMM_TRY(server_create(cfg, &s));
+    *out = reinterpret_cast<cfgmm_server *>(s);
+    return CFGMM_OK;
+}
+
+int cfgmm_server_add_molecule(cfgmm_server *s, int id, const double *dms, const double *phi, const uint8_t *tab,
+                              size_t nao, size_t ngrid, size_t nset, size_t blksize) {
+    clear_error();
+    if (!s) return fail(CFGMM_EINVAL, "server is null");
+    return server_add_molecule(srv(s), id, dms, phi, tab, nao, ngrid, nset, blksize);
+}
+
+int cfgmm_server_warm(cfgmm_server *s, const int *ids, size_t n) {
+    clear_error();
+    if (!s || (n && !ids)) return fail(CFGMM_EINVAL, "null argument");
+    return server_warm(srv(s), ids, n);
+}
+
+int cfgmm_server_run(cfgmm_server *s, const cfgmm_request *reqs, size_t n, cfgmm_request_result *results) {
+    clear_error();
+    if (!s || (n && (!reqs || !results))) return fail(CFGMM_EINVAL, "null argument");
+    return server_run(srv(s), reqs, n, results);
+}
+
+int cfgmm_server_get_stats(const cfgmm_server *s, cfgmm_server_stats *out) {
+    clear_error();
+    if (!s || !out) return fail(CFGMM_EINVAL, "null argument");
+    return server_stats(srv(s), out);
+}
+
+int cfgmm_server_destroy(cfgmm_server *s) {
+    clear_error();
+    if (!s) return CFGMM_OK;
+    return server_destroy(srv(s));
+}
+
 int cfgmm_tune_set(int param, int value) {
     clear_error();
     if (param < 0 || param >= CFGMM_TUNE_COUNT) return fail(CFGMM_EINVAL, "unknown tunable " + std::to_string(param));

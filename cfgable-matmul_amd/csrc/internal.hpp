@@ -89,6 +89,20 @@ size_t resident_nset(const ResidentRho *h);
 size_t resident_blksize(const ResidentRho *h);
 int resident_device(const ResidentRho *h);
 
+// Serving helpers (serve_kernels.hip) and the request loop (server.cpp).
+int launch_fill_uniform(hipStream_t stream, double *x, size_t n, unsigned long long seed);
+int launch_fill_zero(hipStream_t stream, void *x, size_t bytes);
+int launch_checksum(hipStream_t stream, const double *x, size_t rows, size_t cols, size_t ld,
+                    unsigned long long *out);
+struct Server;
+int server_create(const cfgmm_server_config *cfg, Server **out);
+int server_destroy(Server *sv);
+int server_add_molecule(Server *sv, int id, const double *dms, const double *phi, const uint8_t *tab, size_t nao,
+                        size_t ngrid, size_t nset, size_t blksize);
+int server_warm(Server *sv, const int *ids, size_t n);
+int server_run(Server *sv, const cfgmm_request *reqs, size_t n, cfgmm_request_result *res);
+int server_stats(const Server *sv, cfgmm_server_stats *out);
+
 // ---- MFMA fragment types -------------------------------------------------------
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 

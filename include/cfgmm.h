@@ -209,6 +209,65 @@ int cfgmm_rho_resident_bytes(const cfgmm_rho_resident *h, size_t *bytes);
 /* Frees the device memory; no evaluation of the handle may still be running. */
 int cfgmm_rho_resident_destroy(cfgmm_rho_resident *h);
 
+/* ======================================================================== */
+/* 2e. serving loop (no counterpart in the reference)                       */
+/* ======================================================================== */
+
+/* A request loop on the current device: rho over catalogued molecules and
+ * DGEMMs with device-generated operands, admitted as slots (streams) and HBM
+ * budget free up, FP64 rho requests queued together fused into one batched
+ * launch, molecules kept resident across requests and evicted least recently
+ * used when memory is short.  Not thread-safe: one thread drives a server. */
+typedef struct cfgmm_server cfgmm_server;
+
+typedef struct cfgmm_server_config {
+    size_t hbm_budget_bytes; /* device memory for resident molecules + in-flight requests */
+    int slots;               /* concurrent streams, 1..16 */
+    int rho_precision;       /* CFGMM_PREC_FP64 (per-call kernels over resident inputs) or
+                                CFGMM_PREC_OZAKI_INT8 (resident per-block tiles, see 2d) */
+    int rho_nslices;         /* split precision: slices, 2..6 */
+    int max_batch;           /* rho requests dispatched together on one slot, >= 1 */
+} cfgmm_server_config;
+
+enum cfgmm_request_kind { CFGMM_REQ_RHO = 0, CFGMM_REQ_GEMM = 1 };
+
+typedef struct cfgmm_request {
+    int kind;
+    int molecule;            /* rho: catalogue id */
+    size_t ngrid, nset;      /* rho: leading grid points (all, or a multiple of blksize) and density matrices */
+    size_t m, n, k;          /* GEMM: C(m x n) = A(m x k) B(k x n), row-major, C starting at zero */
+    unsigned long long seed; /* GEMM: A[i] = U(seed' = 2*seed), B[i] = U(2*seed + 1), where
+                                U(t)[i] = (splitmix64((t << 32) + i) >> 11) * 2^-52 - 1 */
+    double arrival_s;        /* arrival, seconds after the run starts */
+    double *out;             /* optional host buffer for the result (rho: nset x ngrid, GEMM: m x n), or NULL */
+} cfgmm_request;
+
+typedef struct cfgmm_request_result {
+    double admit_s;               /* dispatched to a slot */
+    double done_s;                /* completion observed (latency = done_s - arrival_s) */
+    unsigned long long checksum;  /* sum over i of splitmix64(bits(x[i]) ^ splitmix64(i)) mod 2^64 */
+    int status;                   /* CFGMM_OK */
+    int batch;                    /* dispatch the request rode in */
+    int batch_size;
+} cfgmm_request_result;
+
+typedef struct cfgmm_server_stats {
+    unsigned long long rho_requests, gemm_requests, batches, hits, misses, uploads, evictions;
+    size_t peak_bytes, held_bytes;
+} cfgmm_server_stats;
+
+int cfgmm_server_create(const cfgmm_server_config *cfg, cfgmm_server **out);
+/* Registers a molecule: host arrays that must stay valid while the server lives
+ * (dms nset x nao x nao, phi nao x ngrid, tab nao x ceil(ngrid/blksize), row-major). */
+int cfgmm_server_add_molecule(cfgmm_server *s, int id, const double *dms, const double *phi, const uint8_t *tab,
+                              size_t nao, size_t ngrid, size_t nset, size_t blksize);
+/* Makes the listed molecules resident, in order, while they fit without evicting. */
+int cfgmm_server_warm(cfgmm_server *s, const int *ids, size_t n);
+/* Replays the requests in real time (open loop) and returns when all are done. */
+int cfgmm_server_run(cfgmm_server *s, const cfgmm_request *reqs, size_t n, cfgmm_request_result *results);
+int cfgmm_server_get_stats(const cfgmm_server *s, cfgmm_server_stats *out);
+int cfgmm_server_destroy(cfgmm_server *s);
+
 /* ---- tunables (set by the autotuner; process-wide) ------------------------ */
 enum cfgmm_tunable {
     CFGMM_TUNE_BATCH_WGS_PER_CU = 0, /* batched rho: persistent workgroups per CU, 1..8 (default 2) */
