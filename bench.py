@@ -18,12 +18,16 @@ The same JSON line carries:
     1031x977x1543 NT and 3527x9583x6581 NN shapes (dgemm_kernel hipEvent time).
 
 Other workloads (one JSON line each):
+  --workload serving   Poisson request mix (70 % rho over 8 resident molecules, 30 % DGEMM) swept over
+                       50..2000 req/s with the rho path in split precision and in FP64: p50/p99 latency
+                       and throughput; results alone vs in the mix; a saturated rho-only trace served
+                       in split precision vs the same requests as plain FP64 per-call launches.
   --workload batched   512 independent rho problems (nao 300..2228, ngrid 64..4096,
                        nset 1..4, density 10..60 %, from a pool of 32 molecules):
                        512 sequential plain FP64 launches vs one batched persistent
                        launch (default and autotuned settings) and the split-precision form.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-dgemm] [--no-cpu] [--workload rho|batched]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-dgemm] [--no-cpu] [--workload rho|batched|serving]
 """
 from __future__ import annotations
 
@@ -273,6 +277,132 @@ def bench_batched(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_serving(args):
+    """Serving mix: Poisson arrivals, 70 % rho over 8 resident molecules / 30 % DGEMM, lambda sweep."""
+    import oracle  # samples only
+    log("[bench] generating the serving pool (8 molecules)")
+    pool = W.serving_pool(n=8)
+    free, total = cfgmm.mem_get_info()
+    budget = int(min(free * 0.85, 220 * 2 ** 30))
+    lams = [50, 100, 200, 500, 1000, 2000]
+    out = {"metric": METRIC, "unit": "rho evals/s", "n_gpus": 1, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64 (rho split precision: 6 int8 slices)", "data": "synthetic",
+           "config": {"workload": "serving: Poisson arrivals, 70 % masked rho (8 molecules, nao 500..4000, ngrid "
+                                  "64..65536, nset 1..4, density 10..60 %), 30 % DGEMM m,n,k log-uniform 256..4096",
+                      "hbm_budget_GiB": round(budget / 2 ** 30, 1), "slots": 4, "max_batch": 16,
+                      "molecules": [{"nao": m.prob.nao, "ngrid": m.prob.ngrid,
+                                     "density": round(float(m.prob.shell_tab.mean()), 3)} for m in pool]}}
+
+    def server(prec):
+        sv = cfgmm.Server(budget, slots=4, rho_precision=prec, rho_nslices=6, max_batch=16)
+        for i, m in enumerate(pool):
+            sv.add_molecule(i, m.prob.dms, m.prob.phi, np.ascontiguousarray(m.prob.tab), m.prob.blksize)
+        t0 = time.perf_counter()
+        sv.warm(list(range(len(pool))))
+        log(f"[bench]   warm {time.perf_counter() - t0:.2f} s, {sv.stats()['held_bytes'] / 2 ** 30:.1f} GiB resident")
+        return sv
+
+    def creq(reqs, outs=None):
+        return [cfgmm.Request(r.kind, r.mol, r.ngrid, r.nset, r.m, r.n, r.k, r.seed, r.arrival_s,
+                              None if outs is None or outs[i] is None else outs[i].ctypes.data)
+                for i, r in enumerate(reqs)]
+
+    def summarize(reqs, res):
+        lat = np.array([res[i].done_s - r.arrival_s for i, r in enumerate(reqs)]) * 1e3
+        rho = np.array([r.kind == 0 for r in reqs])
+        span = max(res[i].done_s for i in range(len(reqs))) - reqs[0].arrival_s
+        fl = sum(r.flops(pool) for r in reqs)
+        d = {"requests": len(reqs), "span_s": round(span, 3), "req_per_s": round(len(reqs) / span, 1),
+             "rho_evals_per_s": round(int(rho.sum()) / span, 1), "tflops": round(fl / span / 1e12, 2),
+             "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3)}
+        for name, sel in (("rho", rho), ("gemm", ~rho)):
+            if sel.any():
+                d[f"{name}_p50_ms"] = round(float(np.percentile(lat[sel], 50)), 3)
+                d[f"{name}_p99_ms"] = round(float(np.percentile(lat[sel], 99)), 3)
+        d["mean_batch"] = round(float(np.mean([res[i].batch_size for i in range(len(reqs))])), 2)
+        return d
+
+    sweeps = {}
+    for prec, name in ((cfgmm.PREC_OZAKI_INT8, "split6"), (cfgmm.PREC_FP64, "fp64")):
+        log(f"[bench] serving, rho in {name}")
+        sv = server(prec)
+        rows = []
+        for lam in lams:
+            n = int(min(2000, max(300, lam * 8)))
+            reqs = W.serving_requests(pool, n, float(lam), seed=1000 + lam)
+            res = sv.run(creq(reqs))
+            row = {"lambda": lam, **summarize(reqs, res)}
+            log(f"[bench]   {row}")
+            rows.append(row)
+        sweeps[name] = rows
+        st = sv.stats()
+        sweeps[name + "_stats"] = {k: int(v) for k, v in st.items()}
+        if name == "split6":
+            # alone vs in the mix, and oracle samples, at lambda = 500
+            reqs = W.serving_requests(pool, 200, 500.0, seed=77)
+            alone = []
+            for r in creq(reqs):
+                r.arrival_s = 0.0
+                alone.append(sv.run([r])[0].checksum)
+            pick = [i for i, r in enumerate(reqs) if r.kind == 0][:2] + [i for i, r in enumerate(reqs) if r.kind == 1][:1]
+            outs = [np.zeros(r.nset * r.ngrid if r.kind == 0 else r.m * r.n) if i in pick else None
+                    for i, r in enumerate(reqs)]
+            res = sv.run(creq(reqs, outs))
+            same = all(res[i].checksum == alone[i] for i in range(len(reqs)))
+            errs = []
+            for i in pick:
+                r = reqs[i]
+                if r.kind == 0:
+                    p = pool[r.mol].prob
+                    ref = np.zeros((r.nset - 1) * p.ngrid + r.ngrid)
+                    assert oracle.aodmao2rho(ref, p.dms.ravel(), p.phi.ravel(), p.phi.ravel(), p.nao, p.ngrid, p.nao,
+                                             r.ngrid, r.nset, np.ascontiguousarray(p.tab), p.nblk, p.blksize) == 0
+                    ref = np.stack([ref[q * p.ngrid: q * p.ngrid + r.ngrid] for q in range(r.nset)]).ravel()
+                    errs.append(["rho", rel_fro(outs[i], ref)])
+                else:
+                    a, b = W.serving_gemm_operands(r.m, r.n, r.k, r.seed)
+                    errs.append(["gemm", rel_fro(outs[i].reshape(r.m, r.n), a @ b)])
+            out["alone_equals_mix"] = {"requests": len(reqs), "bitwise_equal": same}
+            out["oracle_rel_err_samples"] = errs
+            log(f"[bench]   alone == mix: {same}; samples {errs}")
+            # saturated rho-only trace: served split precision vs plain FP64 per-call launches
+            rq = [r for r in W.serving_requests(pool, 1400, 1e6, seed=99) if r.kind == 0][:600]
+            res = sv.run(creq(rq))
+            served = summarize(rq, res)
+        else:
+            served_fp64 = summarize(rq, sv.run(creq(rq)))
+        sv.destroy()
+    # plain FP64 GPU form: the same rho requests, one per-call launch after another on one stream
+    log("[bench] plain FP64 per-call launches on the saturated rho trace")
+    s = cfgmm.Stream()
+    dev = [(cfgmm.DeviceBuffer.from_array(m.prob.dms), cfgmm.DeviceBuffer.from_array(m.prob.phi),
+            cfgmm.DeviceBuffer.from_array(np.ascontiguousarray(m.prob.tab))) for m in pool]
+    outb = cfgmm.DeviceBuffer(8 * max((r.nset - 1) * pool[r.mol].prob.ngrid + r.ngrid for r in rq))
+    e0, e1 = cfgmm.Event(), cfgmm.Event()
+    e0.record(s)
+    for r in rq:
+        p = pool[r.mol].prob
+        d, phi, tab = dev[r.mol]
+        cfgmm.aodmao2rho_dev(s, outb, d, phi, phi, p.nao, p.ngrid, p.nao, r.ngrid, r.nset, tab, p.nblk, p.blksize)
+    e1.record(s)
+    s.synchronize()
+    plain_s = e0.elapsed_ms(e1) * 1e-3
+    plain = round(len(rq) / plain_s, 1)
+    for x in dev:
+        for b in x:
+            b.free()
+    outb.free()
+    out["value"] = served["rho_evals_per_s"]
+    out["steps"] = len(rq)
+    out["warmup"] = 0
+    out["ms_per_step"] = round(1e3 / served["rho_evals_per_s"], 4)
+    out["saturated_rho_trace"] = {"requests": len(rq), "served_split6": served, "served_fp64": served_fp64,
+                                  "plain_fp64_per_call_rho_evals_per_s": plain,
+                                  "speedup_served_split6_vs_plain_fp64": round(served["rho_evals_per_s"] / plain, 2)}
+    out["sweep"] = sweeps
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -280,13 +410,16 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-dgemm", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="rho", choices=["rho", "batched"])
+    ap.add_argument("--workload", default="rho", choices=["rho", "batched", "serving"])
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
     if args.workload == "batched":
         bench_batched(args)
+        return
+    if args.workload == "serving":
+        bench_serving(args)
         return
 
     log("[bench] generating the rho workload (seed 0)")
