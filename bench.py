@@ -15,7 +15,10 @@ The same JSON line carries:
   * cpu_baseline: the oracle (a C restatement of the reference's CPU loop nest)
     timed on this host's cores on a bounded sample of the same workload;
   * dgemm: FP64 DGEMM TFLOP/s and fraction of the FP64 MFMA peak for the
-    1031x977x1543 NT and 3527x9583x6581 NN shapes (dgemm_kernel hipEvent time).
+    1031x977x1543 NT and 3527x9583x6581 NN shapes (dgemm_kernel hipEvent time);
+  * split6_resident: the same evaluation in split precision (6 int8 slices) over
+    resident per-block operands, with its error vs the oracle;
+  * host_pointer_path: the reference-shaped host-buffer call, PCIe included.
 
 Other workloads (one JSON line each):
   --workload serving   Poisson request mix (70 % rho over 8 resident molecules, 30 % DGEMM) swept over
@@ -117,6 +120,46 @@ def bench_rho(p: W.RhoProblem, steps: int, warmup: int, precision=cfgmm.PREC_FP6
     s.destroy()
     return dict(rho_once=rho_once, wall_s=wall, gpu_ms=total_gpu_ms, per_step_ms=per_step,
                 kernel_avg_ms=kern_ms / max(kern_n, 1), kernel_launches=kern_n)
+
+
+def bench_resident(p: W.RhoProblem, steps: int, nslices: int = 6):
+    s = cfgmm.Stream()
+    d_r = cfgmm.DeviceBuffer(8 * p.nset * p.ngrid)
+    d_d = cfgmm.DeviceBuffer.from_array(p.dms)
+    d_phi = cfgmm.DeviceBuffer.from_array(p.phi)
+    d_tab = cfgmm.DeviceBuffer.from_array(np.ascontiguousarray(p.tab))
+    cfgmm.check(cfgmm.lib().cfgmm_device_synchronize())
+    t0 = time.perf_counter()
+    h = cfgmm.RhoResident(s, d_d, d_phi, d_phi, p.nao, p.ngrid, p.nao, p.ngrid, p.nset, d_tab, p.nblk, p.blksize,
+                          cfgmm.PREC_OZAKI_INT8, nslices)
+    s.synchronize()
+    create_ms = (time.perf_counter() - t0) * 1e3
+    d_r.zero()
+    h.eval(s, d_r, p.ngrid)
+    s.synchronize()
+    rho = d_r.to_array(np.float64, (p.nset, p.ngrid))
+    for _ in range(5):
+        h.eval(s, d_r, p.ngrid)
+    e0, e1 = cfgmm.Event(), cfgmm.Event()
+    cfgmm.trace_reset()
+    cfgmm.trace_enable(True)
+    e0.record(s)
+    for _ in range(steps):
+        h.eval(s, d_r, p.ngrid)
+    e1.record(s)
+    s.synchronize()
+    cfgmm.trace_enable(False)
+    km, kn = cfgmm.trace_read("rho_resident_kernel")
+    cfgmm.trace_reset()
+    ms = e0.elapsed_ms(e1) / steps
+    nbytes = h.bytes
+    h.destroy()
+    for b in (d_r, d_d, d_phi, d_tab):
+        b.free()
+    e0.destroy()
+    e1.destroy()
+    s.destroy()
+    return dict(ms=ms, kernel_ms=km / max(kn, 1), create_ms=create_ms, bytes=nbytes, rho=rho)
 
 
 def bench_dgemm(g: W.GemmProblem, reps: int, warmup: int = 2):
@@ -565,6 +608,15 @@ def main():
                      "achieved_compulsory_GBs": round(p.compulsory_bytes() / (ms_per_step * 1e-3) / 1e9, 1)},
     }
 
+    # split precision over resident operands (6 int8 slices): repeated evaluations of the same
+    # (D, phi, mask), as in serving; reported beside the FP64 line, never as `value`
+    log("[bench] split precision, resident operands (6 slices)")
+    rs = bench_resident(p, steps=max(20, args.steps // 2))
+    out_split = {"ms_per_eval": round(rs["ms"], 4), "evals_per_s": round(1e3 / rs["ms"], 2),
+                 "kernel_avg_ms": round(rs["kernel_ms"], 4), "create_ms": round(rs["create_ms"], 2),
+                 "resident_GiB": round(rs["bytes"] / 2 ** 30, 3), "speedup_vs_fp64": round(ms_per_step / rs["ms"], 2),
+                 "rel_err_vs_fp64_gpu": rel_fro(rs["rho"], r["rho_once"])}
+
     # the host-pointer drop-in (reference call shape): PCIe transfers included, never `value`
     log("[bench] host-pointer drop-in path (H2D of phi, D and mask, D2H of rho)")
     hr = np.zeros(p.nset * p.ngrid)
@@ -585,10 +637,13 @@ def main():
         ref, cpu = cpu_baseline_rho(p)
         out["cpu_baseline"] = cpu
         out["rel_err"] = rel_fro(r["rho_once"], ref)
+        out_split["rel_err_vs_oracle"] = rel_fro(rs["rho"], ref)
         log(f"[bench] cpu baseline {cpu['value']} evals/s on {cpu['cores']} threads; rel_err {out['rel_err']:.3e}")
     else:
         out["cpu_baseline"] = None
         out["rel_err"] = None
+
+    out["split6_resident"] = out_split
 
     if not args.no_dgemm:
         dg = {}

@@ -13,7 +13,8 @@
 //   1. build_ao_lists: one workgroup per grid block compacts L_b on the device
 //      (wave ballots + prefix counts) into lists[b][*], counts[b].
 //   2. rho_chunk_kernel: one workgroup per (grid block, 64-row chunk of L_b)
-//      for each group of up to four density matrices.  It streams 16-deep slices of the compacted v list through
+//      for each group of up to four density matrices, the chunks of a block
+//      dispatched consecutively on one XCD.  It streams 16-deep slices of the compacted v list through
 //      double-buffered LDS -- the gathered D_s[u][v] tile (A operand) and the
 //      gathered phi_rhs[v][g] tile (B operand) -- and accumulates
 //      T_s[u][g] = sum_v D_s[u][v] phi_rhs[v][g] with v_mfma_f64_16x16x4_f64.
@@ -96,12 +97,18 @@ template <int NS, int NT>
 __global__ __launch_bounds__(kThreads, 2) void rho_chunk_kernel(
     double *__restrict__ partial, const double *__restrict__ dms, const double *__restrict__ ao_lhs,
     const double *__restrict__ ao_rhs, long stride_ao, long stride_grid, int nao, int ngrid, int nset, int set0,
-    const int *__restrict__ lists, const int *__restrict__ counts, int blksize, int nsub) {
+    const int *__restrict__ lists, const int *__restrict__ counts, int blksize, int nsub, int maxchunks) {
     __shared__ __attribute__((aligned(16))) unsigned char smem[rho::chunk_lds_bytes<NS, NT>()];
-    const int blk = blockIdx.x / nsub, sub = blockIdx.x % nsub;
+    // Workgroup i of the 1-D grid takes task L = start_x + i/8 (x = i%8) of the block-major order
+    // (block, subtile, chunk): the chunks of one block, which share its phi tile, run back to
+    // back on one XCD and find that tile in its L2.
+    const int ntasks = gridDim.x, x = blockIdx.x % 8, q = ntasks / 8, r = ntasks % 8;
+    const int L = x * q + min(x, r) + blockIdx.x / 8;
+    const int chunk = L % maxchunks, rest = L / maxchunks;
+    const int blk = rest / nsub, sub = rest % nsub;
     rho::RhoChunk t;
     t.cnt = counts[blk];
-    t.chunk = blockIdx.y;
+    t.chunk = chunk;
     if (t.chunk * kUC >= t.cnt) return;
     t.g0 = blk * blksize + sub * NT * 16;
     if (t.g0 >= ngrid) return;  // subtile past the end of a partial last block
@@ -121,11 +128,11 @@ __global__ __launch_bounds__(kThreads, 2) void rho_chunk_kernel(
 template <int NS>
 int launch_ns(int nt, dim3 grid, hipStream_t stream, double *partial, const double *dms, const double *ao_lhs,
               const double *ao_rhs, long stride_ao, long stride_grid, int nao, int ngrid, int nset, int set0,
-              const int *lists, const int *counts, int blksize, int nsub) {
+              const int *lists, const int *counts, int blksize, int nsub, int maxchunks) {
     hipEvent_t t0 = trace_start(stream);
 #define CFGMM_RHO_LAUNCH(NT_)                                                                                      \
     hipLaunchKernelGGL((rho_chunk_kernel<NS, NT_>), grid, dim3(kThreads), 0, stream, partial, dms, ao_lhs, ao_rhs, \
-                       stride_ao, stride_grid, nao, ngrid, nset, set0, lists, counts, blksize, nsub)
+                       stride_ao, stride_grid, nao, ngrid, nset, set0, lists, counts, blksize, nsub, maxchunks)
     switch (nt) {
         case 1: CFGMM_RHO_LAUNCH(1); break;
         case 2: CFGMM_RHO_LAUNCH(2); break;
@@ -157,7 +164,7 @@ int launch_rho(hipStream_t stream, double *rhos, const double *dms, const double
     const size_t maxchunks = div_up(nao, kUC);
     const int nt = rho_tile_nt(blksize);
     const int nsub = (int)(blksize / (16 * nt));
-    if (nblk * nsub > (size_t)INT32_MAX || maxchunks > 65535u) return fail(CFGMM_EINVAL, "rho: too many grid blocks");
+    if (nblk * nsub * maxchunks > (size_t)INT32_MAX) return fail(CFGMM_EINVAL, "rho: too many grid blocks");
 
     // workspace: [lists nblk*nao int][counts nblk int][pad][partial maxchunks*nset*ngrid double]
     const size_t lists_bytes = sizeof(int) * nblk * nao;
@@ -176,13 +183,13 @@ int launch_rho(hipStream_t stream, double *rhos, const double *dms, const double
 
     for (size_t set0 = 0; set0 < nset; set0 += 4) {
         const int ns = (int)std::min<size_t>(4, nset - set0);
-        dim3 grid((unsigned)(nblk * nsub), (unsigned)maxchunks, 1);
+        dim3 grid((unsigned)(nblk * nsub * maxchunks), 1, 1);
         const long sa = (long)stride_ao, sg = (long)stride_grid;
         switch (ns) {
-            case 1: st = launch_ns<1>(nt, grid, stream, partial, dms, ao_lhs, ao_rhs, sa, sg, (int)nao, (int)ngrid, (int)nset, (int)set0, lists, counts, (int)blksize, nsub); break;
-            case 2: st = launch_ns<2>(nt, grid, stream, partial, dms, ao_lhs, ao_rhs, sa, sg, (int)nao, (int)ngrid, (int)nset, (int)set0, lists, counts, (int)blksize, nsub); break;
-            case 3: st = launch_ns<3>(nt, grid, stream, partial, dms, ao_lhs, ao_rhs, sa, sg, (int)nao, (int)ngrid, (int)nset, (int)set0, lists, counts, (int)blksize, nsub); break;
-            default: st = launch_ns<4>(nt, grid, stream, partial, dms, ao_lhs, ao_rhs, sa, sg, (int)nao, (int)ngrid, (int)nset, (int)set0, lists, counts, (int)blksize, nsub); break;
+            case 1: st = launch_ns<1>(nt, grid, stream, partial, dms, ao_lhs, ao_rhs, sa, sg, (int)nao, (int)ngrid, (int)nset, (int)set0, lists, counts, (int)blksize, nsub, (int)maxchunks); break;
+            case 2: st = launch_ns<2>(nt, grid, stream, partial, dms, ao_lhs, ao_rhs, sa, sg, (int)nao, (int)ngrid, (int)nset, (int)set0, lists, counts, (int)blksize, nsub, (int)maxchunks); break;
+            case 3: st = launch_ns<3>(nt, grid, stream, partial, dms, ao_lhs, ao_rhs, sa, sg, (int)nao, (int)ngrid, (int)nset, (int)set0, lists, counts, (int)blksize, nsub, (int)maxchunks); break;
+            default: st = launch_ns<4>(nt, grid, stream, partial, dms, ao_lhs, ao_rhs, sa, sg, (int)nao, (int)ngrid, (int)nset, (int)set0, lists, counts, (int)blksize, nsub, (int)maxchunks); break;
         }
         if (st != CFGMM_OK) return st;
     }
