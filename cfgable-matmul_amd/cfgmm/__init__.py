@@ -37,6 +37,7 @@ PREC_OZAKI_INT8 = 1
 TUNE_BATCH_WGS_PER_CU = 0
 TUNE_BATCH_MAX_SETS = 1
 TUNE_DGEMM_GROUP_M = 2
+TUNE_RHO_MAX_SETS = 3
 
 _lib = None
 

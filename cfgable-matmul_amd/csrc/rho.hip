@@ -97,15 +97,18 @@ template <int NS, int NT>
 __global__ __launch_bounds__(kThreads, 2) void rho_chunk_kernel(
     double *__restrict__ partial, const double *__restrict__ dms, const double *__restrict__ ao_lhs,
     const double *__restrict__ ao_rhs, long stride_ao, long stride_grid, int nao, int ngrid, int nset, int set0,
-    const int *__restrict__ lists, const int *__restrict__ counts, int blksize, int nsub, int maxchunks) {
+    const int *__restrict__ lists, const int *__restrict__ counts, int blksize, int nsub, int maxchunks,
+    int ngroups) {
     __shared__ __attribute__((aligned(16))) unsigned char smem[rho::chunk_lds_bytes<NS, NT>()];
     // Workgroup i of the 1-D grid takes task L = start_x + i/8 (x = i%8) of the block-major order
-    // (block, subtile, chunk): the chunks of one block, which share its phi tile, run back to
-    // back on one XCD and find that tile in its L2.
+    // (block, subtile, group of NS density matrices, chunk): the tasks of one block, which share
+    // its phi tile, run back to back on one XCD and find that tile in its L2.
     const int ntasks = gridDim.x, x = blockIdx.x % 8, q = ntasks / 8, r = ntasks % 8;
     const int L = x * q + min(x, r) + blockIdx.x / 8;
     const int chunk = L % maxchunks, rest = L / maxchunks;
-    const int blk = rest / nsub, sub = rest % nsub;
+    const int grp = rest % ngroups, rest2 = rest / ngroups;
+    const int blk = rest2 / nsub, sub = rest2 % nsub;
+    set0 += grp * NS;
     rho::RhoChunk t;
     t.cnt = counts[blk];
     t.chunk = chunk;
@@ -128,11 +131,12 @@ __global__ __launch_bounds__(kThreads, 2) void rho_chunk_kernel(
 template <int NS>
 int launch_ns(int nt, dim3 grid, hipStream_t stream, double *partial, const double *dms, const double *ao_lhs,
               const double *ao_rhs, long stride_ao, long stride_grid, int nao, int ngrid, int nset, int set0,
-              const int *lists, const int *counts, int blksize, int nsub, int maxchunks) {
+              const int *lists, const int *counts, int blksize, int nsub, int maxchunks, int ngroups) {
     hipEvent_t t0 = trace_start(stream);
 #define CFGMM_RHO_LAUNCH(NT_)                                                                                      \
     hipLaunchKernelGGL((rho_chunk_kernel<NS, NT_>), grid, dim3(kThreads), 0, stream, partial, dms, ao_lhs, ao_rhs, \
-                       stride_ao, stride_grid, nao, ngrid, nset, set0, lists, counts, blksize, nsub, maxchunks)
+                       stride_ao, stride_grid, nao, ngrid, nset, set0, lists, counts, blksize, nsub, maxchunks, \
+                       ngroups)
     switch (nt) {
         case 1: CFGMM_RHO_LAUNCH(1); break;
         case 2: CFGMM_RHO_LAUNCH(2); break;
@@ -164,7 +168,6 @@ int launch_rho(hipStream_t stream, double *rhos, const double *dms, const double
     const size_t maxchunks = div_up(nao, kUC);
     const int nt = rho_tile_nt(blksize);
     const int nsub = (int)(blksize / (16 * nt));
-    if (nblk * nsub * maxchunks > (size_t)INT32_MAX) return fail(CFGMM_EINVAL, "rho: too many grid blocks");
 
     // workspace: [lists nblk*nao int][counts nblk int][pad][partial maxchunks*nset*ngrid double]
     const size_t lists_bytes = sizeof(int) * nblk * nao;
@@ -181,16 +184,26 @@ int launch_rho(hipStream_t stream, double *rhos, const double *dms, const double
     st = launch_build_lists(stream, tab, ldtab, nao, nblk, lists, counts);
     if (st != CFGMM_OK) return st;
 
-    for (size_t set0 = 0; set0 < nset; set0 += 4) {
-        const int ns = (int)std::min<size_t>(4, nset - set0);
-        dim3 grid((unsigned)(nblk * nsub * maxchunks), 1, 1);
-        const long sa = (long)stride_ao, sg = (long)stride_grid;
+    // Density matrices per workgroup: up to max_sets.  One launch covers every full group of NS
+    // sets (the group is a task coordinate); a smaller remainder group gets a launch of its own.
+    const int NS = (int)std::min<size_t>((size_t)tune_get(CFGMM_TUNE_RHO_MAX_SETS), nset);
+    const size_t full = nset / NS, rem = nset % NS;
+    const long sa = (long)stride_ao, sg = (long)stride_grid;
+    auto launch_groups = [&](int ns, size_t set0, size_t ngroups) -> int {
+        if (nblk * nsub * maxchunks * ngroups > (size_t)INT32_MAX)
+            return fail(CFGMM_EINVAL, "rho: too many workgroups for one launch");
+        const dim3 grid((unsigned)(nblk * nsub * ngroups * maxchunks), 1, 1);
         switch (ns) {
-            case 1: st = launch_ns<1>(nt, grid, stream, partial, dms, ao_lhs, ao_rhs, sa, sg, (int)nao, (int)ngrid, (int)nset, (int)set0, lists, counts, (int)blksize, nsub, (int)maxchunks); break;
-            case 2: st = launch_ns<2>(nt, grid, stream, partial, dms, ao_lhs, ao_rhs, sa, sg, (int)nao, (int)ngrid, (int)nset, (int)set0, lists, counts, (int)blksize, nsub, (int)maxchunks); break;
-            case 3: st = launch_ns<3>(nt, grid, stream, partial, dms, ao_lhs, ao_rhs, sa, sg, (int)nao, (int)ngrid, (int)nset, (int)set0, lists, counts, (int)blksize, nsub, (int)maxchunks); break;
-            default: st = launch_ns<4>(nt, grid, stream, partial, dms, ao_lhs, ao_rhs, sa, sg, (int)nao, (int)ngrid, (int)nset, (int)set0, lists, counts, (int)blksize, nsub, (int)maxchunks); break;
+            case 1: return launch_ns<1>(nt, grid, stream, partial, dms, ao_lhs, ao_rhs, sa, sg, (int)nao, (int)ngrid, (int)nset, (int)set0, lists, counts, (int)blksize, nsub, (int)maxchunks, (int)ngroups);
+            case 2: return launch_ns<2>(nt, grid, stream, partial, dms, ao_lhs, ao_rhs, sa, sg, (int)nao, (int)ngrid, (int)nset, (int)set0, lists, counts, (int)blksize, nsub, (int)maxchunks, (int)ngroups);
+            case 3: return launch_ns<3>(nt, grid, stream, partial, dms, ao_lhs, ao_rhs, sa, sg, (int)nao, (int)ngrid, (int)nset, (int)set0, lists, counts, (int)blksize, nsub, (int)maxchunks, (int)ngroups);
+            default: return launch_ns<4>(nt, grid, stream, partial, dms, ao_lhs, ao_rhs, sa, sg, (int)nao, (int)ngrid, (int)nset, (int)set0, lists, counts, (int)blksize, nsub, (int)maxchunks, (int)ngroups);
         }
+    };
+    st = launch_groups(NS, 0, full);
+    if (st != CFGMM_OK) return st;
+    if (rem) {
+        st = launch_groups((int)rem, full * NS, 1);
         if (st != CFGMM_OK) return st;
     }
     return launch_rho_reduce(stream, rhos, partial, counts, stride_grid, ngrid, nset, blksize);

@@ -273,7 +273,8 @@ enum cfgmm_tunable {
     CFGMM_TUNE_BATCH_WGS_PER_CU = 0, /* batched rho: persistent workgroups per CU, 1..8 (default 2) */
     CFGMM_TUNE_BATCH_MAX_SETS = 1,   /* batched rho: density matrices per task, 1, 2 or 4 (default 2) */
     CFGMM_TUNE_DGEMM_GROUP_M = 2,    /* DGEMM: tile rows per raster group, 1..64 (default 4) */
-    CFGMM_TUNE_COUNT = 3
+    CFGMM_TUNE_RHO_MAX_SETS = 3,     /* FP64 rho: density matrices per workgroup, 1..4 (default 1) */
+    CFGMM_TUNE_COUNT = 4
 };
 int cfgmm_tune_set(int param, int value);
 int cfgmm_tune_get(int param, int *value);

@@ -139,3 +139,30 @@ def test_rho_deterministic(workload):
                                                      np.ascontiguousarray(p.tab), p.nblk)
         outs.append(rhos)
     assert np.array_equal(outs[0], outs[1])
+
+
+def test_rho_max_sets_tunable_bitwise():
+    """Density matrices per workgroup (1..4) change the launch shape only: results are bitwise equal."""
+    p = W.rho_problem(seed=4, nshell=180, nao=560, ngrid=2048, nset=4, density=0.35)
+    s = cfgmm.Stream()
+    d_d = cfgmm.DeviceBuffer.from_array(p.dms)
+    d_phi = cfgmm.DeviceBuffer.from_array(p.phi)
+    d_tab = cfgmm.DeviceBuffer.from_array(np.ascontiguousarray(p.tab))
+    d_r = cfgmm.DeviceBuffer(8 * p.nset * p.ngrid)
+    old = cfgmm.tune_get(cfgmm.TUNE_RHO_MAX_SETS)
+    outs = []
+    try:
+        for ms in (1, 2, 3, 4):
+            cfgmm.tune_set(cfgmm.TUNE_RHO_MAX_SETS, ms)
+            d_r.zero()
+            cfgmm.aodmao2rho_dev(s, d_r, d_d, d_phi, d_phi, p.nao, p.ngrid, p.nao, p.ngrid, p.nset, d_tab, p.nblk,
+                                 p.blksize)
+            s.synchronize()
+            outs.append(d_r.to_array(np.float64, (p.nset, p.ngrid)))
+    finally:
+        cfgmm.tune_set(cfgmm.TUNE_RHO_MAX_SETS, old)
+    for b in (d_d, d_phi, d_tab, d_r):
+        b.free()
+    s.destroy()
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0])
