@@ -8,8 +8,9 @@
 //      64-row chunk of the block's list, group of density matrices), appended
 //      to a device task array; problems are numbered largest-first by the host.
 //   3. batch_rho_kernel:  a persistent grid (CUs x workgroups-per-CU, both
-//      tunable) pulls tasks from an atomic counter until the array is drained
-//      and runs the same chunk body as the per-problem kernel (rho_body.hpp).
+//      tunable) pulls tasks from per-XCD ranges of the array (atomic heads,
+//      stealing from the other ranges at the end) until all are drained and
+//      runs the same chunk body as the per-problem kernel (rho_body.hpp).
 //   4. batch_reduce:      rho += sum of the chunk partials, in a fixed order.
 // Every task writes its own partial slot, so the result does not depend on
 // which workgroup ran which task: it is bitwise the same as a per-problem call.
@@ -69,14 +70,15 @@ __global__ void batch_make_tasks(const BatchProblem *__restrict__ probs, const i
     if (b >= p.nblk) return;
     const int cnt = counts[p.counts_off + b];
     const int nch = (cnt + kUC - 1) / kUC;
-    for (int sub = 0; sub < p.nsub; ++sub) {
-        if (b * p.blksize + sub * p.nt * 16 >= p.ngrid) break;
-        for (int ch = 0; ch < nch; ++ch)
-            for (int s0 = 0; s0 < p.nset; s0 += nsmax) {
-                const int t = atomicAdd(ntasks, 1);
-                tasks[t] = int4{pi, b, (sub << 16) | ch, s0};
-            }
-    }
+    int nsub = 0;
+    while (nsub < p.nsub && b * p.blksize + nsub * p.nt * 16 < p.ngrid) ++nsub;
+    const int ngrp = (p.nset + nsmax - 1) / nsmax;
+    // one contiguous run per block, ordered (subtile, set group, chunk): the tasks sharing the
+    // block's phi tile are neighbours in the queue
+    int t = atomicAdd(ntasks, nsub * ngrp * nch);
+    for (int sub = 0; sub < nsub; ++sub)
+        for (int g = 0; g < ngrp; ++g)
+            for (int ch = 0; ch < nch; ++ch) tasks[t++] = int4{pi, b, (sub << 16) | ch, g * nsmax};
 }
 
 __global__ void zero_int_kernel(int *p, int n) {
@@ -103,8 +105,25 @@ __global__ __launch_bounds__(kThreads) void batch_rho_kernel(const BatchProblem 
     __shared__ __attribute__((aligned(16))) unsigned char smem[rho::chunk_lds_bytes<NSMAX, 4>()];
     __shared__ int cur;
     const int n = *ntasks;
+    // The queue is cut into 8 contiguous ranges, one per XCD (workgroup i runs on XCD i % 8):
+    // a block's run of tasks stays on one XCD and its L2.  A workgroup drains its own range,
+    // then the others in turn, and leaves after at most 8 empty draws.
+    int range = blockIdx.x % 8, empty = 0;
     for (;;) {
-        if (threadIdx.x == 0) cur = atomicAdd(head, 1);
+        if (threadIdx.x == 0) {
+            int ti = n;
+            while (empty < 8) {
+                const int lo = (int)((long)n * range / 8), hi = (int)((long)n * (range + 1) / 8);
+                const int c = lo + atomicAdd(&head[range], 1);
+                if (c < hi) {
+                    ti = c;
+                    break;
+                }
+                ++empty;
+                range = (range + 1) % 8;
+            }
+            cur = ti;
+        }
         __syncthreads();
         const int ti = cur;
         __syncthreads();  // every thread has read `cur` before thread 0 overwrites it
@@ -223,7 +242,7 @@ int launch_rho_batched(hipStream_t stream, const cfgmm_rho_problem *problems, si
     const size_t o_tasks = o_part + al(sizeof(double) * partial_n);
     const size_t o_ctr = o_tasks + al(sizeof(int4) * tasks_max);
     void *ws = nullptr;
-    int st = workspace_get(stream, o_ctr + 2 * sizeof(int), &ws);
+    int st = workspace_get(stream, o_ctr + 9 * sizeof(int), &ws);
     if (st != CFGMM_OK) return st;
     char *base = static_cast<char *>(ws);
     BatchProblem *d_probs = reinterpret_cast<BatchProblem *>(base);
@@ -231,12 +250,12 @@ int launch_rho_batched(hipStream_t stream, const cfgmm_rho_problem *problems, si
     int *counts = reinterpret_cast<int *>(base + o_counts);
     double *partial = reinterpret_cast<double *>(base + o_part);
     int4 *tasks = reinterpret_cast<int4 *>(base + o_tasks);
-    int *ctr = reinterpret_cast<int *>(base + o_ctr);  // [0] = task count, [1] = queue head
+    int *ctr = reinterpret_cast<int *>(base + o_ctr);  // [0] = task count, [1..8] = heads of the XCD ranges
 
     // Pageable source: the copy is complete when hipMemcpyAsync returns, so `bp` may go out of scope.
     CFGMM_HIP_TRY(hipMemcpyAsync(d_probs, bp.data(), sizeof(BatchProblem) * np, hipMemcpyHostToDevice, stream));
     hipEvent_t t0 = trace_start(stream);
-    hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(64), 0, stream, ctr, 2);
+    hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(64), 0, stream, ctr, 9);
     hipLaunchKernelGGL(batch_build_lists, dim3((unsigned)max_nblk, (unsigned)np), dim3(256), 0, stream, d_probs,
                        lists, counts);
     hipLaunchKernelGGL(batch_make_tasks, dim3((unsigned)div_up(max_nblk, 64), (unsigned)np), dim3(64), 0, stream,
