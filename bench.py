@@ -197,6 +197,30 @@ def bench_dgemm(g: W.GemmProblem, reps: int, warmup: int = 2):
                 dgemm_kernel_avg_ms=round(kavg, 4), kernel_tflops=round(tf_kern, 2))
 
 
+def bench_dgemm_tuned(g: W.GemmProblem, reps: int):
+    """The small-GEMM plan picked by the autotuner (tile edge x K ranges), cached as class dgemm_small."""
+    from cfgmm import autotune
+    s = cfgmm.Stream()
+    dA, dB = cfgmm.DeviceBuffer.from_array(g.a), cfgmm.DeviceBuffer.from_array(g.b)
+    dC = cfgmm.DeviceBuffer(8 * g.m * g.ldc)
+    dC.zero()
+    best, _ = autotune.tune_dgemm_small(
+        lambda: cfgmm.dgemm_dev(s, dC, dA, dB, g.m, g.n, g.k, g.lda, g.ldb, g.ldc, g.transa, g.transb), s)
+    for x in (dA, dB, dC):
+        x.free()
+    s.destroy()
+    autotune.save({"dgemm_small": best})
+    cfgmm.tune_set(cfgmm.TUNE_DGEMM_SMALL_TILE, best["small_tile"])
+    cfgmm.tune_set(cfgmm.TUNE_DGEMM_SMALL_SPLITS, best["small_splits"])
+    try:
+        out = bench_dgemm(g, reps)
+    finally:
+        cfgmm.tune_set(cfgmm.TUNE_DGEMM_SMALL_TILE, 0)
+        cfgmm.tune_set(cfgmm.TUNE_DGEMM_SMALL_SPLITS, 0)
+    out["tuned"] = best
+    return out
+
+
 def cpu_baseline_rho(p: W.RhoProblem, budget_s: float = 15.0):
     import oracle  # the CPU oracle: only this leg of bench.py uses it
     threads = min(16, os.cpu_count() or 1)
@@ -647,9 +671,12 @@ def main():
 
     if not args.no_dgemm:
         dg = {}
-        log("[bench] dgemm 1031x977x1543 NT")
+        log("[bench] dgemm 1031x977x1543 NT (built-in plan)")
         dg["small_nt"] = bench_dgemm(W.dgemm_small_nt(seed=0), reps=50)
         log(f"[bench]   {dg['small_nt']}")
+        log("[bench] dgemm 1031x977x1543 NT: autotuning tile edge x K ranges")
+        dg["small_nt_tuned"] = bench_dgemm_tuned(W.dgemm_small_nt(seed=0), reps=50)
+        log(f"[bench]   {dg['small_nt_tuned']}")
         log("[bench] dgemm 3527x9583x6581 NN")
         dg["large_nn"] = bench_dgemm(W.dgemm_large_nn(seed=0), reps=10)
         log(f"[bench]   {dg['large_nn']}")

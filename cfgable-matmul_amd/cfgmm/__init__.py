@@ -38,6 +38,8 @@ TUNE_BATCH_WGS_PER_CU = 0
 TUNE_BATCH_MAX_SETS = 1
 TUNE_DGEMM_GROUP_M = 2
 TUNE_RHO_MAX_SETS = 3
+TUNE_DGEMM_SMALL_TILE = 4
+TUNE_DGEMM_SMALL_SPLITS = 5
 
 _lib = None
 

@@ -8,7 +8,10 @@ JSON cache keyed by device architecture and workload class
 Tunables:
   batched rho  -- persistent workgroups per CU (1..4) and density matrices per
                   task (1, 2, 4);
-  DGEMM        -- tile rows per raster group (1, 2, 4, 8, 16).
+  DGEMM        -- tile rows per raster group (1, 2, 4, 8, 16); for shapes with
+                  fewer 128x128 tiles than workgroup slots, the tile edge (64,
+                  128) and the number of K ranges per tile (1..8);
+  FP64 rho     -- density matrices per workgroup (1..4).
 """
 from __future__ import annotations
 
@@ -67,6 +70,35 @@ def tune_dgemm_group(run_gemm, stream, groups=(1, 2, 4, 8, 16), reps=3):
     return best[0], results
 
 
+def tune_dgemm_small(run_gemm, stream, tiles=(64, 128), splits=(1, 2, 3, 4, 5, 6, 8), reps=20):
+    """Tile edge and K ranges for a DGEMM too small to fill the chip with 128x128 tiles."""
+    old = {p: cfgmm.tune_get(p) for p in (cfgmm.TUNE_DGEMM_SMALL_TILE, cfgmm.TUNE_DGEMM_SMALL_SPLITS)}
+    results = []
+    try:
+        for t, sp in itertools.product(tiles, splits):
+            cfgmm.tune_set(cfgmm.TUNE_DGEMM_SMALL_TILE, t)
+            cfgmm.tune_set(cfgmm.TUNE_DGEMM_SMALL_SPLITS, sp)
+            results.append(({"small_tile": t, "small_splits": sp}, _time(run_gemm, stream, reps, warmup=3)))
+    finally:
+        for p, v in old.items():
+            cfgmm.tune_set(p, v)
+    best = min(results, key=lambda r: r[1])
+    return best[0], results
+
+
+def tune_rho_sets(run_rho, stream, sets=(1, 2, 3, 4), reps=10):
+    old = cfgmm.tune_get(cfgmm.TUNE_RHO_MAX_SETS)
+    results = []
+    try:
+        for v in sets:
+            cfgmm.tune_set(cfgmm.TUNE_RHO_MAX_SETS, v)
+            results.append(({"rho_max_sets": v}, _time(run_rho, stream, reps)))
+    finally:
+        cfgmm.tune_set(cfgmm.TUNE_RHO_MAX_SETS, old)
+    best = min(results, key=lambda r: r[1])
+    return best[0], results
+
+
 def cache_path(path=None) -> str:
     return path or os.path.join(TUNING_DIR, f"{ARCH}.json")
 
@@ -99,4 +131,10 @@ def apply(workload_class: str, path=None) -> dict:
         cfgmm.tune_set(cfgmm.TUNE_BATCH_MAX_SETS, int(cfg["max_sets"]))
     if "group_m" in cfg:
         cfgmm.tune_set(cfgmm.TUNE_DGEMM_GROUP_M, int(cfg["group_m"]))
+    if "small_tile" in cfg:
+        cfgmm.tune_set(cfgmm.TUNE_DGEMM_SMALL_TILE, int(cfg["small_tile"]))
+    if "small_splits" in cfg:
+        cfgmm.tune_set(cfgmm.TUNE_DGEMM_SMALL_SPLITS, int(cfg["small_splits"]))
+    if "rho_max_sets" in cfg:
+        cfgmm.tune_set(cfgmm.TUNE_RHO_MAX_SETS, int(cfg["rho_max_sets"]))
     return cfg

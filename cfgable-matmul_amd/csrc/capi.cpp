@@ -79,9 +79,9 @@ void trace_stop(hipStream_t stream, const char *name, hipEvent_t start) {
 // ---------------------------------------------------------------------------
 // tunables
 // ---------------------------------------------------------------------------
-static std::atomic<int> g_tune[CFGMM_TUNE_COUNT] = {{2}, {2}, {4}, {1}};
-static const int g_tune_lo[CFGMM_TUNE_COUNT] = {1, 1, 1, 1};
-static const int g_tune_hi[CFGMM_TUNE_COUNT] = {8, 4, 64, 4};
+static std::atomic<int> g_tune[CFGMM_TUNE_COUNT] = {{2}, {2}, {4}, {1}, {0}, {0}};
+static const int g_tune_lo[CFGMM_TUNE_COUNT] = {1, 1, 1, 1, 0, 0};
+static const int g_tune_hi[CFGMM_TUNE_COUNT] = {8, 4, 64, 4, 128, 32};
 
 int tune_get(int param) { return g_tune[param].load(std::memory_order_relaxed); }
 
@@ -633,6 +633,8 @@ int cfgmm_server_destroy(cfgmm_server *s) {
 int cfgmm_tune_set(int param, int value) {
     clear_error();
     if (param < 0 || param >= CFGMM_TUNE_COUNT) return fail(CFGMM_EINVAL, "unknown tunable " + std::to_string(param));
+    if (param == CFGMM_TUNE_DGEMM_SMALL_TILE && value != 0 && value != 64 && value != 128)
+        return fail(CFGMM_EINVAL, "DGEMM small tile must be 0, 64 or 128");
     if (value < g_tune_lo[param] || value > g_tune_hi[param])
         return fail(CFGMM_EINVAL, "tunable " + std::to_string(param) + " out of range: " + std::to_string(value));
     g_tune[param].store(value);

@@ -244,9 +244,12 @@ DgemmPlan plan_dgemm(int m, int n, int k, int cus) {
         const long t64 = (long)div_up(m, 64) * div_up(n, 64);
         const long slots64 = (long)kSlots64 * cus;
         // Few tiles: 128x128 tiles split along K fill the chip with fewer,
-        // larger pieces when that is possible; otherwise 64x64 tiles.
+        // larger pieces when that is possible; otherwise 64x64 tiles.  The
+        // autotuner may fix the tile edge and the number of K ranges.
+        const int force_tile = tune_get(CFGMM_TUNE_DGEMM_SMALL_TILE);
+        const int force_splits = tune_get(CFGMM_TUNE_DGEMM_SMALL_SPLITS);
         const int s128 = split_for(t128, slots128);
-        if (t128 * s128 >= slots128 * 3 / 4) {
+        if (force_tile == 128 || (force_tile == 0 && t128 * s128 >= slots128 * 3 / 4)) {
             p.bm = 128;
             p.tiles = (int)t128;
         } else {
@@ -254,7 +257,10 @@ DgemmPlan plan_dgemm(int m, int n, int k, int cus) {
             p.tiles = (int)t64;
         }
         const long slots = p.bm == 128 ? slots128 : slots64;
-        if (p.tiles >= slots) {
+        if (force_splits > 0) {
+            p.dp_tiles = force_splits == 1 ? p.tiles : 0;
+            p.splits = std::min(force_splits, std::max(1, k / 16));
+        } else if (p.tiles >= slots) {
             p.dp_tiles = p.tiles;
             p.splits = 1;
         } else {

@@ -111,3 +111,21 @@ def test_dgemm_accumulates_twice():
     cfgmm.matmul_anyway_full(c, a, b, m, n, k, k, n, n, False, False)
     cfgmm.matmul_anyway_full(c, a, b, m, n, k, k, n, n, False, False)
     assert rel_fro(c.reshape(m, n), 2 * (a.reshape(m, k) @ b.reshape(k, n))) <= TOL
+
+
+@pytest.mark.parametrize("tile,splits", [(64, 1), (64, 3), (64, 5), (128, 2), (128, 7)])
+@pytest.mark.parametrize("ta,tb", [(False, True), (True, False)])
+def test_dgemm_small_plans(tile, splits, ta, tb):
+    """Every tile edge x K-range plan the autotuner may pick for a small GEMM meets the FP64 bar."""
+    old = (cfgmm.tune_get(cfgmm.TUNE_DGEMM_SMALL_TILE), cfgmm.tune_get(cfgmm.TUNE_DGEMM_SMALL_SPLITS))
+    try:
+        cfgmm.tune_set(cfgmm.TUNE_DGEMM_SMALL_TILE, tile)
+        cfgmm.tune_set(cfgmm.TUNE_DGEMM_SMALL_SPLITS, splits)
+        c, c_ref, c_gpu, ldc = _run_both(301, 277, 899, ta, tb, seed=tile + splits, pad=(3, 1, 2), api="dev")
+    finally:
+        cfgmm.tune_set(cfgmm.TUNE_DGEMM_SMALL_TILE, old[0])
+        cfgmm.tune_set(cfgmm.TUNE_DGEMM_SMALL_SPLITS, old[1])
+    G = c_gpu.reshape(301, ldc)
+    R = c_ref.reshape(301, ldc)
+    assert rel_fro(G[:, :277], R[:, :277]) <= TOL
+    assert np.array_equal(G[:, 277:], c.reshape(301, ldc)[:, 277:])
