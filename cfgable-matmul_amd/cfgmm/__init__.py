@@ -40,6 +40,8 @@ TUNE_DGEMM_GROUP_M = 2
 TUNE_RHO_MAX_SETS = 3
 TUNE_DGEMM_SMALL_TILE = 4
 TUNE_DGEMM_SMALL_SPLITS = 5
+TUNE_RESIDENT_DEPTH = 6
+TUNE_RESIDENT_COLS = 7
 
 _lib = None
 

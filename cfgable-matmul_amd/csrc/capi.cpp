@@ -79,9 +79,9 @@ void trace_stop(hipStream_t stream, const char *name, hipEvent_t start) {
 // ---------------------------------------------------------------------------
 // tunables
 // ---------------------------------------------------------------------------
-static std::atomic<int> g_tune[CFGMM_TUNE_COUNT] = {{2}, {2}, {4}, {1}, {0}, {0}};
-static const int g_tune_lo[CFGMM_TUNE_COUNT] = {1, 1, 1, 1, 0, 0};
-static const int g_tune_hi[CFGMM_TUNE_COUNT] = {8, 4, 64, 4, 128, 32};
+static std::atomic<int> g_tune[CFGMM_TUNE_COUNT] = {{2}, {2}, {4}, {1}, {0}, {0}, {1}, {0}};
+static const int g_tune_lo[CFGMM_TUNE_COUNT] = {1, 1, 1, 1, 0, 0, 1, 0};
+static const int g_tune_hi[CFGMM_TUNE_COUNT] = {8, 4, 64, 4, 128, 32, 2, 64};
 
 int tune_get(int param) { return g_tune[param].load(std::memory_order_relaxed); }
 
@@ -633,6 +633,8 @@ int cfgmm_server_destroy(cfgmm_server *s) {
 int cfgmm_tune_set(int param, int value) {
     clear_error();
     if (param < 0 || param >= CFGMM_TUNE_COUNT) return fail(CFGMM_EINVAL, "unknown tunable " + std::to_string(param));
+    if (param == CFGMM_TUNE_RESIDENT_COLS && value % 16 != 0)
+        return fail(CFGMM_EINVAL, "resident columns must be 0, 16, 32, 48 or 64");
     if (param == CFGMM_TUNE_DGEMM_SMALL_TILE && value != 0 && value != 64 && value != 128)
         return fail(CFGMM_EINVAL, "DGEMM small tile must be 0, 64 or 128");
     if (value < g_tune_lo[param] || value > g_tune_hi[param])

@@ -199,8 +199,8 @@ __device__ inline int xcd_task(int i, int ntasks) {
     return x * q + min(x, r) + i / 8;
 }
 
-template <int S, int NT>
-__global__ __launch_bounds__(kThreads, 2) void rho_resident_kernel(
+template <int S, int NT, int DEPTH>
+__global__ __launch_bounds__(kThreads, DEPTH == 1 ? 2 : 1) void rho_resident_kernel(
     double *__restrict__ partial, const unsigned char *__restrict__ base, const BlockDesc *__restrict__ blocks,
     const int4 *__restrict__ tasks, int ntasks, const int *__restrict__ fg, int blksize, int ngrid, int nset) {
     constexpr int W = NT * 16;
@@ -232,7 +232,7 @@ __global__ __launch_bounds__(kThreads, 2) void rho_resident_kernel(
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) acc[l][nt] = i32x4{0, 0, 0, 0};
 
-    i32x4 fa0[S], fa1[S], breg[kBPer];
+    i32x4 fa0[S], fa1[S], fa2[S], breg[kBPer];
     auto load_a = [&](int ks, i32x4 *dst) {
 #pragma unroll
         for (int p = 0; p < S; ++p) dst[p] = *reinterpret_cast<const i32x4 *>(arow + p * dplane + (long)ks * 1024);
@@ -273,23 +273,39 @@ __global__ __launch_bounds__(kThreads, 2) void rho_resident_kernel(
         }
     };
 
-    // The loads of step ks+1 are issued before step ks computes and stay in flight
-    // meanwhile.  The prefetch index is clamped instead of guarded (a guard makes the
-    // compiler wait for every load at its join): past the last step it re-reads that
-    // step, and the values are not used.
+    // A fragments are prefetched DEPTH k steps ahead (DEPTH 1: two register sets, two waves per
+    // SIMD; DEPTH 2: a ring of three with static roles, the loop unrolled by three, one wave per
+    // SIMD with the whole register file); the B tile one step ahead through LDS.  A prefetch
+    // index is clamped instead of guarded (a guard makes the compiler wait for every load at its
+    // join): past the last step it re-reads that step, and the values are not used.
     const int last = nks - 1;
-    load_b(0);
-    load_a(0, fa0);
-    for (int ks = 0; ks < nks; ++ks) {
+    auto step = [&](int ks, const i32x4 *fa_cur, i32x4 *fa_ahead) {
         const int buf = ks & 1;
         store_b(buf);
         __syncthreads();
         load_b(min(ks + 1, last));
-        load_a(min(ks + 1, last), fa1);
+        load_a(min(ks + DEPTH, last), fa_ahead);
         __builtin_amdgcn_sched_barrier(0);  // keep the prefetch ahead of the MFMAs (not sunk to its use)
-        compute(buf, fa0);
+        compute(buf, fa_cur);
+    };
+    load_b(0);
+    load_a(0, fa0);
+    if constexpr (DEPTH == 1) {
+        for (int ks = 0; ks < nks; ++ks) {
+            step(ks, fa0, fa1);
 #pragma unroll
-        for (int p = 0; p < S; ++p) fa0[p] = fa1[p];
+            for (int p = 0; p < S; ++p) fa0[p] = fa1[p];
+        }
+    } else {
+        load_a(min(1, last), fa1);
+        int ks = 0;
+        for (; ks + 3 <= nks; ks += 3) {
+            step(ks, fa0, fa2);
+            step(ks + 1, fa1, fa0);
+            step(ks + 2, fa2, fa1);
+        }
+        if (ks < nks) step(ks, fa0, fa2);
+        if (ks + 1 < nks) step(ks + 1, fa1, fa0);
     }
 
     // Epilogue.  i32 16x16 layout: lane l, element r holds row 4*(l/16)+r, column l%16.
@@ -389,6 +405,8 @@ int resident_create(hipStream_t stream, const double *dms, const double *ao_lhs,
     h->blksize = blksize;
     h->nblk = ngrid ? div_up(ngrid, blksize) : 0;
     h->nt = tile_nt(blksize);
+    const int want_nt = tune_get(CFGMM_TUNE_RESIDENT_COLS) / 16;  // 0: the widest tile
+    if (want_nt > 0 && want_nt < h->nt && (blksize / 16) % want_nt == 0) h->nt = want_nt;
     h->nsub = (int)(blksize / (16 * h->nt));
     const size_t nblk = h->nblk;
     if (nao == 0 || ngrid == 0 || nset == 0) {
@@ -521,14 +539,24 @@ template <int S>
 static int eval_launch(hipStream_t stream, const ResidentRho *h, double *partial, dim3 grid, int ntasks, int ngrid,
                        int nset) {
     hipEvent_t t0 = trace_start(stream);
-#define CFGMM_RES_LAUNCH(NT_)                                                                                      \
-    hipLaunchKernelGGL((rho_resident_kernel<S, NT_>), grid, dim3(kThreads), 0, stream, partial, h->base, h->d_blocks, \
-                       h->d_tasks, ntasks, h->d_fg, (int)h->blksize, ngrid, nset)
-    switch (h->nt) {
-        case 1: CFGMM_RES_LAUNCH(1); break;
-        case 2: CFGMM_RES_LAUNCH(2); break;
-        case 3: CFGMM_RES_LAUNCH(3); break;
-        default: CFGMM_RES_LAUNCH(4); break;
+    const int depth = tune_get(CFGMM_TUNE_RESIDENT_DEPTH);
+#define CFGMM_RES_LAUNCH(NT_, D_)                                                                                    \
+    hipLaunchKernelGGL((rho_resident_kernel<S, NT_, D_>), grid, dim3(kThreads), 0, stream, partial, h->base,          \
+                       h->d_blocks, h->d_tasks, ntasks, h->d_fg, (int)h->blksize, ngrid, nset)
+    if (depth >= 2) {
+        switch (h->nt) {
+            case 1: CFGMM_RES_LAUNCH(1, 2); break;
+            case 2: CFGMM_RES_LAUNCH(2, 2); break;
+            case 3: CFGMM_RES_LAUNCH(3, 2); break;
+            default: CFGMM_RES_LAUNCH(4, 2); break;
+        }
+    } else {
+        switch (h->nt) {
+            case 1: CFGMM_RES_LAUNCH(1, 1); break;
+            case 2: CFGMM_RES_LAUNCH(2, 1); break;
+            case 3: CFGMM_RES_LAUNCH(3, 1); break;
+            default: CFGMM_RES_LAUNCH(4, 1); break;
+        }
     }
 #undef CFGMM_RES_LAUNCH
     trace_stop(stream, "rho_resident_kernel", t0);

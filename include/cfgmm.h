@@ -276,7 +276,10 @@ enum cfgmm_tunable {
     CFGMM_TUNE_RHO_MAX_SETS = 3,     /* FP64 rho: density matrices per workgroup, 1..4 (default 1) */
     CFGMM_TUNE_DGEMM_SMALL_TILE = 4, /* DGEMM with fewer 128x128 tiles than slots: tile edge 64 or 128, 0 = heuristic */
     CFGMM_TUNE_DGEMM_SMALL_SPLITS = 5, /* ... and K ranges per tile, 1..32, 0 = heuristic */
-    CFGMM_TUNE_COUNT = 6
+    CFGMM_TUNE_RESIDENT_DEPTH = 6,   /* resident rho: k steps of A prefetch, 1 or 2 (default 1) */
+    CFGMM_TUNE_RESIDENT_COLS = 7,    /* resident rho: grid points per workgroup, 16/32/48/64, 0 = widest
+                                        dividing blksize (read at creation) */
+    CFGMM_TUNE_COUNT = 8
 };
 int cfgmm_tune_set(int param, int value);
 int cfgmm_tune_get(int param, int *value);

@@ -170,3 +170,23 @@ def test_resident_errors():
     for b in (d, l, t, out):
         b.free()
     s.destroy()
+
+
+@pytest.mark.parametrize("depth,cols", [(2, 0), (1, 32), (1, 16)])
+def test_resident_launch_tunables_bitwise(depth, cols):
+    """Prefetch depth and columns per workgroup change the schedule only: results are bitwise equal."""
+    p = W.rho_problem(seed=8, nshell=200, nao=640, ngrid=2048, nset=2, density=0.35)
+    base = Resident(p)
+    ref = base.eval()
+    base.close()
+    old = (cfgmm.tune_get(cfgmm.TUNE_RESIDENT_DEPTH), cfgmm.tune_get(cfgmm.TUNE_RESIDENT_COLS))
+    try:
+        cfgmm.tune_set(cfgmm.TUNE_RESIDENT_DEPTH, depth)
+        cfgmm.tune_set(cfgmm.TUNE_RESIDENT_COLS, cols)
+        r = Resident(p)  # the column width is fixed at creation
+        got = r.eval()
+        r.close()
+    finally:
+        cfgmm.tune_set(cfgmm.TUNE_RESIDENT_DEPTH, old[0])
+        cfgmm.tune_set(cfgmm.TUNE_RESIDENT_COLS, old[1])
+    assert np.array_equal(got, ref)
