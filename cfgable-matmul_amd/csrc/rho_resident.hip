@@ -235,7 +235,8 @@ __global__ __launch_bounds__(kThreads, DEPTH == 1 ? 2 : 1) void rho_resident_ker
     i32x4 fa0[S], fa1[S], fa2[S], breg[kBPer];
     auto load_a = [&](int ks, i32x4 *dst) {
 #pragma unroll
-        for (int p = 0; p < S; ++p) dst[p] = *reinterpret_cast<const i32x4 *>(arow + p * dplane + (long)ks * 1024);
+        for (int p = 0; p < S; ++p)  // each A tile is read once per evaluation: stream it past the caches
+            dst[p] = __builtin_nontemporal_load(reinterpret_cast<const i32x4 *>(arow + p * dplane + (long)ks * 1024));
     };
     // B tile of step ks: S planes x W columns x 64 k, contiguous per plane
     auto load_b = [&](int ks) {
