@@ -72,6 +72,7 @@ _SIGNATURES = {
     "cfgmm_aodmao2rho_batched_dev": ([_VP, _VP, _SZ, _I, _I], _I),
     "cfgmm_rho_resident_create": ([_VP, _VP, _VP, _VP] + [_SZ] * 5 + [_VP, _SZ, _SZ, _I, _I, ctypes.POINTER(_VP)], _I),
     "cfgmm_rho_resident_eval": ([_VP, _VP, _VP, _SZ, _SZ, _SZ], _I),
+    "cfgmm_rho_resident_eval_batched": ([_VP, _VP, _SZ], _I),
     "cfgmm_rho_resident_bytes": ([_VP, ctypes.POINTER(_SZ)], _I),
     "cfgmm_rho_resident_destroy": ([_VP], _I),
     "cfgmm_server_create": ([_VP, ctypes.POINTER(_VP)], _I),
@@ -474,6 +475,17 @@ class Server:
         if self.handle:
             check(lib().cfgmm_server_destroy(self.handle))
             self.handle = None
+
+
+class ResidentEval(ctypes.Structure):
+    """cfgmm_rho_resident_job: one entry of a batched resident evaluation."""
+    _fields_ = [("h", _VP), ("rhos", _VP), ("stride_rho", _SZ), ("ngrid_used", _SZ), ("nset_used", _SZ)]
+
+
+def rho_resident_eval_batched(stream, evals):
+    """evals: sequence of (RhoResident, rhos buffer, stride_rho, ngrid, nset)."""
+    arr = (ResidentEval * len(evals))(*[ResidentEval(h.handle, _dev(r), st, ng, ns) for h, r, st, ng, ns in evals])
+    check(lib().cfgmm_rho_resident_eval_batched(_h(stream), ctypes.cast(arr, _VP), len(arr)))
 
 
 def _dev(x):

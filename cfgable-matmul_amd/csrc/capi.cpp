@@ -555,9 +555,8 @@ int cfgmm_rho_resident_create(void *stream, const double *dms, const double *ao_
     return CFGMM_OK;
 }
 
-int cfgmm_rho_resident_eval(void *stream, const cfgmm_rho_resident *h, double *rhos, size_t stride_rho,
-                            size_t ngrid_used, size_t nset_used) {
-    clear_error();
+static int check_resident_eval(const cfgmm_rho_resident *h, const double *rhos, size_t stride_rho, size_t ngrid_used,
+                               size_t nset_used) {
     if (!h) return fail(CFGMM_EINVAL, "resident handle is null");
     const ResidentRho *r = impl(h);
     if (ngrid_used > resident_ngrid(r) || nset_used > resident_nset(r))
@@ -570,7 +569,27 @@ int cfgmm_rho_resident_eval(void *stream, const cfgmm_rho_resident *h, double *r
     int dev = 0;
     CFGMM_HIP_TRY(hipGetDevice(&dev));
     if (dev != resident_device(r)) return fail(CFGMM_EINVAL, "resident rho: handle belongs to another device");
-    return resident_eval(static_cast<hipStream_t>(stream), r, rhos, stride_rho, ngrid_used, nset_used);
+    return CFGMM_OK;
+}
+
+int cfgmm_rho_resident_eval(void *stream, const cfgmm_rho_resident *h, double *rhos, size_t stride_rho,
+                            size_t ngrid_used, size_t nset_used) {
+    clear_error();
+    CFGMM_TRY(check_resident_eval(h, rhos, stride_rho, ngrid_used, nset_used));
+    return resident_eval(static_cast<hipStream_t>(stream), impl(h), rhos, stride_rho, ngrid_used, nset_used);
+}
+
+int cfgmm_rho_resident_eval_batched(void *stream, const cfgmm_rho_resident_job *evals, size_t n) {
+    clear_error();
+    if (n && !evals) return fail(CFGMM_EINVAL, "evals is null");
+    std::vector<ResidentEvalReq> reqs(n);
+    for (size_t i = 0; i < n; ++i) {
+        const cfgmm_rho_resident_job &e = evals[i];
+        const int st = check_resident_eval(e.h, e.rhos, e.stride_rho, e.ngrid_used, e.nset_used);
+        if (st != CFGMM_OK) return fail(st, "evaluation " + std::to_string(i) + ": " + g_last_error);
+        reqs[i] = ResidentEvalReq{impl(e.h), e.rhos, e.stride_rho, e.ngrid_used, e.nset_used};
+    }
+    return resident_eval_batched(static_cast<hipStream_t>(stream), reqs.data(), n);
 }
 
 int cfgmm_rho_resident_bytes(const cfgmm_rho_resident *h, size_t *bytes) {

@@ -83,6 +83,12 @@ int resident_create(hipStream_t stream, const double *dms, const double *ao_lhs,
 int resident_eval(hipStream_t stream, const ResidentRho *h, double *rhos, size_t stride_rho, size_t ngrid_used,
                   size_t nset_used);
 int resident_destroy(ResidentRho *h);
+struct ResidentEvalReq {
+    const ResidentRho *h;
+    double *rhos;
+    size_t stride_rho, ngrid, nset;
+};
+int resident_eval_batched(hipStream_t stream, const ResidentEvalReq *reqs, size_t n);
 size_t resident_bytes(const ResidentRho *h);
 size_t resident_ngrid(const ResidentRho *h);
 size_t resident_nset(const ResidentRho *h);

@@ -199,19 +199,19 @@ __device__ inline int xcd_task(int i, int ntasks) {
     return x * q + min(x, r) + i / 8;
 }
 
+// One task (block b, 64-row tile, column subtile) of density matrix s: partial[tile][s][g] for the
+// subtile's grid points g < ngrid.
 template <int S, int NT, int DEPTH>
-__global__ __launch_bounds__(kThreads, DEPTH == 1 ? 2 : 1) void rho_resident_kernel(
+__device__ __attribute__((always_inline)) inline void resident_tile(
     double *__restrict__ partial, const unsigned char *__restrict__ base, const BlockDesc *__restrict__ blocks,
-    const int4 *__restrict__ tasks, int ntasks, const int *__restrict__ fg, int blksize, int ngrid, int nset) {
+    int4 task, int s, const int *__restrict__ fg, int blksize, int ngrid, int nset) {
     constexpr int W = NT * 16;
     constexpr int kBChunks = S * W * (kK / 16);  // 16-byte chunks of one B tile
     constexpr int kBPer = (kBChunks + kThreads - 1) / kThreads;
     __shared__ __attribute__((aligned(16))) int8_t Bs[2][S][W * kK];
     __shared__ double red[kThreads / 64][W];
 
-    const int4 task = tasks[xcd_task(blockIdx.x, ntasks)];
     const int b = task.x, tile = task.y, sub = task.z;
-    const int s = blockIdx.y;
     const BlockDesc bd = blocks[b];
     const int kpad = bd.kpad;
     const int c0 = sub * W;
@@ -347,6 +347,57 @@ __global__ __launch_bounds__(kThreads, DEPTH == 1 ? 2 : 1) void rho_resident_ker
             partial[((long)tile * nset + s) * ngrid + g] = v;
         }
     }
+}
+
+template <int S, int NT, int DEPTH>
+__global__ __launch_bounds__(kThreads, DEPTH == 1 ? 2 : 1) void rho_resident_kernel(
+    double *__restrict__ partial, const unsigned char *__restrict__ base, const BlockDesc *__restrict__ blocks,
+    const int4 *__restrict__ tasks, int ntasks, const int *__restrict__ fg, int blksize, int ngrid, int nset) {
+    resident_tile<S, NT, DEPTH>(partial, base, blocks, tasks[xcd_task(blockIdx.x, ntasks)], blockIdx.y, fg, blksize,
+                                ngrid, nset);
+}
+
+// One evaluation of a batch (device table).
+struct EvalDesc {
+    const unsigned char *base;
+    const BlockDesc *blocks;
+    const int4 *tasks;
+    const int *fg, *counts;
+    double *partial, *rhos;
+    long stride_rho;
+    int blksize, ngrid, nset, ntasks;
+    int wg_begin;  // first workgroup of this evaluation in the batch grid
+};
+
+// Every (task, density matrix) of every evaluation in one grid.  The XCD-contiguous order runs
+// over the whole batch, evaluations laid out one after another, so the tiles of a block still
+// meet on one XCD.
+template <int S, int NT>
+__global__ __launch_bounds__(kThreads, 2) void rho_resident_batched_kernel(const EvalDesc *__restrict__ evals,
+                                                                           int nevals) {
+    const int L = xcd_task(blockIdx.x, gridDim.x);
+    int lo = 0, hi = nevals - 1;  // last evaluation with wg_begin <= L
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (evals[mid].wg_begin <= L) lo = mid;
+        else hi = mid - 1;
+    }
+    const EvalDesc e = evals[lo];
+    const int t = L - e.wg_begin;
+    resident_tile<S, NT, 1>(e.partial, e.base, e.blocks, e.tasks[t % e.ntasks], t / e.ntasks, e.fg, e.blksize,
+                            e.ngrid, e.nset);
+}
+
+// rho[s*stride + g] += sum over the 64-row tiles of g's block of partial[tile][s][g], per evaluation.
+__global__ void resident_batched_reduce(const EvalDesc *__restrict__ evals) {
+    const EvalDesc e = evals[blockIdx.y];
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)e.nset * e.ngrid) return;
+    const int s = (int)(idx / e.ngrid), g = (int)(idx % e.ngrid);
+    const int nch = (e.counts[g / e.blksize] + kRows - 1) / kRows;
+    double v = 0.0;
+    for (int ch = 0; ch < nch; ++ch) v += e.partial[((long)ch * e.nset + s) * e.ngrid + g];
+    e.rhos[(long)s * e.stride_rho + g] += v;
 }
 
 int tile_nt(size_t blksize) {
@@ -586,6 +637,92 @@ int resident_eval(hipStream_t stream, const ResidentRho *h, double *rhos, size_t
     }
     if (st != CFGMM_OK) return st;
     return launch_rho_reduce(stream, rhos, partial, h->d_counts, stride_rho, ngrid_used, nset_used, h->blksize);
+}
+
+// Evaluations of several handles in one launch per (slices, tile width) group; each exactly as
+// resident_eval would compute it (the same tile code and the same fixed reduction order).
+int resident_eval_batched(hipStream_t stream, const ResidentEvalReq *reqs, size_t n) {
+    std::vector<size_t> todo;
+    for (size_t i = 0; i < n; ++i) {
+        const ResidentRho *h = reqs[i].h;
+        if (reqs[i].ngrid == 0 || reqs[i].nset == 0 || h->nao == 0) continue;
+        if (h->task_prefix[div_up(reqs[i].ngrid, h->blksize)] == 0) continue;
+        todo.push_back(i);
+    }
+    while (!todo.empty()) {
+        const int S = reqs[todo[0]].h->nslices, nt = reqs[todo[0]].h->nt;
+        std::vector<size_t> group, rest;
+        for (size_t i : todo) (reqs[i].h->nslices == S && reqs[i].h->nt == nt ? group : rest).push_back(i);
+        todo.swap(rest);
+        // workspace: [descriptors][partials of each evaluation]
+        auto al = [](size_t x) { return div_up(x, 256) * 256; };
+        std::vector<EvalDesc> d(group.size());
+        size_t off = al(sizeof(EvalDesc) * group.size());
+        long wg = 0;
+        std::vector<size_t> part_off(group.size());
+        for (size_t j = 0; j < group.size(); ++j) {
+            const ResidentEvalReq &q = reqs[group[j]];
+            part_off[j] = off;
+            off += al(sizeof(double) * q.h->max_tiles * q.nset * q.ngrid);
+        }
+        void *ws = nullptr;
+        int st = workspace_get(stream, off, &ws);
+        if (st != CFGMM_OK) return st;
+        char *base = static_cast<char *>(ws);
+        size_t max_elems = 0;
+        for (size_t j = 0; j < group.size(); ++j) {
+            const ResidentEvalReq &q = reqs[group[j]];
+            const ResidentRho *h = q.h;
+            EvalDesc &e = d[j];
+            e.base = h->base;
+            e.blocks = h->d_blocks;
+            e.tasks = h->d_tasks;
+            e.fg = h->d_fg;
+            e.counts = h->d_counts;
+            e.partial = reinterpret_cast<double *>(base + part_off[j]);
+            e.rhos = q.rhos;
+            e.stride_rho = (long)q.stride_rho;
+            e.blksize = (int)h->blksize;
+            e.ngrid = (int)q.ngrid;
+            e.nset = (int)q.nset;
+            e.ntasks = h->task_prefix[div_up(q.ngrid, h->blksize)];
+            e.wg_begin = (int)wg;
+            wg += (long)e.ntasks * e.nset;
+            max_elems = std::max(max_elems, q.nset * q.ngrid);
+        }
+        if (wg > INT32_MAX || group.size() > 65535) return fail(CFGMM_EINVAL, "resident rho: batch too large");
+        // pageable source: the copy is complete when hipMemcpyAsync returns
+        CFGMM_HIP_TRY(hipMemcpyAsync(base, d.data(), sizeof(EvalDesc) * d.size(), hipMemcpyHostToDevice, stream));
+        const EvalDesc *dd = reinterpret_cast<const EvalDesc *>(base);
+        hipEvent_t t0 = trace_start(stream);
+#define CFGMM_RESB(S_, NT_)                                                                                  \
+    hipLaunchKernelGGL((rho_resident_batched_kernel<S_, NT_>), dim3((unsigned)wg), dim3(kThreads), 0, stream, dd, \
+                       (int)d.size())
+#define CFGMM_RESB_NT(S_)               \
+    switch (nt) {                       \
+        case 1: CFGMM_RESB(S_, 1); break; \
+        case 2: CFGMM_RESB(S_, 2); break; \
+        case 3: CFGMM_RESB(S_, 3); break; \
+        default: CFGMM_RESB(S_, 4); break; \
+    }
+        switch (S) {
+            case 2: CFGMM_RESB_NT(2); break;
+            case 3: CFGMM_RESB_NT(3); break;
+            case 4: CFGMM_RESB_NT(4); break;
+            case 5: CFGMM_RESB_NT(5); break;
+            default: CFGMM_RESB_NT(6); break;
+        }
+#undef CFGMM_RESB_NT
+#undef CFGMM_RESB
+        trace_stop(stream, "rho_resident_kernel", t0);
+        st = hip_status(hipGetLastError(), "rho_resident_batched_kernel launch");
+        if (st != CFGMM_OK) return st;
+        hipLaunchKernelGGL(resident_batched_reduce, dim3((unsigned)div_up(max_elems, 256), (unsigned)d.size()),
+                           dim3(256), 0, stream, dd);
+        st = hip_status(hipGetLastError(), "resident_batched_reduce launch");
+        if (st != CFGMM_OK) return st;
+    }
+    return CFGMM_OK;
 }
 
 }  // namespace cfgmm

@@ -205,6 +205,15 @@ int cfgmm_rho_resident_create(void *stream, const double *dms, const double *ao_
  * streams may run concurrently (each uses its stream's workspace). */
 int cfgmm_rho_resident_eval(void *stream, const cfgmm_rho_resident *h, double *rhos, size_t stride_rho,
                             size_t ngrid_used, size_t nset_used);
+/* Several evaluations (of one or many handles) in one launch per (nslices, tile width) group;
+ * each entry with the contract of cfgmm_rho_resident_eval and bitwise its result.  Entries
+ * must not write overlapping rho ranges. */
+typedef struct cfgmm_rho_resident_job {
+    const cfgmm_rho_resident *h;
+    double *rhos;
+    size_t stride_rho, ngrid_used, nset_used;
+} cfgmm_rho_resident_job;
+int cfgmm_rho_resident_eval_batched(void *stream, const cfgmm_rho_resident_job *evals, size_t n);
 int cfgmm_rho_resident_bytes(const cfgmm_rho_resident *h, size_t *bytes);
 /* Frees the device memory; no evaluation of the handle may still be running. */
 int cfgmm_rho_resident_destroy(cfgmm_rho_resident *h);

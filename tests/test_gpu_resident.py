@@ -190,3 +190,34 @@ def test_resident_launch_tunables_bitwise(depth, cols):
         cfgmm.tune_set(cfgmm.TUNE_RESIDENT_DEPTH, old[0])
         cfgmm.tune_set(cfgmm.TUNE_RESIDENT_COLS, old[1])
     assert np.array_equal(got, ref)
+
+
+def test_resident_eval_batched_bitwise():
+    """One batched launch over several handles (different sizes, leading ranges, strides, slices)
+    equals separate evaluations bitwise; empty entries are no-ops."""
+    probs = [W.rho_problem(seed=30 + i, nshell=90 + 40 * i, nao=270 + 120 * i, ngrid=64 * (12 + 7 * i), nset=1 + i % 3,
+                           density=0.25 + 0.1 * i) for i in range(3)]
+    rs = [Resident(p, ns=6 if i != 1 else 5) for i, p in enumerate(probs)]
+    jobs = [(0, probs[0].ngrid, probs[0].nset, probs[0].ngrid), (1, 64 * 5, 1, 64 * 5 + 3), (2, probs[2].ngrid, 2, probs[2].ngrid),
+            (0, 0, 1, 64), (2, 64 * 3, probs[2].nset, 64 * 3)]
+    refs, bufs, entries = [], [], []
+    for k, (i, ng, ns, stride) in enumerate(jobs):
+        r = rs[i]
+        if ng:
+            refs.append(r.eval(rhos0=np.zeros(r.p.nset * stride) + k, stride=stride, ngrid=ng, nset=ns))
+        else:
+            refs.append(np.zeros(r.p.nset * stride) + k)
+        buf = cfgmm.DeviceBuffer.from_array(np.zeros(r.p.nset * stride) + k)
+        bufs.append(buf)
+        entries.append((r.h, buf, stride, ng, ns))
+    s = cfgmm.Stream()
+    cfgmm.rho_resident_eval_batched(s, entries)
+    s.synchronize()
+    outs = [b.to_array(np.float64, ref.shape) for b, ref in zip(bufs, refs)]
+    for b in bufs:
+        b.free()
+    for r in rs:
+        r.close()
+    s.destroy()
+    for o, ref in zip(outs, refs):
+        assert np.array_equal(o, ref)
