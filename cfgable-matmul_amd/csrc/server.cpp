@@ -8,8 +8,8 @@
 // tiles) and in-flight requests.  Molecules that no in-flight request uses are
 // evicted least-recently-used first when memory is short.  At every dispatch
 // the pending FP64 rho requests are fused into one batched persistent launch
-// (up to max_batch of them); split-precision rho requests queued together
-// run over their molecules' resident tiles in one batched launch.  Work on different slots overlaps on the device.  Each result is
+// (up to max_batch of them); split-precision rho requests run over their
+// molecules' resident tiles, one evaluation each, several per slot.  Work on different slots overlaps on the device.  Each result is
 // hashed on the device (cfgmm checksum: order-independent over the result
 // bits), so a caller can compare a request's result alone and in the mix
 // exactly; full results are copied back on request after the run.
@@ -408,7 +408,6 @@ int server_run(Server *sv, const cfgmm_request *reqs, size_t n, cfgmm_request_re
                 sv->stats.gemm_requests++;
             } else {
                 std::vector<cfgmm_rho_problem> probs;
-                std::vector<ResidentEvalReq> evals;
                 std::vector<double *> outs;
                 for (size_t r : batch) {
                     const cfgmm_request &q = reqs[r];
@@ -446,10 +445,12 @@ int server_run(Server *sv, const cfgmm_request *reqs, size_t n, cfgmm_request_re
                         p.blksize = m.blksize;
                         probs.push_back(p);
                     } else {
-                        evals.push_back(ResidentEvalReq{m.tiles, o, sv->rho_ld(q), q.ngrid, q.nset});
+                        // one evaluation per request: measured faster on the serving mix than one
+                        // fused launch per dispatch (work of different slots overlaps instead)
+                        st = resident_eval(s, m.tiles, o, sv->rho_ld(q), q.ngrid, q.nset);
+                        if (st != CFGMM_OK) break;
                     }
                 }
-                if (st == CFGMM_OK && !evals.empty()) st = resident_eval_batched(s, evals.data(), evals.size());
                 if (st == CFGMM_OK && !probs.empty()) {
                     st = probs.size() == 1 ? launch_rho(s, const_cast<double *>(probs[0].rhos), probs[0].dms,
                                                         probs[0].ao_lhs, probs[0].ao_rhs, probs[0].stride_ao,
