@@ -280,7 +280,7 @@ int cfgmm_server_destroy(cfgmm_server *s);
 /* ---- tunables (set by the autotuner; process-wide) ------------------------ */
 enum cfgmm_tunable {
     CFGMM_TUNE_BATCH_WGS_PER_CU = 0, /* batched rho: persistent workgroups per CU, 1..8 (default 2) */
-    CFGMM_TUNE_BATCH_MAX_SETS = 1,   /* batched rho: density matrices per task, 1, 2 or 4 (default 2) */
+    CFGMM_TUNE_BATCH_MAX_SETS = 1,   /* batched rho: density matrices per task, 1, 2 or 4 (default 1) */
     CFGMM_TUNE_DGEMM_GROUP_M = 2,    /* DGEMM: tile rows per raster group, 1..64 (default 4) */
     CFGMM_TUNE_RHO_MAX_SETS = 3,     /* FP64 rho: density matrices per workgroup, 1..4 (default 1) */
     CFGMM_TUNE_DGEMM_SMALL_TILE = 4, /* DGEMM with fewer 128x128 tiles than slots: tile edge 64 or 128, 0 = heuristic */
