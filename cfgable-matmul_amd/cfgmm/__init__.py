@@ -64,6 +64,7 @@ _SIGNATURES = {
         [_VP, _SZ, _VP, _SZ, _VP, _SZ] + [_SZ] * 6 + [_I, _I, _VP, _SZ, _SZ, _SZ], _I),
     "cfgmm_matmul_anyway_full_non0tab": ([_VP, _SZ, _VP, _SZ, _VP, _SZ] + [_SZ] * 6 + [_I, _I, _VP, _SZ, _SZ], _I),
     "cfgmm_dgemm_non0tab_dev": ([_VP, _VP, _VP, _VP] + [_SZ] * 6 + [_I, _I, _VP, _SZ, _SZ], _I),
+    "cfgmm_dgemm_non0tab_ex_dev": ([_VP, _VP, _VP, _VP] + [_SZ] * 6 + [_I, _I, _VP, _SZ, _SZ], _I),
     "cfgmm_aodmao2rho_loop_parallel": ([_VP, _SZ, _VP, _SZ, _VP, _SZ, _VP, _SZ] + [_SZ] * 5 + [_VP, _SZ, _SZ, _SZ],
                                        _I),
     "cfgmm_aodmao2rho_anyway": ([_VP, _SZ, _VP, _SZ, _VP, _SZ, _VP, _SZ] + [_SZ] * 5 + [_VP, _SZ, _SZ], _I),
@@ -517,9 +518,13 @@ def dgemm_dev(stream, c: DeviceBuffer, a: DeviceBuffer, b: DeviceBuffer, m, n, k
                                        int(transb), precision, nslices))
 
 
-def dgemm_non0tab_dev(stream, c, a, b, m, n, k, lda, ldb, ldc, transa, transb, tab: DeviceBuffer, ldtab, blksize):
-    check(lib().cfgmm_dgemm_non0tab_dev(_h(stream), c.ptr, a.ptr, b.ptr, m, n, k, lda, ldb, ldc, int(transa),
-                                        int(transb), tab.ptr, ldtab, blksize))
+def dgemm_non0tab_dev(stream, c, a, b, m, n, k, lda, ldb, ldc, transa, transb, tab: DeviceBuffer, ldtab, blksize,
+                      allow_transb=False):
+    """Masked GEMM on device buffers; allow_transb selects cfgmm_dgemm_non0tab_ex_dev, which takes transb
+    (the reference-shaped entry point rejects it, as the reference does)."""
+    fn = lib().cfgmm_dgemm_non0tab_ex_dev if allow_transb else lib().cfgmm_dgemm_non0tab_dev
+    check(fn(_h(stream), c.ptr, a.ptr, b.ptr, m, n, k, lda, ldb, ldc, int(transa), int(transb), tab.ptr, ldtab,
+             blksize))
 
 
 def aodmao2rho_dev(stream, rhos, dms, ao_lhs, ao_rhs, stride_ao, stride_grid, nao, ngrid, nset, tab, ldtab, blksize,

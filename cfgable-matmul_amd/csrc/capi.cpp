@@ -327,8 +327,8 @@ static int gemm_host(double *c, size_t c_len, const double *a, size_t a_len, con
     if (masked) {
         uint8_t *dt;
         CFGMM_TRY(stage_in(3, tab, tab_ext, &dt));
-        CFGMM_TRY(launch_dgemm_non0tab(t_stage.stream, dc, da, db, m, n, k, lda, ldb, ldc, transa != 0, dt, ldtab,
-                                       blksize));
+        CFGMM_TRY(launch_dgemm_non0tab(t_stage.stream, dc, da, db, m, n, k, lda, ldb, ldc, transa != 0, false, dt,
+                                       ldtab, blksize));
     } else {
         CFGMM_TRY(launch_dgemm(t_stage.stream, dc, da, db, m, n, k, lda, ldb, ldc, transa != 0, transb != 0));
     }
@@ -441,6 +441,19 @@ int cfgmm_matmul_anyway_full_non0tab(double *c, size_t c_len, const double *a, s
     return gemm_host(c, c_len, a, a_len, b, b_len, m, n, k, lda, ldb, ldc, transa, transb, tab, tab_len, ldtab, 48);
 }
 
+static int masked_dev_common(void *stream, double *c, const double *a, const double *b, size_t m, size_t n, size_t k,
+                             size_t lda, size_t ldb, size_t ldc, int transa, int transb, const uint8_t *tab,
+                             size_t ldtab, size_t blksize) {
+    CFGMM_TRY(check_blksize(blksize));
+    CFGMM_TRY(check_dev_gemm(c, a, b, m, n, k, lda, ldb, ldc, transa, transb));
+    if (m && n && k) {
+        CFGMM_TRY(check_ld("ldtab", ldtab, div_up(n, blksize), true));
+        if (!tab) return fail(CFGMM_EINVAL, "tab is null");
+    }
+    return launch_dgemm_non0tab(static_cast<hipStream_t>(stream), c, a, b, m, n, k, lda, ldb, ldc, transa != 0,
+                                transb != 0, tab, ldtab, blksize);
+}
+
 int cfgmm_dgemm_non0tab_dev(void *stream, double *c, const double *a, const double *b, size_t m, size_t n, size_t k,
                             size_t lda, size_t ldb, size_t ldc, int transa, int transb, const uint8_t *tab,
                             size_t ldtab, size_t blksize) {
@@ -448,14 +461,14 @@ int cfgmm_dgemm_non0tab_dev(void *stream, double *c, const double *a, const doub
     if (transb)
         return fail(CFGMM_EINVAL_TRANSB_MASK,
                     "transb=true is not implemented in matmul_loop_parallel_mnk_pack_a_non0tab");
-    CFGMM_TRY(check_blksize(blksize));
-    CFGMM_TRY(check_dev_gemm(c, a, b, m, n, k, lda, ldb, ldc, transa, 0));
-    if (m && n && k) {
-        CFGMM_TRY(check_ld("ldtab", ldtab, div_up(n, blksize), true));
-        if (!tab) return fail(CFGMM_EINVAL, "tab is null");
-    }
-    return launch_dgemm_non0tab(static_cast<hipStream_t>(stream), c, a, b, m, n, k, lda, ldb, ldc, transa != 0, tab,
-                                ldtab, blksize);
+    return masked_dev_common(stream, c, a, b, m, n, k, lda, ldb, ldc, transa, 0, tab, ldtab, blksize);
+}
+
+int cfgmm_dgemm_non0tab_ex_dev(void *stream, double *c, const double *a, const double *b, size_t m, size_t n,
+                               size_t k, size_t lda, size_t ldb, size_t ldc, int transa, int transb,
+                               const uint8_t *tab, size_t ldtab, size_t blksize) {
+    clear_error();
+    return masked_dev_common(stream, c, a, b, m, n, k, lda, ldb, ldc, transa, transb, tab, ldtab, blksize);
 }
 
 // ---- rho ------------------------------------------------------------------------

@@ -13,7 +13,10 @@
 // gathered op(A) columns and gathered B rows are staged through
 // double-buffered LDS and accumulated with v_mfma_f64_16x16x4_f64.  A tile
 // whose K list is empty is skipped outright (the reference's `count == 0`
-// skip, 200-202).  transb is not implemented, as in the reference (228-230).
+// skip, 200-202).  The reference leaves transb unimplemented (228-230), and
+// so do the drop-in entry points; the additive cfgmm_dgemm_non0tab_ex_dev takes
+// it: B(p, j) = b[j*ldb + p], loaded with p fastest so that a column's listed
+// entries are read along its row of b.
 #include "internal.hpp"
 
 namespace cfgmm {
@@ -28,7 +31,7 @@ constexpr int kBM = 128;  // rows per workgroup (4 waves x 32)
 constexpr int kBK = 16;   // compacted-K slice depth
 constexpr int kSeg = 2048;  // compacted-K list entries staged in LDS at a time
 
-template <int NT, bool TA>
+template <int NT, bool TA, bool TB>
 __global__ __launch_bounds__(kThreads, 2) void masked_gemm_kernel(double *__restrict__ c, const double *__restrict__ a,
                                                                   const double *__restrict__ b, int m, int n,
                                                                   long lda, long ldb, long ldc, int kmax,
@@ -90,9 +93,10 @@ __global__ __launch_bounds__(kThreads, 2) void masked_gemm_kernel(double *__rest
 #pragma unroll
             for (int e = 0; e < B_PER_T; ++e) {
                 const int idx = tid + kThreads * e;
-                const int pp = idx / W, gj = n0 + idx % W;
+                const int pp = TB ? idx % kBK : idx / W, gj = n0 + (TB ? idx / kBK : idx % W);
                 const bool ok = gj < n && k0 + pp < segn;
-                const double *src = ok ? b + (long)vls[ok ? k0 + pp : 0] * ldb + gj : kZerosM;
+                const long p = vls[ok ? k0 + pp : 0];
+                const double *src = ok ? (TB ? b + (long)gj * ldb + p : b + p * ldb + gj) : kZerosM;
                 rb[e] = *src;
             }
         };
@@ -107,7 +111,10 @@ __global__ __launch_bounds__(kThreads, 2) void masked_gemm_kernel(double *__rest
 #pragma unroll
             for (int e = 0; e < B_PER_T; ++e) {
                 const int idx = tid + kThreads * e;
-                Bs[buf][idx / W][idx % W] = rb[e];
+                if (TB)
+                    Bs[buf][idx % kBK][idx / kBK] = rb[e];
+                else
+                    Bs[buf][idx / W][idx % W] = rb[e];
             }
         };
 
@@ -149,15 +156,22 @@ __global__ __launch_bounds__(kThreads, 2) void masked_gemm_kernel(double *__rest
 }
 
 template <int NT>
-int launch_nt(bool ta, dim3 grid, hipStream_t stream, double *c, const double *a, const double *b, int m, int n,
-              long lda, long ldb, long ldc, int kmax, const int *lists, const int *counts, int blksize, int nsub) {
+int launch_nt(bool ta, bool tb, dim3 grid, hipStream_t stream, double *c, const double *a, const double *b, int m,
+              int n, long lda, long ldb, long ldc, int kmax, const int *lists, const int *counts, int blksize,
+              int nsub) {
     hipEvent_t t0 = trace_start(stream);
-    if (ta)
-        hipLaunchKernelGGL((masked_gemm_kernel<NT, true>), grid, dim3(kThreads), 0, stream, c, a, b, m, n, lda, ldb,
-                           ldc, kmax, lists, counts, blksize, nsub);
+#define CFGMM_MASKED(TA_, TB_)                                                                                    \
+    hipLaunchKernelGGL((masked_gemm_kernel<NT, TA_, TB_>), grid, dim3(kThreads), 0, stream, c, a, b, m, n, lda, ldb, \
+                       ldc, kmax, lists, counts, blksize, nsub)
+    if (ta && tb)
+        CFGMM_MASKED(true, true);
+    else if (ta)
+        CFGMM_MASKED(true, false);
+    else if (tb)
+        CFGMM_MASKED(false, true);
     else
-        hipLaunchKernelGGL((masked_gemm_kernel<NT, false>), grid, dim3(kThreads), 0, stream, c, a, b, m, n, lda, ldb,
-                           ldc, kmax, lists, counts, blksize, nsub);
+        CFGMM_MASKED(false, false);
+#undef CFGMM_MASKED
     trace_stop(stream, "masked_gemm_kernel", t0);
     return hip_status(hipGetLastError(), "masked_gemm_kernel launch");
 }
@@ -165,8 +179,8 @@ int launch_nt(bool ta, dim3 grid, hipStream_t stream, double *c, const double *a
 }  // namespace
 
 int launch_dgemm_non0tab(hipStream_t stream, double *c, const double *a, const double *b, size_t m, size_t n,
-                         size_t k, size_t lda, size_t ldb, size_t ldc, bool transa, const uint8_t *tab, size_t ldtab,
-                         size_t blksize) {
+                         size_t k, size_t lda, size_t ldb, size_t ldc, bool transa, bool transb, const uint8_t *tab,
+                         size_t ldtab, size_t blksize) {
     if (m == 0 || n == 0 || k == 0) return CFGMM_OK;
     if (m > INT32_MAX || n > INT32_MAX || k > INT32_MAX) return fail(CFGMM_EINVAL, "masked dgemm: size out of range");
     const size_t nblk = div_up(n, blksize);
@@ -189,10 +203,10 @@ int launch_dgemm_non0tab(hipStream_t stream, double *c, const double *a, const d
     dim3 grid((unsigned)(nblk * nsub), (unsigned)mtiles, 1);
     const int im = (int)m, in = (int)n, ik = (int)k, ib = (int)blksize;
     switch (nt) {
-        case 1: return launch_nt<1>(transa, grid, stream, c, a, b, im, in, (long)lda, (long)ldb, (long)ldc, ik, lists, counts, ib, nsub);
-        case 2: return launch_nt<2>(transa, grid, stream, c, a, b, im, in, (long)lda, (long)ldb, (long)ldc, ik, lists, counts, ib, nsub);
-        case 3: return launch_nt<3>(transa, grid, stream, c, a, b, im, in, (long)lda, (long)ldb, (long)ldc, ik, lists, counts, ib, nsub);
-        default: return launch_nt<4>(transa, grid, stream, c, a, b, im, in, (long)lda, (long)ldb, (long)ldc, ik, lists, counts, ib, nsub);
+        case 1: return launch_nt<1>(transa, transb, grid, stream, c, a, b, im, in, (long)lda, (long)ldb, (long)ldc, ik, lists, counts, ib, nsub);
+        case 2: return launch_nt<2>(transa, transb, grid, stream, c, a, b, im, in, (long)lda, (long)ldb, (long)ldc, ik, lists, counts, ib, nsub);
+        case 3: return launch_nt<3>(transa, transb, grid, stream, c, a, b, im, in, (long)lda, (long)ldb, (long)ldc, ik, lists, counts, ib, nsub);
+        default: return launch_nt<4>(transa, transb, grid, stream, c, a, b, im, in, (long)lda, (long)ldb, (long)ldc, ik, lists, counts, ib, nsub);
     }
 }
 

@@ -123,6 +123,14 @@ int cfgmm_dgemm_non0tab_dev(void *stream, double *c, const double *a, const doub
                             size_t lda, size_t ldb, size_t ldc, int transa, int transb, const uint8_t *tab,
                             size_t ldtab, size_t blksize);
 
+/* Masked GEMM with transb allowed (no counterpart in the reference, which leaves
+ * transb unimplemented, src/impl_matmul_non0tab.rs:181-183,228-230):
+ *   C(i,j) += sum_{p : tab[p*ldtab + j/blksize]} op(A)(i,p) * op(B)(p,j),
+ * op(B)(p,j) = transb ? b[j*ldb + p] : b[p*ldb + j]. */
+int cfgmm_dgemm_non0tab_ex_dev(void *stream, double *c, const double *a, const double *b, size_t m, size_t n,
+                               size_t k, size_t lda, size_t ldb, size_t ldc, int transa, int transb,
+                               const uint8_t *tab, size_t ldtab, size_t blksize);
+
 /* ======================================================================== */
 /* 2b. fused masked AO-DM-AO -> rho                                         */
 /* ======================================================================== */

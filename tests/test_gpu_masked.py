@@ -53,3 +53,39 @@ def test_masked_gemm_empty_mask_leaves_c():
     tab = np.zeros((k, 2), np.uint8)
     cfgmm.MatmulLoops().matmul_loop_parallel_mnk_pack_a_non0tab(64, c, a, b, m, n, k, k, n, n, False, False, tab, 2)
     assert np.array_equal(c, c0)
+
+
+@pytest.mark.parametrize("blksize", [16, 48, 64])
+@pytest.mark.parametrize("ta", [False, True])
+def test_masked_gemm_transb_ex(blksize, ta):
+    """cfgmm_dgemm_non0tab_ex_dev with transb: checked against the oracle on an explicitly transposed
+    copy of B (op(B)(p, j) = b[j*ldb + p]); padded leading dimensions, a partial last column block,
+    nonzero B under off bits."""
+    rng = np.random.default_rng(100 + blksize + ta)
+    m, n, k = 211, 700, 389
+    lda = (m if ta else k) + 3
+    ldb = k + 5
+    ldc = n + 2
+    a = rng.uniform(-1, 1, size=(k if ta else m) * lda)
+    b = rng.uniform(-1, 1, size=n * ldb)
+    nblk = (n + blksize - 1) // blksize
+    tab = W.periodic_mask(k, nblk + 1, W.MASK_30)
+    c = rng.uniform(-1, 1, size=m * ldc)
+    bt = np.ascontiguousarray(b.reshape(n, ldb)[:, :k].T).ravel()  # k x n, row-major
+    c_ref = c.copy()
+    assert oracle.matmul_non0tab(c_ref, a, bt, m, n, k, lda, n, ldc, ta, False, tab, nblk + 1, blksize) == 0
+    s = cfgmm.Stream()
+    dA, dB, dC, dT = (cfgmm.DeviceBuffer.from_array(np.ascontiguousarray(x)) for x in (a, b, c, tab))
+    cfgmm.dgemm_non0tab_dev(s, dC, dA, dB, m, n, k, lda, ldb, ldc, ta, True, dT, nblk + 1, blksize,
+                            allow_transb=True)
+    s.synchronize()
+    got = dC.to_array(np.float64, (m * ldc,))
+    with pytest.raises(cfgmm.CfgmmError) as e:  # the reference-shaped entry point still rejects transb
+        cfgmm.dgemm_non0tab_dev(s, dC, dA, dB, m, n, k, lda, ldb, ldc, ta, True, dT, nblk + 1, blksize)
+    assert e.value.status == cfgmm.EINVAL_TRANSB_MASK
+    for x in (dA, dB, dC, dT):
+        x.free()
+    s.destroy()
+    G, R = got.reshape(m, ldc), c_ref.reshape(m, ldc)
+    assert rel_fro(G[:, :n], R[:, :n]) <= TOL
+    assert np.array_equal(G[:, n:], c.reshape(m, ldc)[:, n:])
