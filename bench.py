@@ -18,6 +18,8 @@ The same JSON line carries:
     1031x977x1543 NT and 3527x9583x6581 NN shapes (dgemm_kernel hipEvent time);
   * split6_resident: the same evaluation in split precision (6 int8 slices) over
     resident per-block operands, with its error vs the oracle;
+  * rho_4_components: four left-hand factors (GGA-shaped: phi and three gradient-like
+    arrays) over one D.phi pass vs four separate evaluations;
   * host_pointer_path: the reference-shaped host-buffer call, PCIe included.
 
 Other workloads (one JSON line each):
@@ -160,6 +162,49 @@ def bench_resident(p: W.RhoProblem, steps: int, nslices: int = 6):
     e1.destroy()
     s.destroy()
     return dict(ms=ms, kernel_ms=km / max(kn, 1), create_ms=create_ms, bytes=nbytes, rho=rho)
+
+
+def bench_multi(p: W.RhoProblem, ncomp: int = 4, reps: int = 20):
+    """ncomp left-hand components over one D.rhs (cfgmm_aodmao2rho_multi_dev) vs ncomp separate evaluations."""
+    rng = np.random.default_rng(5)
+    lhs = np.empty((ncomp, p.nao, p.ngrid))
+    lhs[0] = p.phi
+    for c in range(1, ncomp):
+        lhs[c] = p.phi * rng.uniform(-2.0, 2.0, size=p.phi.shape)
+    s = cfgmm.Stream()
+    d_d, d_l = cfgmm.DeviceBuffer.from_array(p.dms), cfgmm.DeviceBuffer.from_array(lhs)
+    d_rhs = cfgmm.DeviceBuffer.from_array(p.phi)
+    d_t = cfgmm.DeviceBuffer.from_array(np.ascontiguousarray(p.tab))
+    d_r = cfgmm.DeviceBuffer(8 * ncomp * p.nset * p.ngrid)
+    d_r.zero()
+    per = p.nao * p.ngrid
+
+    def fused():
+        cfgmm.aodmao2rho_multi_dev(s, d_r, p.nset * p.ngrid, d_d, d_l, per, ncomp, d_rhs, p.nao, p.ngrid, p.nao,
+                                   p.ngrid, p.nset, d_t, p.nblk, p.blksize)
+
+    def separate():
+        for c in range(ncomp):
+            cfgmm.check(cfgmm.lib().cfgmm_aodmao2rho_dev(s.handle, d_r.ptr + 8 * c * p.nset * p.ngrid, d_d.ptr,
+                                                         d_l.ptr + 8 * c * per, d_rhs.ptr, p.nao, p.ngrid, p.nao,
+                                                         p.ngrid, p.nset, d_t.ptr, p.nblk, p.blksize))
+
+    def timed(fn):
+        fn()
+        s.synchronize()
+        e0, e1 = cfgmm.Event(), cfgmm.Event()
+        e0.record(s)
+        for _ in range(reps):
+            fn()
+        e1.record(s)
+        s.synchronize()
+        return e0.elapsed_ms(e1) / reps
+    t_f, t_s = timed(fused), timed(separate)
+    for b in (d_d, d_l, d_rhs, d_t, d_r):
+        b.free()
+    s.destroy()
+    return {"components": ncomp, "fused_ms": round(t_f, 4), "separate_ms": round(t_s, 4),
+            "speedup": round(t_s / t_f, 2)}
 
 
 def bench_dgemm(g: W.GemmProblem, reps: int, warmup: int = 2):
@@ -656,6 +701,10 @@ def main():
                  "resident_GiB": round(rs["bytes"] / 2 ** 30, 3), "speedup_vs_fp64": round(ms_per_step / rs["ms"], 2),
                  "rel_err_vs_fp64_gpu": rel_fro(rs["rho"], r["rho_once"])}
 
+    # GGA-shaped evaluation: 4 left-hand components (phi and 3 gradient-like factors) sharing D.phi
+    log("[bench] 4-component (GGA-shaped) rho: one fused pass vs 4 evaluations")
+    out_gga = bench_multi(p, ncomp=4)
+
     # the host-pointer drop-in (reference call shape): PCIe transfers included, never `value`
     log("[bench] host-pointer drop-in path (H2D of phi, D and mask, D2H of rho)")
     hr = np.zeros(p.nset * p.ngrid)
@@ -683,6 +732,7 @@ def main():
         out["rel_err"] = None
 
     out["split6_resident"] = out_split
+    out["rho_4_components"] = out_gga
 
     if not args.no_dgemm:
         dg = {}

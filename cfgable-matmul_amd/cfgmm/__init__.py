@@ -70,6 +70,7 @@ _SIGNATURES = {
     "cfgmm_aodmao2rho_anyway": ([_VP, _SZ, _VP, _SZ, _VP, _SZ, _VP, _SZ] + [_SZ] * 5 + [_VP, _SZ, _SZ], _I),
     "cfgmm_aodmao2rho_dev": ([_VP, _VP, _VP, _VP, _VP] + [_SZ] * 5 + [_VP, _SZ, _SZ], _I),
     "cfgmm_aodmao2rho_ex_dev": ([_VP, _VP, _VP, _VP, _VP] + [_SZ] * 5 + [_VP, _SZ, _SZ, _I, _I], _I),
+    "cfgmm_aodmao2rho_multi_dev": ([_VP, _VP, _SZ, _VP, _VP, _SZ, _SZ, _VP] + [_SZ] * 5 + [_VP, _SZ, _SZ], _I),
     "cfgmm_aodmao2rho_batched_dev": ([_VP, _VP, _SZ, _I, _I], _I),
     "cfgmm_rho_resident_create": ([_VP, _VP, _VP, _VP] + [_SZ] * 5 + [_VP, _SZ, _SZ, _I, _I, ctypes.POINTER(_VP)], _I),
     "cfgmm_rho_resident_eval": ([_VP, _VP, _VP, _SZ, _SZ, _SZ], _I),
@@ -360,6 +361,14 @@ def trace_read(name: str) -> tuple[float, int]:
     ms, n = ctypes.c_double(0.0), ctypes.c_longlong(0)
     check(lib().cfgmm_trace_read(name.encode(), ctypes.byref(ms), ctypes.byref(n)))
     return ms.value, n.value
+
+
+def aodmao2rho_multi_dev(stream, rhos, rho_comp_stride, dms, ao_lhs, lhs_comp_stride, ncomp, ao_rhs, stride_ao,
+                         stride_grid, nao, ngrid, nset, tab, ldtab, blksize):
+    """ncomp left-hand components (ao_lhs + c*lhs_comp_stride) sharing D, rhs and the mask."""
+    check(lib().cfgmm_aodmao2rho_multi_dev(_h(stream), _dev(rhos), rho_comp_stride, _dev(dms), _dev(ao_lhs),
+                                           lhs_comp_stride, ncomp, _dev(ao_rhs), stride_ao, stride_grid, nao, ngrid,
+                                           nset, _dev(tab), ldtab, blksize))
 
 
 class RhoProblem(ctypes.Structure):

@@ -515,6 +515,23 @@ int cfgmm_aodmao2rho_dev(void *stream, double *rhos, const double *dms, const do
                           blksize, CFGMM_PREC_FP64, 0);
 }
 
+int cfgmm_aodmao2rho_multi_dev(void *stream, double *rhos, size_t rho_comp_stride, const double *dms,
+                               const double *ao_lhs, size_t lhs_comp_stride, size_t ncomp, const double *ao_rhs,
+                               size_t stride_ao, size_t stride_grid, size_t nao, size_t ngrid, size_t nset,
+                               const uint8_t *tab, size_t ldtab, size_t blksize) {
+    clear_error();
+    RhoExtents ext;
+    CFGMM_TRY(rho_extents(stride_ao, stride_grid, nao, ngrid, nset, ldtab, blksize, &ext));
+    if (ncomp > 16) return fail(CFGMM_EINVAL, "at most 16 left-hand components");
+    const bool any = nao && ngrid && nset && ncomp;
+    if (any && (!rhos || !dms || !ao_lhs || !ao_rhs || !tab)) return fail(CFGMM_EINVAL, "null device pointer");
+    if (any && ncomp > 1 && rho_comp_stride < (nset - 1) * stride_grid + ngrid)
+        return fail(CFGMM_EINVAL_DIMS, "rho_comp_stride: the components' rho ranges overlap");
+    CFGMM_TRY(check_device());
+    return launch_rho_multi(static_cast<hipStream_t>(stream), rhos, rho_comp_stride, dms, ao_lhs, lhs_comp_stride,
+                            ncomp, ao_rhs, stride_ao, stride_grid, nao, ngrid, nset, tab, ldtab, blksize);
+}
+
 int cfgmm_aodmao2rho_batched_dev(void *stream, const cfgmm_rho_problem *problems, size_t nproblems, int precision,
                                  int nslices) {
     clear_error();

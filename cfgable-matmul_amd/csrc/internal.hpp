@@ -68,6 +68,9 @@ int launch_dgemm_non0tab(hipStream_t stream, double *c, const double *a, const d
 int launch_rho(hipStream_t stream, double *rhos, const double *dms, const double *ao_lhs, const double *ao_rhs,
                size_t stride_ao, size_t stride_grid, size_t nao, size_t ngrid, size_t nset, const uint8_t *tab,
                size_t ldtab, size_t blksize);
+int launch_rho_multi(hipStream_t stream, double *rhos, size_t rho_cs, const double *dms, const double *ao_lhs,
+                     size_t lhs_cs, size_t ncomp, const double *ao_rhs, size_t stride_ao, size_t stride_grid,
+                     size_t nao, size_t ngrid, size_t nset, const uint8_t *tab, size_t ldtab, size_t blksize);
 int launch_dgemm_ozaki(hipStream_t stream, double *c, const double *a, const double *b, size_t m, size_t n, size_t k,
                        size_t lda, size_t ldb, size_t ldc, bool transa, bool transb, int nslices);
 int launch_rho_batched(hipStream_t stream, const cfgmm_rho_problem *problems, size_t nprob);

@@ -98,7 +98,7 @@ __global__ __launch_bounds__(kThreads, 2) void rho_chunk_kernel(
     double *__restrict__ partial, const double *__restrict__ dms, const double *__restrict__ ao_lhs,
     const double *__restrict__ ao_rhs, long stride_ao, long stride_grid, int nao, int ngrid, int nset, int set0,
     const int *__restrict__ lists, const int *__restrict__ counts, int blksize, int nsub, int maxchunks,
-    int ngroups) {
+    int ngroups, int ncomp, long lhs_comp_stride, long part_comp_stride) {
     __shared__ __attribute__((aligned(16))) unsigned char smem[rho::chunk_lds_bytes<NS, NT>()];
     // Workgroup i of the 1-D grid takes task L = start_x + i/8 (x = i%8) of the block-major order
     // (block, subtile, group of NS density matrices, chunk): the tasks of one block, which share
@@ -125,18 +125,22 @@ __global__ __launch_bounds__(kThreads, 2) void rho_chunk_kernel(
     t.nset = nset;
     t.set0 = set0;
     t.list = lists + (long)blk * nao;
+    t.ncomp = ncomp;
+    t.lhs_comp_stride = lhs_comp_stride;
+    t.part_comp_stride = part_comp_stride;
     rho::rho_chunk_body<NS, NT>(t, smem);
 }
 
 template <int NS>
 int launch_ns(int nt, dim3 grid, hipStream_t stream, double *partial, const double *dms, const double *ao_lhs,
               const double *ao_rhs, long stride_ao, long stride_grid, int nao, int ngrid, int nset, int set0,
-              const int *lists, const int *counts, int blksize, int nsub, int maxchunks, int ngroups) {
+              const int *lists, const int *counts, int blksize, int nsub, int maxchunks, int ngroups, int ncomp,
+              long lhs_cs, long part_cs) {
     hipEvent_t t0 = trace_start(stream);
 #define CFGMM_RHO_LAUNCH(NT_)                                                                                      \
     hipLaunchKernelGGL((rho_chunk_kernel<NS, NT_>), grid, dim3(kThreads), 0, stream, partial, dms, ao_lhs, ao_rhs, \
                        stride_ao, stride_grid, nao, ngrid, nset, set0, lists, counts, blksize, nsub, maxchunks, \
-                       ngroups)
+                       ngroups, ncomp, lhs_cs, part_cs)
     switch (nt) {
         case 1: CFGMM_RHO_LAUNCH(1); break;
         case 2: CFGMM_RHO_LAUNCH(2); break;
@@ -162,18 +166,28 @@ static int rho_tile_nt(size_t blksize) {
 int launch_rho(hipStream_t stream, double *rhos, const double *dms, const double *ao_lhs, const double *ao_rhs,
                size_t stride_ao, size_t stride_grid, size_t nao, size_t ngrid, size_t nset, const uint8_t *tab,
                size_t ldtab, size_t blksize) {
-    if (nao == 0 || ngrid == 0 || nset == 0) return CFGMM_OK;
+    return launch_rho_multi(stream, rhos, 0, dms, ao_lhs, 0, 1, ao_rhs, stride_ao, stride_grid, nao, ngrid, nset, tab,
+                            ldtab, blksize);
+}
+
+// ncomp evaluations sharing D, rhs and the mask: rho_c = rhos + c*rho_cs, lhs_c = ao_lhs + c*lhs_cs.
+int launch_rho_multi(hipStream_t stream, double *rhos, size_t rho_cs, const double *dms, const double *ao_lhs,
+                     size_t lhs_cs, size_t ncomp, const double *ao_rhs, size_t stride_ao, size_t stride_grid,
+                     size_t nao, size_t ngrid, size_t nset, const uint8_t *tab, size_t ldtab, size_t blksize) {
+    if (nao == 0 || ngrid == 0 || nset == 0 || ncomp == 0) return CFGMM_OK;
+    if (ncomp > 16) return fail(CFGMM_EINVAL, "rho: at most 16 left-hand components");
     if (nao > INT32_MAX || ngrid > INT32_MAX || nset > 65535) return fail(CFGMM_EINVAL, "rho: size out of range");
     const size_t nblk = div_up(ngrid, blksize);
     const size_t maxchunks = div_up(nao, kUC);
     const int nt = rho_tile_nt(blksize);
     const int nsub = (int)(blksize / (16 * nt));
 
-    // workspace: [lists nblk*nao int][counts nblk int][pad][partial maxchunks*nset*ngrid double]
+    // workspace: [lists nblk*nao int][counts nblk int][pad][partial ncomp*maxchunks*nset*ngrid double]
     const size_t lists_bytes = sizeof(int) * nblk * nao;
     const size_t counts_bytes = sizeof(int) * nblk;
     const size_t part_off = div_up(lists_bytes + counts_bytes, 256) * 256;
-    const size_t part_bytes = sizeof(double) * maxchunks * nset * ngrid;
+    const size_t part_cs = maxchunks * nset * ngrid;  // doubles per component
+    const size_t part_bytes = sizeof(double) * part_cs * ncomp;
     void *ws = nullptr;
     int st = workspace_get(stream, part_off + part_bytes, &ws);
     if (st != CFGMM_OK) return st;
@@ -194,10 +208,10 @@ int launch_rho(hipStream_t stream, double *rhos, const double *dms, const double
             return fail(CFGMM_EINVAL, "rho: too many workgroups for one launch");
         const dim3 grid((unsigned)(nblk * nsub * ngroups * maxchunks), 1, 1);
         switch (ns) {
-            case 1: return launch_ns<1>(nt, grid, stream, partial, dms, ao_lhs, ao_rhs, sa, sg, (int)nao, (int)ngrid, (int)nset, (int)set0, lists, counts, (int)blksize, nsub, (int)maxchunks, (int)ngroups);
-            case 2: return launch_ns<2>(nt, grid, stream, partial, dms, ao_lhs, ao_rhs, sa, sg, (int)nao, (int)ngrid, (int)nset, (int)set0, lists, counts, (int)blksize, nsub, (int)maxchunks, (int)ngroups);
-            case 3: return launch_ns<3>(nt, grid, stream, partial, dms, ao_lhs, ao_rhs, sa, sg, (int)nao, (int)ngrid, (int)nset, (int)set0, lists, counts, (int)blksize, nsub, (int)maxchunks, (int)ngroups);
-            default: return launch_ns<4>(nt, grid, stream, partial, dms, ao_lhs, ao_rhs, sa, sg, (int)nao, (int)ngrid, (int)nset, (int)set0, lists, counts, (int)blksize, nsub, (int)maxchunks, (int)ngroups);
+            case 1: return launch_ns<1>(nt, grid, stream, partial, dms, ao_lhs, ao_rhs, sa, sg, (int)nao, (int)ngrid, (int)nset, (int)set0, lists, counts, (int)blksize, nsub, (int)maxchunks, (int)ngroups, (int)ncomp, (long)lhs_cs, (long)part_cs);
+            case 2: return launch_ns<2>(nt, grid, stream, partial, dms, ao_lhs, ao_rhs, sa, sg, (int)nao, (int)ngrid, (int)nset, (int)set0, lists, counts, (int)blksize, nsub, (int)maxchunks, (int)ngroups, (int)ncomp, (long)lhs_cs, (long)part_cs);
+            case 3: return launch_ns<3>(nt, grid, stream, partial, dms, ao_lhs, ao_rhs, sa, sg, (int)nao, (int)ngrid, (int)nset, (int)set0, lists, counts, (int)blksize, nsub, (int)maxchunks, (int)ngroups, (int)ncomp, (long)lhs_cs, (long)part_cs);
+            default: return launch_ns<4>(nt, grid, stream, partial, dms, ao_lhs, ao_rhs, sa, sg, (int)nao, (int)ngrid, (int)nset, (int)set0, lists, counts, (int)blksize, nsub, (int)maxchunks, (int)ngroups, (int)ncomp, (long)lhs_cs, (long)part_cs);
         }
     };
     st = launch_groups(NS, 0, full);
@@ -206,7 +220,12 @@ int launch_rho(hipStream_t stream, double *rhos, const double *dms, const double
         st = launch_groups((int)rem, full * NS, 1);
         if (st != CFGMM_OK) return st;
     }
-    return launch_rho_reduce(stream, rhos, partial, counts, stride_grid, ngrid, nset, blksize);
+    for (size_t c = 0; c < ncomp; ++c) {
+        st = launch_rho_reduce(stream, rhos + c * rho_cs, partial + c * part_cs, counts, stride_grid, ngrid, nset,
+                               blksize);
+        if (st != CFGMM_OK) return st;
+    }
+    return CFGMM_OK;
 }
 
 }  // namespace cfgmm

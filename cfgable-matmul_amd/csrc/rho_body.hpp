@@ -23,6 +23,10 @@ struct RhoChunk {
     const int *list;  // compacted AO list of the block
     int cnt;          // its length
     int chunk, g0;
+    // several left-hand factors sharing D and rhs (e.g. phi and its gradient): component c
+    // reads ao_lhs + c*lhs_comp_stride and writes partial + c*part_comp_stride
+    int ncomp = 1;
+    long lhs_comp_stride = 0, part_comp_stride = 0;
 };
 
 template <int NS, int NT>
@@ -175,40 +179,46 @@ __device__ inline void rho_chunk_body(const RhoChunk &task, unsigned char *__res
         }
     }
 
-    // Epilogue: rho_partial[g] = sum_u lhs[u][g] * T[u][g] over this chunk's rows.
+    // Epilogue: rho_partial[g] = sum_u lhs[u][g] * T[u][g] over this chunk's rows, for each
+    // left-hand component (T is shared).
     double *red = As_raw;  // [4 waves][NS][W]; all slice reads finished at the last barrier
+    for (int comp = 0; comp < task.ncomp; ++comp) {
+        const double *__restrict__ lhs_c = ao_lhs + comp * task.lhs_comp_stride;
+        double *__restrict__ part_c = partial + comp * task.part_comp_stride;
+        if (comp > 0) __syncthreads();  // the previous component's reads of red are done
 #pragma unroll
-    for (int nt = 0; nt < NT; ++nt) {
-        const int g = g0 + nt * 16 + lrow;
-        const bool g_ok = g < ngrid;
-        double lhs[4];
-        bool row_ok[4];
+        for (int nt = 0; nt < NT; ++nt) {
+            const int g = g0 + nt * 16 + lrow;
+            const bool g_ok = g < ngrid;
+            double lhs[4];
+            bool row_ok[4];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int i = wave * 16 + lk + 4 * r;  // D[l/16 + 4r][l%16]
-            row_ok[r] = i < nrows;
-            lhs[r] = ao_lhs[(long)ulist[i] * stride_grid + (g_ok ? g : ngrid - 1)];
+            for (int r = 0; r < 4; ++r) {
+                const int i = wave * 16 + lk + 4 * r;  // D[l/16 + 4r][l%16]
+                row_ok[r] = i < nrows;
+                lhs[r] = lhs_c[(long)ulist[i] * stride_grid + (g_ok ? g : ngrid - 1)];
+            }
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                // rows past the chunk end (fallback rows) contribute nothing
+                double part = 0.0;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) part = row_ok[r] ? fma(acc[s][nt][r], lhs[r], part) : part;
+                part += __shfl_xor(part, 16);
+                part += __shfl_xor(part, 32);
+                if (lk == 0) red[(wave * NS + s) * W + nt * 16 + lrow] = part;
+            }
         }
+        __syncthreads();
+        for (int idx = tid; idx < NS * W; idx += kThreads) {
+            const int s = idx / W, col = idx % W;
+            const int g = g0 + col;
+            if (g < ngrid) {
+                double v = red[s * W + col];
 #pragma unroll
-        for (int s = 0; s < NS; ++s) {
-            // rows past the chunk end (fallback rows) contribute nothing
-            double part = 0.0;
-#pragma unroll
-            for (int r = 0; r < 4; ++r) part = row_ok[r] ? fma(acc[s][nt][r], lhs[r], part) : part;
-            part += __shfl_xor(part, 16);
-            part += __shfl_xor(part, 32);
-            if (lk == 0) red[(wave * NS + s) * W + nt * 16 + lrow] = part;
-        }
-    }
-    __syncthreads();
-    for (int idx = tid; idx < NS * W; idx += kThreads) {
-        const int s = idx / W, col = idx % W;
-        const int g = g0 + col;
-        if (g < ngrid) {
-            double v = red[s * W + col];
-#pragma unroll
-            for (int w = 1; w < kThreads / 64; ++w) v += red[(w * NS + s) * W + col];
-            partial[((long)chunk * nset + set0 + s) * ngrid + g] = v;
+                for (int w = 1; w < kThreads / 64; ++w) v += red[(w * NS + s) * W + col];
+                part_c[((long)chunk * nset + set0 + s) * ngrid + g] = v;
+            }
         }
     }
 }

@@ -163,6 +163,18 @@ int cfgmm_aodmao2rho_ex_dev(void *stream, double *rhos, const double *dms, const
                             size_t nset, const uint8_t *tab, size_t ldtab, size_t blksize, int precision,
                             int nslices);
 
+/* Several left-hand factors over the same density matrices, rhs and mask, in one pass
+ * (no counterpart in the reference; a GGA density needs phi and its three gradient
+ * components as lhs with phi as rhs):
+ *   rho_c[s*stride_grid + g] += sum_{u,v in L_b} lhs_c[u][g] D_s[u][v] rhs[v][g],  c < ncomp,
+ * rho_c = rhos + c*rho_comp_stride, lhs_c = ao_lhs + c*lhs_comp_stride (doubles).  T = D.rhs
+ * is formed once; each component is bitwise the result of cfgmm_aodmao2rho_dev with its lhs.
+ * ncomp <= 16; the components' rho ranges must not overlap. */
+int cfgmm_aodmao2rho_multi_dev(void *stream, double *rhos, size_t rho_comp_stride, const double *dms,
+                               const double *ao_lhs, size_t lhs_comp_stride, size_t ncomp, const double *ao_rhs,
+                               size_t stride_ao, size_t stride_grid, size_t nao, size_t ngrid, size_t nset,
+                               const uint8_t *tab, size_t ldtab, size_t blksize);
+
 /* ======================================================================== */
 /* 2c. batched rho (no counterpart in the reference)                        */
 /* ======================================================================== */

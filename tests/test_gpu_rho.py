@@ -166,3 +166,41 @@ def test_rho_max_sets_tunable_bitwise():
     s.destroy()
     for o in outs[1:]:
         assert np.array_equal(o, outs[0])
+
+
+def test_rho_multi_component_bitwise_and_oracle():
+    """Four left-hand components (phi and three gradient-like factors) over one D.rhs: each component
+    equals a separate evaluation bitwise and the oracle to 1e-12; rho accumulates into start values."""
+    p = W.rho_problem(seed=12, nshell=150, nao=480, ngrid=1600, nset=2, density=0.35)
+    rng = np.random.default_rng(3)
+    comps = [p.phi] + [p.phi * rng.uniform(-2, 2, size=p.phi.shape) for _ in range(3)]
+    lhs = np.ascontiguousarray(np.stack(comps))  # (4, nao, ngrid)
+    r0 = rng.standard_normal((4, p.nset, p.ngrid))
+    s = cfgmm.Stream()
+    d_d = cfgmm.DeviceBuffer.from_array(p.dms)
+    d_l = cfgmm.DeviceBuffer.from_array(lhs)
+    d_rhs = cfgmm.DeviceBuffer.from_array(p.phi)
+    d_t = cfgmm.DeviceBuffer.from_array(np.ascontiguousarray(p.tab))
+    d_r = cfgmm.DeviceBuffer.from_array(r0)
+    cfgmm.aodmao2rho_multi_dev(s, d_r, p.nset * p.ngrid, d_d, d_l, p.nao * p.ngrid, 4, d_rhs, p.nao, p.ngrid, p.nao,
+                               p.ngrid, p.nset, d_t, p.nblk, p.blksize)
+    s.synchronize()
+    multi = d_r.to_array(np.float64, r0.shape)
+    singles = []
+    for c in range(4):
+        d_one = cfgmm.DeviceBuffer.from_array(np.ascontiguousarray(r0[c]))
+        d_lc = cfgmm.DeviceBuffer.from_array(np.ascontiguousarray(lhs[c]))
+        cfgmm.aodmao2rho_dev(s, d_one, d_d, d_lc, d_rhs, p.nao, p.ngrid, p.nao, p.ngrid, p.nset, d_t, p.nblk, p.blksize)
+        s.synchronize()
+        singles.append(d_one.to_array(np.float64, (p.nset, p.ngrid)))
+        d_one.free()
+        d_lc.free()
+    for b in (d_d, d_l, d_rhs, d_t, d_r):
+        b.free()
+    s.destroy()
+    for c in range(4):
+        assert np.array_equal(multi[c], singles[c])
+    ref = r0[2].ravel().copy()
+    assert oracle.aodmao2rho(ref, p.dms.ravel(), lhs[2].ravel(), p.phi.ravel(), p.nao, p.ngrid, p.nao, p.ngrid,
+                             p.nset, np.ascontiguousarray(p.tab), p.nblk, p.blksize) == 0
+    assert rel_fro(multi[2].ravel() - r0[2].ravel(), ref - r0[2].ravel()) <= 1e-12
