@@ -20,6 +20,7 @@ The same JSON line carries:
     resident per-block operands, with its error vs the oracle;
   * rho_4_components: four left-hand factors (GGA-shaped: phi and three gradient-like
     arrays) over one D.phi pass vs four separate evaluations;
+  * aogridao2mat: the contraction after rho, V_uv += sum_g phi_ug w_g phi_vg (masked);
   * host_pointer_path: the reference-shaped host-buffer call, PCIe included.
 
 Other workloads (one JSON line each):
@@ -205,6 +206,35 @@ def bench_multi(p: W.RhoProblem, ncomp: int = 4, reps: int = 20):
     s.destroy()
     return {"components": ncomp, "fused_ms": round(t_f, 4), "separate_ms": round(t_s, 4),
             "speedup": round(t_s / t_f, 2)}
+
+
+def bench_vxc(p: W.RhoProblem, reps: int = 10):
+    """V_uv += sum_g phi_ug w_g phi_vg over the (u, v) active in g's block: flops sum_b npts_b 2 n_b^2."""
+    rng = np.random.default_rng(6)
+    w = rng.uniform(0.1, 2.0, p.ngrid)
+    s = cfgmm.Stream()
+    d_phi, d_w = cfgmm.DeviceBuffer.from_array(p.phi), cfgmm.DeviceBuffer.from_array(w)
+    d_t = cfgmm.DeviceBuffer.from_array(np.ascontiguousarray(p.tab))
+    d_v = cfgmm.DeviceBuffer(8 * p.nao * p.nao)
+    d_v.zero()
+
+    def run():
+        cfgmm.aogridao2mat_dev(s, d_v, p.nao, d_phi, d_phi, d_w, p.ngrid, p.nao, p.ngrid, d_t, p.nblk, p.blksize)
+    run()
+    s.synchronize()
+    e0, e1 = cfgmm.Event(), cfgmm.Event()
+    e0.record(s)
+    for _ in range(reps):
+        run()
+    e1.record(s)
+    s.synchronize()
+    ms = e0.elapsed_ms(e1) / reps
+    for b in (d_phi, d_w, d_t, d_v):
+        b.free()
+    s.destroy()
+    n_b = p.counts().astype(np.float64)
+    flops = float(np.sum(p.blksize * 2.0 * n_b * n_b))
+    return {"ms": round(ms, 4), "masked_gflop": round(flops / 1e9, 2), "tflops": round(flops / (ms * 1e-3) / 1e12, 2)}
 
 
 def bench_dgemm(g: W.GemmProblem, reps: int, warmup: int = 2):
@@ -705,6 +735,10 @@ def main():
     log("[bench] 4-component (GGA-shaped) rho: one fused pass vs 4 evaluations")
     out_gga = bench_multi(p, ncomp=4)
 
+    # the contraction after rho: V += sum_g phi_u w_g phi_v over the block-active pairs
+    log("[bench] AO-grid-AO contraction (V_uv from phi, weights, mask)")
+    out_vxc = bench_vxc(p)
+
     # the host-pointer drop-in (reference call shape): PCIe transfers included, never `value`
     log("[bench] host-pointer drop-in path (H2D of phi, D and mask, D2H of rho)")
     hr = np.zeros(p.nset * p.ngrid)
@@ -733,6 +767,7 @@ def main():
 
     out["split6_resident"] = out_split
     out["rho_4_components"] = out_gga
+    out["aogridao2mat"] = out_vxc
 
     if not args.no_dgemm:
         dg = {}

@@ -71,6 +71,7 @@ _SIGNATURES = {
     "cfgmm_aodmao2rho_dev": ([_VP, _VP, _VP, _VP, _VP] + [_SZ] * 5 + [_VP, _SZ, _SZ], _I),
     "cfgmm_aodmao2rho_ex_dev": ([_VP, _VP, _VP, _VP, _VP] + [_SZ] * 5 + [_VP, _SZ, _SZ, _I, _I], _I),
     "cfgmm_aodmao2rho_multi_dev": ([_VP, _VP, _SZ, _VP, _VP, _SZ, _SZ, _VP] + [_SZ] * 5 + [_VP, _SZ, _SZ], _I),
+    "cfgmm_aogridao2mat_dev": ([_VP, _VP, _SZ, _VP, _VP, _VP] + [_SZ] * 3 + [_VP, _SZ, _SZ], _I),
     "cfgmm_aodmao2rho_batched_dev": ([_VP, _VP, _SZ, _I, _I], _I),
     "cfgmm_rho_resident_create": ([_VP, _VP, _VP, _VP] + [_SZ] * 5 + [_VP, _SZ, _SZ, _I, _I, ctypes.POINTER(_VP)], _I),
     "cfgmm_rho_resident_eval": ([_VP, _VP, _VP, _SZ, _SZ, _SZ], _I),
@@ -369,6 +370,13 @@ def aodmao2rho_multi_dev(stream, rhos, rho_comp_stride, dms, ao_lhs, lhs_comp_st
     check(lib().cfgmm_aodmao2rho_multi_dev(_h(stream), _dev(rhos), rho_comp_stride, _dev(dms), _dev(ao_lhs),
                                            lhs_comp_stride, ncomp, _dev(ao_rhs), stride_ao, stride_grid, nao, ngrid,
                                            nset, _dev(tab), ldtab, blksize))
+
+
+def aogridao2mat_dev(stream, V, ldv, ao_lhs, ao_rhs, w, stride_grid, nao, ngrid, tab, ldtab, blksize):
+    """V += sum over grid points of lhs[u][g] w[g] rhs[v][g] for (u, v) active in g's block (w may be None)."""
+    check(lib().cfgmm_aogridao2mat_dev(_h(stream), _dev(V), ldv, _dev(ao_lhs), _dev(ao_rhs),
+                                       None if w is None else _dev(w), stride_grid, nao, ngrid, _dev(tab), ldtab,
+                                       blksize))
 
 
 class RhoProblem(ctypes.Structure):

@@ -532,6 +532,21 @@ int cfgmm_aodmao2rho_multi_dev(void *stream, double *rhos, size_t rho_comp_strid
                             ncomp, ao_rhs, stride_ao, stride_grid, nao, ngrid, nset, tab, ldtab, blksize);
 }
 
+int cfgmm_aogridao2mat_dev(void *stream, double *V, size_t ldv, const double *ao_lhs, const double *ao_rhs,
+                           const double *w, size_t stride_grid, size_t nao, size_t ngrid, const uint8_t *tab,
+                           size_t ldtab, size_t blksize) {
+    clear_error();
+    CFGMM_TRY(check_blksize(blksize));
+    const bool any = nao && ngrid;
+    CFGMM_TRY(check_ld("ldv", ldv, nao, any));
+    CFGMM_TRY(check_ld("stride_grid", stride_grid, ngrid, any));
+    CFGMM_TRY(check_ld("ldtab", ldtab, ngrid ? div_up(ngrid, blksize) : 0, any));
+    if (any && (!V || !ao_lhs || !ao_rhs || !tab)) return fail(CFGMM_EINVAL, "null device pointer");
+    CFGMM_TRY(check_device());
+    return launch_vxc(static_cast<hipStream_t>(stream), V, ldv, ao_lhs, ao_rhs, w, stride_grid, nao, ngrid, tab,
+                      ldtab, blksize);
+}
+
 int cfgmm_aodmao2rho_batched_dev(void *stream, const cfgmm_rho_problem *problems, size_t nproblems, int precision,
                                  int nslices) {
     clear_error();

@@ -98,6 +98,10 @@ size_t resident_nset(const ResidentRho *h);
 size_t resident_blksize(const ResidentRho *h);
 int resident_device(const ResidentRho *h);
 
+// The AO-grid-AO contraction after rho (vxc.hip).
+int launch_vxc(hipStream_t stream, double *V, size_t ldv, const double *ao_lhs, const double *ao_rhs, const double *w,
+               size_t stride_grid, size_t nao, size_t ngrid, const uint8_t *tab, size_t ldtab, size_t blksize);
+
 // Serving helpers (serve_kernels.hip) and the request loop (server.cpp).
 int launch_fill_uniform(hipStream_t stream, double *x, size_t n, unsigned long long seed);
 int launch_fill_zero(hipStream_t stream, void *x, size_t bytes);

@@ -175,6 +175,16 @@ int cfgmm_aodmao2rho_multi_dev(void *stream, double *rhos, size_t rho_comp_strid
                                size_t stride_ao, size_t stride_grid, size_t nao, size_t ngrid, size_t nset,
                                const uint8_t *tab, size_t ldtab, size_t blksize);
 
+/* The contraction that follows rho in a DFT step (no counterpart in the reference):
+ *   V[u*ldv + v] += sum_b sum_{g in block b} [u in L_b][v in L_b] lhs[u][g] * w[g] * rhs[v][g],
+ * with the mask, strides and blocks of cfgmm_aodmao2rho_dev (lhs, rhs: nao x stride_grid;
+ * w: ngrid weights, or NULL for all ones; V: nao x ldv).  Deterministic (per-block products
+ * summed in block order); uses sum_b kpad_b^2 doubles of workspace, kpad_b = |L_b| rounded
+ * up to 64. */
+int cfgmm_aogridao2mat_dev(void *stream, double *V, size_t ldv, const double *ao_lhs, const double *ao_rhs,
+                           const double *w, size_t stride_grid, size_t nao, size_t ngrid, const uint8_t *tab,
+                           size_t ldtab, size_t blksize);
+
 /* ======================================================================== */
 /* 2c. batched rho (no counterpart in the reference)                        */
 /* ======================================================================== */
