@@ -10,6 +10,7 @@ Modes:
   dgemm_large    FP64 DGEMM 3527x9583x6581 NN (dgemm_kernel)
   dgemm_small    FP64 DGEMM 1031x977x1543 NT
   gemm_split<S>  the large GEMM in split precision (ozaki_gemm_kernel)
+  vxc            V += phi w phi^T over the same mask (vxc_block_kernel, vxc_reduce_kernel)
   copy           a 256 MiB device-to-device copy (byte-counter calibration)
 
 Each pass of a hardware-counter profiler is a separate process running this
@@ -62,6 +63,16 @@ def main(argv: list[str]) -> int:
         d_c.zero()
         for _ in range(reps):
             cfgmm.dgemm_dev(s, d_c, d_a, d_b, g.m, g.n, g.k, g.lda, g.ldb, g.ldc, g.transa, g.transb, prec, ns)
+    elif mode == "vxc":
+        p = W.rho_problem(seed=0)
+        w = np.random.default_rng(6).uniform(0.1, 2.0, p.ngrid)
+        d_v = cfgmm.DeviceBuffer(8 * p.nao * p.nao)
+        d_v.zero()
+        d_phi = cfgmm.DeviceBuffer.from_array(p.phi)
+        d_w = cfgmm.DeviceBuffer.from_array(w)
+        d_tab = cfgmm.DeviceBuffer.from_array(np.ascontiguousarray(p.tab))
+        for _ in range(reps):
+            cfgmm.aogridao2mat_dev(s, d_v, p.nao, d_phi, d_phi, d_w, p.ngrid, p.nao, p.ngrid, d_tab, p.nblk, p.blksize)
     elif mode == "copy":
         nbytes = 256 << 20
         a = cfgmm.DeviceBuffer(nbytes)

@@ -5,15 +5,18 @@
 //
 // No counterpart in the reference (it has the density side, aodmao2rho); the mask, strides and
 // block semantics are those of src/impl_aodmao2rho.rs:171-244, read the other way round: the
-// contraction runs over grid points instead of AOs.  Deterministic, in two launches:
+// contraction runs over grid points instead of AOs.  Deterministic, in two main launches:
 //   1. vxc_block_kernel: for each grid block b, the compacted block product
 //        S_b[i][j] = sum_{g in b} lhs[L_b[i]][g] w[g] rhs[L_b[j]][g],   i, j < |L_b|,
-//      one 64x64 tile per workgroup (FP64 MFMA over the block's points, K <= blksize), stored
-//      in a workspace of sum_b kpad_b^2 doubles (kpad_b = |L_b| rounded up to 64).
-//   2. vxc_reduce_kernel: each 64x64 tile of V sums, over the blocks in increasing order, the
-//      entries S_b[pos_b(u)][pos_b(v)] of the (u, v) active in block b (pos_b(x) = index of x
-//      in L_b, or -1), then adds the sum into V.  No float atomics: the result does not depend
-//      on scheduling.
+//      by FP64 MFMA, each workgroup a strip of up to twelve 64x64 tiles of one row of tiles (the
+//      row operand in registers, the next tile's column operand loaded during the current
+//      tile's MFMAs); S_b lives in a workspace of sum_b kpad_b^2 doubles (kpad_b = |L_b|
+//      rounded up to 64).  The strips of one block run back to back on one XCD.
+//   2. vxc_reduce_kernel: each wave owns one row u of V over a 512-column tile and streams, for
+//      the blocks in increasing order where u is active, the contiguous segment of row
+//      pos_b(u) of S_b whose columns L_b[k] fall in the tile, adding it into an LDS copy of
+//      the row (pos_b(x) = index of x in L_b).  Every element of S_b is read once, by full
+//      lanes; no float atomics, so the result does not depend on scheduling.
 #include "internal.hpp"
 
 #include <vector>
@@ -22,127 +25,292 @@ namespace cfgmm {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kT = 64;   // output tile edge (rows of S_b / of V)
-constexpr int kKB = 16;  // grid points per LDS slice
+constexpr int kT = 64;      // tile edge of S_b
+constexpr int kKC = 64;     // grid points staged per LDS pass of the block kernel
+constexpr int kStrip = 12;  // S_b tiles per block-kernel workgroup (at most)
+static_assert(kStrip * kT % kThreads == 0, "column indexes of a strip: whole passes of the workgroup");
+constexpr int kRW = 4;      // V rows per reduce workgroup (one per wave)
+constexpr int kRC = 512;    // V columns per reduce tile
+constexpr int kSeg = kRC / 64;  // 64-wide pieces of one S_b row segment
 
 // pos[b*nao + x] = index of x in L_b, or -1.
 __global__ void vxc_positions_kernel(const int *__restrict__ lists, const int *__restrict__ counts, int nao, int nblk,
                                      int *__restrict__ pos) {
-    const int b = blockIdx.y;
+    const int b = blockIdx.x;  // one workgroup per block: it owns the whole row of pos
     if (b >= nblk) return;
     int *__restrict__ pb = pos + (long)b * nao;
-    for (int x = blockIdx.x * blockDim.x + threadIdx.x; x < nao; x += gridDim.x * blockDim.x) pb[x] = -1;
-    __syncthreads();  // the whole row belongs to this workgroup when gridDim.x == 1
+    for (int x = threadIdx.x; x < nao; x += blockDim.x) pb[x] = -1;
+    __syncthreads();
     const int n = counts[b];
     const int *__restrict__ lb = lists + (long)b * nao;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) pb[lb[i]] = i;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) pb[lb[i]] = i;
 }
 
-// One 64x64 tile (ti, tj) of S_b.  Task: x = block, y = ti, z = tj; w unused.
+// kr[b*(nct+1) + c] = #{ i < |L_b| : L_b[i] < c*kRC },  c = 0..nct  (binary search of the sorted list).
+__global__ void vxc_ranges_kernel(const int *__restrict__ lists, const int *__restrict__ counts, int nao, int nblk,
+                                  int nct, int *__restrict__ kr) {
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)nblk * (nct + 1)) return;
+    const int b = (int)(idx / (nct + 1)), c = (int)(idx % (nct + 1));
+    const int *__restrict__ lb = lists + (long)b * nao;
+    const long key = (long)c * kRC;
+    int lo = 0, hi = counts[b];
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (lb[mid] < key) lo = mid + 1;
+        else hi = mid;
+    }
+    kr[idx] = lo;
+}
+
+// A strip of S_b tiles: task {b, ti, tj0, ntj} -> tiles (ti, tj0 .. tj0+ntj-1).  Wave w
+// computes rows 32*(w/2) .. +31 and columns 32*(w%2) .. +31 of each tile.
+//
+// Blocks of 4m <= 64 points: on gfx950 every VALU instruction takes issue cycles from the MFMA
+// pipe of its SIMD, so the per-tile loop keeps VALU work out: row bases are uniform (scalar
+// unit) with one per-lane offset for all loads; LDS and store addresses are per-strip bases
+// plus immediates; the first MFMA of a tile takes a zero accumulator; rows outside L_b are
+// zeroed by uniform branches.  The waves keep their A fragments (lhs*w) in registers for the
+// whole strip, and tile t+1's column rows are loaded during tile t's MFMAs into the other half
+// of a double-buffered LDS tile (one barrier per tile).  Other blocks (longer, or a partial
+// last block of 4m+1..4m+3 points) take 64-point passes with both operands through LDS.
 __global__ __launch_bounds__(kThreads, 2) void vxc_block_kernel(
     const int4 *__restrict__ tasks, const long *__restrict__ s_off, const int *__restrict__ kpads,
     const int *__restrict__ lists, const int *__restrict__ counts, const double *__restrict__ lhs,
     const double *__restrict__ rhs, const double *__restrict__ w, long stride_grid, int nao, int ngrid, int blksize,
     double *__restrict__ S) {
-    __shared__ double As[kKB][kT + 1];  // lhs*w [point][row]
-    __shared__ double Bs[kKB][kT + 1];  // rhs   [point][col]
-    __shared__ int ui[kT], vi[kT];
-    const int4 t = tasks[blockIdx.x];
-    const int b = t.x, ti = t.y, tj = t.z;
+    __shared__ double Bs[2][kKC][kT + 1];  // [buffer][point][row or column]
+    __shared__ int ui[kT], vi[kStrip * kT];
+    // workgroup i takes task x*q + min(x, r) + i/8 (x = i%8): each XCD a contiguous run of the
+    // block-major task order
+    const int ntasks = gridDim.x, xcd = blockIdx.x % 8, q = ntasks / 8, rr = ntasks % 8;
+    const int4 t = tasks[xcd * q + min(xcd, rr) + blockIdx.x / 8];
+    const int b = t.x, ti = t.y, tj0 = t.z, ntj = t.w;
     const int n = counts[b], kpad = kpads[b];
     const int *__restrict__ lb = lists + (long)b * nao;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int lrow = lane & 15, lk = lane >> 4;
     if (tid < kT) {
-        const int i = ti * kT + tid, j = tj * kT + tid;
+        const int i = ti * kT + tid;
         ui[tid] = i < n ? lb[i] : -1;
-        vi[tid] = j < n ? lb[j] : -1;
+    }
+#pragma unroll
+    for (int e = 0; e < kStrip * kT / kThreads; ++e) {
+        const int jj = e * kThreads + tid, j = tj0 * kT + jj;
+        vi[jj] = (jj < ntj * kT && j < n) ? lb[j] : -1;
     }
     __syncthreads();
     const int g0 = b * blksize, npts = min(blksize, ngrid - g0);
-    // wave w computes rows 32*(w/2) .. +31 and columns 32*(w%2) .. +31 of the tile
     const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+    double *__restrict__ Sb = S + s_off[b];
+
+    if (npts <= kKC && npts % 4 == 0) {
+        const int nkk = npts / 4;
+        // wave w stages operand rows 16w .. 16w+15 (lane = point); lanes past the block read
+        // its last point (in bounds) and those points are never used: the MFMA steps stop at
+        // npts
+        const unsigned loff = (unsigned)min(lane, npts - 1) * 8u;
+        const double wg = w ? w[g0 + min(lane, npts - 1)] : 1.0;
+        auto load16 = [&](const double *__restrict__ src, const int *idx, double *r, int *rows) {
+#pragma unroll
+            for (int q4 = 0; q4 < 16; ++q4) {
+                rows[q4] = __builtin_amdgcn_readfirstlane(idx[q4]);
+                const char *base = (const char *)(src + (long)max(rows[q4], 0) * stride_grid + g0);
+                r[q4] = *(const double *)(base + loff);
+            }
+        };
+        auto put16 = [&](int buf, const double *r, const int *rows, bool scale) {
+#pragma unroll
+            for (int q4 = 0; q4 < 16; ++q4) {
+                double x = 0.0;
+                if (rows[q4] >= 0) x = scale ? r[q4] * wg : r[q4];
+                Bs[buf][lane][wave * 16 + q4] = x;
+            }
+        };
+        double rg[16];
+        int rows[16];
+        load16(lhs, ui + wave * 16, rg, rows);
+        put16(1, rg, rows, true);  // A staged through buffer 1, B of tile 0 in buffer 0
+        load16(rhs, vi + wave * 16, rg, rows);
+        put16(0, rg, rows, false);
+        __syncthreads();
+        double fa[kKC / 4][2];
+#pragma unroll
+        for (int kk = 0; kk < kKC / 4; ++kk)
+#pragma unroll
+            for (int a = 0; a < 2; ++a) fa[kk][a] = Bs[1][4 * kk + lk][wr + a * 16 + lrow];
+        __syncthreads();  // every wave holds its A fragments before buffer 1 is reused
+        // per-lane store offsets (bytes) of D[l/16 + 4r][l%16] for the a-blocks; +128 B per c
+        unsigned soff[2][4];
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) soff[a][r] = (unsigned)(((wr + a * 16 + lk + 4 * r) * kpad + wc + lrow) * 8);
+        char *const srow = (char *)(Sb + (long)ti * kT * kpad);
+        // kq = 4-point steps to run; the full-block call is inlined with kq = 16 (no guards)
+        auto mfma_tile = [&](int cur, int kq, f64x4 (&acc)[2][2]) {
+#pragma unroll
+            for (int kk = 0; kk < kKC / 4; ++kk) {
+                if (kk < kq) {
+                    double fb[2];
+#pragma unroll
+                    for (int c = 0; c < 2; ++c) fb[c] = Bs[cur][4 * kk + lk][wc + c * 16 + lrow];
+#pragma unroll
+                    for (int a = 0; a < 2; ++a)
+#pragma unroll
+                        for (int c = 0; c < 2; ++c)
+                            acc[a][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                                fa[kk][a], fb[c], kk == 0 ? f64x4{0.0, 0.0, 0.0, 0.0} : acc[a][c], 0, 0, 0);
+                }
+            }
+        };
+        for (int jt = 0; jt < ntj; ++jt) {
+            const int cur = jt & 1;
+            if (jt + 1 < ntj) load16(rhs, vi + (jt + 1) * kT + wave * 16, rg, rows);  // during the MFMAs
+            f64x4 acc[2][2];
+            if (nkk == kKC / 4) mfma_tile(cur, kKC / 4, acc);
+            else mfma_tile(cur, nkk, acc);
+            char *const st = srow + (long)(tj0 + jt) * kT * 8;
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int c = 0; c < 2; ++c)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) *(double *)(st + soff[a][r] + c * 128) = acc[a][c][r];
+            if (jt + 1 < ntj) put16(cur ^ 1, rg, rows, false);  // cur^1 was last read before the previous barrier
+            __syncthreads();
+        }
+        return;
+    }
+
+    // general blocks: step = (tile jt, 64-point pass p), both operands through LDS; lane =
+    // point, rows wave + 4q; rows outside L_b and points past the block stage as zero
+    const int npass = (npts + kKC - 1) / kKC, nsteps = ntj * npass;
     f64x4 acc[2][2];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int c = 0; c < 2; ++c) acc[a][c] = f64x4{0.0, 0.0, 0.0, 0.0};
-    for (int k0 = 0; k0 < npts; k0 += kKB) {
-        // stage 64 rows x 16 points of lhs*w and of rhs (rows outside L_b and points past the
-        // block are zero)
-        for (int idx = tid; idx < kT * kKB; idx += kThreads) {
-            const int r = idx / kKB, p = idx % kKB;
-            const int g = g0 + k0 + p;
-            const bool pok = k0 + p < npts;
-            const int u = ui[r], v = vi[r];
-            const double wg = pok ? (w ? w[g] : 1.0) : 0.0;
-            As[p][r] = (pok && u >= 0) ? lhs[(long)u * stride_grid + g] * wg : 0.0;
-            Bs[p][r] = (pok && v >= 0) ? rhs[(long)v * stride_grid + g] : 0.0;
+    for (int step = 0; step < nsteps; ++step) {
+        const int jt = step / npass, pass = step % npass, k0 = pass * kKC;
+        const bool pok = k0 + lane < npts;
+        const int g = pok ? g0 + k0 + lane : g0;
+        const double wg = pok ? (w ? w[g] : 1.0) : 0.0;
+        if (step > 0) __syncthreads();  // the previous step has finished reading both buffers
+#pragma unroll
+        for (int q4 = 0; q4 < kT / 4; ++q4) {
+            const int u = ui[wave + 4 * q4], v = vi[jt * kT + wave + 4 * q4];
+            Bs[1][lane][wave + 4 * q4] = u >= 0 ? lhs[(long)u * stride_grid + g] * wg : 0.0;
+            Bs[0][lane][wave + 4 * q4] = (pok && v >= 0) ? rhs[(long)v * stride_grid + g] : 0.0;
         }
         __syncthreads();
-#pragma unroll
-        for (int kk = 0; kk < kKB / 4; ++kk) {
+        const int kend = min(kKC, npts - k0);  // zero-padded up to a multiple of 4
+        for (int kk = 0; kk < kend; kk += 4) {
             double fa[2], fb[2];
 #pragma unroll
-            for (int a = 0; a < 2; ++a) fa[a] = As[kk * 4 + lk][wr + a * 16 + lrow];
+            for (int a = 0; a < 2; ++a) fa[a] = Bs[1][kk + lk][wr + a * 16 + lrow];
 #pragma unroll
-            for (int c = 0; c < 2; ++c) fb[c] = Bs[kk * 4 + lk][wc + c * 16 + lrow];
+            for (int c = 0; c < 2; ++c) fb[c] = Bs[0][kk + lk][wc + c * 16 + lrow];
 #pragma unroll
             for (int a = 0; a < 2; ++a)
 #pragma unroll
                 for (int c = 0; c < 2; ++c)
                     acc[a][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[a], fb[c], acc[a][c], 0, 0, 0);
         }
-        __syncthreads();
-    }
-    double *__restrict__ Sb = S + s_off[b];
+        if (pass == npass - 1) {
+            const int tj = tj0 + jt;
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+            for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int c = 0; c < 2; ++c)
+                for (int c = 0; c < 2; ++c) {
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int i = ti * kT + wr + a * 16 + lk + 4 * r;  // D[l/16 + 4r][l%16]
-                const int j = tj * kT + wc + c * 16 + lrow;
-                Sb[(long)i * kpad + j] = acc[a][c][r];  // i, j < kpad: the padding is written too
-            }
-}
-
-// One 64x64 tile of V: V[u][v] += sum over blocks, in increasing order, of S_b[pos_b(u)][pos_b(v)].
-__global__ __launch_bounds__(kThreads) void vxc_reduce_kernel(const int *__restrict__ pos,
-                                                               const long *__restrict__ s_off,
-                                                               const int *__restrict__ kpads, int nao, int nblk,
-                                                               const double *__restrict__ S, double *__restrict__ V,
-                                                               long ldv) {
-    __shared__ int pu[kT], pv[kT];
-    const int u0 = blockIdx.y * kT, v0 = blockIdx.x * kT;
-    const int tid = threadIdx.x;
-    // thread -> column v0 + tid%64, rows u0 + tid/64 + 4e (e < 16)
-    const int cv = tid % kT, r0 = tid / kT;
-    double acc[kT / 4];
-#pragma unroll
-    for (int e = 0; e < kT / 4; ++e) acc[e] = 0.0;
-    for (int b = 0; b < nblk; ++b) {
-        __syncthreads();
-        if (tid < kT) pu[tid] = u0 + tid < nao ? pos[(long)b * nao + u0 + tid] : -1;
-        else if (tid < 2 * kT) pv[tid - kT] = v0 + tid - kT < nao ? pos[(long)b * nao + v0 + tid - kT] : -1;
-        __syncthreads();
-        const int jv = pv[cv];
-        if (jv < 0) continue;
-        const double *__restrict__ Sb = S + s_off[b];
-        const int kpad = kpads[b];
-#pragma unroll
-        for (int e = 0; e < kT / 4; ++e) {
-            const int iu = pu[r0 + 4 * e];
-            if (iu >= 0) acc[e] += Sb[(long)iu * kpad + jv];
+                    for (int r = 0; r < 4; ++r) {
+                        const int i = ti * kT + wr + a * 16 + lk + 4 * r;  // D[l/16 + 4r][l%16]
+                        const int j = tj * kT + wc + c * 16 + lrow;
+                        Sb[(long)i * kpad + j] = acc[a][c][r];  // i, j < kpad: the padding is written too
+                    }
+                    acc[a][c] = f64x4{0.0, 0.0, 0.0, 0.0};
+                }
         }
     }
-    const int v = v0 + cv;
-    if (v >= nao) return;
+}
+
+// Rows 4*blockIdx.x .. +3 of V (one per wave) over columns v0 .. v0+511, v0 = 512 * blockIdx.y.
+__global__ __launch_bounds__(kThreads) void vxc_reduce_kernel(const int *__restrict__ pos,
+                                                               const int *__restrict__ lists,
+                                                               const int *__restrict__ kr,
+                                                               const long *__restrict__ s_off,
+                                                               const int *__restrict__ kpads, int nao, int nblk,
+                                                               int nct, const double *__restrict__ S,
+                                                               double *__restrict__ V, long ldv) {
+    __shared__ double Vt[kRW][kRC];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int ct = blockIdx.y, v0 = ct * kRC, u = blockIdx.x * kRW + wave;
+    double *__restrict__ row = Vt[wave];  // private to this wave: no barriers
+    if (u >= nao) return;
 #pragma unroll
-    for (int e = 0; e < kT / 4; ++e) {
-        const int u = u0 + r0 + 4 * e;
-        if (u < nao) V[(long)u * ldv + v] += acc[e];
+    for (int c = 0; c < kSeg; ++c) row[c * 64 + lane] = 0.0;
+    for (int c0 = 0; c0 < nblk; c0 += 64) {
+        // lane l describes block c0 + l: the row's position in L_b and the segment [klo, khi)
+        // of L_b whose columns lie in this tile (clamped indexes: every load is in bounds)
+        const bool ok = c0 + lane < nblk;
+        const int bq = ok ? c0 + lane : 0;
+        int iu = pos[(long)bq * nao + u];
+        const int klo = kr[(long)bq * (nct + 1) + ct], khi = kr[(long)bq * (nct + 1) + ct + 1];
+        const long so = s_off[bq];
+        const int kp = kpads[bq];
+        iu = ok ? iu : -1;
+        unsigned long long act = __ballot(iu >= 0 && khi > klo);
+        while (act) {  // two active blocks per step, in increasing order
+            int ls[2];
+            ls[0] = __builtin_ctzll(act);
+            act &= act - 1;
+            ls[1] = act ? __builtin_ctzll(act) : -1;
+            if (act) act &= act - 1;
+            double xs[2][kSeg];
+            int xc[2][kSeg];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+#pragma unroll
+                for (int c = 0; c < kSeg; ++c) {
+                    xs[h][c] = 0.0;
+                    xc[h][c] = -1;
+                }
+                const int l = ls[h];
+                if (l < 0) continue;
+                const int iuh = __builtin_amdgcn_readlane(iu, l);
+                const int klh = __builtin_amdgcn_readlane(klo, l), khh = __builtin_amdgcn_readlane(khi, l);
+                const int kph = __builtin_amdgcn_readlane(kp, l);
+                const long soh = (long)(((unsigned long)(unsigned)__builtin_amdgcn_readlane((int)(so >> 32), l) << 32) |
+                                        (unsigned)__builtin_amdgcn_readlane((int)so, l));
+                const double *__restrict__ src = S + soh + (long)iuh * kph;
+                const int *__restrict__ lst = lists + (long)(c0 + l) * nao;
+#pragma unroll
+                for (int c = 0; c < kSeg; ++c) {
+                    if (klh + 64 * c < khh) {
+                        const int k = klh + 64 * c + lane;
+                        if (k < khh) {  // raw loads only: the wait comes at the first use below
+                            xs[h][c] = src[k];
+                            xc[h][c] = lst[k];
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+#pragma unroll
+                for (int c = 0; c < kSeg; ++c)
+                    if (xc[h][c] >= 0)  // ds_add_f64: only this wave touches the row, in program order
+                        __hip_atomic_fetch_add(&row[xc[h][c] - v0], xs[h][c], __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WAVEFRONT);
+        }
+    }
+    double *__restrict__ vrow = V + (long)u * ldv;
+#pragma unroll
+    for (int c = 0; c < kSeg; ++c) {
+        const int v = v0 + c * 64 + lane;
+        if (v < nao) vrow[v] += row[c * 64 + lane];
     }
 }
 
@@ -152,12 +320,13 @@ int launch_vxc(hipStream_t stream, double *V, size_t ldv, const double *ao_lhs, 
                size_t stride_grid, size_t nao, size_t ngrid, const uint8_t *tab, size_t ldtab, size_t blksize) {
     if (nao == 0 || ngrid == 0) return CFGMM_OK;
     if (nao > INT32_MAX || ngrid > INT32_MAX) return fail(CFGMM_EINVAL, "vxc: size out of range");
-    const size_t nblk = div_up(ngrid, blksize);
+    const size_t nblk = div_up(ngrid, blksize), nct = div_up(nao, kRC);
     auto al = [](size_t x) { return div_up(x, 256) * 256; };
-    // workspace: lists | counts | pos ; then the tables and S (sizes need the counts)
+    // workspace: lists | counts | pos | kr ; then the tables and S (sizes need the counts)
     const size_t o_counts = al(sizeof(int) * nblk * nao);
     const size_t o_pos = o_counts + al(sizeof(int) * nblk);
-    const size_t head = o_pos + al(sizeof(int) * nblk * nao);
+    const size_t o_kr = o_pos + al(sizeof(int) * nblk * nao);
+    const size_t head = o_kr + al(sizeof(int) * nblk * (nct + 1));
     void *ws = nullptr;
     int st = workspace_get(stream, head, &ws);
     if (st != CFGMM_OK) return st;
@@ -168,6 +337,7 @@ int launch_vxc(hipStream_t stream, double *V, size_t ldv, const double *ao_lhs, 
     std::vector<int> hc(nblk);
     CFGMM_HIP_TRY(hipMemcpyAsync(hc.data(), counts, sizeof(int) * nblk, hipMemcpyDeviceToHost, stream));
     CFGMM_HIP_TRY(hipStreamSynchronize(stream));
+    // S_b offsets and the strip tasks, block-major (block, tile row, strip)
     std::vector<long> soff(nblk);
     std::vector<int> kp(nblk);
     std::vector<int4> tasks;
@@ -177,34 +347,44 @@ int launch_vxc(hipStream_t stream, double *V, size_t ldv, const double *ao_lhs, 
         soff[b] = (long)s_elems;
         s_elems += (size_t)kp[b] * kp[b];
         const int nt = kp[b] / kT;
+        const int ns = (nt + kStrip - 1) / kStrip;  // strips per tile row, of near-equal length
         for (int i = 0; i < nt; ++i)
-            for (int j = 0; j < nt; ++j) tasks.push_back(int4{(int)b, i, j, 0});
+            for (int q = 0; q < ns; ++q) {
+                const int j0 = q * nt / ns, j1 = (q + 1) * nt / ns;
+                tasks.push_back(int4{(int)b, i, j0, j1 - j0});
+            }
     }
     if (tasks.size() > (size_t)INT32_MAX) return fail(CFGMM_EINVAL, "vxc: too many tiles");
     const size_t o_soff = head;
     const size_t o_kp = o_soff + al(sizeof(long) * nblk);
     const size_t o_tasks = o_kp + al(sizeof(int) * nblk);
     const size_t o_S = o_tasks + al(sizeof(int4) * std::max<size_t>(tasks.size(), 1));
+    void *const ws_lists = ws;
     st = workspace_get(stream, o_S + sizeof(double) * std::max<size_t>(s_elems, 1), &ws);  // may move: re-derive
     if (st != CFGMM_OK) return st;
     char *base = static_cast<char *>(ws);
-    // a grown workspace no longer holds the lists: build them again in the new buffer
     lists = reinterpret_cast<int *>(base);
     counts = reinterpret_cast<int *>(base + o_counts);
     int *pos = reinterpret_cast<int *>(base + o_pos);
+    int *kr = reinterpret_cast<int *>(base + o_kr);
     long *d_soff = reinterpret_cast<long *>(base + o_soff);
     int *d_kp = reinterpret_cast<int *>(base + o_kp);
     int4 *d_tasks = reinterpret_cast<int4 *>(base + o_tasks);
     double *S = reinterpret_cast<double *>(base + o_S);
-    st = launch_build_lists(stream, tab, ldtab, nao, nblk, lists, counts);
-    if (st != CFGMM_OK) return st;
+    if (ws != ws_lists) {  // a grown workspace no longer holds the lists: build them again
+        st = launch_build_lists(stream, tab, ldtab, nao, nblk, lists, counts);
+        if (st != CFGMM_OK) return st;
+    }
     CFGMM_HIP_TRY(hipMemcpyAsync(d_soff, soff.data(), sizeof(long) * nblk, hipMemcpyHostToDevice, stream));
     CFGMM_HIP_TRY(hipMemcpyAsync(d_kp, kp.data(), sizeof(int) * nblk, hipMemcpyHostToDevice, stream));
     if (!tasks.empty())
         CFGMM_HIP_TRY(hipMemcpyAsync(d_tasks, tasks.data(), sizeof(int4) * tasks.size(), hipMemcpyHostToDevice, stream));
     hipEvent_t t0 = trace_start(stream);
-    hipLaunchKernelGGL(vxc_positions_kernel, dim3(1, (unsigned)nblk), dim3(kThreads), 0, stream, lists, counts,
+    hipLaunchKernelGGL(vxc_positions_kernel, dim3((unsigned)nblk), dim3(kThreads), 0, stream, lists, counts,
                        (int)nao, (int)nblk, pos);
+    const size_t nkr = nblk * (nct + 1);
+    hipLaunchKernelGGL(vxc_ranges_kernel, dim3((unsigned)div_up(nkr, kThreads)), dim3(kThreads), 0, stream, lists,
+                       counts, (int)nao, (int)nblk, (int)nct, kr);
     if (!tasks.empty())
         hipLaunchKernelGGL(vxc_block_kernel, dim3((unsigned)tasks.size()), dim3(kThreads), 0, stream, d_tasks, d_soff,
                            d_kp, lists, counts, ao_lhs, ao_rhs, w, (long)stride_grid, (int)nao, (int)ngrid,
@@ -212,10 +392,9 @@ int launch_vxc(hipStream_t stream, double *V, size_t ldv, const double *ao_lhs, 
     trace_stop(stream, "vxc_block_kernel", t0);
     st = hip_status(hipGetLastError(), "vxc_block_kernel launch");
     if (st != CFGMM_OK) return st;
-    const unsigned tiles = (unsigned)div_up(nao, kT);
     t0 = trace_start(stream);
-    hipLaunchKernelGGL(vxc_reduce_kernel, dim3(tiles, tiles), dim3(kThreads), 0, stream, pos, d_soff, d_kp, (int)nao,
-                       (int)nblk, S, V, (long)ldv);
+    hipLaunchKernelGGL(vxc_reduce_kernel, dim3((unsigned)div_up(nao, kRW), (unsigned)nct), dim3(kThreads), 0, stream,
+                       pos, lists, kr, d_soff, d_kp, (int)nao, (int)nblk, (int)nct, S, V, (long)ldv);
     trace_stop(stream, "vxc_reduce_kernel", t0);
     st = hip_status(hipGetLastError(), "vxc_reduce_kernel launch");
     if (st != CFGMM_OK) return st;
