@@ -71,15 +71,18 @@ __global__ void vxc_ranges_kernel(const int *__restrict__ lists, const int *__re
 // unit) with one per-lane offset for all loads; LDS and store addresses are per-strip bases
 // plus immediates; the first MFMA of a tile takes a zero accumulator; rows outside L_b are
 // zeroed by uniform branches.  The waves keep their A fragments (lhs*w) in registers for the
-// whole strip, and tile t+1's column rows are loaded during tile t's MFMAs into the other half
-// of a double-buffered LDS tile (one barrier per tile).  Other blocks (longer, or a partial
-// last block of 4m+1..4m+3 points) take 64-point passes with both operands through LDS.
-__global__ __launch_bounds__(kThreads, 2) void vxc_block_kernel(
+// whole strip, and tile t+1's column rows are loaded during tile t's MFMAs.  Other blocks
+// (longer, or a partial last block of 4m+1..4m+3 points) take 32-point passes with both
+// operands through LDS.
+__global__ __launch_bounds__(kThreads, 3) void vxc_block_kernel(
     const int4 *__restrict__ tasks, const long *__restrict__ s_off, const int *__restrict__ kpads,
     const int *__restrict__ lists, const int *__restrict__ counts, const double *__restrict__ lhs,
     const double *__restrict__ rhs, const double *__restrict__ w, long stride_grid, int nao, int ngrid, int blksize,
     double *__restrict__ S) {
-    __shared__ double Bs[2][kKC][kT + 1];  // [buffer][point][row or column]
+    // One 64-point x 64-row tile [point][row] (33 KB, so that three workgroups fit a CU); the
+    // general path splits it into two 32-point tiles, A and B of a pass.
+    __shared__ double smem[kKC * (kT + 1)];
+    double(*__restrict__ T1)[kT + 1] = reinterpret_cast<double(*)[kT + 1]>(smem);
     __shared__ int ui[kT], vi[kStrip * kT];
     // workgroup i takes task x*q + min(x, r) + i/8 (x = i%8): each XCD a contiguous run of the
     // block-major task order
@@ -120,27 +123,28 @@ __global__ __launch_bounds__(kThreads, 2) void vxc_block_kernel(
                 r[q4] = *(const double *)(base + loff);
             }
         };
-        auto put16 = [&](int buf, const double *r, const int *rows, bool scale) {
+        auto put16 = [&](const double *r, const int *rows, bool scale) {
 #pragma unroll
             for (int q4 = 0; q4 < 16; ++q4) {
                 double x = 0.0;
                 if (rows[q4] >= 0) x = scale ? r[q4] * wg : r[q4];
-                Bs[buf][lane][wave * 16 + q4] = x;
+                T1[lane][wave * 16 + q4] = x;
             }
         };
         double rg[16];
         int rows[16];
         load16(lhs, ui + wave * 16, rg, rows);
-        put16(1, rg, rows, true);  // A staged through buffer 1, B of tile 0 in buffer 0
+        put16(rg, rows, true);  // A through the tile first
         load16(rhs, vi + wave * 16, rg, rows);
-        put16(0, rg, rows, false);
         __syncthreads();
         double fa[kKC / 4][2];
 #pragma unroll
         for (int kk = 0; kk < kKC / 4; ++kk)
 #pragma unroll
-            for (int a = 0; a < 2; ++a) fa[kk][a] = Bs[1][4 * kk + lk][wr + a * 16 + lrow];
-        __syncthreads();  // every wave holds its A fragments before buffer 1 is reused
+            for (int a = 0; a < 2; ++a) fa[kk][a] = T1[4 * kk + lk][wr + a * 16 + lrow];
+        __syncthreads();  // every wave holds its A fragments before the tile is reused
+        put16(rg, rows, false);
+        __syncthreads();
         // per-lane store offsets (bytes) of D[l/16 + 4r][l%16] for the a-blocks; +128 B per c
         unsigned soff[2][4];
 #pragma unroll
@@ -149,13 +153,13 @@ __global__ __launch_bounds__(kThreads, 2) void vxc_block_kernel(
             for (int r = 0; r < 4; ++r) soff[a][r] = (unsigned)(((wr + a * 16 + lk + 4 * r) * kpad + wc + lrow) * 8);
         char *const srow = (char *)(Sb + (long)ti * kT * kpad);
         // kq = 4-point steps to run; the full-block call is inlined with kq = 16 (no guards)
-        auto mfma_tile = [&](int cur, int kq, f64x4 (&acc)[2][2]) {
+        auto mfma_tile = [&](int kq, f64x4 (&acc)[2][2]) {
 #pragma unroll
             for (int kk = 0; kk < kKC / 4; ++kk) {
                 if (kk < kq) {
                     double fb[2];
 #pragma unroll
-                    for (int c = 0; c < 2; ++c) fb[c] = Bs[cur][4 * kk + lk][wc + c * 16 + lrow];
+                    for (int c = 0; c < 2; ++c) fb[c] = T1[4 * kk + lk][wc + c * 16 + lrow];
 #pragma unroll
                     for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -166,11 +170,10 @@ __global__ __launch_bounds__(kThreads, 2) void vxc_block_kernel(
             }
         };
         for (int jt = 0; jt < ntj; ++jt) {
-            const int cur = jt & 1;
             if (jt + 1 < ntj) load16(rhs, vi + (jt + 1) * kT + wave * 16, rg, rows);  // during the MFMAs
             f64x4 acc[2][2];
-            if (nkk == kKC / 4) mfma_tile(cur, kKC / 4, acc);
-            else mfma_tile(cur, nkk, acc);
+            if (nkk == kKC / 4) mfma_tile(kKC / 4, acc);
+            else mfma_tile(nkk, acc);
             char *const st = srow + (long)(tj0 + jt) * kT * 8;
 #pragma unroll
             for (int a = 0; a < 2; ++a)
@@ -178,40 +181,50 @@ __global__ __launch_bounds__(kThreads, 2) void vxc_block_kernel(
                 for (int c = 0; c < 2; ++c)
 #pragma unroll
                     for (int r = 0; r < 4; ++r) *(double *)(st + soff[a][r] + c * 128) = acc[a][c][r];
-            if (jt + 1 < ntj) put16(cur ^ 1, rg, rows, false);  // cur^1 was last read before the previous barrier
-            __syncthreads();
+            if (jt + 1 < ntj) {
+                __syncthreads();  // every wave has read tile jt
+                put16(rg, rows, false);
+                __syncthreads();
+            }
         }
         return;
     }
 
-    // general blocks: step = (tile jt, 64-point pass p), both operands through LDS; lane =
-    // point, rows wave + 4q; rows outside L_b and points past the block stage as zero
-    const int npass = (npts + kKC - 1) / kKC, nsteps = ntj * npass;
+    // general blocks: step = (tile jt, 32-point pass p), both operands through LDS (A in the
+    // tile's first 32 point rows, B in the last 32); rows outside L_b and points past the
+    // block stage as zero
+    constexpr int kHP = kKC / 2;
+    double(*__restrict__ As)[kT + 1] = T1;
+    double(*__restrict__ Bs)[kT + 1] = T1 + kHP;
+    const int npass = (npts + kHP - 1) / kHP, nsteps = ntj * npass;
     f64x4 acc[2][2];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int c = 0; c < 2; ++c) acc[a][c] = f64x4{0.0, 0.0, 0.0, 0.0};
     for (int step = 0; step < nsteps; ++step) {
-        const int jt = step / npass, pass = step % npass, k0 = pass * kKC;
-        const bool pok = k0 + lane < npts;
-        const int g = pok ? g0 + k0 + lane : g0;
-        const double wg = pok ? (w ? w[g] : 1.0) : 0.0;
-        if (step > 0) __syncthreads();  // the previous step has finished reading both buffers
+        const int jt = step / npass, pass = step % npass, k0 = pass * kHP;
+        // lanes 0..31 stage point k0 + lane of A, lanes 32..63 the same point of B
+        const int pt = lane & (kHP - 1);
+        const bool isb = lane >= kHP;
+        const bool pok = k0 + pt < npts;
+        const int g = pok ? g0 + k0 + pt : g0;
+        const double wg = isb ? 1.0 : (w ? w[g] : 1.0);
+        if (step > 0) __syncthreads();  // the previous step has finished reading both halves
 #pragma unroll
         for (int q4 = 0; q4 < kT / 4; ++q4) {
-            const int u = ui[wave + 4 * q4], v = vi[jt * kT + wave + 4 * q4];
-            Bs[1][lane][wave + 4 * q4] = u >= 0 ? lhs[(long)u * stride_grid + g] * wg : 0.0;
-            Bs[0][lane][wave + 4 * q4] = (pok && v >= 0) ? rhs[(long)v * stride_grid + g] : 0.0;
+            const int x = isb ? vi[jt * kT + wave + 4 * q4] : ui[wave + 4 * q4];
+            const double *__restrict__ src = isb ? rhs : lhs;
+            T1[lane][wave + 4 * q4] = (pok && x >= 0) ? src[(long)x * stride_grid + g] * wg : 0.0;
         }
         __syncthreads();
-        const int kend = min(kKC, npts - k0);  // zero-padded up to a multiple of 4
+        const int kend = min(kHP, npts - k0);  // zero-padded up to a multiple of 4
         for (int kk = 0; kk < kend; kk += 4) {
             double fa[2], fb[2];
 #pragma unroll
-            for (int a = 0; a < 2; ++a) fa[a] = Bs[1][kk + lk][wr + a * 16 + lrow];
+            for (int a = 0; a < 2; ++a) fa[a] = As[kk + lk][wr + a * 16 + lrow];
 #pragma unroll
-            for (int c = 0; c < 2; ++c) fb[c] = Bs[0][kk + lk][wc + c * 16 + lrow];
+            for (int c = 0; c < 2; ++c) fb[c] = Bs[kk + lk][wc + c * 16 + lrow];
 #pragma unroll
             for (int a = 0; a < 2; ++a)
 #pragma unroll

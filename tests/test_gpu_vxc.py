@@ -51,17 +51,21 @@ def test_vxc_generated_molecule():
     ref = vxc_numpy(p.phi, p.phi, w, tab, p.nao, p.ngrid, p.blksize)
     assert rel_fro(got, ref) <= 1e-12
     assert rel_fro(got, got.T) <= 1e-14  # lhs = rhs: symmetric up to rounding
+    # weights on the other operand: lhs*w . rhs^T == (rhs*w . lhs^T)^T up to rounding
+    got_t = run(p.phi * w, p.phi, None, tab, p.nao, p.ngrid, p.blksize)
+    assert rel_fro(got_t, ref) <= 1e-12
     again = run(p.phi, p.phi, w, tab, p.nao, p.ngrid, p.blksize)
     assert np.array_equal(got, again)
 
 
-@pytest.mark.parametrize("blksize", [16, 48, 64])
-def test_vxc_strided_ragged_mask_authority(blksize):
-    """lhs != rhs, rows strided past ngrid, V padded and accumulated into, a partial last block,
-    a periodic mask with huge values under its off bits, no weights."""
-    nao, ngrid = 333, 1000
+@pytest.mark.parametrize("blksize,ngrid", [(16, 1000), (48, 1000), (64, 1000), (64, 1001), (96, 1000), (128, 1000)])
+def test_vxc_strided_ragged_mask_authority(blksize, ngrid):
+    """lhs != rhs, rows strided past ngrid, V padded and accumulated into, a partial last block
+    (41 points for ngrid 1001: not a multiple of 4), blocks longer than 64 points, a periodic
+    mask with huge values under its off bits, no weights."""
+    nao = 333
     stride = ngrid + 7
-    rng = np.random.default_rng(blksize)
+    rng = np.random.default_rng(blksize + ngrid)
     nblk = -(-ngrid // blksize)
     tab = W.periodic_mask(nao, nblk + 2, W.MASK_30)
     lhs = rng.standard_normal((nao, stride))
