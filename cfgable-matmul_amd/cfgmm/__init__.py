@@ -42,6 +42,7 @@ TUNE_DGEMM_SMALL_TILE = 4
 TUNE_DGEMM_SMALL_SPLITS = 5
 TUNE_RESIDENT_DEPTH = 6
 TUNE_RESIDENT_COLS = 7
+TUNE_RHO_ORDER = 8
 
 _lib = None
 

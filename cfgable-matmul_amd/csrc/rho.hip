@@ -12,9 +12,11 @@
 //
 //   1. build_ao_lists: one workgroup per grid block compacts L_b on the device
 //      (wave ballots + prefix counts) into lists[b][*], counts[b].
-//   2. rho_chunk_kernel: one workgroup per (grid block, 64-row chunk of L_b)
-//      for each group of up to four density matrices, the chunks of a block
-//      dispatched consecutively on one XCD.  It streams 16-deep slices of the compacted v list through
+//   2. rho_plan_kernel orders the (block, chunk) tasks by the AO range of their
+//      rows, so that tasks sharing rows of D run together on one XCD (block-major
+//      order, a tunable, reads every chunk's D rows from memory instead).
+//   3. rho_chunk_kernel: one workgroup per (grid block, 64-row chunk of L_b)
+//      for each group of up to four density matrices.  It streams 16-deep slices of the compacted v list through
 //      double-buffered LDS -- the gathered D_s[u][v] tile (A operand) and the
 //      gathered phi_rhs[v][g] tile (B operand) -- and accumulates
 //      T_s[u][g] = sum_v D_s[u][v] phi_rhs[v][g] with v_mfma_f64_16x16x4_f64.
@@ -22,7 +24,7 @@
 //      sums the four rows a lane holds, the four lane groups (cross-lane
 //      shuffles) and the four waves (LDS), and writes one partial rho row per
 //      chunk.  No ngrid x nao intermediate is written to HBM.
-//   3. rho_reduce_kernel: rho[s][g] += sum over the chunks of g's block, in
+//   4. rho_reduce_kernel: rho[s][g] += sum over the chunks of g's block, in
 //      increasing chunk order -- deterministic, no float atomics.
 #include "internal.hpp"
 #include "rho_body.hpp"
@@ -93,21 +95,94 @@ namespace {
 using rho::kThreads;
 using rho::kUC;
 
+constexpr int kPlanThreads = 1024;
+constexpr int kPlanBuckets = 1024;
+
+// Orders the (block, chunk) tasks of one evaluation by the AO range their rows cover (the
+// bucketed midpoint of the chunk's first and last row): tasks that run together on an XCD then
+// share rows of D in its L2, where block-major order reads each chunk's D rows from memory
+// (the columns of a chunk are a third of each row, spread over all its cache lines).
+// perm[0 .. *ntask) = slot b*maxchunks + c of the non-empty chunks, bucket-ordered; the order
+// inside a bucket follows atomics and varies from call to call, which the result does not
+// depend on (each task writes its own partial slot, reduced in fixed chunk order).
+__global__ __launch_bounds__(kPlanThreads) void rho_plan_kernel(const int *__restrict__ lists,
+                                                                const int *__restrict__ counts, int nao, int nblk,
+                                                                int maxchunks, int *__restrict__ perm,
+                                                                int *__restrict__ ntask) {
+    __shared__ int hist[kPlanBuckets];
+    const int tid = threadIdx.x;
+    const long nslot = (long)nblk * maxchunks;
+    auto key = [&](long sl) -> int {  // bucket of slot sl, or -1 for an empty chunk
+        const int b = (int)(sl / maxchunks), c = (int)(sl % maxchunks);
+        const int n = counts[b];
+        if (c * kUC >= n) return -1;
+        const int *__restrict__ lb = lists + (long)b * nao;
+        const long mid = ((long)lb[c * kUC] + lb[min(c * kUC + kUC, n) - 1]) / 2;
+        return (int)(mid * kPlanBuckets / nao);
+    };
+    for (int i = tid; i < kPlanBuckets; i += kPlanThreads) hist[i] = 0;
+    __syncthreads();
+    for (long sl = tid; sl < nslot; sl += kPlanThreads) {
+        const int k = key(sl);
+        if (k >= 0) atomicAdd(&hist[k], 1);
+    }
+    __syncthreads();
+    // exclusive scan of the bucket counts (Hillis-Steele over one bucket per thread)
+    static_assert(kPlanBuckets == kPlanThreads, "one bucket per thread");
+    const int mine = hist[tid];
+    int incl = mine;
+    for (int d = 1; d < kPlanBuckets; d <<= 1) {
+        __syncthreads();
+        const int add = tid >= d ? hist[tid - d] : 0;
+        __syncthreads();
+        incl += add;
+        hist[tid] = incl;
+    }
+    __syncthreads();
+    hist[tid] = incl - mine;
+    if (tid == kPlanThreads - 1) *ntask = incl;
+    __syncthreads();
+    for (long sl = tid; sl < nslot; sl += kPlanThreads) {
+        const int k = key(sl);
+        if (k >= 0) perm[atomicAdd(&hist[k], 1)] = (int)sl;
+    }
+}
+
 template <int NS, int NT>
 __global__ __launch_bounds__(kThreads, 2) void rho_chunk_kernel(
     double *__restrict__ partial, const double *__restrict__ dms, const double *__restrict__ ao_lhs,
     const double *__restrict__ ao_rhs, long stride_ao, long stride_grid, int nao, int ngrid, int nset, int set0,
     const int *__restrict__ lists, const int *__restrict__ counts, int blksize, int nsub, int maxchunks,
-    int ngroups, int ncomp, long lhs_comp_stride, long part_comp_stride) {
+    int ngroups, int ncomp, long lhs_comp_stride, long part_comp_stride, const int *__restrict__ perm,
+    const int *__restrict__ ntask) {
     __shared__ __attribute__((aligned(16))) unsigned char smem[rho::chunk_lds_bytes<NS, NT>()];
-    // Workgroup i of the 1-D grid takes task L = start_x + i/8 (x = i%8) of the block-major order
-    // (block, subtile, group of NS density matrices, chunk): the tasks of one block, which share
-    // its phi tile, run back to back on one XCD and find that tile in its L2.
-    const int ntasks = gridDim.x, x = blockIdx.x % 8, q = ntasks / 8, r = ntasks % 8;
-    const int L = x * q + min(x, r) + blockIdx.x / 8;
-    const int chunk = L % maxchunks, rest = L / maxchunks;
-    const int grp = rest % ngroups, rest2 = rest / ngroups;
-    const int blk = rest2 / nsub, sub = rest2 % nsub;
+    int blk, chunk, grp, sub;
+    const int x = blockIdx.x % 8, k = blockIdx.x / 8;
+    if (perm) {
+        // AO-ordered tasks (group of NS density matrices, planned chunk j, subtile), the
+        // sequence cut into eight contiguous parts, one per XCD (workgroup i runs on XCD i%8;
+        // the grid is a multiple of 8 with at least as many workgroups as tasks)
+        const long per = (long)*ntask * nsub, total = per * ngroups;
+        const long part = (total + 7) / 8, tt = x * part + k;
+        if (k >= part || tt >= total) return;
+        grp = (int)(tt / per);
+        const long rj = tt % per;
+        sub = (int)(rj % nsub);
+        const int sl = perm[rj / nsub];
+        blk = sl / maxchunks;
+        chunk = sl % maxchunks;
+    } else {
+        // block-major (block, subtile, group, chunk): the tasks of one block, which share its
+        // phi tile, run back to back on one XCD; empty chunk slots exit
+        const int ntasks = gridDim.x, q = ntasks / 8, r = ntasks % 8;
+        const int L = x * q + min(x, r) + k;
+        chunk = L % maxchunks;
+        const int rest = L / maxchunks;
+        grp = rest % ngroups;
+        const int rest2 = rest / ngroups;
+        blk = rest2 / nsub;
+        sub = rest2 % nsub;
+    }
     set0 += grp * NS;
     rho::RhoChunk t;
     t.cnt = counts[blk];
@@ -135,12 +210,12 @@ template <int NS>
 int launch_ns(int nt, dim3 grid, hipStream_t stream, double *partial, const double *dms, const double *ao_lhs,
               const double *ao_rhs, long stride_ao, long stride_grid, int nao, int ngrid, int nset, int set0,
               const int *lists, const int *counts, int blksize, int nsub, int maxchunks, int ngroups, int ncomp,
-              long lhs_cs, long part_cs) {
+              long lhs_cs, long part_cs, const int *perm, const int *ntask) {
     hipEvent_t t0 = trace_start(stream);
 #define CFGMM_RHO_LAUNCH(NT_)                                                                                      \
     hipLaunchKernelGGL((rho_chunk_kernel<NS, NT_>), grid, dim3(kThreads), 0, stream, partial, dms, ao_lhs, ao_rhs, \
                        stride_ao, stride_grid, nao, ngrid, nset, set0, lists, counts, blksize, nsub, maxchunks, \
-                       ngroups, ncomp, lhs_cs, part_cs)
+                       ngroups, ncomp, lhs_cs, part_cs, perm, ntask)
     switch (nt) {
         case 1: CFGMM_RHO_LAUNCH(1); break;
         case 2: CFGMM_RHO_LAUNCH(2); break;
@@ -182,10 +257,12 @@ int launch_rho_multi(hipStream_t stream, double *rhos, size_t rho_cs, const doub
     const int nt = rho_tile_nt(blksize);
     const int nsub = (int)(blksize / (16 * nt));
 
-    // workspace: [lists nblk*nao int][counts nblk int][pad][partial ncomp*maxchunks*nset*ngrid double]
+    // workspace: [lists nblk*nao int][counts nblk int][perm nblk*maxchunks int][ntask int][pad]
+    //            [partial ncomp*maxchunks*nset*ngrid double]
     const size_t lists_bytes = sizeof(int) * nblk * nao;
     const size_t counts_bytes = sizeof(int) * nblk;
-    const size_t part_off = div_up(lists_bytes + counts_bytes, 256) * 256;
+    const size_t perm_bytes = sizeof(int) * (nblk * maxchunks + 1);
+    const size_t part_off = div_up(lists_bytes + counts_bytes + perm_bytes, 256) * 256;
     const size_t part_cs = maxchunks * nset * ngrid;  // doubles per component
     const size_t part_bytes = sizeof(double) * part_cs * ncomp;
     void *ws = nullptr;
@@ -193,10 +270,21 @@ int launch_rho_multi(hipStream_t stream, double *rhos, size_t rho_cs, const doub
     if (st != CFGMM_OK) return st;
     int *lists = static_cast<int *>(ws);
     int *counts = lists + nblk * nao;
+    int *perm = counts + nblk;
+    int *ntask = perm + nblk * maxchunks;
     double *partial = reinterpret_cast<double *>(static_cast<char *>(ws) + part_off);
 
     st = launch_build_lists(stream, tab, ldtab, nao, nblk, lists, counts);
     if (st != CFGMM_OK) return st;
+    const bool ordered = tune_get(CFGMM_TUNE_RHO_ORDER) == 0;
+    if (ordered) {
+        hipEvent_t t0 = trace_start(stream);
+        hipLaunchKernelGGL(rho_plan_kernel, dim3(1), dim3(kPlanThreads), 0, stream, lists, counts, (int)nao,
+                           (int)nblk, (int)maxchunks, perm, ntask);
+        trace_stop(stream, "rho_plan_kernel", t0);
+        st = hip_status(hipGetLastError(), "rho_plan_kernel launch");
+        if (st != CFGMM_OK) return st;
+    }
 
     // Density matrices per workgroup: up to max_sets.  One launch covers every full group of NS
     // sets (the group is a task coordinate); a smaller remainder group gets a launch of its own.
@@ -204,14 +292,16 @@ int launch_rho_multi(hipStream_t stream, double *rhos, size_t rho_cs, const doub
     const size_t full = nset / NS, rem = nset % NS;
     const long sa = (long)stride_ao, sg = (long)stride_grid;
     auto launch_groups = [&](int ns, size_t set0, size_t ngroups) -> int {
-        if (nblk * nsub * maxchunks * ngroups > (size_t)INT32_MAX)
+        if (nblk * nsub * maxchunks * ngroups + 8 > (size_t)INT32_MAX)
             return fail(CFGMM_EINVAL, "rho: too many workgroups for one launch");
-        const dim3 grid((unsigned)(nblk * nsub * ngroups * maxchunks), 1, 1);
+        const size_t slots = nblk * nsub * ngroups * maxchunks;
+        const dim3 grid((unsigned)(ordered ? div_up(slots, 8) * 8 : slots), 1, 1);
+        const int *pp = ordered ? perm : nullptr;
         switch (ns) {
-            case 1: return launch_ns<1>(nt, grid, stream, partial, dms, ao_lhs, ao_rhs, sa, sg, (int)nao, (int)ngrid, (int)nset, (int)set0, lists, counts, (int)blksize, nsub, (int)maxchunks, (int)ngroups, (int)ncomp, (long)lhs_cs, (long)part_cs);
-            case 2: return launch_ns<2>(nt, grid, stream, partial, dms, ao_lhs, ao_rhs, sa, sg, (int)nao, (int)ngrid, (int)nset, (int)set0, lists, counts, (int)blksize, nsub, (int)maxchunks, (int)ngroups, (int)ncomp, (long)lhs_cs, (long)part_cs);
-            case 3: return launch_ns<3>(nt, grid, stream, partial, dms, ao_lhs, ao_rhs, sa, sg, (int)nao, (int)ngrid, (int)nset, (int)set0, lists, counts, (int)blksize, nsub, (int)maxchunks, (int)ngroups, (int)ncomp, (long)lhs_cs, (long)part_cs);
-            default: return launch_ns<4>(nt, grid, stream, partial, dms, ao_lhs, ao_rhs, sa, sg, (int)nao, (int)ngrid, (int)nset, (int)set0, lists, counts, (int)blksize, nsub, (int)maxchunks, (int)ngroups, (int)ncomp, (long)lhs_cs, (long)part_cs);
+            case 1: return launch_ns<1>(nt, grid, stream, partial, dms, ao_lhs, ao_rhs, sa, sg, (int)nao, (int)ngrid, (int)nset, (int)set0, lists, counts, (int)blksize, nsub, (int)maxchunks, (int)ngroups, (int)ncomp, (long)lhs_cs, (long)part_cs, pp, ntask);
+            case 2: return launch_ns<2>(nt, grid, stream, partial, dms, ao_lhs, ao_rhs, sa, sg, (int)nao, (int)ngrid, (int)nset, (int)set0, lists, counts, (int)blksize, nsub, (int)maxchunks, (int)ngroups, (int)ncomp, (long)lhs_cs, (long)part_cs, pp, ntask);
+            case 3: return launch_ns<3>(nt, grid, stream, partial, dms, ao_lhs, ao_rhs, sa, sg, (int)nao, (int)ngrid, (int)nset, (int)set0, lists, counts, (int)blksize, nsub, (int)maxchunks, (int)ngroups, (int)ncomp, (long)lhs_cs, (long)part_cs, pp, ntask);
+            default: return launch_ns<4>(nt, grid, stream, partial, dms, ao_lhs, ao_rhs, sa, sg, (int)nao, (int)ngrid, (int)nset, (int)set0, lists, counts, (int)blksize, nsub, (int)maxchunks, (int)ngroups, (int)ncomp, (long)lhs_cs, (long)part_cs, pp, ntask);
         }
     };
     st = launch_groups(NS, 0, full);

@@ -76,7 +76,11 @@ __device__ inline void rho_chunk_body(const RhoChunk &task, unsigned char *__res
     // Fixed per-thread D coordinates: column slot p_d, rows tid/16 + 16e.
     const int p_d = tid % kKV;
 
-    double rphi[PHI_PER_T], rd[NS][D_ROWS_PER_T];
+    // Register sets of slice operands: a slice's loads are issued PF slices before it is
+    // stored into LDS (a loaded global access takes a few microseconds, several slices' worth
+    // of MFMAs).  Sets cost VGPRs, so the wide many-DM tiles keep one.
+    constexpr int PF = NS == 1 ? 3 : (NS == 2 ? 2 : 1);
+    double rphi[PF][PHI_PER_T], rd[PF][NS][D_ROWS_PER_T];
     f64x4 acc[NS][NT];
 #pragma unroll
     for (int s = 0; s < NS; ++s)
@@ -102,17 +106,18 @@ __device__ inline void rho_chunk_body(const RhoChunk &task, unsigned char *__res
         // are clamped (their outputs are never written).  The last, partial
         // slice of a segment is loaded by load_tail, after the main loop, so
         // that no select on loaded values sits in the pipelined path.
-        auto load_full = [&](int k0) {
+        // (set = register set; a compile-time index wherever the loop below is unrolled)
+        auto load_full = [&](int k0, int set) {
             const long vcol = vls[k0 + p_d];
 #pragma unroll
             for (int e = 0; e < D_ROWS_PER_T; ++e)
 #pragma unroll
-                for (int s = 0; s < NS; ++s) rd[s][e] = drow[s][e][vcol];
+                for (int s = 0; s < NS; ++s) rd[set][s][e] = drow[s][e][vcol];
 #pragma unroll
             for (int e = 0; e < PHI_PER_T; ++e) {
                 const int idx = tid + kThreads * e;
                 const int g = min(g0 + idx % W, ngrid - 1);
-                rphi[e] = ao_rhs[(long)vls[k0 + idx / W] * stride_grid + g];
+                rphi[set][e] = ao_rhs[(long)vls[k0 + idx / W] * stride_grid + g];
             }
         };
         auto load_tail = [&](int k0) {
@@ -121,25 +126,25 @@ __device__ inline void rho_chunk_body(const RhoChunk &task, unsigned char *__res
 #pragma unroll
             for (int e = 0; e < D_ROWS_PER_T; ++e)
 #pragma unroll
-                for (int s = 0; s < NS; ++s) rd[s][e] = pv < segn ? drow[s][e][vcol] : 0.0;
+                for (int s = 0; s < NS; ++s) rd[0][s][e] = pv < segn ? drow[s][e][vcol] : 0.0;
 #pragma unroll
             for (int e = 0; e < PHI_PER_T; ++e) {
                 const int idx = tid + kThreads * e;
                 const int pr = k0 + idx / W;
                 const int g = min(g0 + idx % W, ngrid - 1);
-                rphi[e] = pr < segn ? ao_rhs[(long)vls[pr] * stride_grid + g] : 0.0;
+                rphi[0][e] = pr < segn ? ao_rhs[(long)vls[pr] * stride_grid + g] : 0.0;
             }
         };
-        auto store_slice = [&](int buf) {
+        auto store_slice = [&](int buf, int set) {
 #pragma unroll
             for (int e = 0; e < PHI_PER_T; ++e) {
                 const int idx = tid + kThreads * e;
-                Bs[buf][idx / W][idx % W] = rphi[e];
+                Bs[buf][idx / W][idx % W] = rphi[set][e];
             }
 #pragma unroll
             for (int e = 0; e < D_ROWS_PER_T; ++e)
 #pragma unroll
-                for (int s = 0; s < NS; ++s) As(buf, s, p_d, tid / kKV + (kThreads / kKV) * e) = rd[s][e];
+                for (int s = 0; s < NS; ++s) As(buf, s, p_d, tid / kKV + (kThreads / kKV) * e) = rd[set][s][e];
         };
         auto compute_slice = [&](int cur) {
 #pragma unroll
@@ -157,22 +162,44 @@ __device__ inline void rho_chunk_body(const RhoChunk &task, unsigned char *__res
             }
         };
 
+        // Slice t lives in register set t % PF, then in LDS buffer t % 2.  Iteration t refills
+        // set t % PF (slice t is already in LDS) with slice t+PF, computes slice t and stores
+        // slice t+1 (loaded PF iterations earlier) into the other buffer.  In the main part the
+        // refills are unconditional (past the end they reload the last slice, unused) so that
+        // the compiler can count the loads in flight and wait only for the slice it stores;
+        // the last nfull % PF slices were loaded by then.
         const int nfull = segn / kKV;
         if (nfull > 0) {
-            load_full(0);
-            store_slice(0);
+#pragma unroll
+            for (int u = 0; u < PF; ++u) load_full(min(u, nfull - 1) * kKV, u);
+            store_slice(0, 0);
         }
         __syncthreads();
-        for (int t = 0; t < nfull; ++t) {
-            const int cur = t & 1;
-            if (t + 1 < nfull) load_full((t + 1) * kKV);
-            compute_slice(cur);
-            if (t + 1 < nfull) store_slice(cur ^ 1);
-            __syncthreads();
+        // main part: whole rounds of PF slices, straight-line (a store past the end writes
+        // the reloaded last slice into the idle buffer, never computed)
+        int t0 = 0;
+        for (; t0 + PF <= nfull; t0 += PF) {
+#pragma unroll
+            for (int u = 0; u < PF; ++u) {
+                const int t = t0 + u;
+                load_full(min(t + PF, nfull - 1) * kKV, u);
+                compute_slice(t & 1);
+                store_slice((t + 1) & 1, (u + 1) % PF);
+                __syncthreads();
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < PF; ++u) {  // the last nfull % PF slices
+            const int t = t0 + u;
+            if (t < nfull) {
+                compute_slice(t & 1);
+                if (t + 1 < nfull) store_slice((t + 1) & 1, (u + 1) % PF);
+                __syncthreads();
+            }
         }
         if (nfull * kKV < segn) {
             load_tail(nfull * kKV);
-            store_slice(0);
+            store_slice(0, 0);
             __syncthreads();
             compute_slice(0);
             __syncthreads();

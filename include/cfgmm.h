@@ -318,7 +318,9 @@ enum cfgmm_tunable {
     CFGMM_TUNE_RESIDENT_DEPTH = 6,   /* resident rho: k steps of A prefetch, 1 or 2 (default 1) */
     CFGMM_TUNE_RESIDENT_COLS = 7,    /* resident rho: grid points per workgroup, 16/32/48/64, 0 = widest
                                         dividing blksize (read at creation) */
-    CFGMM_TUNE_COUNT = 8
+    CFGMM_TUNE_RHO_ORDER = 8,        /* FP64 rho: 0 = chunks ordered by the AO range of their rows
+                                        (default), 1 = block-major */
+    CFGMM_TUNE_COUNT = 9
 };
 int cfgmm_tune_set(int param, int value);
 int cfgmm_tune_get(int param, int *value);

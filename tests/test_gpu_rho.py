@@ -141,6 +141,42 @@ def test_rho_deterministic(workload):
     assert np.array_equal(outs[0], outs[1])
 
 
+@pytest.mark.parametrize("blksize", [64, 96])
+def test_rho_task_order_bitwise(blksize):
+    """AO-ordered and block-major task orders (CFGMM_TUNE_RHO_ORDER) with 1 and 3 density
+    matrices per workgroup give bitwise-equal results, repeated AO-ordered calls too (the order
+    inside a plan bucket varies), and match the oracle."""
+    p = W.rho_problem(seed=6, nshell=200, nao=640, ngrid=blksize * 23 + 17, nset=3, blksize=blksize, density=0.3)
+    s = cfgmm.Stream()
+    d_d = cfgmm.DeviceBuffer.from_array(p.dms)
+    d_phi = cfgmm.DeviceBuffer.from_array(p.phi)
+    d_tab = cfgmm.DeviceBuffer.from_array(np.ascontiguousarray(p.tab))
+    d_r = cfgmm.DeviceBuffer(8 * p.nset * p.ngrid)
+    old = (cfgmm.tune_get(cfgmm.TUNE_RHO_ORDER), cfgmm.tune_get(cfgmm.TUNE_RHO_MAX_SETS))
+    outs = []
+    try:
+        for order, ms in ((0, 1), (0, 1), (1, 1), (0, 3), (1, 3)):
+            cfgmm.tune_set(cfgmm.TUNE_RHO_ORDER, order)
+            cfgmm.tune_set(cfgmm.TUNE_RHO_MAX_SETS, ms)
+            d_r.zero()
+            cfgmm.aodmao2rho_dev(s, d_r, d_d, d_phi, d_phi, p.nao, p.ngrid, p.nao, p.ngrid, p.nset, d_tab, p.nblk,
+                                 p.blksize)
+            s.synchronize()
+            outs.append(d_r.to_array(np.float64, (p.nset * p.ngrid,)))
+    finally:
+        cfgmm.tune_set(cfgmm.TUNE_RHO_ORDER, old[0])
+        cfgmm.tune_set(cfgmm.TUNE_RHO_MAX_SETS, old[1])
+    for b in (d_d, d_phi, d_tab, d_r):
+        b.free()
+    s.destroy()
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0])
+    ref = np.zeros(p.nset * p.ngrid)
+    assert oracle.aodmao2rho(ref, p.dms.ravel(), p.phi.ravel(), p.phi.ravel(), p.nao, p.ngrid, p.nao, p.ngrid,
+                             p.nset, np.ascontiguousarray(p.tab), p.nblk, p.blksize) == 0
+    assert rel_fro(outs[0], ref) <= TOL
+
+
 def test_rho_max_sets_tunable_bitwise():
     """Density matrices per workgroup (1..4) change the launch shape only: results are bitwise equal."""
     p = W.rho_problem(seed=4, nshell=180, nao=560, ngrid=2048, nset=4, density=0.35)
