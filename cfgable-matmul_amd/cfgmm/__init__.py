@@ -73,6 +73,8 @@ _SIGNATURES = {
     "cfgmm_aodmao2rho_ex_dev": ([_VP, _VP, _VP, _VP, _VP] + [_SZ] * 5 + [_VP, _SZ, _SZ, _I, _I], _I),
     "cfgmm_aodmao2rho_multi_dev": ([_VP, _VP, _SZ, _VP, _VP, _SZ, _SZ, _VP] + [_SZ] * 5 + [_VP, _SZ, _SZ], _I),
     "cfgmm_aogridao2mat_dev": ([_VP, _VP, _SZ, _VP, _VP, _VP] + [_SZ] * 3 + [_VP, _SZ, _SZ], _I),
+    "cfgmm_eval_gto_dev": ([_VP, _VP, _SZ, _VP, _SZ, _VP, _SZ, _VP, _VP, _SZ, _VP, _SZ, _SZ, _SZ,
+                            ctypes.c_double], _I),
     "cfgmm_aodmao2rho_batched_dev": ([_VP, _VP, _SZ, _I, _I], _I),
     "cfgmm_rho_resident_create": ([_VP, _VP, _VP, _VP] + [_SZ] * 5 + [_VP, _SZ, _SZ, _I, _I, ctypes.POINTER(_VP)], _I),
     "cfgmm_rho_resident_eval": ([_VP, _VP, _VP, _SZ, _SZ, _SZ], _I),
@@ -378,6 +380,32 @@ def aogridao2mat_dev(stream, V, ldv, ao_lhs, ao_rhs, w, stride_grid, nao, ngrid,
     check(lib().cfgmm_aogridao2mat_dev(_h(stream), _dev(V), ldv, _dev(ao_lhs), _dev(ao_rhs),
                                        None if w is None else _dev(w), stride_grid, nao, ngrid, _dev(tab), ldtab,
                                        blksize))
+
+
+class GtoShell(ctypes.Structure):
+    """cfgmm_gto_shell: a Cartesian GTO shell (centre, l <= 4, primitive range, first AO)."""
+    _fields_ = [("center", ctypes.c_double * 3), ("l", _I), ("nprim", _I), ("prim_off", _I), ("ao_off", _I)]
+
+
+def gto_shells(centers, ls, nprims, prim_offs, ao_offs):
+    """A ctypes array of GtoShell from per-shell sequences."""
+    arr = (GtoShell * len(ls))()
+    for i, (c, l, n, po, ao) in enumerate(zip(centers, ls, nprims, prim_offs, ao_offs)):
+        arr[i].center[:] = [float(x) for x in c]
+        arr[i].l, arr[i].nprim, arr[i].prim_off, arr[i].ao_off = int(l), int(n), int(po), int(ao)
+    return arr
+
+
+def eval_gto_dev(stream, phi, stride_grid, tab, ldtab, shells, exps, coefs, coords, ngrid, nao, blksize, cutoff):
+    """Tabulates the Cartesian GTO shells (a GtoShell array; exps/coefs: host float64 arrays) at the
+    device grid coords (ngrid x 3) into phi (nao x stride_grid) and the per-AO block mask tab."""
+    exps = np.ascontiguousarray(exps, dtype=np.float64)
+    coefs = np.ascontiguousarray(coefs, dtype=np.float64)
+    if exps.shape != coefs.shape:
+        raise ValueError("exps and coefs differ in length")
+    check(lib().cfgmm_eval_gto_dev(_h(stream), _dev(phi), stride_grid, _dev(tab), ldtab, ctypes.addressof(shells), len(shells),
+                                   exps.ctypes.data, coefs.ctypes.data, exps.size, _dev(coords), ngrid, nao,
+                                   blksize, float(cutoff)))
 
 
 class RhoProblem(ctypes.Structure):

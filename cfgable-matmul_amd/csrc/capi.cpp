@@ -547,6 +547,35 @@ int cfgmm_aogridao2mat_dev(void *stream, double *V, size_t ldv, const double *ao
                       ldtab, blksize);
 }
 
+int cfgmm_eval_gto_dev(void *stream, double *phi, size_t stride_grid, uint8_t *tab, size_t ldtab,
+                       const cfgmm_gto_shell *shells, size_t nshell, const double *exps, const double *coefs,
+                       size_t nprim_total, const double *coords, size_t ngrid, size_t nao, size_t blksize,
+                       double cutoff) {
+    clear_error();
+    CFGMM_TRY(check_blksize(blksize));
+    const bool any = nshell && ngrid;
+    CFGMM_TRY(check_ld("stride_grid", stride_grid, ngrid, any));
+    CFGMM_TRY(check_ld("ldtab", ldtab, ngrid ? div_up(ngrid, blksize) : 0, any));
+    if (!(cutoff >= 0.0)) return fail(CFGMM_EINVAL, "cutoff must be >= 0");
+    if (nshell && !shells) return fail(CFGMM_EINVAL, "shells is null");
+    if (nprim_total && (!exps || !coefs)) return fail(CFGMM_EINVAL, "exps / coefs are null");
+    if (nao > INT32_MAX || ngrid > INT32_MAX || nshell > INT32_MAX || nprim_total > INT32_MAX)
+        return fail(CFGMM_EINVAL, "gto: size out of range");
+    for (size_t i = 0; i < nshell; ++i) {
+        const cfgmm_gto_shell &sh = shells[i];
+        const std::string who = "shell " + std::to_string(i) + ": ";
+        if (sh.l < 0 || sh.l > 4) return fail(CFGMM_EINVAL, who + "l must be 0..4");
+        if (sh.nprim < 1 || sh.prim_off < 0 || (size_t)sh.prim_off + sh.nprim > nprim_total)
+            return fail(CFGMM_EINVAL, who + "primitives outside exps / coefs");
+        const int nf = (sh.l + 1) * (sh.l + 2) / 2;
+        if (sh.ao_off < 0 || (size_t)sh.ao_off + nf > nao) return fail(CFGMM_EINVAL, who + "AOs outside nao");
+    }
+    if (any && (!phi || !tab || !coords)) return fail(CFGMM_EINVAL, "null device pointer");
+    CFGMM_TRY(check_device());
+    return launch_gto(static_cast<hipStream_t>(stream), phi, stride_grid, tab, ldtab, shells, nshell, exps, coefs,
+                      nprim_total, coords, ngrid, blksize, cutoff);
+}
+
 int cfgmm_aodmao2rho_batched_dev(void *stream, const cfgmm_rho_problem *problems, size_t nproblems, int precision,
                                  int nslices) {
     clear_error();

@@ -101,6 +101,10 @@ int resident_device(const ResidentRho *h);
 // The AO-grid-AO contraction after rho (vxc.hip).
 int launch_vxc(hipStream_t stream, double *V, size_t ldv, const double *ao_lhs, const double *ao_rhs, const double *w,
                size_t stride_grid, size_t nao, size_t ngrid, const uint8_t *tab, size_t ldtab, size_t blksize);
+// cfgmm_eval_gto_dev (shells, exps, coefs: host arrays, validated by the caller)
+int launch_gto(hipStream_t stream, double *phi, size_t stride_grid, uint8_t *tab, size_t ldtab,
+               const cfgmm_gto_shell *shells, size_t nshell, const double *exps, const double *coefs,
+               size_t nprim_total, const double *coords, size_t ngrid, size_t blksize, double cutoff);
 
 // Serving helpers (serve_kernels.hip) and the request loop (server.cpp).
 int launch_fill_uniform(hipStream_t stream, double *x, size_t n, unsigned long long seed);

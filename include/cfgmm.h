@@ -185,6 +185,26 @@ int cfgmm_aogridao2mat_dev(void *stream, double *V, size_t ldv, const double *ao
                            const double *w, size_t stride_grid, size_t nao, size_t ngrid, const uint8_t *tab,
                            size_t ldtab, size_t blksize);
 
+/* AO tabulation on the grid, the step before rho (no counterpart in the reference, which
+ * takes phi and the mask as inputs; the mask plays the role of PySCF's non0tab, README.md:49).
+ * Cartesian GTO shells: shell s (host array, nshell entries) centred at R = center, angular
+ * momentum l <= 4, contracted over exps/coefs[prim_off .. prim_off + nprim) (host arrays, the
+ * coefficients as given: any normalisation is the caller's), owning AOs ao_off .. ao_off +
+ * (l+1)(l+2)/2 - 1 in the order lx = l..0, then ly = l-lx..0, lz = l-lx-ly (PySCF's Cartesian
+ * order):  phi[(ao_off+c)*stride_grid + g] = dx^lx dy^ly dz^lz sum_k coefs[k] exp(-exps[k] r^2),
+ * d = coords[g] - R (coords: device, ngrid x 3).  The mask: tab[(ao_off+c)*ldtab + b] = 1 when
+ * max over the points of block b and the components of the shell of |phi| > cutoff, else 0, and
+ * then phi is zeroed on that block.  Rows of AOs owned by no shell, and columns g >= ngrid, are
+ * left untouched.  Synchronises the stream before returning (the shell tables are staged). */
+typedef struct cfgmm_gto_shell {
+    double center[3];
+    int l, nprim, prim_off, ao_off;
+} cfgmm_gto_shell;
+int cfgmm_eval_gto_dev(void *stream, double *phi, size_t stride_grid, uint8_t *tab, size_t ldtab,
+                       const cfgmm_gto_shell *shells, size_t nshell, const double *exps, const double *coefs,
+                       size_t nprim_total, const double *coords, size_t ngrid, size_t nao, size_t blksize,
+                       double cutoff);
+
 /* ======================================================================== */
 /* 2c. batched rho (no counterpart in the reference)                        */
 /* ======================================================================== */
