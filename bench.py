@@ -21,6 +21,8 @@ The same JSON line carries:
   * rho_4_components: four left-hand factors (GGA-shaped: phi and three gradient-like
     arrays) over one D.phi pass vs four separate evaluations;
   * aogridao2mat: the contraction after rho, V_uv += sum_g phi_ug w_g phi_vg (masked);
+  * gto_tabulation: the step before rho, phi and the mask from GTO shells on the device, and
+    tabulation + rho with everything device-resident;
   * host_pointer_path: the reference-shaped host-buffer call, PCIe included.
 
 Other workloads (one JSON line each):
@@ -235,6 +237,69 @@ def bench_vxc(p: W.RhoProblem, reps: int = 10):
     n_b = p.counts().astype(np.float64)
     flops = float(np.sum(p.blksize * 2.0 * n_b * n_b))
     return {"ms": round(ms, 4), "masked_gflop": round(flops / 1e9, 2), "tflops": round(flops / (ms * 1e-3) / 1e12, 2)}
+
+
+def bench_gto(p: W.RhoProblem, reps: int = 10):
+    """On-device tabulation of a Cartesian basis on a grid like the rho workload's (700 shells,
+    40 % s / 40 % p / 20 % d, 1-3 primitives, 16384 points in 256 blocks of a 25 A box), then
+    rho on the tabulated phi and mask with D already on the device: the device-resident form
+    of the step the host-pointer path feeds over PCIe.  Algorithmic bytes: phi and the mask
+    written once (the kernel reads only the grid and the shell tables)."""
+    rng = np.random.default_rng(11)
+    nshell, box = 700, 25.0
+    ls = rng.choice(3, size=nshell, p=[0.4, 0.4, 0.2])
+    nprim = rng.integers(1, 4, size=nshell)
+    poff = np.concatenate([[0], np.cumsum(nprim)[:-1]])
+    nf = (ls + 1) * (ls + 2) // 2
+    aoff = np.concatenate([[0], np.cumsum(nf)[:-1]])
+    nao = int(nf.sum())
+    centers = rng.uniform(0.0, box, (nshell, 3))
+    exps = 10.0 ** rng.uniform(-0.7, 0.5, int(nprim.sum()))
+    coefs = rng.uniform(0.3, 1.2, int(nprim.sum()))
+    nblk = p.nblk
+    blk_c = rng.uniform(0.0, box, (nblk, 3))
+    coords = np.ascontiguousarray(blk_c[np.arange(p.ngrid) // p.blksize] + rng.uniform(-0.5, 0.5, (p.ngrid, 3)))
+    shells = cfgmm.gto_shells(centers, ls, nprim, poff, aoff)
+    s = cfgmm.Stream()
+    d_phi = cfgmm.DeviceBuffer(8 * nao * p.ngrid)
+    d_tab = cfgmm.DeviceBuffer(nao * nblk)
+    d_xyz = cfgmm.DeviceBuffer.from_array(coords)
+    cutoff = 1e-8
+
+    def tab_once():
+        cfgmm.eval_gto_dev(s, d_phi, p.ngrid, d_tab, nblk, shells, exps, coefs, d_xyz, p.ngrid, nao, p.blksize, cutoff)
+    tab_once()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        tab_once()  # returns after a stream synchronisation
+    ms = (time.perf_counter() - t0) / reps * 1e3
+    tab = d_tab.to_array(np.uint8, (nao, nblk))
+    # rho on the tabulated operands (D of the basis size, on the device)
+    C = rng.standard_normal((p.nset, nao, 100))
+    d_d = cfgmm.DeviceBuffer.from_array(np.einsum("sik,sjk->sij", C, C) / 100)
+    d_r = cfgmm.DeviceBuffer(8 * p.nset * p.ngrid)
+    d_r.zero()
+
+    def pipeline():
+        tab_once()
+        cfgmm.aodmao2rho_dev(s, d_r, d_d, d_phi, d_phi, nao, p.ngrid, nao, p.ngrid, p.nset, d_tab, nblk, p.blksize)
+    pipeline()
+    s.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        pipeline()
+    s.synchronize()
+    pipe_ms = (time.perf_counter() - t0) / reps * 1e3
+    for b in (d_phi, d_tab, d_xyz, d_d, d_r):
+        b.free()
+    s.destroy()
+    nbytes = 8.0 * nao * p.ngrid + nao * nblk
+    return {"nshell": nshell, "nao": nao, "ngrid": p.ngrid, "mask_density": round(float(tab.mean()), 3),
+            "ms": round(ms, 4), "written_GB": round(nbytes / 1e9, 3),
+            "achieved_GBs": round(nbytes / (ms * 1e-3) / 1e9, 1),
+            "frac_hbm": round(nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 3),
+            "tabulate_then_rho_ms": round(pipe_ms, 4),
+            "note": "ms includes staging the shell tables and one stream synchronisation"}
 
 
 def bench_dgemm(g: W.GemmProblem, reps: int, warmup: int = 2):
@@ -739,6 +804,10 @@ def main():
     log("[bench] AO-grid-AO contraction (V_uv from phi, weights, mask)")
     out_vxc = bench_vxc(p)
 
+    # the step before rho on the device: phi and the mask tabulated from a basis and the grid
+    log("[bench] AO tabulation on the grid (GTO shells -> phi, mask), then rho")
+    out_gto = bench_gto(p)
+
     # the host-pointer drop-in (reference call shape): PCIe transfers included, never `value`
     log("[bench] host-pointer drop-in path (H2D of phi, D and mask, D2H of rho)")
     hr = np.zeros(p.nset * p.ngrid)
@@ -768,6 +837,7 @@ def main():
     out["split6_resident"] = out_split
     out["rho_4_components"] = out_gga
     out["aogridao2mat"] = out_vxc
+    out["gto_tabulation"] = out_gto
 
     if not args.no_dgemm:
         dg = {}
