@@ -12,7 +12,7 @@
 //      row operand in registers, the next tile's column operand loaded during the current
 //      tile's MFMAs); S_b lives in a workspace of sum_b kpad_b^2 doubles (kpad_b = |L_b|
 //      rounded up to 64).  The strips of one block run back to back on one XCD.
-//   2. vxc_reduce_kernel: each wave owns one row u of V over a 512-column tile and streams, for
+//   2. vxc_reduce_kernel: each wave owns one row u of V over a 256-column tile and streams, for
 //      the blocks in increasing order where u is active, the contiguous segment of row
 //      pos_b(u) of S_b whose columns L_b[k] fall in the tile, adding it into an LDS copy of
 //      the row (pos_b(x) = index of x in L_b).  Every element of S_b is read once, by full
@@ -30,7 +30,8 @@ constexpr int kKC = 64;     // grid points staged per LDS pass of the block kern
 constexpr int kStrip = 12;  // S_b tiles per block-kernel workgroup (at most)
 static_assert(kStrip * kT % kThreads == 0, "column indexes of a strip: whole passes of the workgroup");
 constexpr int kRW = 4;      // V rows per reduce workgroup (one per wave)
-constexpr int kRC = 512;    // V columns per reduce tile
+constexpr int kRC = 256;    // V columns per reduce tile
+constexpr int kPH = 4;      // active blocks whose segments are loaded together in the reduction
 constexpr int kSeg = kRC / 64;  // 64-wide pieces of one S_b row segment
 
 // pos[b*nao + x] = index of x in L_b, or -1.
@@ -249,7 +250,7 @@ __global__ __launch_bounds__(kThreads, 3) void vxc_block_kernel(
     }
 }
 
-// Rows 4*blockIdx.x .. +3 of V (one per wave) over columns v0 .. v0+511, v0 = 512 * blockIdx.y.
+// Rows 4*blockIdx.x .. +3 of V (one per wave) over columns v0 .. v0+255, v0 = 256 * blockIdx.y.
 __global__ __launch_bounds__(kThreads) void vxc_reduce_kernel(const int *__restrict__ pos,
                                                                const int *__restrict__ lists,
                                                                const int *__restrict__ kr,
@@ -275,16 +276,17 @@ __global__ __launch_bounds__(kThreads) void vxc_reduce_kernel(const int *__restr
         const int kp = kpads[bq];
         iu = ok ? iu : -1;
         unsigned long long act = __ballot(iu >= 0 && khi > klo);
-        while (act) {  // two active blocks per step, in increasing order
-            int ls[2];
-            ls[0] = __builtin_ctzll(act);
-            act &= act - 1;
-            ls[1] = act ? __builtin_ctzll(act) : -1;
-            if (act) act &= act - 1;
-            double xs[2][kSeg];
-            int xc[2][kSeg];
+        while (act) {  // kPH active blocks per step, in increasing order
+            int ls[kPH];
 #pragma unroll
-            for (int h = 0; h < 2; ++h) {
+            for (int h = 0; h < kPH; ++h) {
+                ls[h] = act ? __builtin_ctzll(act) : -1;
+                act &= act - 1;  // (0 stays 0)
+            }
+            double xs[kPH][kSeg];
+            int xc[kPH][kSeg];
+#pragma unroll
+            for (int h = 0; h < kPH; ++h) {
 #pragma unroll
                 for (int c = 0; c < kSeg; ++c) {
                     xs[h][c] = 0.0;
@@ -311,7 +313,7 @@ __global__ __launch_bounds__(kThreads) void vxc_reduce_kernel(const int *__restr
                 }
             }
 #pragma unroll
-            for (int h = 0; h < 2; ++h)
+            for (int h = 0; h < kPH; ++h)
 #pragma unroll
                 for (int c = 0; c < kSeg; ++c)
                     if (xc[h][c] >= 0)  // ds_add_f64: only this wave touches the row, in program order
