@@ -4,9 +4,11 @@
 // src/impl_aodmao2rho.rs:171-244); each problem keeps exactly that contract.
 //
 //   1. batch_build_lists: the compacted AO lists of every (problem, grid block).
-//   2. batch_make_tasks:  one task per (problem, block, 64-point subtile,
-//      64-row chunk of the block's list, group of density matrices), appended
-//      to a device task array; problems are numbered largest-first by the host.
+//   2. batch_count_chunks / batch_scan_offsets / batch_fill_tasks: one task per
+//      (problem, block, 64-point subtile, 64-row chunk of the block's list, group
+//      of density matrices) in a device queue; problems are numbered
+//      largest-first by the host, each problem's tasks are contiguous and ordered
+//      by the AO range of the chunk rows.
 //   3. batch_rho_kernel:  a persistent grid (CUs x workgroups-per-CU, both
 //      tunable) pulls tasks from per-XCD ranges of the array (atomic heads,
 //      stealing from the other ranges at the end) until all are drained and
@@ -62,23 +64,112 @@ __global__ __launch_bounds__(256) void batch_build_lists(const BatchProblem *__r
 }
 
 // Task word: x = problem, y = block, z = subtile << 16 | chunk, w = first density matrix.
-__global__ void batch_make_tasks(const BatchProblem *__restrict__ probs, const int *__restrict__ counts, int nsmax,
-                                 int4 *__restrict__ tasks, int *__restrict__ ntasks) {
-    const int pi = blockIdx.y;
+//
+// The queue holds the problems one after another (largest first); inside a problem the tasks
+// are ordered (set group, chunk in AO order, subtile): the chunks are sorted by the AO range of
+// their rows (bucketed midpoint of the first and last row), so that tasks sharing rows of D
+// run together on one XCD and find them in its L2 (the per-problem kernel's plan, rho.hip).
+// Three steps: chunk counts per problem, an exclusive scan into queue offsets, the fill.  The
+// order inside an AO bucket follows atomics; no result depends on it.
+constexpr int kFillThreads = 256;
+constexpr int kFillBuckets = 256;
+
+// cnum[pi] = number of non-empty chunks of problem pi
+__global__ __launch_bounds__(kFillThreads) void batch_count_chunks(const BatchProblem *__restrict__ probs,
+                                                                   const int *__restrict__ counts,
+                                                                   int *__restrict__ cnum) {
+    const BatchProblem p = probs[blockIdx.x];
+    __shared__ int tot;
+    if (threadIdx.x == 0) tot = 0;
+    __syncthreads();
+    int mine = 0;
+    for (int b = threadIdx.x; b < p.nblk; b += kFillThreads) mine += (counts[p.counts_off + b] + kUC - 1) / kUC;
+    atomicAdd(&tot, mine);
+    __syncthreads();
+    if (threadIdx.x == 0) cnum[blockIdx.x] = tot;
+}
+
+// toff[pi] = first queue slot of problem pi (tasks: groups x chunks x subtiles); ctr[0] = total
+__global__ __launch_bounds__(1024) void batch_scan_offsets(const BatchProblem *__restrict__ probs, int np, int nsmax,
+                                                           const int *__restrict__ cnum, int *__restrict__ toff,
+                                                           int *__restrict__ ctr) {
+    __shared__ long part[1024];
+    const int tid = threadIdx.x;
+    // each thread owns a contiguous run of problems
+    const int per = (np + 1023) / 1024, lo = min(np, tid * per), hi = min(np, lo + per);
+    long sum = 0;
+    for (int i = lo; i < hi; ++i) {
+        const BatchProblem &p = probs[i];
+        sum += (long)cnum[i] * ((p.nset + nsmax - 1) / nsmax) * p.nsub;
+    }
+    part[tid] = sum;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {
+        const long add = tid >= d ? part[tid - d] : 0;
+        __syncthreads();
+        part[tid] += add;
+        __syncthreads();
+    }
+    long run = part[tid] - sum;
+    for (int i = lo; i < hi; ++i) {
+        const BatchProblem &p = probs[i];
+        toff[i] = (int)run;
+        run += (long)cnum[i] * ((p.nset + nsmax - 1) / nsmax) * p.nsub;
+    }
+    if (tid == 1023) ctr[0] = (int)part[1023];
+}
+
+__global__ __launch_bounds__(kFillThreads) void batch_fill_tasks(const BatchProblem *__restrict__ probs,
+                                                                 const int *__restrict__ lists,
+                                                                 const int *__restrict__ counts, int nsmax,
+                                                                 const int *__restrict__ cnum,
+                                                                 const int *__restrict__ toff,
+                                                                 int4 *__restrict__ tasks) {
+    const int pi = blockIdx.x;
     const BatchProblem p = probs[pi];
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= p.nblk) return;
-    const int cnt = counts[p.counts_off + b];
-    const int nch = (cnt + kUC - 1) / kUC;
-    int nsub = 0;
-    while (nsub < p.nsub && b * p.blksize + nsub * p.nt * 16 < p.ngrid) ++nsub;
-    const int ngrp = (p.nset + nsmax - 1) / nsmax;
-    // one contiguous run per block, ordered (subtile, set group, chunk): the tasks sharing the
-    // block's phi tile are neighbours in the queue
-    int t = atomicAdd(ntasks, nsub * ngrp * nch);
-    for (int sub = 0; sub < nsub; ++sub)
+    __shared__ int hist[kFillBuckets];
+    const int tid = threadIdx.x;
+    const int maxch = (p.nao + kUC - 1) / kUC;
+    const long nslot = (long)p.nblk * maxch;
+    auto key = [&](long sl) -> int {  // AO bucket of chunk slot sl, -1 if empty
+        const int b = (int)(sl / maxch), c = (int)(sl % maxch);
+        const int n = counts[p.counts_off + b];
+        if (c * kUC >= n) return -1;
+        const int *__restrict__ lb = lists + p.lists_off + (long)b * p.nao;
+        const long mid = ((long)lb[c * kUC] + lb[min(c * kUC + kUC, n) - 1]) / 2;
+        return (int)(mid * kFillBuckets / p.nao);
+    };
+    static_assert(kFillBuckets == kFillThreads, "one bucket per thread");
+    hist[tid] = 0;
+    __syncthreads();
+    for (long sl = tid; sl < nslot; sl += kFillThreads) {
+        const int k = key(sl);
+        if (k >= 0) atomicAdd(&hist[k], 1);
+    }
+    __syncthreads();
+    const int mine = hist[tid];
+    int incl = mine;
+    for (int d = 1; d < kFillBuckets; d <<= 1) {
+        __syncthreads();
+        const int add = tid >= d ? hist[tid - d] : 0;
+        __syncthreads();
+        incl += add;
+        hist[tid] = incl;
+    }
+    __syncthreads();
+    hist[tid] = incl - mine;
+    __syncthreads();
+    const int nchunks = cnum[pi], ngrp = (p.nset + nsmax - 1) / nsmax;
+    int4 *__restrict__ out = tasks + toff[pi];
+    for (long sl = tid; sl < nslot; sl += kFillThreads) {
+        const int k = key(sl);
+        if (k < 0) continue;
+        const int pos = atomicAdd(&hist[k], 1);
+        const int b = (int)(sl / maxch), c = (int)(sl % maxch);
         for (int g = 0; g < ngrp; ++g)
-            for (int ch = 0; ch < nch; ++ch) tasks[t++] = int4{pi, b, (sub << 16) | ch, g * nsmax};
+            for (int sub = 0; sub < p.nsub; ++sub)
+                out[((long)g * nchunks + pos) * p.nsub + sub] = int4{pi, b, (sub << 16) | c, g * nsmax};
+    }
 }
 
 __global__ void zero_int_kernel(int *p, int n) {
@@ -144,6 +235,7 @@ __global__ __launch_bounds__(kThreads) void batch_rho_kernel(const BatchProblem 
         t.cnt = counts[p.counts_off + w.y];
         t.chunk = w.z & 0xffff;
         t.g0 = w.y * p.blksize + (w.z >> 16) * p.nt * 16;
+        if (t.g0 >= p.ngrid) continue;  // subtile past the end of a partial last block (uniform)
         const int ns = min(NSMAX, p.nset - w.w);
         if (NSMAX >= 4 && ns == 4)
             run_chunk_ns<(NSMAX >= 4 ? 4 : 1)>(p.nt, t, smem);
@@ -240,7 +332,9 @@ int launch_rho_batched(hipStream_t stream, const cfgmm_rho_problem *problems, si
     const size_t o_counts = o_lists + al(sizeof(int) * lists_n);
     const size_t o_part = o_counts + al(sizeof(int) * counts_n);
     const size_t o_tasks = o_part + al(sizeof(double) * partial_n);
-    const size_t o_ctr = o_tasks + al(sizeof(int4) * tasks_max);
+    const size_t o_cnum = o_tasks + al(sizeof(int4) * tasks_max);
+    const size_t o_toff = o_cnum + al(sizeof(int) * np);
+    const size_t o_ctr = o_toff + al(sizeof(int) * np);
     void *ws = nullptr;
     int st = workspace_get(stream, o_ctr + 9 * sizeof(int), &ws);
     if (st != CFGMM_OK) return st;
@@ -250,6 +344,8 @@ int launch_rho_batched(hipStream_t stream, const cfgmm_rho_problem *problems, si
     int *counts = reinterpret_cast<int *>(base + o_counts);
     double *partial = reinterpret_cast<double *>(base + o_part);
     int4 *tasks = reinterpret_cast<int4 *>(base + o_tasks);
+    int *cnum = reinterpret_cast<int *>(base + o_cnum);
+    int *toff = reinterpret_cast<int *>(base + o_toff);
     int *ctr = reinterpret_cast<int *>(base + o_ctr);  // [0] = task count, [1..8] = heads of the XCD ranges
 
     // Pageable source: the copy is complete when hipMemcpyAsync returns, so `bp` may go out of scope.
@@ -258,8 +354,10 @@ int launch_rho_batched(hipStream_t stream, const cfgmm_rho_problem *problems, si
     hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(64), 0, stream, ctr, 9);
     hipLaunchKernelGGL(batch_build_lists, dim3((unsigned)max_nblk, (unsigned)np), dim3(256), 0, stream, d_probs,
                        lists, counts);
-    hipLaunchKernelGGL(batch_make_tasks, dim3((unsigned)div_up(max_nblk, 64), (unsigned)np), dim3(64), 0, stream,
-                       d_probs, counts, nsmax, tasks, ctr);
+    hipLaunchKernelGGL(batch_count_chunks, dim3((unsigned)np), dim3(kFillThreads), 0, stream, d_probs, counts, cnum);
+    hipLaunchKernelGGL(batch_scan_offsets, dim3(1), dim3(1024), 0, stream, d_probs, np, nsmax, cnum, toff, ctr);
+    hipLaunchKernelGGL(batch_fill_tasks, dim3((unsigned)np), dim3(kFillThreads), 0, stream, d_probs, lists, counts,
+                       nsmax, cnum, toff, tasks);
     trace_stop(stream, "batch_prepare", t0);
     st = hip_status(hipGetLastError(), "batched rho prepare launch");
     if (st != CFGMM_OK) return st;
