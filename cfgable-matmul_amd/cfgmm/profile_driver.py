@@ -11,6 +11,7 @@ Modes:
   dgemm_small    FP64 DGEMM 1031x977x1543 NT
   gemm_split<S>  the large GEMM in split precision (ozaki_gemm_kernel)
   vxc            V += phi w phi^T over the same mask (vxc_block_kernel, vxc_reduce_kernel)
+  gto            AO tabulation of workloads.gto_workload (700 shells, 1890 AOs, 16384 points)
   copy           a 256 MiB device-to-device copy (byte-counter calibration)
 
 Each pass of a hardware-counter profiler is a separate process running this
@@ -73,6 +74,15 @@ def main(argv: list[str]) -> int:
         d_tab = cfgmm.DeviceBuffer.from_array(np.ascontiguousarray(p.tab))
         for _ in range(reps):
             cfgmm.aogridao2mat_dev(s, d_v, p.nao, d_phi, d_phi, d_w, p.ngrid, p.nao, p.ngrid, d_tab, p.nblk, p.blksize)
+    elif mode == "gto":
+        gw = W.gto_workload()
+        shells = cfgmm.gto_shells(gw["centers"], gw["ls"], gw["nprim"], gw["poff"], gw["aoff"])
+        d_phi = cfgmm.DeviceBuffer(8 * gw["nao"] * gw["ngrid"])
+        d_tab = cfgmm.DeviceBuffer(gw["nao"] * gw["nblk"])
+        d_xyz = cfgmm.DeviceBuffer.from_array(gw["coords"])
+        for _ in range(reps):
+            cfgmm.eval_gto_dev(s, d_phi, gw["ngrid"], d_tab, gw["nblk"], shells, gw["exps"], gw["coefs"], d_xyz,
+                               gw["ngrid"], gw["nao"], gw["blksize"], gw["cutoff"])
     elif mode == "copy":
         nbytes = 256 << 20
         a = cfgmm.DeviceBuffer(nbytes)

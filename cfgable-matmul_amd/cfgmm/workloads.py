@@ -399,3 +399,23 @@ def serving_requests(pool, n, rate, seed=11, rho_fraction=0.7, gemm_range=(256, 
             m, nn, k = (int(round(np.exp(rng.uniform(lo, hi)))) for _ in range(3))
             out.append(ServedRequest(1, float(t[i]), m=m, n=nn, k=k, seed=int(rng.integers(1, 2 ** 30))))
     return out
+
+
+def gto_workload(ngrid=16384, blksize=64, nshell=700, box=25.0, seed=11):
+    """A Cartesian GTO basis on a grid like the rho workload's: nshell shells uniform in a box,
+    40 % s / 40 % p / 20 % d, 1-3 primitives (exponents 10^U(-0.7, 0.5), coefficients
+    U(0.3, 1.2)), grid points jittered by U(-0.5, 0.5) around ngrid/blksize random block centres."""
+    rng = np.random.default_rng(seed)
+    ls = rng.choice(3, size=nshell, p=[0.4, 0.4, 0.2])
+    nprim = rng.integers(1, 4, size=nshell)
+    poff = np.concatenate([[0], np.cumsum(nprim)[:-1]])
+    nf = (ls + 1) * (ls + 2) // 2
+    aoff = np.concatenate([[0], np.cumsum(nf)[:-1]])
+    centers = rng.uniform(0.0, box, (nshell, 3))
+    exps = 10.0 ** rng.uniform(-0.7, 0.5, int(nprim.sum()))
+    coefs = rng.uniform(0.3, 1.2, int(nprim.sum()))
+    nblk = -(-ngrid // blksize)
+    blk_c = rng.uniform(0.0, box, (nblk, 3))
+    coords = np.ascontiguousarray(blk_c[np.arange(ngrid) // blksize] + rng.uniform(-0.5, 0.5, (ngrid, 3)))
+    return dict(centers=centers, ls=ls, nprim=nprim, poff=poff, aoff=aoff, nao=int(nf.sum()), exps=exps,
+                coefs=coefs, coords=coords, ngrid=ngrid, blksize=blksize, nblk=nblk, cutoff=1e-8)
