@@ -66,6 +66,7 @@ _SIGNATURES = {
     "cfgmm_matmul_anyway_full_non0tab": ([_VP, _SZ, _VP, _SZ, _VP, _SZ] + [_SZ] * 6 + [_I, _I, _VP, _SZ, _SZ], _I),
     "cfgmm_dgemm_non0tab_dev": ([_VP, _VP, _VP, _VP] + [_SZ] * 6 + [_I, _I, _VP, _SZ, _SZ], _I),
     "cfgmm_dgemm_non0tab_ex_dev": ([_VP, _VP, _VP, _VP] + [_SZ] * 6 + [_I, _I, _VP, _SZ, _SZ], _I),
+    "cfgmm_sgemm_non0tab_dev": ([_VP, _VP, _VP, _VP] + [_SZ] * 6 + [_I, _I, _VP, _SZ, _SZ], _I),
     "cfgmm_aodmao2rho_loop_parallel": ([_VP, _SZ, _VP, _SZ, _VP, _SZ, _VP, _SZ] + [_SZ] * 5 + [_VP, _SZ, _SZ, _SZ],
                                        _I),
     "cfgmm_aodmao2rho_anyway": ([_VP, _SZ, _VP, _SZ, _VP, _SZ, _VP, _SZ] + [_SZ] * 5 + [_VP, _SZ, _SZ], _I),
@@ -571,6 +572,12 @@ def dgemm_non0tab_dev(stream, c, a, b, m, n, k, lda, ldb, ldc, transa, transb, t
     fn = lib().cfgmm_dgemm_non0tab_ex_dev if allow_transb else lib().cfgmm_dgemm_non0tab_dev
     check(fn(_h(stream), c.ptr, a.ptr, b.ptr, m, n, k, lda, ldb, ldc, int(transa), int(transb), tab.ptr, ldtab,
              blksize))
+
+
+def sgemm_non0tab_dev(stream, c, a, b, m, n, k, lda, ldb, ldc, transa, transb, tab: DeviceBuffer, ldtab, blksize):
+    """The masked GEMM with float32 buffers (cfgmm_sgemm_non0tab_dev; transb allowed, f32 accumulation)."""
+    check(lib().cfgmm_sgemm_non0tab_dev(_h(stream), c.ptr, a.ptr, b.ptr, m, n, k, lda, ldb, ldc, int(transa),
+                                        int(transb), tab.ptr, ldtab, blksize))
 
 
 def aodmao2rho_dev(stream, rhos, dms, ao_lhs, ao_rhs, stride_ao, stride_grid, nao, ngrid, nset, tab, ldtab, blksize,

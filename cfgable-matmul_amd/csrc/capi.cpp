@@ -471,6 +471,22 @@ int cfgmm_dgemm_non0tab_ex_dev(void *stream, double *c, const double *a, const d
     return masked_dev_common(stream, c, a, b, m, n, k, lda, ldb, ldc, transa, transb, tab, ldtab, blksize);
 }
 
+int cfgmm_sgemm_non0tab_dev(void *stream, float *c, const float *a, const float *b, size_t m, size_t n, size_t k,
+                            size_t lda, size_t ldb, size_t ldc, int transa, int transb, const uint8_t *tab,
+                            size_t ldtab, size_t blksize) {
+    clear_error();
+    CFGMM_TRY(check_blksize(blksize));
+    // extents and null checks do not depend on the element type
+    CFGMM_TRY(check_dev_gemm(reinterpret_cast<const double *>(c), reinterpret_cast<const double *>(a),
+                             reinterpret_cast<const double *>(b), m, n, k, lda, ldb, ldc, transa, transb));
+    if (m && n && k) {
+        CFGMM_TRY(check_ld("ldtab", ldtab, div_up(n, blksize), true));
+        if (!tab) return fail(CFGMM_EINVAL, "tab is null");
+    }
+    return launch_sgemm_non0tab(static_cast<hipStream_t>(stream), c, a, b, m, n, k, lda, ldb, ldc, transa != 0,
+                                transb != 0, tab, ldtab, blksize);
+}
+
 // ---- rho ------------------------------------------------------------------------
 int cfgmm_aodmao2rho_loop_parallel(double *rhos, size_t rhos_len, const double *dms, size_t dms_len,
                                    const double *ao_lhs, size_t ao_lhs_len, const double *ao_rhs, size_t ao_rhs_len,

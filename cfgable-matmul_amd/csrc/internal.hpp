@@ -65,6 +65,10 @@ int launch_dgemm(hipStream_t stream, double *c, const double *a, const double *b
 int launch_dgemm_non0tab(hipStream_t stream, double *c, const double *a, const double *b, size_t m, size_t n,
                          size_t k, size_t lda, size_t ldb, size_t ldc, bool transa, bool transb, const uint8_t *tab,
                          size_t ldtab, size_t blksize);
+// the same masked GEMM in f32 (v_mfma_f32_16x16x4f32)
+int launch_sgemm_non0tab(hipStream_t stream, float *c, const float *a, const float *b, size_t m, size_t n, size_t k,
+                         size_t lda, size_t ldb, size_t ldc, bool transa, bool transb, const uint8_t *tab,
+                         size_t ldtab, size_t blksize);
 int launch_rho(hipStream_t stream, double *rhos, const double *dms, const double *ao_lhs, const double *ao_rhs,
                size_t stride_ao, size_t stride_grid, size_t nao, size_t ngrid, size_t nset, const uint8_t *tab,
                size_t ldtab, size_t blksize);

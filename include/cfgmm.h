@@ -131,6 +131,13 @@ int cfgmm_dgemm_non0tab_ex_dev(void *stream, double *c, const double *a, const d
                                size_t k, size_t lda, size_t ldb, size_t ldc, int transa, int transb,
                                const uint8_t *tab, size_t ldtab, size_t blksize);
 
+/* The same masked GEMM with T = f32: the reference's element type is generic
+ * (src/impl_matmul_non0tab.rs, `T: TySimd`), but only f64 is implemented there
+ * (src/structs.rs:19-21).  Accumulates in f32 (v_mfma_f32_16x16x4f32); transb allowed. */
+int cfgmm_sgemm_non0tab_dev(void *stream, float *c, const float *a, const float *b, size_t m, size_t n, size_t k,
+                            size_t lda, size_t ldb, size_t ldc, int transa, int transb, const uint8_t *tab,
+                            size_t ldtab, size_t blksize);
+
 /* ======================================================================== */
 /* 2b. fused masked AO-DM-AO -> rho                                         */
 /* ======================================================================== */

@@ -89,3 +89,42 @@ def test_masked_gemm_transb_ex(blksize, ta):
     G, R = got.reshape(m, ldc), c_ref.reshape(m, ldc)
     assert rel_fro(G[:, :n], R[:, :n]) <= TOL
     assert np.array_equal(G[:, n:], c.reshape(m, ldc)[:, n:])
+
+
+@pytest.mark.parametrize("blksize", [16, 64])
+@pytest.mark.parametrize("ta,tb", [(False, False), (True, False), (False, True), (True, True)])
+def test_masked_sgemm_f32(blksize, ta, tb):
+    """cfgmm_sgemm_non0tab_dev (T = f32, not implemented in the reference): against the f64 oracle on
+    the same float32 values (relative Frobenius <= 1e-5, f32 accumulation over ~160 listed terms),
+    padded leading dimensions, a partial last column block, nonzero B under off bits; bitwise
+    repeatable."""
+    rng = np.random.default_rng(300 + blksize + 2 * ta + tb)
+    m, n, k = 211, 700, 530
+    lda = (m if ta else k) + 3
+    ldb = (k if tb else n) + 5
+    ldc = n + 2
+    a = rng.uniform(-1, 1, size=(k if ta else m) * lda).astype(np.float32)
+    b = rng.uniform(-1, 1, size=(n if tb else k) * ldb).astype(np.float32)
+    nblk = (n + blksize - 1) // blksize
+    tab = W.periodic_mask(k, nblk + 1, W.MASK_30)
+    c = rng.uniform(-1, 1, size=m * ldc).astype(np.float32)
+    bk = b.reshape(n, ldb)[:, :k].T if tb else b.reshape(k, ldb)[:, :n]  # op(B), k x n
+    c_ref = c.astype(np.float64)
+    assert oracle.matmul_non0tab(c_ref, a.astype(np.float64), np.ascontiguousarray(bk, dtype=np.float64).ravel(), m,
+                                 n, k, lda, n, ldc, ta, False, tab, nblk + 1, blksize) == 0
+    s = cfgmm.Stream()
+    dA, dB, dT = (cfgmm.DeviceBuffer.from_array(np.ascontiguousarray(x)) for x in (a, b, tab))
+    outs = []
+    for _ in range(2):
+        dC = cfgmm.DeviceBuffer.from_array(c)
+        cfgmm.sgemm_non0tab_dev(s, dC, dA, dB, m, n, k, lda, ldb, ldc, ta, tb, dT, nblk + 1, blksize)
+        s.synchronize()
+        outs.append(dC.to_array(np.float32, (m, ldc)))
+        dC.free()
+    for x in (dA, dB, dT):
+        x.free()
+    s.destroy()
+    assert np.array_equal(outs[0], outs[1])
+    G, R = outs[0].astype(np.float64), c_ref.reshape(m, ldc)
+    assert rel_fro(G[:, :n], R[:, :n]) <= 1e-5
+    assert np.array_equal(outs[0][:, n:], c.reshape(m, ldc)[:, n:])
