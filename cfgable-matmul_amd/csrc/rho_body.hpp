@@ -77,9 +77,11 @@ __device__ inline void rho_chunk_body(const RhoChunk &task, unsigned char *__res
     const int p_d = tid % kKV;
 
     // Register sets of slice operands: a slice's loads are issued PF slices before it is
-    // stored into LDS (a loaded global access takes a few microseconds, several slices' worth
-    // of MFMAs).  Sets cost VGPRs, so the wide many-DM tiles keep one.
-    constexpr int PF = NS == 1 ? 3 : (NS == 2 ? 2 : 1);
+    // stored into LDS.  A loaded global access takes a few microseconds, several slices' worth
+    // of MFMAs, but every set costs VGPRs: on the config-2 workload one set at four waves per
+    // SIMD (1.05 ms) beats two (1.07 ms) and three (1.11 ms) at three waves per SIMD, the
+    // waves of the other workgroups covering the latency.
+    constexpr int PF = 1;
     double rphi[PF][PHI_PER_T], rd[PF][NS][D_ROWS_PER_T];
     f64x4 acc[NS][NT];
 #pragma unroll
