@@ -11,7 +11,7 @@ namespace rho {
 constexpr int kThreads = 256;
 constexpr int kUC = 64;     // rows of L_b per workgroup (4 waves x 16)
 constexpr int kKV = 16;     // v slice depth per LDS stage
-constexpr int kSeg = 2048;  // compacted-list entries staged in LDS at a time
+constexpr int kSeg = 1536;  // compacted-list entries staged in LDS at a time (keeps <1,4> under 40 KB: 4 WGs per CU)
 
 // One unit of work: rows [chunk*64, chunk*64+64) of the compacted AO list of
 // one grid block, grid points [g0, g0 + 16*NT), density matrices [set0, set0+NS).

@@ -240,3 +240,32 @@ def test_rho_multi_component_bitwise_and_oracle():
     assert oracle.aodmao2rho(ref, p.dms.ravel(), lhs[2].ravel(), p.phi.ravel(), p.nao, p.ngrid, p.nao, p.ngrid,
                              p.nset, np.ascontiguousarray(p.tab), p.nblk, p.blksize) == 0
     assert rel_fro(multi[2].ravel() - r0[2].ravel(), ref - r0[2].ravel()) <= 1e-12
+
+
+def test_rho_long_lists_span_segments():
+    """Blocks with more active AOs than one LDS segment of the compacted list holds (~2000 of 3000,
+    staged 1536 at a time): both task orders against the oracle, bitwise equal to each other."""
+    rng = np.random.default_rng(9)
+    nao, ngrid, blksize, nset = 3000, 256 + 40, 64, 2
+    nblk = (ngrid + blksize - 1) // blksize
+    tab = (rng.random((nao, nblk)) < 0.65).astype(np.uint8)
+    phi = rng.standard_normal((nao, ngrid))
+    C = rng.standard_normal((nset, nao, 40))
+    dms = np.einsum("sik,sjk->sij", C, C) / 40
+    assert tab.sum(axis=0).max() > 1536
+    ref = np.zeros(nset * ngrid)
+    assert oracle.aodmao2rho(ref, dms.ravel(), phi.ravel(), phi.ravel(), nao, ngrid, nao, ngrid, nset, tab, nblk,
+                             blksize) == 0
+    old = cfgmm.tune_get(cfgmm.TUNE_RHO_ORDER)
+    outs = []
+    try:
+        for order in (0, 1):
+            cfgmm.tune_set(cfgmm.TUNE_RHO_ORDER, order)
+            rhos = np.zeros(nset * ngrid)
+            cfgmm.MatmulLoops().aodmao2rho_loop_parallel(blksize, rhos, dms.ravel(), phi.ravel(), phi.ravel(),
+                                                         [nao, ngrid], [nao, ngrid], nset, tab, nblk)
+            outs.append(rhos)
+    finally:
+        cfgmm.tune_set(cfgmm.TUNE_RHO_ORDER, old)
+    assert np.array_equal(outs[0], outs[1])
+    assert rel_fro(outs[0], ref) <= TOL
