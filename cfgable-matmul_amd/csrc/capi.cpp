@@ -146,6 +146,28 @@ int workspace_get(hipStream_t stream, size_t bytes, void **out) {
     return CFGMM_OK;
 }
 
+int host_staging_get(hipStream_t stream, size_t bytes, void **out) {
+    int dev = 0;
+    CFGMM_HIP_TRY(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(g_ws_mutex);
+    Workspace &ws = g_ws[std::make_pair(dev, stream)];
+    if (ws.host_bytes < bytes) {
+        if (ws.host) {
+            CFGMM_HIP_TRY(hipStreamSynchronize(stream));
+            CFGMM_HIP_TRY(hipHostFree(ws.host));
+            ws.host = nullptr;
+            ws.host_bytes = 0;
+        }
+        const size_t want = std::max(bytes + bytes / 4, (size_t)1 << 16);
+        void *p = nullptr;
+        CFGMM_HIP_TRY(hipHostMalloc(&p, want, hipHostMallocDefault));
+        ws.host = p;
+        ws.host_bytes = want;
+    }
+    *out = ws.host;
+    return CFGMM_OK;
+}
+
 // ---------------------------------------------------------------------------
 // argument validation (extents the reference would index)
 // ---------------------------------------------------------------------------
@@ -822,6 +844,7 @@ int cfgmm_stream_destroy(void *stream) {
             if (it != g_ws.end()) {
                 (void)hipStreamSynchronize(static_cast<hipStream_t>(stream));
                 if (it->second.ptr) (void)hipFree(it->second.ptr);
+                if (it->second.host) (void)hipHostFree(it->second.host);
                 g_ws.erase(it);
             }
         }

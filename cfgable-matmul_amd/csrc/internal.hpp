@@ -34,8 +34,14 @@ int hip_status(hipError_t e, const char *what);
 struct Workspace {
     void *ptr = nullptr;
     size_t bytes = 0;
+    void *host = nullptr;  // pinned host staging for tables copied to the device
+    size_t host_bytes = 0;
 };
 int workspace_get(hipStream_t stream, size_t bytes, void **out);
+// Pinned host memory of the stream, for host tables that a call copies to the device.  The
+// caller fills it only after every copy that read it has completed (a call that uses it
+// synchronises the stream before returning).
+int host_staging_get(hipStream_t stream, size_t bytes, void **out);
 
 // ---- tracing -----------------------------------------------------------------
 // trace_start returns nullptr unless tracing is on; trace_stop(nullptr) is a no-op.

@@ -64,6 +64,56 @@ def test_gto_vs_oracle_strided_ragged(blksize):
     assert 0.2 < ref_tab.mean() < 0.98  # a mask with both kinds of block
 
 
+def block_maxima(raw, shells, ngrid, blksize):
+    nblk = -(-ngrid // blksize)
+    out = []
+    for _, l, _, _, ao in shells:
+        nf = (l + 1) * (l + 2) // 2
+        out.append([np.max(np.abs(raw[ao:ao + nf, b * blksize:(b + 1) * blksize])) for b in range(nblk)])
+    return np.array(out)
+
+
+@pytest.mark.parametrize("blksize", [16, 64, 96])
+def test_gto_clustered_blocks_cutoff_sweep(blksize):
+    """Blocks of points clustered as on a molecular grid, so that most (block, shell) pairs are far
+    apart and can be skipped without evaluation; cutoffs at quantiles of the block maxima put many
+    pairs near the threshold.  Every tab bit and every value equals the oracle's.  The basis holds
+    shells of l = 0..4, negative coefficients, a shell of 70 primitives (more than one group's
+    record holds), shells of 30 primitives (groups cut short by primitives) and a shell with a zero
+    exponent (no decay: never skipped)."""
+    rng = np.random.default_rng(100 + blksize)
+    shells, exps, coefs, nao = gto.random_basis(rng, 48, box=12.0)
+    exps, coefs = list(exps), list(coefs)
+    coefs = [c * (-1.0 if i % 5 == 0 else 1.0) for i, c in enumerate(coefs)]
+    extra = [(70, 4), (30, 1), (30, 2), (30, 0), (1, 1)]
+    for n, l in extra:
+        shells.append((rng.uniform(0.0, 12.0, 3), l, n, len(exps), nao))
+        e = list(10.0 ** rng.uniform(-1.0, 1.0, n))
+        if n == 1:
+            e = [0.0]
+        exps.extend(e)
+        coefs.extend(rng.uniform(0.2, 1.5, n) * (1e-6 if n == 1 else 1.0))
+        nao += (l + 1) * (l + 2) // 2
+    exps, coefs = np.array(exps), np.array(coefs)
+    ngrid = blksize * 23 - 5
+    nblk = -(-ngrid // blksize)
+    centres = rng.uniform(0.0, 12.0, (nblk, 3))
+    coords = centres[np.arange(ngrid) // blksize] + rng.uniform(-0.6, 0.6, (ngrid, 3))
+    raw = gto.eval_gto(shells, exps, coefs, coords, nao)
+    maxima = block_maxima(raw, shells, ngrid, blksize)
+    srt = np.sort(maxima[maxima > 0])
+    for q in (0.5, 0.7, 0.9):
+        i = int(q * len(srt))
+        cutoff = float(np.sqrt(srt[i] * srt[i + 1]))  # between two block maxima: no ties
+        assert np.min(np.abs(maxima - cutoff)) > 1e-9 * cutoff
+        phi, tab = tabulate(shells, exps, coefs, coords, nao, ngrid, blksize, cutoff)
+        ref, ref_tab = gto.block_mask(raw, shells, ngrid, blksize, cutoff)
+        assert np.array_equal(tab, ref_tab), f"cutoff quantile {q}"
+        assert rel_fro(phi, ref) <= 1e-13
+        assert np.all(phi[~np.repeat(ref_tab.astype(bool), blksize, axis=1)[:, :ngrid]] == 0.0)
+        assert 0.4 < 1.0 - ref_tab.mean() < 0.97  # mostly off: screening has work to skip
+
+
 def test_gto_normalisation_integrals_on_device():
     """int phi_c^2 = prod (2l-1)!! / (4a)^l (pi/2a)^(3/2) for one primitive of each l <= 4, by
     Gauss-Hermite quadrature of the device values."""
