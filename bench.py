@@ -64,7 +64,7 @@ README_RHO_EVALS_PER_S = 1000.0 / 120.0  # README.md:51, 16-core Ryzen 7945HX
 # Bytes moved past L2 by one rho_chunk_kernel launch on the default workload, from two rocprofv3
 # --pmc passes (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, factors from a 256 MiB device copy):
 # profiles/rho_chunk_kernel_fp64_nao2228_ngrid16384_nset3.txt.  Only valid for that workload.
-RHO_DEFAULT_TRAFFIC_BYTES = 2891333 * 1024 * 2.000 + 4167 * 1024 * 1.000
+RHO_DEFAULT_TRAFFIC_BYTES = 2953547 * 1024 * 2.000 + 4167 * 1024 * 1.000
 
 
 def log(msg: str) -> None:
