@@ -43,6 +43,7 @@ TUNE_DGEMM_SMALL_SPLITS = 5
 TUNE_RESIDENT_DEPTH = 6
 TUNE_RESIDENT_COLS = 7
 TUNE_RHO_ORDER = 8
+TUNE_RHO_SYMMETRIC = 9
 
 _lib = None
 

@@ -26,6 +26,11 @@
 //      chunk.  No ngrid x nao intermediate is written to HBM.
 //   4. rho_reduce_kernel: rho[s][g] += sum over the chunks of g's block, in
 //      increasing chunk order -- deterministic, no float atomics.
+//
+// When lhs and rhs are one array (the density of one set of orbitals), dm_symmetry_kernel tests
+// every D_s for D_s == D_s^T, bitwise, before step 3; if all are, each chunk contracts only the
+// list positions from its own rows on, those past its rows twice: half the gathers and MFMAs.
+// The flag stays on the device (no host synchronisation); the chunk kernel reads it.
 #include "internal.hpp"
 #include "rho_body.hpp"
 
@@ -105,12 +110,18 @@ constexpr int kPlanBuckets = 1024;
 // perm[0 .. *ntask) = slot b*maxchunks + c of the non-empty chunks, bucket-ordered; the order
 // inside a bucket follows atomics and varies from call to call, which the result does not
 // depend on (each task writes its own partial slot, reduced in fixed chunk order).
+// bounds[0..8] (general form) and bounds[9..17] (symmetric form): the sequence cut into eight
+// ranges of equal work, one per XCD; a task's work is rows x (contracted positions + 64).
 __global__ __launch_bounds__(kPlanThreads) void rho_plan_kernel(const int *__restrict__ lists,
                                                                 const int *__restrict__ counts, int nao, int nblk,
                                                                 int maxchunks, int *__restrict__ perm,
-                                                                int *__restrict__ ntask) {
+                                                                int *__restrict__ ntask, int *__restrict__ sym,
+                                                                int *__restrict__ bounds) {
     __shared__ int hist[kPlanBuckets];
+    __shared__ long wsum[2][kPlanThreads];
+    __shared__ int s_ntask;
     const int tid = threadIdx.x;
+    if (tid == 0) *sym = 1;  // cleared by dm_symmetry_kernel, which runs after this kernel
     const long nslot = (long)nblk * maxchunks;
     auto key = [&](long sl) -> int {  // bucket of slot sl, or -1 for an empty chunk
         const int b = (int)(sl / maxchunks), c = (int)(sl % maxchunks);
@@ -140,12 +151,90 @@ __global__ __launch_bounds__(kPlanThreads) void rho_plan_kernel(const int *__res
     }
     __syncthreads();
     hist[tid] = incl - mine;
-    if (tid == kPlanThreads - 1) *ntask = incl;
+    if (tid == kPlanThreads - 1) {
+        *ntask = incl;
+        s_ntask = incl;
+    }
     __syncthreads();
     for (long sl = tid; sl < nslot; sl += kPlanThreads) {
         const int k = key(sl);
         if (k >= 0) perm[atomicAdd(&hist[k], 1)] = (int)sl;
     }
+    __syncthreads();  // perm complete (the global writes of this workgroup are visible to it)
+
+    // equal-work XCD ranges: each thread takes a contiguous piece of the sequence
+    const int n = s_ntask;
+    const int per = (n + kPlanThreads - 1) / kPlanThreads;
+    const int p0 = min(n, tid * per), p1 = min(n, p0 + per);
+    auto work = [&](int pos, int form) -> long {
+        const int sl = perm[pos];
+        const int b = sl / maxchunks, c = sl % maxchunks, cnt = counts[b];
+        const long rows = min(kUC, cnt - c * kUC);
+        return rows * ((form ? cnt - c * kUC : cnt) + kUC);
+    };
+    long mine2[2] = {0, 0};
+    for (int pos = p0; pos < p1; ++pos) {
+        mine2[0] += work(pos, 0);
+        mine2[1] += work(pos, 1);
+    }
+    // inclusive scan of the per-thread sums (Hillis-Steele)
+    long inc2[2] = {mine2[0], mine2[1]};
+    wsum[0][tid] = inc2[0];
+    wsum[1][tid] = inc2[1];
+    for (int d = 1; d < kPlanThreads; d <<= 1) {
+        __syncthreads();
+        const long a0 = tid >= d ? wsum[0][tid - d] : 0, a1 = tid >= d ? wsum[1][tid - d] : 0;
+        __syncthreads();
+        inc2[0] += a0;
+        inc2[1] += a1;
+        wsum[0][tid] = inc2[0];
+        wsum[1][tid] = inc2[1];
+    }
+    __syncthreads();
+    const long tot[2] = {wsum[0][kPlanThreads - 1], wsum[1][kPlanThreads - 1]};
+    if (tid < 2) {
+        bounds[9 * tid] = 0;
+        for (int x = 1; x <= 8; ++x) bounds[9 * tid + x] = n;
+    }
+    __syncthreads();
+    // position pos goes to XCD floor(8 * (work before pos) / total); bounds[x] = first position of XCD >= x
+#pragma unroll
+    for (int form = 0; form < 2; ++form) {
+        if (tot[form] == 0) continue;
+        long before = inc2[form] - mine2[form];
+        // the XCD of the position before this thread's piece (position 0 starts XCD 0)
+        int prev = p0 > 0 && p0 < p1 ? (int)min(7L, (before - work(p0 - 1, form)) * 8 / tot[form]) : 0;
+        for (int pos = p0; pos < p1; ++pos) {
+            const int xcd = (int)min(7L, before * 8 / tot[form]);
+            for (int x = prev + 1; x <= xcd; ++x) bounds[9 * form + x] = pos;
+            prev = xcd;
+            before += work(pos, form);
+        }
+    }
+}
+
+// *sym = 0 unless every D_s (s < nset) equals its transpose bitwise on [0, nao)^2.  Workgroup
+// (tj, ti, s) with ti <= tj compares the 64x64 tile (ti, tj) with the transpose of (tj, ti)
+// through LDS; the others exit.
+__global__ __launch_bounds__(256) void dm_symmetry_kernel(const double *__restrict__ dms, long stride_ao, int nao,
+                                                          int *__restrict__ sym) {
+    const int tj = blockIdx.x, ti = blockIdx.y, s = blockIdx.z;
+    if (ti > tj) return;
+    __shared__ unsigned long long t[64][65];
+    const double *__restrict__ d = dms + (long)s * stride_ao * stride_ao;
+    const int c = threadIdx.x & 63, r0 = threadIdx.x >> 6;
+    const int i0 = ti * 64, j0 = tj * 64;
+    for (int r = r0; r < 64; r += 4) {
+        const int i = i0 + r, j = j0 + c;
+        t[r][c] = i < nao && j < nao ? (unsigned long long)__double_as_longlong(d[(long)i * stride_ao + j]) : 0ull;
+    }
+    __syncthreads();
+    bool differ = false;
+    for (int r = r0; r < 64; r += 4) {  // row j0 + r, column i0 + c of D against t[c][r]
+        const int j = j0 + r, i = i0 + c;
+        if (i < nao && j < nao) differ |= (unsigned long long)__double_as_longlong(d[(long)j * stride_ao + i]) != t[c][r];
+    }
+    if (__ballot(differ) != 0 && (threadIdx.x & 63) == 0) __hip_atomic_store(sym, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 template <int NS, int NT>
@@ -154,68 +243,73 @@ __global__ __launch_bounds__(kThreads, 2) void rho_chunk_kernel(
     const double *__restrict__ ao_rhs, long stride_ao, long stride_grid, int nao, int ngrid, int nset, int set0,
     const int *__restrict__ lists, const int *__restrict__ counts, int blksize, int nsub, int maxchunks,
     int ngroups, int ncomp, long lhs_comp_stride, long part_comp_stride, const int *__restrict__ perm,
-    const int *__restrict__ ntask) {
+    const int *__restrict__ ntask, const int *__restrict__ sym, const int *__restrict__ bounds) {
     __shared__ __attribute__((aligned(16))) unsigned char smem[rho::chunk_lds_bytes<NS, NT>()];
-    int blk, chunk, grp, sub;
+    const int symmetric = sym ? *sym : 0;
+    bool first = true;
+    // one (block, chunk, group of NS density matrices, subtile) task
+    auto run = [&](int blk, int chunk, int grp, int sub) {
+        rho::RhoChunk t;
+        t.cnt = counts[blk];
+        t.chunk = chunk;
+        t.g0 = blk * blksize + sub * NT * 16;
+        if (t.chunk * kUC >= t.cnt || t.g0 >= ngrid) return;  // empty chunk slot, subtile past the grid
+        if (!first) __syncthreads();  // the previous task's epilogue is done with LDS
+        first = false;
+        t.partial = partial;
+        t.dms = dms;
+        t.ao_lhs = ao_lhs;
+        t.ao_rhs = ao_rhs;
+        t.stride_ao = stride_ao;
+        t.stride_grid = stride_grid;
+        t.ngrid = ngrid;
+        t.nset = nset;
+        t.set0 = set0 + grp * NS;
+        t.list = lists + (long)blk * nao;
+        t.ncomp = ncomp;
+        t.lhs_comp_stride = lhs_comp_stride;
+        t.part_comp_stride = part_comp_stride;
+        t.sym = symmetric;
+        rho::rho_chunk_body<NS, NT>(t, smem);
+    };
     const int x = blockIdx.x % 8, k = blockIdx.x / 8;
     if (perm) {
-        // AO-ordered tasks (group of NS density matrices, planned chunk j, subtile), the
-        // sequence cut into eight contiguous parts, one per XCD (workgroup i runs on XCD i%8;
-        // the grid is a multiple of 8 with at least as many workgroups as tasks)
-        const long per = (long)*ntask * nsub, total = per * ngroups;
-        const long part = (total + 7) / 8, tt = x * part + k;
-        if (k >= part || tt >= total) return;
-        grp = (int)(tt / per);
-        const long rj = tt % per;
-        sub = (int)(rj % nsub);
-        const int sl = perm[rj / nsub];
-        blk = sl / maxchunks;
-        chunk = sl % maxchunks;
+        // AO-ordered tasks: XCD x (workgroup i runs on XCD i % 8) takes the planned positions
+        // [bounds[x], bounds[x+1]) -- equal work per XCD -- for every group and subtile, group-major;
+        // its workgroups stride over them (several tasks each where short tasks are many)
+        const int *__restrict__ bd = bounds + (symmetric ? 9 : 0);
+        const int b0 = bd[x], npos = bd[x + 1] - b0;
+        const long per = (long)npos * nsub, items = per * ngroups;
+        for (long kk = k; kk < items; kk += gridDim.x / 8) {
+            const int grp = (int)(kk / per);
+            const long rj = kk % per;
+            const int sl = perm[b0 + rj / nsub];
+            run(sl / maxchunks, sl % maxchunks, grp, (int)(rj % nsub));
+        }
     } else {
         // block-major (block, subtile, group, chunk): the tasks of one block, which share its
         // phi tile, run back to back on one XCD; empty chunk slots exit
         const int ntasks = gridDim.x, q = ntasks / 8, r = ntasks % 8;
         const int L = x * q + min(x, r) + k;
-        chunk = L % maxchunks;
+        const int chunk = L % maxchunks;
         const int rest = L / maxchunks;
-        grp = rest % ngroups;
+        const int grp = rest % ngroups;
         const int rest2 = rest / ngroups;
-        blk = rest2 / nsub;
-        sub = rest2 % nsub;
+        run(rest2 / nsub, chunk, grp, rest2 % nsub);
     }
-    set0 += grp * NS;
-    rho::RhoChunk t;
-    t.cnt = counts[blk];
-    t.chunk = chunk;
-    if (t.chunk * kUC >= t.cnt) return;
-    t.g0 = blk * blksize + sub * NT * 16;
-    if (t.g0 >= ngrid) return;  // subtile past the end of a partial last block
-    t.partial = partial;
-    t.dms = dms;
-    t.ao_lhs = ao_lhs;
-    t.ao_rhs = ao_rhs;
-    t.stride_ao = stride_ao;
-    t.stride_grid = stride_grid;
-    t.ngrid = ngrid;
-    t.nset = nset;
-    t.set0 = set0;
-    t.list = lists + (long)blk * nao;
-    t.ncomp = ncomp;
-    t.lhs_comp_stride = lhs_comp_stride;
-    t.part_comp_stride = part_comp_stride;
-    rho::rho_chunk_body<NS, NT>(t, smem);
+    (void)ntask;
 }
 
 template <int NS>
 int launch_ns(int nt, dim3 grid, hipStream_t stream, double *partial, const double *dms, const double *ao_lhs,
               const double *ao_rhs, long stride_ao, long stride_grid, int nao, int ngrid, int nset, int set0,
               const int *lists, const int *counts, int blksize, int nsub, int maxchunks, int ngroups, int ncomp,
-              long lhs_cs, long part_cs, const int *perm, const int *ntask) {
+              long lhs_cs, long part_cs, const int *perm, const int *ntask, const int *sym, const int *bounds) {
     hipEvent_t t0 = trace_start(stream);
 #define CFGMM_RHO_LAUNCH(NT_)                                                                                      \
     hipLaunchKernelGGL((rho_chunk_kernel<NS, NT_>), grid, dim3(kThreads), 0, stream, partial, dms, ao_lhs, ao_rhs, \
                        stride_ao, stride_grid, nao, ngrid, nset, set0, lists, counts, blksize, nsub, maxchunks, \
-                       ngroups, ncomp, lhs_cs, part_cs, perm, ntask)
+                       ngroups, ncomp, lhs_cs, part_cs, perm, ntask, sym, bounds)
     switch (nt) {
         case 1: CFGMM_RHO_LAUNCH(1); break;
         case 2: CFGMM_RHO_LAUNCH(2); break;
@@ -257,11 +351,12 @@ int launch_rho_multi(hipStream_t stream, double *rhos, size_t rho_cs, const doub
     const int nt = rho_tile_nt(blksize);
     const int nsub = (int)(blksize / (16 * nt));
 
-    // workspace: [lists nblk*nao int][counts nblk int][perm nblk*maxchunks int][ntask int][pad]
+    // workspace: [lists nblk*nao int][counts nblk int][perm nblk*maxchunks int][ntask int][sym int]
+    //            [bounds 18 int][pad]
     //            [partial ncomp*maxchunks*nset*ngrid double]
     const size_t lists_bytes = sizeof(int) * nblk * nao;
     const size_t counts_bytes = sizeof(int) * nblk;
-    const size_t perm_bytes = sizeof(int) * (nblk * maxchunks + 1);
+    const size_t perm_bytes = sizeof(int) * (nblk * maxchunks + 2 + 18);
     const size_t part_off = div_up(lists_bytes + counts_bytes + perm_bytes, 256) * 256;
     const size_t part_cs = maxchunks * nset * ngrid;  // doubles per component
     const size_t part_bytes = sizeof(double) * part_cs * ncomp;
@@ -272,6 +367,8 @@ int launch_rho_multi(hipStream_t stream, double *rhos, size_t rho_cs, const doub
     int *counts = lists + nblk * nao;
     int *perm = counts + nblk;
     int *ntask = perm + nblk * maxchunks;
+    int *sym = ntask + 1;
+    int *bounds = sym + 1;
     double *partial = reinterpret_cast<double *>(static_cast<char *>(ws) + part_off);
 
     st = launch_build_lists(stream, tab, ldtab, nao, nblk, lists, counts);
@@ -280,11 +377,24 @@ int launch_rho_multi(hipStream_t stream, double *rhos, size_t rho_cs, const doub
     if (ordered) {
         hipEvent_t t0 = trace_start(stream);
         hipLaunchKernelGGL(rho_plan_kernel, dim3(1), dim3(kPlanThreads), 0, stream, lists, counts, (int)nao,
-                           (int)nblk, (int)maxchunks, perm, ntask);
+                           (int)nblk, (int)maxchunks, perm, ntask, sym, bounds);
         trace_stop(stream, "rho_plan_kernel", t0);
         st = hip_status(hipGetLastError(), "rho_plan_kernel launch");
         if (st != CFGMM_OK) return st;
     }
+    // the symmetric form: one set of orbitals on both sides, every D_s tested on the device
+    const bool try_sym = ordered && ncomp == 1 && ao_lhs == ao_rhs && tune_get(CFGMM_TUNE_RHO_SYMMETRIC) == 0 &&
+                         div_up(nao, 64) <= 65535;
+    if (try_sym) {
+        const unsigned nt64 = (unsigned)div_up(nao, 64);
+        hipEvent_t t0 = trace_start(stream);
+        hipLaunchKernelGGL(dm_symmetry_kernel, dim3(nt64, nt64, (unsigned)nset), dim3(256), 0, stream, dms,
+                           (long)stride_ao, (int)nao, sym);
+        trace_stop(stream, "dm_symmetry_kernel", t0);
+        st = hip_status(hipGetLastError(), "dm_symmetry_kernel launch");
+        if (st != CFGMM_OK) return st;
+    }
+    const int *symp = try_sym ? sym : nullptr;
 
     // Density matrices per workgroup: up to max_sets.  One launch covers every full group of NS
     // sets (the group is a task coordinate); a smaller remainder group gets a launch of its own.
@@ -298,10 +408,10 @@ int launch_rho_multi(hipStream_t stream, double *rhos, size_t rho_cs, const doub
         const dim3 grid((unsigned)(ordered ? div_up(slots, 8) * 8 : slots), 1, 1);
         const int *pp = ordered ? perm : nullptr;
         switch (ns) {
-            case 1: return launch_ns<1>(nt, grid, stream, partial, dms, ao_lhs, ao_rhs, sa, sg, (int)nao, (int)ngrid, (int)nset, (int)set0, lists, counts, (int)blksize, nsub, (int)maxchunks, (int)ngroups, (int)ncomp, (long)lhs_cs, (long)part_cs, pp, ntask);
-            case 2: return launch_ns<2>(nt, grid, stream, partial, dms, ao_lhs, ao_rhs, sa, sg, (int)nao, (int)ngrid, (int)nset, (int)set0, lists, counts, (int)blksize, nsub, (int)maxchunks, (int)ngroups, (int)ncomp, (long)lhs_cs, (long)part_cs, pp, ntask);
-            case 3: return launch_ns<3>(nt, grid, stream, partial, dms, ao_lhs, ao_rhs, sa, sg, (int)nao, (int)ngrid, (int)nset, (int)set0, lists, counts, (int)blksize, nsub, (int)maxchunks, (int)ngroups, (int)ncomp, (long)lhs_cs, (long)part_cs, pp, ntask);
-            default: return launch_ns<4>(nt, grid, stream, partial, dms, ao_lhs, ao_rhs, sa, sg, (int)nao, (int)ngrid, (int)nset, (int)set0, lists, counts, (int)blksize, nsub, (int)maxchunks, (int)ngroups, (int)ncomp, (long)lhs_cs, (long)part_cs, pp, ntask);
+            case 1: return launch_ns<1>(nt, grid, stream, partial, dms, ao_lhs, ao_rhs, sa, sg, (int)nao, (int)ngrid, (int)nset, (int)set0, lists, counts, (int)blksize, nsub, (int)maxchunks, (int)ngroups, (int)ncomp, (long)lhs_cs, (long)part_cs, pp, ntask, symp, bounds);
+            case 2: return launch_ns<2>(nt, grid, stream, partial, dms, ao_lhs, ao_rhs, sa, sg, (int)nao, (int)ngrid, (int)nset, (int)set0, lists, counts, (int)blksize, nsub, (int)maxchunks, (int)ngroups, (int)ncomp, (long)lhs_cs, (long)part_cs, pp, ntask, symp, bounds);
+            case 3: return launch_ns<3>(nt, grid, stream, partial, dms, ao_lhs, ao_rhs, sa, sg, (int)nao, (int)ngrid, (int)nset, (int)set0, lists, counts, (int)blksize, nsub, (int)maxchunks, (int)ngroups, (int)ncomp, (long)lhs_cs, (long)part_cs, pp, ntask, symp, bounds);
+            default: return launch_ns<4>(nt, grid, stream, partial, dms, ao_lhs, ao_rhs, sa, sg, (int)nao, (int)ngrid, (int)nset, (int)set0, lists, counts, (int)blksize, nsub, (int)maxchunks, (int)ngroups, (int)ncomp, (long)lhs_cs, (long)part_cs, pp, ntask, symp, bounds);
         }
     };
     st = launch_groups(NS, 0, full);

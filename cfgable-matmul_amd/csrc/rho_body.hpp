@@ -27,6 +27,10 @@ struct RhoChunk {
     // reads ao_lhs + c*lhs_comp_stride and writes partial + c*part_comp_stride
     int ncomp = 1;
     long lhs_comp_stride = 0, part_comp_stride = 0;
+    // D_s symmetric and lhs == rhs (one component): the chunk contracts only list positions from
+    // its own first row on, the positions past its 64 rows twice (the pairs of those positions
+    // with earlier rows are the mirror images of pairs that the earlier chunks count)
+    int sym = 0;
 };
 
 template <int NS, int NT>
@@ -89,122 +93,136 @@ __device__ inline void rho_chunk_body(const RhoChunk &task, unsigned char *__res
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) acc[s][nt] = f64x4{0.0, 0.0, 0.0, 0.0};
 
-    for (int seg0 = 0; seg0 < cnt; seg0 += kSeg) {
-        const int segn = min(kSeg, cnt - seg0);
-        __syncthreads();  // previous segment's readers of vls / slices are done
-        for (int j = tid; j < segn; j += kThreads) vls[j] = list[seg0 + j];
-        __syncthreads();
-        const double *__restrict__ drow[NS][D_ROWS_PER_T];
+    // Symmetric form: positions past the chunk's rows first, the accumulators doubled, then the
+    // chunk's own 64 positions.  General form: the whole list.
+    const int own_end = min(u_begin + kUC, cnt);
+    const int nranges = task.sym ? 2 : 1;
+    for (int rg = 0; rg < nranges; ++rg) {
+        const int kb = task.sym ? (rg == 0 ? own_end : u_begin) : 0;
+        const int ke = task.sym && rg == 1 ? own_end : cnt;
+        for (int seg0 = kb; seg0 < ke; seg0 += kSeg) {
+            const int segn = min(kSeg, ke - seg0);
+            __syncthreads();  // previous segment's readers of vls / slices are done
+            for (int j = tid; j < segn; j += kThreads) vls[j] = list[seg0 + j];
+            __syncthreads();
+            const double *__restrict__ drow[NS][D_ROWS_PER_T];
 #pragma unroll
-        for (int e = 0; e < D_ROWS_PER_T; ++e) {
-            const int i = tid / kKV + (kThreads / kKV) * e;
+            for (int e = 0; e < D_ROWS_PER_T; ++e) {
+                const int i = tid / kKV + (kThreads / kKV) * e;
 #pragma unroll
-            for (int s = 0; s < NS; ++s) drow[s][e] = dms + (long)(set0 + s) * dstride + (long)ulist[i] * stride_ao;
-        }
+                for (int s = 0; s < NS; ++s) drow[s][e] = dms + (long)(set0 + s) * dstride + (long)ulist[i] * stride_ao;
+            }
 
-        // Loads of a full 16-deep slice are unconditional: every list entry is
-        // valid, rows past the chunk end read a valid fallback row (their
-        // products are dropped in the epilogue) and columns past the grid end
-        // are clamped (their outputs are never written).  The last, partial
-        // slice of a segment is loaded by load_tail, after the main loop, so
-        // that no select on loaded values sits in the pipelined path.
-        // (set = register set; a compile-time index wherever the loop below is unrolled)
-        auto load_full = [&](int k0, int set) {
-            const long vcol = vls[k0 + p_d];
+            // Loads of a full 16-deep slice are unconditional: every list entry is
+            // valid, rows past the chunk end read a valid fallback row (their
+            // products are dropped in the epilogue) and columns past the grid end
+            // are clamped (their outputs are never written).  The last, partial
+            // slice of a segment is loaded by load_tail, after the main loop, so
+            // that no select on loaded values sits in the pipelined path.
+            // (set = register set; a compile-time index wherever the loop below is unrolled)
+            auto load_full = [&](int k0, int set) {
+                const long vcol = vls[k0 + p_d];
 #pragma unroll
-            for (int e = 0; e < D_ROWS_PER_T; ++e)
+                for (int e = 0; e < D_ROWS_PER_T; ++e)
 #pragma unroll
-                for (int s = 0; s < NS; ++s) rd[set][s][e] = drow[s][e][vcol];
+                    for (int s = 0; s < NS; ++s) rd[set][s][e] = drow[s][e][vcol];
 #pragma unroll
-            for (int e = 0; e < PHI_PER_T; ++e) {
-                const int idx = tid + kThreads * e;
-                const int g = min(g0 + idx % W, ngrid - 1);
-                rphi[set][e] = ao_rhs[(long)vls[k0 + idx / W] * stride_grid + g];
-            }
-        };
-        auto load_tail = [&](int k0) {
-            const int pv = k0 + p_d;
-            const long vcol = vls[pv < segn ? pv : 0];
+                for (int e = 0; e < PHI_PER_T; ++e) {
+                    const int idx = tid + kThreads * e;
+                    const int g = min(g0 + idx % W, ngrid - 1);
+                    rphi[set][e] = ao_rhs[(long)vls[k0 + idx / W] * stride_grid + g];
+                }
+            };
+            auto load_tail = [&](int k0) {
+                const int pv = k0 + p_d;
+                const long vcol = vls[pv < segn ? pv : 0];
 #pragma unroll
-            for (int e = 0; e < D_ROWS_PER_T; ++e)
+                for (int e = 0; e < D_ROWS_PER_T; ++e)
 #pragma unroll
-                for (int s = 0; s < NS; ++s) rd[0][s][e] = pv < segn ? drow[s][e][vcol] : 0.0;
+                    for (int s = 0; s < NS; ++s) rd[0][s][e] = pv < segn ? drow[s][e][vcol] : 0.0;
 #pragma unroll
-            for (int e = 0; e < PHI_PER_T; ++e) {
-                const int idx = tid + kThreads * e;
-                const int pr = k0 + idx / W;
-                const int g = min(g0 + idx % W, ngrid - 1);
-                rphi[0][e] = pr < segn ? ao_rhs[(long)vls[pr] * stride_grid + g] : 0.0;
-            }
-        };
-        auto store_slice = [&](int buf, int set) {
+                for (int e = 0; e < PHI_PER_T; ++e) {
+                    const int idx = tid + kThreads * e;
+                    const int pr = k0 + idx / W;
+                    const int g = min(g0 + idx % W, ngrid - 1);
+                    rphi[0][e] = pr < segn ? ao_rhs[(long)vls[pr] * stride_grid + g] : 0.0;
+                }
+            };
+            auto store_slice = [&](int buf, int set) {
 #pragma unroll
-            for (int e = 0; e < PHI_PER_T; ++e) {
-                const int idx = tid + kThreads * e;
-                Bs[buf][idx / W][idx % W] = rphi[set][e];
-            }
+                for (int e = 0; e < PHI_PER_T; ++e) {
+                    const int idx = tid + kThreads * e;
+                    Bs[buf][idx / W][idx % W] = rphi[set][e];
+                }
 #pragma unroll
-            for (int e = 0; e < D_ROWS_PER_T; ++e)
+                for (int e = 0; e < D_ROWS_PER_T; ++e)
 #pragma unroll
-                for (int s = 0; s < NS; ++s) As(buf, s, p_d, tid / kKV + (kThreads / kKV) * e) = rd[set][s][e];
-        };
-        auto compute_slice = [&](int cur) {
+                    for (int s = 0; s < NS; ++s) As(buf, s, p_d, tid / kKV + (kThreads / kKV) * e) = rd[set][s][e];
+            };
+            auto compute_slice = [&](int cur) {
 #pragma unroll
-            for (int kk = 0; kk < kKV / 4; ++kk) {
-                double fa[NS], fb[NT];
+                for (int kk = 0; kk < kKV / 4; ++kk) {
+                    double fa[NS], fb[NT];
 #pragma unroll
-                for (int s = 0; s < NS; ++s) fa[s] = As(cur, s, kk * 4 + lk, wave * 16 + lrow);
+                    for (int s = 0; s < NS; ++s) fa[s] = As(cur, s, kk * 4 + lk, wave * 16 + lrow);
 #pragma unroll
-                for (int nt = 0; nt < NT; ++nt) fb[nt] = Bs[cur][kk * 4 + lk][nt * 16 + lrow];
+                    for (int nt = 0; nt < NT; ++nt) fb[nt] = Bs[cur][kk * 4 + lk][nt * 16 + lrow];
 #pragma unroll
-                for (int s = 0; s < NS; ++s)
+                    for (int s = 0; s < NS; ++s)
 #pragma unroll
-                    for (int nt = 0; nt < NT; ++nt)
-                        acc[s][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[s], fb[nt], acc[s][nt], 0, 0, 0);
-            }
-        };
+                        for (int nt = 0; nt < NT; ++nt)
+                            acc[s][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[s], fb[nt], acc[s][nt], 0, 0, 0);
+                }
+            };
 
-        // Slice t lives in register set t % PF, then in LDS buffer t % 2.  Iteration t refills
-        // set t % PF (slice t is already in LDS) with slice t+PF, computes slice t and stores
-        // slice t+1 (loaded PF iterations earlier) into the other buffer.  In the main part the
-        // refills are unconditional (past the end they reload the last slice, unused) so that
-        // the compiler can count the loads in flight and wait only for the slice it stores;
-        // the last nfull % PF slices were loaded by then.
-        const int nfull = segn / kKV;
-        if (nfull > 0) {
+            // Slice t lives in register set t % PF, then in LDS buffer t % 2.  Iteration t refills
+            // set t % PF (slice t is already in LDS) with slice t+PF, computes slice t and stores
+            // slice t+1 (loaded PF iterations earlier) into the other buffer.  In the main part the
+            // refills are unconditional (past the end they reload the last slice, unused) so that
+            // the compiler can count the loads in flight and wait only for the slice it stores;
+            // the last nfull % PF slices were loaded by then.
+            const int nfull = segn / kKV;
+            if (nfull > 0) {
 #pragma unroll
-            for (int u = 0; u < PF; ++u) load_full(min(u, nfull - 1) * kKV, u);
-            store_slice(0, 0);
-        }
-        __syncthreads();
-        // main part: whole rounds of PF slices, straight-line (a store past the end writes
-        // the reloaded last slice into the idle buffer, never computed)
-        int t0 = 0;
-        for (; t0 + PF <= nfull; t0 += PF) {
+                for (int u = 0; u < PF; ++u) load_full(min(u, nfull - 1) * kKV, u);
+                store_slice(0, 0);
+            }
+            __syncthreads();
+            // main part: whole rounds of PF slices, straight-line (a store past the end writes
+            // the reloaded last slice into the idle buffer, never computed)
+            int t0 = 0;
+            for (; t0 + PF <= nfull; t0 += PF) {
 #pragma unroll
-            for (int u = 0; u < PF; ++u) {
+                for (int u = 0; u < PF; ++u) {
+                    const int t = t0 + u;
+                    load_full(min(t + PF, nfull - 1) * kKV, u);
+                    compute_slice(t & 1);
+                    store_slice((t + 1) & 1, (u + 1) % PF);
+                    __syncthreads();
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < PF; ++u) {  // the last nfull % PF slices
                 const int t = t0 + u;
-                load_full(min(t + PF, nfull - 1) * kKV, u);
-                compute_slice(t & 1);
-                store_slice((t + 1) & 1, (u + 1) % PF);
+                if (t < nfull) {
+                    compute_slice(t & 1);
+                    if (t + 1 < nfull) store_slice((t + 1) & 1, (u + 1) % PF);
+                    __syncthreads();
+                }
+            }
+            if (nfull * kKV < segn) {
+                load_tail(nfull * kKV);
+                store_slice(0, 0);
+                __syncthreads();
+                compute_slice(0);
                 __syncthreads();
             }
         }
+        if (task.sym && rg == 0) {
 #pragma unroll
-        for (int u = 0; u < PF; ++u) {  // the last nfull % PF slices
-            const int t = t0 + u;
-            if (t < nfull) {
-                compute_slice(t & 1);
-                if (t + 1 < nfull) store_slice((t + 1) & 1, (u + 1) % PF);
-                __syncthreads();
-            }
-        }
-        if (nfull * kKV < segn) {
-            load_tail(nfull * kKV);
-            store_slice(0, 0);
-            __syncthreads();
-            compute_slice(0);
-            __syncthreads();
+            for (int s = 0; s < NS; ++s)
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt) acc[s][nt] *= 2.0;
         }
     }
 

@@ -347,7 +347,10 @@ enum cfgmm_tunable {
                                         dividing blksize (read at creation) */
     CFGMM_TUNE_RHO_ORDER = 8,        /* FP64 rho: 0 = chunks ordered by the AO range of their rows
                                         (default), 1 = block-major */
-    CFGMM_TUNE_COUNT = 9
+    CFGMM_TUNE_RHO_SYMMETRIC = 9,    /* FP64 rho: 0 = when ao_lhs == ao_rhs, test every D_s for
+                                        D_s == D_s^T on the device and, if so, contract only the
+                                        upper triangle of each block (default); 1 = never */
+    CFGMM_TUNE_COUNT = 10
 };
 int cfgmm_tune_set(int param, int value);
 int cfgmm_tune_get(int param, int *value);

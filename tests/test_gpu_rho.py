@@ -144,7 +144,7 @@ def test_rho_deterministic(workload):
 @pytest.mark.parametrize("blksize", [64, 96])
 def test_rho_task_order_bitwise(blksize):
     """AO-ordered and block-major task orders (CFGMM_TUNE_RHO_ORDER) with 1 and 3 density
-    matrices per workgroup give bitwise-equal results, repeated AO-ordered calls too (the order
+    matrices per workgroup give bitwise-equal results in the general (non-symmetric) form, repeated AO-ordered calls too (the order
     inside a plan bucket varies), and match the oracle."""
     p = W.rho_problem(seed=6, nshell=200, nao=640, ngrid=blksize * 23 + 17, nset=3, blksize=blksize, density=0.3)
     s = cfgmm.Stream()
@@ -152,9 +152,11 @@ def test_rho_task_order_bitwise(blksize):
     d_phi = cfgmm.DeviceBuffer.from_array(p.phi)
     d_tab = cfgmm.DeviceBuffer.from_array(np.ascontiguousarray(p.tab))
     d_r = cfgmm.DeviceBuffer(8 * p.nset * p.ngrid)
-    old = (cfgmm.tune_get(cfgmm.TUNE_RHO_ORDER), cfgmm.tune_get(cfgmm.TUNE_RHO_MAX_SETS))
+    old = (cfgmm.tune_get(cfgmm.TUNE_RHO_ORDER), cfgmm.tune_get(cfgmm.TUNE_RHO_MAX_SETS),
+           cfgmm.tune_get(cfgmm.TUNE_RHO_SYMMETRIC))
     outs = []
     try:
+        cfgmm.tune_set(cfgmm.TUNE_RHO_SYMMETRIC, 1)  # the general form in both orders
         for order, ms in ((0, 1), (0, 1), (1, 1), (0, 3), (1, 3)):
             cfgmm.tune_set(cfgmm.TUNE_RHO_ORDER, order)
             cfgmm.tune_set(cfgmm.TUNE_RHO_MAX_SETS, ms)
@@ -166,6 +168,7 @@ def test_rho_task_order_bitwise(blksize):
     finally:
         cfgmm.tune_set(cfgmm.TUNE_RHO_ORDER, old[0])
         cfgmm.tune_set(cfgmm.TUNE_RHO_MAX_SETS, old[1])
+        cfgmm.tune_set(cfgmm.TUNE_RHO_SYMMETRIC, old[2])
     for b in (d_d, d_phi, d_tab, d_r):
         b.free()
     s.destroy()
@@ -252,20 +255,85 @@ def test_rho_long_lists_span_segments():
     phi = rng.standard_normal((nao, ngrid))
     C = rng.standard_normal((nset, nao, 40))
     dms = np.einsum("sik,sjk->sij", C, C) / 40
+    dms = (dms + dms.transpose(0, 2, 1)) / 2  # bitwise symmetric
     assert tab.sum(axis=0).max() > 1536
     ref = np.zeros(nset * ngrid)
     assert oracle.aodmao2rho(ref, dms.ravel(), phi.ravel(), phi.ravel(), nao, ngrid, nao, ngrid, nset, tab, nblk,
                              blksize) == 0
-    old = cfgmm.tune_get(cfgmm.TUNE_RHO_ORDER)
+    old = (cfgmm.tune_get(cfgmm.TUNE_RHO_ORDER), cfgmm.tune_get(cfgmm.TUNE_RHO_SYMMETRIC))
     outs = []
     try:
-        for order in (0, 1):
+        for order, sym_off in ((0, 1), (1, 1), (0, 0)):
             cfgmm.tune_set(cfgmm.TUNE_RHO_ORDER, order)
+            cfgmm.tune_set(cfgmm.TUNE_RHO_SYMMETRIC, sym_off)
             rhos = np.zeros(nset * ngrid)
             cfgmm.MatmulLoops().aodmao2rho_loop_parallel(blksize, rhos, dms.ravel(), phi.ravel(), phi.ravel(),
                                                          [nao, ngrid], [nao, ngrid], nset, tab, nblk)
             outs.append(rhos)
     finally:
-        cfgmm.tune_set(cfgmm.TUNE_RHO_ORDER, old)
+        cfgmm.tune_set(cfgmm.TUNE_RHO_ORDER, old[0])
+        cfgmm.tune_set(cfgmm.TUNE_RHO_SYMMETRIC, old[1])
     assert np.array_equal(outs[0], outs[1])
     assert rel_fro(outs[0], ref) <= TOL
+    assert rel_fro(outs[2], ref) <= TOL  # symmetric form, ranges past the chunk spanning segments
+    assert not np.array_equal(outs[2], outs[0])  # it did take the symmetric form (other rounding)
+
+
+def _rho_dev(p, dms, lhs, rhs, same_buffer=True, blksize=None):
+    blksize = blksize or p.blksize
+    s = cfgmm.Stream()
+    d_d = cfgmm.DeviceBuffer.from_array(np.ascontiguousarray(dms))
+    d_l = cfgmm.DeviceBuffer.from_array(np.ascontiguousarray(lhs))
+    d_rr = d_l if same_buffer else cfgmm.DeviceBuffer.from_array(np.ascontiguousarray(rhs))
+    d_t = cfgmm.DeviceBuffer.from_array(np.ascontiguousarray(p.tab))
+    d_r = cfgmm.DeviceBuffer(8 * p.nset * p.ngrid)
+    d_r.zero()
+    cfgmm.aodmao2rho_dev(s, d_r, d_d, d_l, d_rr, p.nao, p.ngrid, p.nao, p.ngrid, p.nset, d_t, p.nblk, blksize)
+    s.synchronize()
+    out = d_r.to_array(np.float64, (p.nset * p.ngrid,))
+    for b in {id(x): x for x in (d_d, d_l, d_rr, d_t, d_r)}.values():
+        b.free()
+    s.destroy()
+    return out
+
+
+def _oracle(p, dms, lhs, rhs):
+    ref = np.zeros(p.nset * p.ngrid)
+    assert oracle.aodmao2rho(ref, np.ascontiguousarray(dms).ravel(), lhs.ravel(), rhs.ravel(), p.nao, p.ngrid, p.nao,
+                             p.ngrid, p.nset, np.ascontiguousarray(p.tab), p.nblk, p.blksize) == 0
+    return ref
+
+
+@pytest.mark.parametrize("blksize,nset", [(64, 3), (96, 5)])
+def test_rho_symmetric_form(blksize, nset):
+    """Bitwise-symmetric D_s with lhs and rhs one buffer take the upper-triangle form: the oracle
+    to 1e-12, the general form (CFGMM_TUNE_RHO_SYMMETRIC = 1) to 1e-13, bitwise repeatable.  The
+    general form is kept whenever the symmetric one would be wrong: one large asymmetric entry
+    inside the mask, one asymmetric set among symmetric ones, lhs and rhs in separate buffers."""
+    p = W.rho_problem(seed=21, nshell=260, nao=800, ngrid=blksize * 21 + 13, nset=nset, blksize=blksize,
+                      density=0.4)
+    assert all(np.array_equal(d, d.T) for d in p.dms)
+    ref = _oracle(p, p.dms, p.phi, p.phi)
+    old = cfgmm.tune_get(cfgmm.TUNE_RHO_SYMMETRIC)
+    try:
+        cfgmm.tune_set(cfgmm.TUNE_RHO_SYMMETRIC, 0)
+        sym = _rho_dev(p, p.dms, p.phi, p.phi)
+        sym2 = _rho_dev(p, p.dms, p.phi, p.phi)
+        cfgmm.tune_set(cfgmm.TUNE_RHO_SYMMETRIC, 1)
+        gen = _rho_dev(p, p.dms, p.phi, p.phi)
+        cfgmm.tune_set(cfgmm.TUNE_RHO_SYMMETRIC, 0)
+        assert rel_fro(sym, ref) <= TOL
+        assert rel_fro(sym, gen) <= 1e-13
+        assert np.array_equal(sym, sym2)
+        assert not np.array_equal(sym, gen)  # the symmetric form ran
+        # one entry D[u][v] (u, v both active in block 0, in different chunks) made asymmetric
+        lst = np.flatnonzero(p.tab[:, 0])
+        u, v = int(lst[3]), int(lst[min(len(lst) - 1, 70)])
+        bad = p.dms.copy()
+        bad[nset - 1, u, v] += 1000.0
+        got = _rho_dev(p, bad, p.phi, p.phi)
+        assert rel_fro(got, _oracle(p, bad, p.phi, p.phi)) <= TOL
+        # lhs and rhs equal in value but separate buffers: the general form (bitwise)
+        assert np.array_equal(_rho_dev(p, p.dms, p.phi, p.phi, same_buffer=False), gen)
+    finally:
+        cfgmm.tune_set(cfgmm.TUNE_RHO_SYMMETRIC, old)
