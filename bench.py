@@ -16,8 +16,8 @@ The same JSON line carries:
     timed on this host's cores on a bounded sample of the same workload;
   * dgemm: FP64 DGEMM TFLOP/s and fraction of the FP64 MFMA peak for the
     1031x977x1543 NT and 3527x9583x6581 NN shapes (dgemm_kernel hipEvent time);
-  * split6_resident: the same evaluation in split precision (6 int8 slices) over
-    resident per-block operands, with its error vs the oracle;
+  * split_resident: the same evaluation in split precision (RESIDENT_SLICES balanced int8
+    digits) over resident per-block operands, with its error vs the oracle;
   * rho_4_components: four left-hand factors (GGA-shaped: phi and three gradient-like
     arrays) over one D.phi pass vs four separate evaluations;
   * aogridao2mat: the contraction after rho, V_uv += sum_g phi_ug w_g phi_vg (masked);
@@ -61,6 +61,9 @@ FP64_MFMA_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
 METRIC = "FP64 DGEMM TFLOP/s (% peak); rho evals/s + p99 latency at ≤1e-10 rel err"
 README_RHO_EVALS_PER_S = 1000.0 / 120.0  # README.md:51, 16-core Ryzen 7945HX
+# int8 slices of the resident split-precision rho (balanced digits, about 8 bits each): the fewest
+# that keep rho within 1e-10 of the FP64 oracle on the workloads here (five: ~1e-11)
+RESIDENT_SLICES = 5
 # Bytes moved past L2 by one rho_chunk_kernel launch on the default workload, from two rocprofv3
 # --pmc passes (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, factors from a 256 MiB device copy):
 # profiles/rho_chunk_kernel_fp64_nao2228_ngrid16384_nset3.txt.  Only valid for that workload.
@@ -127,7 +130,7 @@ def bench_rho(p: W.RhoProblem, steps: int, warmup: int, precision=cfgmm.PREC_FP6
                 kernel_avg_ms=kern_ms / max(kern_n, 1), kernel_launches=kern_n)
 
 
-def bench_resident(p: W.RhoProblem, steps: int, nslices: int = 6):
+def bench_resident(p: W.RhoProblem, steps: int, nslices: int = RESIDENT_SLICES):
     s = cfgmm.Stream()
     d_r = cfgmm.DeviceBuffer(8 * p.nset * p.ngrid)
     d_d = cfgmm.DeviceBuffer.from_array(p.dms)
@@ -498,7 +501,7 @@ def bench_serving(args):
     budget = int(min(free * 0.85, 220 * 2 ** 30))
     lams = [50, 100, 200, 500, 1000, 2000]
     out = {"metric": METRIC, "unit": "rho evals/s", "n_gpus": 1, "higher_is_better": True, "scaling": "weak",
-           "vs_baseline": None, "dtype": "f64 (rho split precision: 6 int8 slices)", "data": "synthetic",
+           "vs_baseline": None, "dtype": f"f64 (rho split precision: {RESIDENT_SLICES} int8 slices)", "data": "synthetic",
            "config": {"workload": "serving: Poisson arrivals, 70 % masked rho (8 molecules, nao 500..4000, ngrid "
                                   "64..65536, nset 1..4, density 10..60 %), 30 % DGEMM m,n,k log-uniform 256..4096",
                       "hbm_budget_GiB": round(budget / 2 ** 30, 1), "slots": 4, "max_batch": 16,
@@ -506,7 +509,7 @@ def bench_serving(args):
                                      "density": round(float(m.prob.shell_tab.mean()), 3)} for m in pool]}}
 
     def server(prec):
-        sv = cfgmm.Server(budget, slots=4, rho_precision=prec, rho_nslices=6, max_batch=16)
+        sv = cfgmm.Server(budget, slots=4, rho_precision=prec, rho_nslices=RESIDENT_SLICES, max_batch=16)
         for i, m in enumerate(pool):
             sv.add_molecule(i, m.prob.dms, m.prob.phi, np.ascontiguousarray(m.prob.tab), m.prob.blksize)
         t0 = time.perf_counter()
@@ -535,7 +538,7 @@ def bench_serving(args):
         return d
 
     sweeps = {}
-    for prec, name in ((cfgmm.PREC_OZAKI_INT8, "split6"), (cfgmm.PREC_FP64, "fp64")):
+    for prec, name in ((cfgmm.PREC_OZAKI_INT8, "split"), (cfgmm.PREC_FP64, "fp64")):
         log(f"[bench] serving, rho in {name}")
         sv = server(prec)
         rows = []
@@ -549,7 +552,7 @@ def bench_serving(args):
         sweeps[name] = rows
         st = sv.stats()
         sweeps[name + "_stats"] = {k: int(v) for k, v in st.items()}
-        if name == "split6":
+        if name == "split":
             # alone vs in the mix, and oracle samples, at lambda = 500
             reqs = W.serving_requests(pool, 200, 500.0, seed=77)
             alone = []
@@ -585,7 +588,7 @@ def bench_serving(args):
             # the FP64 served form at its best of two dispatch modes: fused batched launches
             # (max_batch 16) or one per-call launch per request on each of the 4 slots
             served_fp64 = summarize(rq, sv.run(creq(rq)))
-            sv1 = cfgmm.Server(budget, slots=4, rho_precision=prec, rho_nslices=6, max_batch=1)
+            sv1 = cfgmm.Server(budget, slots=4, rho_precision=prec, rho_nslices=RESIDENT_SLICES, max_batch=1)
             for i, m in enumerate(pool):
                 sv1.add_molecule(i, m.prob.dms, m.prob.phi, np.ascontiguousarray(m.prob.tab), m.prob.blksize)
             sv1.warm(list(range(len(pool))))
@@ -621,10 +624,10 @@ def bench_serving(args):
     out["steps"] = len(rq)
     out["warmup"] = 0
     out["ms_per_step"] = round(1e3 / served["rho_evals_per_s"], 4)
-    out["saturated_rho_trace"] = {"requests": len(rq), "served_split6": served, "served_fp64": served_fp64,
-                                  "plain_fp64_per_call_rho_evals_per_s": plain,
-                                  "speedup_served_split6_vs_plain_fp64": round(served["rho_evals_per_s"] / plain, 2),
-                                  "speedup_served_split6_vs_served_fp64":
+    out["saturated_rho_trace"] = {"requests": len(rq), "split_slices": RESIDENT_SLICES, "served_split": served,
+                                  "served_fp64": served_fp64, "plain_fp64_per_call_rho_evals_per_s": plain,
+                                  "speedup_served_split_vs_plain_fp64": round(served["rho_evals_per_s"] / plain, 2),
+                                  "speedup_served_split_vs_served_fp64":
                                       round(served["rho_evals_per_s"] / served_fp64["rho_evals_per_s"], 2)}
     out["sweep"] = sweeps
     print(json.dumps(out), flush=True)
@@ -783,11 +786,11 @@ def main():
                      "achieved_compulsory_GBs": round(p.compulsory_bytes() / (ms_per_step * 1e-3) / 1e9, 1)},
     }
 
-    # split precision over resident operands (6 int8 slices): repeated evaluations of the same
+    # split precision over resident operands (RESIDENT_SLICES int8 slices): repeated evaluations of the same
     # (D, phi, mask), as in serving; reported beside the FP64 line, never as `value`
-    log("[bench] split precision, resident operands (6 slices)")
+    log(f"[bench] split precision, resident operands ({RESIDENT_SLICES} slices)")
     rs = bench_resident(p, steps=max(20, args.steps // 2))
-    out_split = {"ms_per_eval": round(rs["ms"], 4), "evals_per_s": round(1e3 / rs["ms"], 2),
+    out_split = {"slices": RESIDENT_SLICES, "ms_per_eval": round(rs["ms"], 4), "evals_per_s": round(1e3 / rs["ms"], 2),
                  "kernel_avg_ms": round(rs["kernel_ms"], 4), "create_ms": round(rs["create_ms"], 2),
                  "resident_GiB": round(rs["bytes"] / 2 ** 30, 3), "speedup_vs_fp64": round(ms_per_step / rs["ms"], 2),
                  "rel_err_vs_fp64_gpu": rel_fro(rs["rho"], r["rho_once"])}
@@ -830,7 +833,7 @@ def main():
         out["cpu_baseline"] = None
         out["rel_err"] = None
 
-    out["split6_resident"] = out_split
+    out["split_resident"] = out_split
     out["rho_4_components"] = out_gga
     out["aogridao2mat"] = out_vxc
     out["gto_tabulation"] = out_gto

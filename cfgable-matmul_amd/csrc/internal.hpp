@@ -60,6 +60,9 @@ __host__ __device__ inline size_t div_up(size_t a, size_t b) { return (a + b - 1
 // ---- kernel launchers (one per .hip file) -----------------------------------
 // Compacts, for each column block b < nblk, the rows x < nrows with
 // tab[x*ldtab + b] != 0 into lists[b*nrows + *] (ascending) and counts[b].
+// *flag = 0 unless every D_s (s < nset) equals its transpose bitwise on [0, nao)^2; the caller
+// sets *flag beforehand (nothing is written when all are symmetric)
+int launch_dm_symmetry(hipStream_t stream, const double *dms, size_t stride_ao, size_t nao, size_t nset, int *flag);
 int launch_build_lists(hipStream_t stream, const uint8_t *tab, size_t ldtab, size_t nrows, size_t nblk, int *lists,
                        int *counts);
 // rho[s*stride_grid + g] += sum over the 64-row chunks of g's block of partial[chunk][s][g]

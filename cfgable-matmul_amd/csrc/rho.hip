@@ -111,7 +111,8 @@ constexpr int kPlanBuckets = 1024;
 // inside a bucket follows atomics and varies from call to call, which the result does not
 // depend on (each task writes its own partial slot, reduced in fixed chunk order).
 // bounds[0..8] (general form) and bounds[9..17] (symmetric form): the sequence cut into eight
-// ranges of equal work, one per XCD; a task's work is rows x (contracted positions + 64).
+// ranges of equal work, one per XCD; a task's work is rows x (contracted positions + 64).  The
+// general form's work is even enough along the sequence that equal counts serve it better.
 __global__ __launch_bounds__(kPlanThreads) void rho_plan_kernel(const int *__restrict__ lists,
                                                                 const int *__restrict__ counts, int nao, int nblk,
                                                                 int maxchunks, int *__restrict__ perm,
@@ -221,20 +222,9 @@ __global__ __launch_bounds__(256) void dm_symmetry_kernel(const double *__restri
     const int tj = blockIdx.x, ti = blockIdx.y, s = blockIdx.z;
     if (ti > tj) return;
     __shared__ unsigned long long t[64][65];
-    const double *__restrict__ d = dms + (long)s * stride_ao * stride_ao;
-    const int c = threadIdx.x & 63, r0 = threadIdx.x >> 6;
-    const int i0 = ti * 64, j0 = tj * 64;
-    for (int r = r0; r < 64; r += 4) {
-        const int i = i0 + r, j = j0 + c;
-        t[r][c] = i < nao && j < nao ? (unsigned long long)__double_as_longlong(d[(long)i * stride_ao + j]) : 0ull;
-    }
-    __syncthreads();
-    bool differ = false;
-    for (int r = r0; r < 64; r += 4) {  // row j0 + r, column i0 + c of D against t[c][r]
-        const int j = j0 + r, i = i0 + c;
-        if (i < nao && j < nao) differ |= (unsigned long long)__double_as_longlong(d[(long)j * stride_ao + i]) != t[c][r];
-    }
-    if (__ballot(differ) != 0 && (threadIdx.x & 63) == 0) __hip_atomic_store(sym, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (rho::tile_pair_differs(dms + (long)s * stride_ao * stride_ao, stride_ao, nao, ti, tj, t) &&
+        threadIdx.x == 0)
+        __hip_atomic_store(sym, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 template <int NS, int NT>
@@ -269,16 +259,18 @@ __global__ __launch_bounds__(kThreads, 2) void rho_chunk_kernel(
         t.ncomp = ncomp;
         t.lhs_comp_stride = lhs_comp_stride;
         t.part_comp_stride = part_comp_stride;
-        t.sym = symmetric;
-        rho::rho_chunk_body<NS, NT>(t, smem);
+        if (symmetric)
+            rho::rho_chunk_body<NS, NT, true>(t, smem);
+        else
+            rho::rho_chunk_body<NS, NT, false>(t, smem);
     };
     const int x = blockIdx.x % 8, k = blockIdx.x / 8;
-    if (perm) {
-        // AO-ordered tasks: XCD x (workgroup i runs on XCD i % 8) takes the planned positions
-        // [bounds[x], bounds[x+1]) -- equal work per XCD -- for every group and subtile, group-major;
-        // its workgroups stride over them (several tasks each where short tasks are many)
-        const int *__restrict__ bd = bounds + (symmetric ? 9 : 0);
-        const int b0 = bd[x], npos = bd[x + 1] - b0;
+    if (perm && symmetric) {
+        // AO-ordered tasks of the symmetric form, whose work falls with the AO range: XCD x
+        // (workgroup i runs on XCD i % 8) takes the planned positions [bounds[x], bounds[x+1]) --
+        // equal work per XCD -- for every group and subtile, group-major; its workgroups stride
+        // over them (several tasks each where short tasks are many)
+        const int b0 = bounds[9 + x], npos = bounds[9 + x + 1] - b0;
         const long per = (long)npos * nsub, items = per * ngroups;
         for (long kk = k; kk < items; kk += gridDim.x / 8) {
             const int grp = (int)(kk / per);
@@ -286,6 +278,16 @@ __global__ __launch_bounds__(kThreads, 2) void rho_chunk_kernel(
             const int sl = perm[b0 + rj / nsub];
             run(sl / maxchunks, sl % maxchunks, grp, (int)(rj % nsub));
         }
+    } else if (perm) {
+        // AO-ordered tasks (group of NS density matrices, planned chunk j, subtile), the
+        // sequence cut into eight contiguous parts of equal count, one per XCD (the grid is a
+        // multiple of 8 with at least as many workgroups as tasks)
+        const long per = (long)*ntask * nsub, total = per * ngroups;
+        const long part = (total + 7) / 8, tt = x * part + k;
+        if (k >= part || tt >= total) return;
+        const long rj = tt % per;
+        const int sl = perm[rj / nsub];
+        run(sl / maxchunks, sl % maxchunks, (int)(tt / per), (int)(rj % nsub));
     } else {
         // block-major (block, subtile, group, chunk): the tasks of one block, which share its
         // phi tile, run back to back on one XCD; empty chunk slots exit
@@ -297,7 +299,6 @@ __global__ __launch_bounds__(kThreads, 2) void rho_chunk_kernel(
         const int rest2 = rest / ngroups;
         run(rest2 / nsub, chunk, grp, rest2 % nsub);
     }
-    (void)ntask;
 }
 
 template <int NS>
@@ -322,6 +323,17 @@ int launch_ns(int nt, dim3 grid, hipStream_t stream, double *partial, const doub
 }
 
 }  // namespace
+
+int launch_dm_symmetry(hipStream_t stream, const double *dms, size_t stride_ao, size_t nao, size_t nset, int *flag) {
+    if (nao == 0 || nset == 0) return CFGMM_OK;
+    const size_t nt64 = div_up(nao, 64);
+    if (nt64 > 65535 || nset > 65535) return fail(CFGMM_EINVAL, "dm symmetry: matrix too large");
+    hipEvent_t t0 = trace_start(stream);
+    hipLaunchKernelGGL(dm_symmetry_kernel, dim3((unsigned)nt64, (unsigned)nt64, (unsigned)nset), dim3(256), 0, stream,
+                       dms, (long)stride_ao, (int)nao, flag);
+    trace_stop(stream, "dm_symmetry_kernel", t0);
+    return hip_status(hipGetLastError(), "dm_symmetry_kernel launch");
+}
 
 // Columns per workgroup: the widest of 64/48/32/16 grid points that tiles a
 // block exactly (blksize is a multiple of 16).
@@ -386,12 +398,7 @@ int launch_rho_multi(hipStream_t stream, double *rhos, size_t rho_cs, const doub
     const bool try_sym = ordered && ncomp == 1 && ao_lhs == ao_rhs && tune_get(CFGMM_TUNE_RHO_SYMMETRIC) == 0 &&
                          div_up(nao, 64) <= 65535;
     if (try_sym) {
-        const unsigned nt64 = (unsigned)div_up(nao, 64);
-        hipEvent_t t0 = trace_start(stream);
-        hipLaunchKernelGGL(dm_symmetry_kernel, dim3(nt64, nt64, (unsigned)nset), dim3(256), 0, stream, dms,
-                           (long)stride_ao, (int)nao, sym);
-        trace_stop(stream, "dm_symmetry_kernel", t0);
-        st = hip_status(hipGetLastError(), "dm_symmetry_kernel launch");
+        st = launch_dm_symmetry(stream, dms, stride_ao, nao, nset, sym);
         if (st != CFGMM_OK) return st;
     }
     const int *symp = try_sym ? sym : nullptr;

@@ -14,6 +14,8 @@
 //      stealing from the other ranges at the end) until all are drained and
 //      runs the same chunk body as the per-problem kernel (rho_body.hpp).
 //   4. batch_reduce:      rho += sum of the chunk partials, in a fixed order.
+// A problem whose lhs and rhs are one array and whose D_s are all symmetric (batch_symmetry, on
+// the device) takes the upper-triangle form of the chunk body, as the per-problem call does.
 // Every task writes its own partial slot, so the result does not depend on
 // which workgroup ran which task: it is bitwise the same as a per-problem call.
 #include "internal.hpp"
@@ -36,7 +38,34 @@ struct BatchProblem {
     long stride_ao, stride_grid, ldtab;
     int nao, ngrid, nset, blksize, nblk, nsub, nt;
     long lists_off, counts_off, partial_off;
+    int sym;  // 1: lhs is rhs; cleared by batch_symmetry unless every D_s is symmetric
 };
+
+// Workgroup (k, problem): the k-th pair (ti <= tj) of 64x64 tiles of the problem's D_s, every s;
+// clears the problem's sym flag at a difference.
+__global__ __launch_bounds__(256) void batch_symmetry(BatchProblem *__restrict__ probs) {
+    BatchProblem &p = probs[blockIdx.y];
+    // read once for the workgroup (another workgroup may clear it meanwhile): all threads leave
+    // together or none does -- the comparison below has barriers
+    __shared__ int go;
+    if (threadIdx.x == 0) go = __hip_atomic_load(&p.sym, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    if (!go) return;
+    const long k = blockIdx.x;
+    // upper-triangle index k = tj (tj + 1) / 2 + ti, 0 <= ti <= tj
+    long tj = (long)((sqrt(8.0 * (double)k + 1.0) - 1.0) / 2.0);
+    while (tj * (tj + 1) / 2 > k) --tj;
+    while ((tj + 1) * (tj + 2) / 2 <= k) ++tj;
+    const int ti = (int)(k - tj * (tj + 1) / 2);
+    if (tj * 64 >= p.nao) return;
+    __shared__ unsigned long long t[64][65];
+    for (int s = 0; s < p.nset; ++s) {
+        if (rho::tile_pair_differs(p.dms + (long)s * p.stride_ao * p.stride_ao, p.stride_ao, p.nao, ti, (int)tj, t)) {
+            if (threadIdx.x == 0) __hip_atomic_store(&p.sym, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return;  // the answer is the workgroup's: every thread leaves together
+        }
+    }
+}
 
 __global__ __launch_bounds__(256) void batch_build_lists(const BatchProblem *__restrict__ probs, int *__restrict__ lists,
                                                          int *__restrict__ counts) {
@@ -177,14 +206,21 @@ __global__ void zero_int_kernel(int *p, int n) {
     if (i < n) p[i] = 0;
 }
 
-template <int NS>
-__device__ inline void run_chunk_ns(int nt, const rho::RhoChunk &t, unsigned char *smem) {
+template <int NS, bool SYM>
+__device__ inline void run_chunk_nt(int nt, const rho::RhoChunk &t, unsigned char *smem) {
     switch (nt) {
-        case 1: rho::rho_chunk_body<NS, 1>(t, smem); break;
-        case 2: rho::rho_chunk_body<NS, 2>(t, smem); break;
-        case 3: rho::rho_chunk_body<NS, 3>(t, smem); break;
-        default: rho::rho_chunk_body<NS, 4>(t, smem); break;
+        case 1: rho::rho_chunk_body<NS, 1, SYM>(t, smem); break;
+        case 2: rho::rho_chunk_body<NS, 2, SYM>(t, smem); break;
+        case 3: rho::rho_chunk_body<NS, 3, SYM>(t, smem); break;
+        default: rho::rho_chunk_body<NS, 4, SYM>(t, smem); break;
     }
+}
+template <int NS>
+__device__ inline void run_chunk_ns(int nt, bool sym, const rho::RhoChunk &t, unsigned char *smem) {
+    if (sym)
+        run_chunk_nt<NS, true>(nt, t, smem);
+    else
+        run_chunk_nt<NS, false>(nt, t, smem);
 }
 
 template <int NSMAX>
@@ -235,16 +271,17 @@ __global__ __launch_bounds__(kThreads) void batch_rho_kernel(const BatchProblem 
         t.cnt = counts[p.counts_off + w.y];
         t.chunk = w.z & 0xffff;
         t.g0 = w.y * p.blksize + (w.z >> 16) * p.nt * 16;
+        const bool sym = p.sym != 0;
         if (t.g0 >= p.ngrid) continue;  // subtile past the end of a partial last block (uniform)
         const int ns = min(NSMAX, p.nset - w.w);
         if (NSMAX >= 4 && ns == 4)
-            run_chunk_ns<(NSMAX >= 4 ? 4 : 1)>(p.nt, t, smem);
+            run_chunk_ns<(NSMAX >= 4 ? 4 : 1)>(p.nt, sym, t, smem);
         else if (NSMAX >= 3 && ns == 3)
-            run_chunk_ns<(NSMAX >= 3 ? 3 : 1)>(p.nt, t, smem);
+            run_chunk_ns<(NSMAX >= 3 ? 3 : 1)>(p.nt, sym, t, smem);
         else if (NSMAX >= 2 && ns == 2)
-            run_chunk_ns<(NSMAX >= 2 ? 2 : 1)>(p.nt, t, smem);
+            run_chunk_ns<(NSMAX >= 2 ? 2 : 1)>(p.nt, sym, t, smem);
         else
-            run_chunk_ns<1>(p.nt, t, smem);
+            run_chunk_ns<1>(p.nt, sym, t, smem);
     }
 }
 
@@ -286,7 +323,8 @@ int launch_rho_batched(hipStream_t stream, const cfgmm_rho_problem *problems, si
     });
     std::vector<BatchProblem> bp;
     bp.reserve(nprob);
-    long lists_n = 0, counts_n = 0, partial_n = 0, tasks_max = 0;
+    long lists_n = 0, counts_n = 0, partial_n = 0, tasks_max = 0, max_pairs = 0;
+    bool any_sym = false;
     int max_nblk = 0;
     long max_elems = 0;
     for (size_t oi : order) {
@@ -313,6 +351,9 @@ int launch_rho_batched(hipStream_t stream, const cfgmm_rho_problem *problems, si
         p.lists_off = lists_n;
         p.counts_off = counts_n;
         p.partial_off = partial_n;
+        p.sym = q.ao_lhs == q.ao_rhs && tune_get(CFGMM_TUNE_RHO_SYMMETRIC) == 0 ? 1 : 0;
+        any_sym |= p.sym != 0;
+        max_pairs = std::max(max_pairs, (long)div_up(q.nao, 64) * ((long)div_up(q.nao, 64) + 1) / 2);
         const long maxchunks = (long)div_up(q.nao, kUC);
         if (maxchunks > 0xffff || p.nsub > 0x7fff) return fail(CFGMM_EINVAL, "batched rho: problem too large");
         lists_n += (long)p.nblk * p.nao;
@@ -358,6 +399,8 @@ int launch_rho_batched(hipStream_t stream, const cfgmm_rho_problem *problems, si
     hipLaunchKernelGGL(batch_scan_offsets, dim3(1), dim3(1024), 0, stream, d_probs, np, nsmax, cnum, toff, ctr);
     hipLaunchKernelGGL(batch_fill_tasks, dim3((unsigned)np), dim3(kFillThreads), 0, stream, d_probs, lists, counts,
                        nsmax, cnum, toff, tasks);
+    if (any_sym)  // the symmetric form where lhs is rhs and every D_s is symmetric
+        hipLaunchKernelGGL(batch_symmetry, dim3((unsigned)max_pairs, (unsigned)np), dim3(256), 0, stream, d_probs);
     trace_stop(stream, "batch_prepare", t0);
     st = hip_status(hipGetLastError(), "batched rho prepare launch");
     if (st != CFGMM_OK) return st;

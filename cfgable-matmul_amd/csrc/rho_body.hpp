@@ -27,18 +27,17 @@ struct RhoChunk {
     // reads ao_lhs + c*lhs_comp_stride and writes partial + c*part_comp_stride
     int ncomp = 1;
     long lhs_comp_stride = 0, part_comp_stride = 0;
-    // D_s symmetric and lhs == rhs (one component): the chunk contracts only list positions from
-    // its own first row on, the positions past its 64 rows twice (the pairs of those positions
-    // with earlier rows are the mirror images of pairs that the earlier chunks count)
-    int sym = 0;
 };
+// rho_chunk_body<NS, NT, true>: D_s symmetric and lhs == rhs (one component): the chunk contracts
+// only list positions from its own first row on, the positions past its 64 rows twice (the pairs
+// of those positions with earlier rows are the mirror images of pairs that the earlier chunks count).
 
 template <int NS, int NT>
 constexpr size_t chunk_lds_bytes() {
     return sizeof(double) * (2 * NS * kKV * (kUC + 1) + 2 * kKV * NT * 16) + sizeof(int) * (kSeg + kUC);
 }
 
-template <int NS, int NT>
+template <int NS, int NT, bool SYM = false>
 __device__ inline void rho_chunk_body(const RhoChunk &task, unsigned char *__restrict__ smem) {
     constexpr int W = NT * 16;                          // grid points per workgroup
     constexpr int PHI_PER_T = kKV * W / kThreads;       // phi_rhs elements per thread per slice
@@ -96,10 +95,10 @@ __device__ inline void rho_chunk_body(const RhoChunk &task, unsigned char *__res
     // Symmetric form: positions past the chunk's rows first, the accumulators doubled, then the
     // chunk's own 64 positions.  General form: the whole list.
     const int own_end = min(u_begin + kUC, cnt);
-    const int nranges = task.sym ? 2 : 1;
+    constexpr int nranges = SYM ? 2 : 1;
     for (int rg = 0; rg < nranges; ++rg) {
-        const int kb = task.sym ? (rg == 0 ? own_end : u_begin) : 0;
-        const int ke = task.sym && rg == 1 ? own_end : cnt;
+        const int kb = SYM ? (rg == 0 ? own_end : u_begin) : 0;
+        const int ke = SYM && rg == 1 ? own_end : cnt;
         for (int seg0 = kb; seg0 < ke; seg0 += kSeg) {
             const int segn = min(kSeg, ke - seg0);
             __syncthreads();  // previous segment's readers of vls / slices are done
@@ -218,7 +217,7 @@ __device__ inline void rho_chunk_body(const RhoChunk &task, unsigned char *__res
                 __syncthreads();
             }
         }
-        if (task.sym && rg == 0) {
+        if (SYM && rg == 0) {
 #pragma unroll
             for (int s = 0; s < NS; ++s)
 #pragma unroll
@@ -270,6 +269,27 @@ __device__ inline void rho_chunk_body(const RhoChunk &task, unsigned char *__res
     }
 }
 
+
+// Whether the 64x64 tiles (ti, tj) and (tj, ti) of the nao x nao matrix d (row stride ld) are
+// not each other's transpose, bitwise; the same answer on every thread of the workgroup (256
+// threads, all of which must call it).  t: LDS scratch of 64 x 65 words.
+__device__ inline bool tile_pair_differs(const double *__restrict__ d, long ld, int nao, int ti, int tj,
+                                         unsigned long long (*t)[65]) {
+    const int c = threadIdx.x & 63, r0 = threadIdx.x >> 6;
+    const int i0 = ti * 64, j0 = tj * 64;
+    __syncthreads();  // an earlier call's readers of t are done
+    for (int r = r0; r < 64; r += 4) {
+        const int i = i0 + r, j = j0 + c;
+        t[r][c] = i < nao && j < nao ? (unsigned long long)__double_as_longlong(d[(long)i * ld + j]) : 0ull;
+    }
+    __syncthreads();
+    bool differ = false;
+    for (int r = r0; r < 64; r += 4) {  // row j0 + r, column i0 + c of d against t[c][r]
+        const int j = j0 + r, i = i0 + c;
+        if (i < nao && j < nao) differ |= (unsigned long long)__double_as_longlong(d[(long)j * ld + i]) != t[c][r];
+    }
+    return __syncthreads_or(differ ? 1 : 0) != 0;
+}
 
 }  // namespace rho
 }  // namespace cfgmm

@@ -10,7 +10,7 @@
 // case, where a molecule stays resident across requests -- the gather is done
 // once, at creation, into one contiguous tile per block:
 //
-//   Dq[b][s][p](i, j)  p-th 7-bit slice of D_s[L_b[i]][L_b[j]] * 2^-ed[b][s][i],
+//   Dq[b][s][p](i, j)  p-th balanced int8 digit of D_s[L_b[i]][L_b[j]] * 2^-ed[b][s][i] (slice4),
 //                      i, j < kpad_b = round_up(|L_b|, 64) (zero padded),
 //                      ed = exponent of the row maximum over the block's own v;
 //                      stored as 1 KiB tiles of 16 rows x 64 k, [i/16][j/64], each
@@ -28,8 +28,14 @@
 // slice), shares the B tile of the block through double-buffered LDS, and
 // accumulates every slice product (sa + sb < S) exactly in int32 with
 // v_mfma_i32_16x16x64_i8.  The epilogue recombines the levels in f64 with the
-// power-of-two scales, multiplies by lhs and reduces the 64 rows; per-tile
+// digit weights and the power-of-two scales, multiplies by lhs and reduces the 64 rows; per-tile
 // partial rows are summed in a fixed order by launch_rho_reduce.
+//
+// Symmetric form (every D_s == D_s^T bitwise, lhs and rhs one array, kpad <= 8192): only the
+// tiles (row tile T of 64 rows, k step ks >= T) are stored, a plane being [T][4 row groups]
+// [ks - T] 1 KiB tiles; a workgroup on row tile T streams its steps past the diagonal, doubles
+// its int32 sums (exact: 2 * 6 * 8192 * 127^2 < 2^31), then adds the diagonal step -- the pairs
+// below the diagonal are the mirror images of those it doubles.  Half the bytes and MFMAs.
 //
 // A row of D_s or a block column of rhs holding Inf/NaN makes the rho values
 // it touches NaN.
@@ -49,14 +55,25 @@ constexpr int kRows = 64;   // compacted rows per workgroup (4 waves x 16)
 constexpr int kK = 64;      // k per v_mfma_i32_16x16x64_i8
 constexpr int kExpNonFinite = 0x7fffffff;
 constexpr int kMaxKpad = 16384;  // keeps every int32 level sum below 2^31: 6 * 16384 * 127^2 < 2^31
+constexpr int kMaxKpadSym = 8192;  // the same with the off-diagonal sums doubled
 
 struct BlockDesc {
-    long d_off;  // bytes: Dq of set 0 ([nset][S] planes of kpad*kpad int8, 1 KiB tiles)
+    long d_off;  // bytes: Dq of set 0 ([nset][S] planes of 1 KiB tiles)
     long r_off;  // bytes: Rq ([S][kpad/64][blksize][64] int8)
     long l_off;  // bytes: Lc ([kpad][blksize] double)
     long e_off;  // bytes: ed ([nset][kpad] int)
     int n, kpad;
+    int sym;  // upper-triangle tiles only
 };
+
+// 1 KiB tiles before row tile T in a plane of the symmetric form (nks k steps): 4 * sum_{t<T} (nks - t)
+__host__ __device__ inline long sym_tiles_before(int T, int nks) {
+    return 4L * ((long)T * nks - (long)T * (T - 1) / 2);
+}
+__host__ __device__ inline long dq_plane_bytes(int kpad, int sym) {
+    const long nks = kpad / kK;
+    return sym ? 1024L * sym_tiles_before((int)nks, (int)nks) : (long)kpad * kpad;
+}
 
 __device__ inline int exponent_of(u64 maxbits) {
     const double m = __longlong_as_double((long long)maxbits);
@@ -67,6 +84,17 @@ __device__ inline int exponent_of(u64 maxbits) {
     return e;
 }
 
+// Balanced digits: x * 2^-e = sum_p d_p / (127 * 254^p) + r, d_p = rint of the running value,
+// |d_p| <= 127, |r| <= 1 / (2 * 127 * 254^(S-1)).  |x * 2^-e| < 1 makes the first value
+// 127 x 2^-e lie in (-127, 127); every later one is 254 times a remainder in [-1/2, 1/2].  About
+// 8 bits per int8 slice (7 with truncated digits of 128): five slices hold what six did.  The
+// product of digits of levels a and b weighs 1 / (127^2 254^(a+b)): a function of a + b only.
+__device__ __host__ constexpr double level_weight(int l) {
+    double w = 1.0 / (127.0 * 127.0);
+    for (int i = 0; i < l; ++i) w /= 254.0;
+    return w;
+}
+
 // byte `q` of the result is slice `p` of x * 2^-e, for four values
 template <int S>
 __device__ inline void slice4(const double x[4], const int e[4], unsigned w[S]) {
@@ -74,12 +102,11 @@ __device__ inline void slice4(const double x[4], const int e[4], unsigned w[S]) 
     for (int p = 0; p < S; ++p) w[p] = 0;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        double rest = e[q] == kExpNonFinite ? 0.0 : ldexp(x[q], -e[q]);
+        double v = e[q] == kExpNonFinite ? 0.0 : ldexp(x[q], -e[q]) * 127.0;
 #pragma unroll
         for (int p = 0; p < S; ++p) {
-            const double v = rest * 128.0;
-            const double t = trunc(v);
-            rest = v - t;
+            const double t = rint(v);
+            v = (v - t) * 254.0;
             w[p] |= ((unsigned)(int)t & 0xffu) << (8 * q);
         }
     }
@@ -96,15 +123,18 @@ __global__ __launch_bounds__(kThreads) void prep_dq_kernel(unsigned char *__rest
     const int lane = threadIdx.x & 63;
     const int i = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
     if (i >= bd.kpad) return;
-    const long plane = (long)bd.kpad * bd.kpad;
+    const long plane = dq_plane_bytes(bd.kpad, bd.sym);
     const int nks = bd.kpad / kK;
-    // byte of (i, j) inside a plane: 1 KiB tile (i/16, j/64), chunk (j%64)/16 * 16 + i%16, byte j%16
-    unsigned char *__restrict__ out = base + bd.d_off + (long)s * S * plane + (long)(i >> 4) * nks * 1024 + (i & 15) * 16;
-    auto at = [&](int j) { return (long)(j >> 6) * 1024 + ((j & 63) >> 4) * 256 + (j & 15); };
+    // byte of (i, j) inside a plane: 1 KiB tile (i/16, j/64), chunk (j%64)/16 * 16 + i%16, byte j%16;
+    // the symmetric form keeps the tiles j/64 >= T = i/64 of row group i/16, [T][(i/16)%4][j/64 - T]
+    const int T = i >> 6, j0 = bd.sym ? T * kK : 0;
+    const long row_tile = bd.sym ? sym_tiles_before(T, nks) + (long)((i >> 4) & 3) * (nks - T) : (long)(i >> 4) * nks;
+    unsigned char *__restrict__ out = base + bd.d_off + (long)s * S * plane + row_tile * 1024 + (i & 15) * 16;
+    auto at = [&](int j) { return (long)((j - j0) >> 6) * 1024 + ((j & 63) >> 4) * 256 + (j & 15); };
     int *__restrict__ ed = reinterpret_cast<int *>(base + bd.e_off) + (long)s * bd.kpad;
     const int *__restrict__ list = lists + (long)b * nao;
     if (i >= bd.n) {
-        for (int j = 4 * lane; j < bd.kpad; j += 256)
+        for (int j = j0 + 4 * lane; j < bd.kpad; j += 256)
 #pragma unroll
             for (int p = 0; p < S; ++p) *reinterpret_cast<unsigned *>(out + p * plane + at(j)) = 0u;
         if (lane == 0) ed[i] = 0;
@@ -122,7 +152,7 @@ __global__ __launch_bounds__(kThreads) void prep_dq_kernel(unsigned char *__rest
     }
     const int e = exponent_of(m);
     if (lane == 0) ed[i] = e;
-    for (int j = 4 * lane; j < bd.kpad; j += 256) {
+    for (int j = j0 + 4 * lane; j < bd.kpad; j += 256) {
         double x[4];
         int ee[4];
 #pragma unroll
@@ -199,8 +229,10 @@ __device__ inline int xcd_task(int i, int ntasks) {
     return x * q + min(x, r) + i / 8;
 }
 
-// One task (block b, 64-row tile, column subtile) of density matrix s: partial[tile][s][g] for the
-// subtile's grid points g < ngrid.
+// One task of density matrix s: block b, column subtile, and 64-row tile task.y -- or, for a
+// task of the symmetric form with task.w = 1, the tiles task.y and ntiles-1-task.y one after the
+// other (a constant number of k steps per task, one pipeline through both): partial[tile][s][g]
+// for the subtile's grid points g < ngrid.
 template <int S, int NT, int DEPTH>
 __device__ __attribute__((always_inline)) inline void resident_tile(
     double *__restrict__ partial, const unsigned char *__restrict__ base, const BlockDesc *__restrict__ blocks,
@@ -211,35 +243,59 @@ __device__ __attribute__((always_inline)) inline void resident_tile(
     __shared__ __attribute__((aligned(16))) int8_t Bs[2][S][W * kK];
     __shared__ double red[kThreads / 64][W];
 
-    const int b = task.x, tile = task.y, sub = task.z;
+    const int b = task.x, sub = task.z;
     const BlockDesc bd = blocks[b];
     const int kpad = bd.kpad;
     const int c0 = sub * W;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int lrow = lane & 15, lk = lane >> 4;
 
-    const long dplane = (long)kpad * kpad;
+    const long dplane = dq_plane_bytes(kpad, bd.sym);
     const int nks = kpad / kK;
-    // this wave's 16 rows: 1 KiB A tiles at (row tile, k step); lane l reads bytes 16l..16l+15
-    const int8_t *__restrict__ arow = reinterpret_cast<const int8_t *>(base + bd.d_off) + (long)s * S * dplane +
-                                      (long)(tile * (kRows / 16) + wave) * nks * 1024 + lane * 16;
+    const bool sym = bd.sym != 0;
+    // the task's tiles and their k steps: all, or (symmetric form) those past the diagonal, then it
+    const int T1 = task.y, T2 = sym && task.w ? kpad / kRows - 1 - task.y : -1;
+    const int n1 = sym ? nks - T1 : nks, n2 = T2 > T1 ? nks - T2 : 0;
+    const int nsteps = n1 + n2;
+    // this wave's 16 rows of tile T: 1 KiB A tiles at (row tile, k step); lane l reads bytes 16l..16l+15
+    auto arow_of = [&](int T) {
+        const long row_tiles = sym ? sym_tiles_before(T, nks) + (long)wave * (nks - T)
+                                   : (long)(T * (kRows / 16) + wave) * nks;
+        return reinterpret_cast<const int8_t *>(base + bd.d_off) + (long)s * S * dplane + row_tiles * 1024 + lane * 16;
+    };
+    const int8_t *__restrict__ arow1 = arow_of(T1);
+    const int8_t *__restrict__ arow2 = arow_of(T2 > T1 ? T2 : T1);
+    // step t: tile, its local step, k step and A tile within the wave's row of tiles
+    auto ks_of = [&](int t) {
+        const int T = t < n1 ? T1 : T2, l = t < n1 ? t : t - n1, n = t < n1 ? n1 : n2;
+        return sym ? (l < n - 1 ? T + 1 + l : T) : l;
+    };
+    auto a_of = [&](int t) {
+        const int l = t < n1 ? t : t - n1, n = t < n1 ? n1 : n2;
+        return sym ? (l < n - 1 ? l + 1 : 0) : l;
+    };
     const long rplane = (long)blksize * kpad;
     const int8_t *__restrict__ rq = reinterpret_cast<const int8_t *>(base + bd.r_off) + (long)c0 * kK;
 
     i32x4 acc[S][NT];
+    auto zero_acc = [&]() {
 #pragma unroll
-    for (int l = 0; l < S; ++l)
+        for (int l = 0; l < S; ++l)
 #pragma unroll
-        for (int nt = 0; nt < NT; ++nt) acc[l][nt] = i32x4{0, 0, 0, 0};
+            for (int nt = 0; nt < NT; ++nt) acc[l][nt] = i32x4{0, 0, 0, 0};
+    };
+    zero_acc();
 
     i32x4 fa0[S], fa1[S], fa2[S], breg[kBPer];
-    auto load_a = [&](int ks, i32x4 *dst) {
+    auto load_a = [&](int t, i32x4 *dst) {
+        const int8_t *__restrict__ ar = (t < n1 ? arow1 : arow2) + (long)a_of(t) * 1024;
 #pragma unroll
         for (int p = 0; p < S; ++p)  // each A tile is read once per evaluation: stream it past the caches
-            dst[p] = __builtin_nontemporal_load(reinterpret_cast<const i32x4 *>(arow + p * dplane + (long)ks * 1024));
+            dst[p] = __builtin_nontemporal_load(reinterpret_cast<const i32x4 *>(ar + p * dplane));
     };
-    // B tile of step ks: S planes x W columns x 64 k, contiguous per plane
-    auto load_b = [&](int ks) {
+    // B tile of step t: S planes x W columns x 64 k, contiguous per plane
+    auto load_b = [&](int t) {
+        const int ks = ks_of(t);
 #pragma unroll
         for (int e = 0; e < kBPer; ++e) {
             const int q = tid + e * kThreads;
@@ -274,87 +330,120 @@ __device__ __attribute__((always_inline)) inline void resident_tile(
         }
     };
 
-    // A fragments are prefetched DEPTH k steps ahead (DEPTH 1: two register sets, two waves per
-    // SIMD; DEPTH 2: a ring of three with static roles, the loop unrolled by three, one wave per
-    // SIMD with the whole register file); the B tile one step ahead through LDS.  A prefetch
-    // index is clamped instead of guarded (a guard makes the compiler wait for every load at its
-    // join): past the last step it re-reads that step, and the values are not used.
-    const int last = nks - 1;
-    auto step = [&](int ks, const i32x4 *fa_cur, i32x4 *fa_ahead) {
-        const int buf = ks & 1;
+    // Epilogue of tile T.  i32 16x16 layout: lane l, element r holds row 4*(l/16)+r, column l%16.
+    const int *__restrict__ ed = reinterpret_cast<const int *>(base + bd.e_off) + (long)s * kpad;
+    const double *__restrict__ lc = reinterpret_cast<const double *>(base + bd.l_off);
+    const int gblk = b * blksize;
+    auto epilogue = [&](int T) {
+        const double nan = __builtin_nan("");
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+            const int c = c0 + nt * 16 + lrow;
+            const int g = gblk + c;
+            const bool g_ok = g < ngrid;
+            const int eg = g_ok ? fg[g] : 0;
+            double part = 0.0;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = T * kRows + wave * 16 + 4 * lk + r;
+                if (i < bd.n && g_ok) {
+                    const int eu = ed[i];
+                    double t = 0.0;
+#pragma unroll
+                    for (int l = S - 1; l >= 0; --l) t = fma((double)acc[l][nt][r], level_weight(l), t);
+                    t = (eu == kExpNonFinite || eg == kExpNonFinite) ? nan : ldexp(t, eu + eg);
+                    part = fma(lc[(long)i * blksize + c], t, part);
+                }
+            }
+            part += __shfl_xor(part, 16);
+            part += __shfl_xor(part, 32);
+            if (lk == 0) red[wave][nt * 16 + lrow] = part;
+        }
+        __syncthreads();
+        for (int c = tid; c < W; c += kThreads) {
+            const int g = gblk + c0 + c;
+            if (g < ngrid) {
+                double v = red[0][c];
+#pragma unroll
+                for (int w = 1; w < kThreads / 64; ++w) v += red[w][c];
+                partial[((long)T * nset + s) * ngrid + g] = v;
+            }
+        }
+    };
+
+    // A fragments are prefetched DEPTH k steps ahead (DEPTH 1: two register sets; DEPTH 2: a ring
+    // of three; the roles static in a loop unrolled by two or three); the B tile one step ahead
+    // through LDS.  A prefetch index is clamped instead of guarded (a guard makes the compiler wait for
+    // every load at its join): past the last step it re-reads that step, and the values are not
+    // used.  In the symmetric form the int32 sums of a tile are doubled before its diagonal step.
+    const int last = nsteps - 1;
+    auto step = [&](int t, const i32x4 *fa_cur, i32x4 *fa_ahead) {
+        const int buf = t & 1;
         store_b(buf);
         __syncthreads();
-        load_b(min(ks + 1, last));
-        load_a(min(ks + DEPTH, last), fa_ahead);
+        load_b(min(t + 1, last));
+        load_a(min(t + DEPTH, last), fa_ahead);
         __builtin_amdgcn_sched_barrier(0);  // keep the prefetch ahead of the MFMAs (not sunk to its use)
+        if (sym && (t == n1 - 1 || t == last)) {  // the steps past the diagonal count twice
+#pragma unroll
+            for (int l = 0; l < S; ++l)
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt) acc[l][nt] <<= 1;
+        }
         compute(buf, fa_cur);
+    };
+    auto after = [&](int t) {
+        if (t == n1 - 1 && n2 > 0) {  // first tile of a pair done; the pipeline runs on into the second
+            epilogue(T1);
+            zero_acc();
+        }
     };
     load_b(0);
     load_a(0, fa0);
     if constexpr (DEPTH == 1) {
-        for (int ks = 0; ks < nks; ++ks) {
-            step(ks, fa0, fa1);
-#pragma unroll
-            for (int p = 0; p < S; ++p) fa0[p] = fa1[p];
+        // unrolled by two, the register sets swapping roles (no copy that would wait for the prefetch)
+        int t = 0;
+        for (; t + 2 <= nsteps; t += 2) {
+            step(t, fa0, fa1);
+            after(t);
+            step(t + 1, fa1, fa0);
+            after(t + 1);
+        }
+        if (t < nsteps) {
+            step(t, fa0, fa1);
+            after(t);
         }
     } else {
         load_a(min(1, last), fa1);
-        int ks = 0;
-        for (; ks + 3 <= nks; ks += 3) {
-            step(ks, fa0, fa2);
-            step(ks + 1, fa1, fa0);
-            step(ks + 2, fa2, fa1);
+        int t = 0;
+        for (; t + 3 <= nsteps; t += 3) {
+            step(t, fa0, fa2);
+            after(t);
+            step(t + 1, fa1, fa0);
+            after(t + 1);
+            step(t + 2, fa2, fa1);
+            after(t + 2);
         }
-        if (ks < nks) step(ks, fa0, fa2);
-        if (ks + 1 < nks) step(ks + 1, fa1, fa0);
-    }
-
-    // Epilogue.  i32 16x16 layout: lane l, element r holds row 4*(l/16)+r, column l%16.
-    const double nan = __builtin_nan("");
-    const int *__restrict__ ed = reinterpret_cast<const int *>(base + bd.e_off) + (long)s * kpad;
-    const double *__restrict__ lc = reinterpret_cast<const double *>(base + bd.l_off);
-    const int gblk = b * blksize;
-#pragma unroll
-    for (int nt = 0; nt < NT; ++nt) {
-        const int c = c0 + nt * 16 + lrow;
-        const int g = gblk + c;
-        const bool g_ok = g < ngrid;
-        const int eg = g_ok ? fg[g] : 0;
-        double part = 0.0;
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int i = tile * kRows + wave * 16 + 4 * lk + r;
-            if (i < bd.n && g_ok) {
-                const int eu = ed[i];
-                double t = 0.0;
-#pragma unroll
-                for (int l = S - 1; l >= 0; --l) t += ldexp((double)acc[l][nt][r], -7 * (l + 2));
-                t = (eu == kExpNonFinite || eg == kExpNonFinite) ? nan : ldexp(t, eu + eg);
-                part = fma(lc[(long)i * blksize + c], t, part);
-            }
+        if (t < nsteps) {
+            step(t, fa0, fa2);
+            after(t);
         }
-        part += __shfl_xor(part, 16);
-        part += __shfl_xor(part, 32);
-        if (lk == 0) red[wave][nt * 16 + lrow] = part;
-    }
-    __syncthreads();
-    for (int c = tid; c < W; c += kThreads) {
-        const int g = gblk + c0 + c;
-        if (g < ngrid) {
-            double v = red[0][c];
-#pragma unroll
-            for (int w = 1; w < kThreads / 64; ++w) v += red[w][c];
-            partial[((long)tile * nset + s) * ngrid + g] = v;
+        if (t + 1 < nsteps) {
+            step(t + 1, fa1, fa0);
+            after(t + 1);
         }
     }
+    epilogue(n2 > 0 ? T2 : T1);
 }
 
 template <int S, int NT, int DEPTH>
 __global__ __launch_bounds__(kThreads, DEPTH == 1 ? 2 : 1) void rho_resident_kernel(
     double *__restrict__ partial, const unsigned char *__restrict__ base, const BlockDesc *__restrict__ blocks,
     const int4 *__restrict__ tasks, int ntasks, const int *__restrict__ fg, int blksize, int ngrid, int nset) {
-    resident_tile<S, NT, DEPTH>(partial, base, blocks, tasks[xcd_task(blockIdx.x, ntasks)], blockIdx.y, fg, blksize,
-                                ngrid, nset);
+    // (task, density matrix) pairs, task-major: the sets of a block's tasks, which share its Rq and
+    // Lc, sit next to each other in the XCD-contiguous order
+    const int idx = xcd_task(blockIdx.x, ntasks * nset);
+    resident_tile<S, NT, DEPTH>(partial, base, blocks, tasks[idx / nset], idx % nset, fg, blksize, ngrid, nset);
 }
 
 // One evaluation of a batch (device table).
@@ -383,9 +472,9 @@ __global__ __launch_bounds__(kThreads, 2) void rho_resident_batched_kernel(const
         else hi = mid - 1;
     }
     const EvalDesc e = evals[lo];
-    const int t = L - e.wg_begin;
-    resident_tile<S, NT, 1>(e.partial, e.base, e.blocks, e.tasks[t % e.ntasks], t / e.ntasks, e.fg, e.blksize,
-                            e.ngrid, e.nset);
+    const int t = L - e.wg_begin;  // task-major (task, density matrix), as in the single evaluation
+    resident_tile<S, NT, 1>(e.partial, e.base, e.blocks, e.tasks[t / e.nset], t % e.nset, e.fg, e.blksize, e.ngrid,
+                            e.nset);
 }
 
 // rho[s*stride + g] += sum over the 64-row tiles of g's block of partial[tile][s][g], per evaluation.
@@ -418,6 +507,7 @@ struct ResidentRho {
     size_t nao = 0, ngrid = 0, nset = 0, blksize = 0, nblk = 0;
     int nt = 4, nsub = 1;
     int max_tiles = 0;
+    bool symmetric = false;         // the upper-triangle form
     unsigned char *base = nullptr;  // one allocation: tiles, tables, exponents
     size_t bytes = 0;
     BlockDesc *d_blocks = nullptr;
@@ -467,17 +557,28 @@ int resident_create(hipStream_t stream, const double *dms, const double *ao_lhs,
         return CFGMM_OK;
     }
 
-    // 1. compacted AO lists of every block (device), counts to the host
+    // 1. compacted AO lists of every block (device), counts to the host; with one array on both
+    //    sides, whether every D_s is symmetric
     void *ws = nullptr;
     const size_t lists_bytes = sizeof(int) * nblk * nao;
-    int st = workspace_get(stream, lists_bytes + sizeof(int) * nblk, &ws);
+    int st = workspace_get(stream, lists_bytes + sizeof(int) * (nblk + 1), &ws);
     if (st != CFGMM_OK) {
         delete h;
         return st;
     }
     int *lists = static_cast<int *>(ws);
     int *counts = lists + nblk * nao;
+    int *dflag = counts + nblk;
+    const bool try_sym = ao_lhs == ao_rhs && tune_get(CFGMM_TUNE_RHO_SYMMETRIC) == 0;
+    int hflag = try_sym ? 1 : 0;
     st = launch_build_lists(stream, tab, ldtab, nao, nblk, lists, counts);
+    if (st == CFGMM_OK && try_sym) {
+        st = hip_status(hipMemcpyAsync(dflag, &hflag, sizeof(int), hipMemcpyHostToDevice, stream), "resident rho: flag");
+        if (st == CFGMM_OK) st = launch_dm_symmetry(stream, dms, stride_ao, nao, nset, dflag);
+        if (st == CFGMM_OK)
+            st = hip_status(hipMemcpyAsync(&hflag, dflag, sizeof(int), hipMemcpyDeviceToHost, stream),
+                            "resident rho: flag D2H");
+    }
     std::vector<int> hcounts(nblk);
     if (st == CFGMM_OK)
         st = hip_status(hipMemcpyAsync(hcounts.data(), counts, sizeof(int) * nblk, hipMemcpyDeviceToHost, stream),
@@ -487,6 +588,8 @@ int resident_create(hipStream_t stream, const double *dms, const double *ao_lhs,
         delete h;
         return st;
     }
+    const int max_n = nblk ? *std::max_element(hcounts.begin(), hcounts.end()) : 0;
+    h->symmetric = hflag != 0 && (int)div_up((size_t)max_n, kK) * kK <= kMaxKpadSym;
 
     // 2. layout
     auto al = [](size_t x) { return div_up(x, 256) * 256; };
@@ -507,9 +610,9 @@ int resident_create(hipStream_t stream, const double *dms, const double *ao_lhs,
         BlockDesc &bd = blocks[b];
         bd.n = n;
         bd.kpad = kpad;
-        const size_t k2 = (size_t)kpad * kpad;
+        bd.sym = h->symmetric ? 1 : 0;
         bd.d_off = (long)off;
-        off += al(nset * S * k2);
+        off += al(nset * S * (size_t)dq_plane_bytes(kpad, bd.sym));
         bd.r_off = (long)off;
         off += al((size_t)S * blksize * kpad);
         bd.l_off = (long)off;
@@ -518,8 +621,10 @@ int resident_create(hipStream_t stream, const double *dms, const double *ao_lhs,
         off += al(sizeof(int) * nset * kpad);
         const int ntiles = kpad / kRows;
         h->max_tiles = std::max(h->max_tiles, ntiles);
-        for (int t = 0; t < ntiles; ++t)
-            for (int sub = 0; sub < h->nsub; ++sub) tasks.push_back(int4{(int)b, t, sub, 0});
+        // symmetric form: tile t paired with ntiles-1-t (nks+1 k steps per task)
+        const int ntask_tiles = h->symmetric ? (ntiles + 1) / 2 : ntiles;
+        for (int t = 0; t < ntask_tiles; ++t)
+            for (int sub = 0; sub < h->nsub; ++sub) tasks.push_back(int4{(int)b, t, sub, h->symmetric ? 1 : 0});
         h->task_prefix[b + 1] = (int)tasks.size();
     }
     const size_t o_blocks = off;
@@ -627,7 +732,8 @@ int resident_eval(hipStream_t stream, const ResidentRho *h, double *rhos, size_t
     int st = workspace_get(stream, sizeof(double) * h->max_tiles * nset_used * ngrid_used, &ws);
     if (st != CFGMM_OK) return st;
     double *partial = static_cast<double *>(ws);
-    const dim3 grid((unsigned)ntasks, (unsigned)nset_used, 1);
+    if ((size_t)ntasks * nset_used > INT32_MAX) return fail(CFGMM_EINVAL, "resident rho: too many workgroups");
+    const dim3 grid((unsigned)(ntasks * nset_used), 1, 1);
     switch (h->nslices) {
         case 2: st = eval_launch<2>(stream, h, partial, grid, ntasks, (int)ngrid_used, (int)nset_used); break;
         case 3: st = eval_launch<3>(stream, h, partial, grid, ntasks, (int)ngrid_used, (int)nset_used); break;

@@ -28,11 +28,16 @@ def _oracle(p, rhos0=None, lhs=None, rhs=None, dms=None):
 
 
 class Resident:
-    def __init__(self, p, ns=6, lhs=None, rhs=None, dms=None):
+    """same: lhs and rhs in one device buffer (the symmetric form when D is symmetric)."""
+
+    def __init__(self, p, ns=6, lhs=None, rhs=None, dms=None, same=False):
         self.s = cfgmm.Stream()
         self.bufs = [cfgmm.DeviceBuffer.from_array(np.ascontiguousarray(x)) for x in
                      (p.dms if dms is None else dms, p.phi if lhs is None else lhs, p.phi if rhs is None else rhs,
                       p.tab)]
+        if same:
+            self.bufs[2].free()
+            self.bufs[2] = self.bufs[1]
         d, l, r, t = self.bufs
         self.h = cfgmm.RhoResident(self.s, d, l, r, p.nao, p.ngrid, p.nao, p.ngrid, p.nset, t, p.nblk, p.blksize,
                                    cfgmm.PREC_OZAKI_INT8, ns)
@@ -52,27 +57,30 @@ class Resident:
 
     def close(self):
         self.h.destroy()
-        for b in self.bufs:
+        for b in {id(b): b for b in self.bufs}.values():
             b.free()
         self.s.destroy()
 
 
-def test_resident_full_workload():
+@pytest.mark.parametrize("ns,same", [(6, False), (5, True)])
+def test_resident_full_workload(ns, same):
+    """The full workload with six slices (general form) and with five (symmetric form)."""
     p = W.rho_problem(seed=0)
     ref = _oracle(p)
-    r = Resident(p)
+    r = Resident(p, ns=ns, same=same)
     got = r.eval().reshape(p.nset, p.ngrid)
     again = r.eval().reshape(p.nset, p.ngrid)
     nbytes = r.h.bytes
     r.close()
     err = rel_fro(got, ref)
-    print(f"resident ns=6 rel_fro {err:.3e}, {nbytes / 2**30:.2f} GiB")
+    print(f"resident ns={ns} rel_fro {err:.3e}, {nbytes / 2**30:.2f} GiB")
     assert err <= 1e-10
     assert np.array_equal(got, again)
 
 
-@pytest.mark.parametrize("ns,tol", [(4, 1e-6), (5, 1e-8)])
+@pytest.mark.parametrize("ns,tol", [(4, 1e-7), (5, 1e-10)])
 def test_resident_fewer_slices(ns, tol):
+    """Balanced digits (about 8 bits per int8 slice): five slices within 1e-10, four within 1e-7."""
     p = W.rho_problem(seed=3, nshell=200, nao=640, ngrid=2048, nset=2, density=0.3)
     r = Resident(p, ns=ns)
     got = r.eval().reshape(p.nset, p.ngrid)
@@ -221,3 +229,54 @@ def test_resident_eval_batched_bitwise():
     s.destroy()
     for o, ref in zip(outs, refs):
         assert np.array_equal(o, ref)
+
+
+def test_resident_symmetric_form():
+    """Symmetric D with lhs and rhs one buffer: the upper-triangle tiles only (about half the bytes),
+    the oracle to 1e-10 over a partial last block, a leading range, both prefetch depths (bitwise
+    equal) and a batched launch beside a general-form handle (bitwise equal to separate calls).
+    D magnitudes 10^U(-8, 2) too.  One asymmetric entry keeps the general form."""
+    p = W.rho_problem(seed=41, nshell=240, nao=760, ngrid=64 * 30 + 20, nset=3, density=0.35)
+    ref = _oracle(p)
+    rs, rg = Resident(p, same=True), Resident(p)
+    assert rs.h.bytes < 0.75 * rg.h.bytes  # the symmetric layout was taken
+    got = rs.eval().reshape(p.nset, p.ngrid)
+    assert rel_fro(got, ref) <= 1e-10
+    assert rel_fro(rg.eval().reshape(p.nset, p.ngrid), ref) <= 1e-10
+    part = rs.eval(ngrid=64 * 7, nset=2).reshape(p.nset, p.ngrid)
+    assert rel_fro(part[:2, :448], ref[:2, :448]) <= 1e-10 and np.all(part[:, 448:] == 0.0) and np.all(part[2] == 0.0)
+    old = cfgmm.tune_get(cfgmm.TUNE_RESIDENT_DEPTH)
+    try:
+        cfgmm.tune_set(cfgmm.TUNE_RESIDENT_DEPTH, 2)
+        assert np.array_equal(rs.eval().reshape(p.nset, p.ngrid), got)
+    finally:
+        cfgmm.tune_set(cfgmm.TUNE_RESIDENT_DEPTH, old)
+    want = [rs.eval(), rg.eval()]
+    bufs = [cfgmm.DeviceBuffer(8 * p.nset * p.ngrid) for _ in range(2)]
+    for b in bufs:
+        b.zero()
+    s = cfgmm.Stream()
+    cfgmm.rho_resident_eval_batched(s, [(rs.h, bufs[0], p.ngrid, p.ngrid, p.nset), (rg.h, bufs[1], p.ngrid, p.ngrid,
+                                                                                    p.nset)])
+    s.synchronize()
+    for b, w in zip(bufs, want):
+        assert np.array_equal(b.to_array(np.float64, w.shape), w)
+        b.free()
+    s.destroy()
+    general_bytes = rg.h.bytes
+    rs.close()
+    rg.close()
+    # one asymmetric entry inside the mask: the general form, still right
+    lst = np.flatnonzero(p.tab[:, 0])
+    bad = p.dms.copy()
+    bad[1, lst[2], lst[min(len(lst) - 1, 90)]] += 5.0
+    rb = Resident(p, same=True, dms=bad)
+    assert rb.h.bytes == general_bytes
+    assert rel_fro(rb.eval().reshape(p.nset, p.ngrid), _oracle(p, dms=bad)) <= 1e-10
+    rb.close()
+    # wide-range D
+    pw = W.rho_problem(seed=42, nshell=200, nao=640, ngrid=2048, nset=2, density=0.3, dm_logspan=(-8, 2))
+    for ns in (6, 5):
+        rw = Resident(pw, ns=ns, same=True)
+        assert rel_fro(rw.eval().reshape(pw.nset, pw.ngrid), _oracle(pw)) <= 1e-10
+        rw.close()
