@@ -146,6 +146,13 @@ int workspace_get(hipStream_t stream, size_t bytes, void **out) {
     return CFGMM_OK;
 }
 
+unsigned workspace_epoch(hipStream_t stream) {
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    std::lock_guard<std::mutex> lock(g_ws_mutex);
+    return ++g_ws[std::make_pair(dev, stream)].epoch;
+}
+
 int host_staging_get(hipStream_t stream, size_t bytes, void **out) {
     int dev = 0;
     CFGMM_HIP_TRY(hipGetDevice(&dev));

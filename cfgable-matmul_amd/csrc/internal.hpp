@@ -36,8 +36,12 @@ struct Workspace {
     size_t bytes = 0;
     void *host = nullptr;  // pinned host staging for tables copied to the device
     size_t host_bytes = 0;
+    unsigned epoch = 0;
 };
 int workspace_get(hipStream_t stream, size_t bytes, void **out);
+// A new value per call on the stream (wraps after 2^32): a flag in the workspace written with
+// the current value during a call was written by that call.
+unsigned workspace_epoch(hipStream_t stream);
 // Pinned host memory of the stream, for host tables that a call copies to the device.  The
 // caller fills it only after every copy that read it has completed (a call that uses it
 // synchronises the stream before returning).
@@ -58,13 +62,16 @@ int device_cu_count();  // compute units of the current device (cached per devic
 __host__ __device__ inline size_t div_up(size_t a, size_t b) { return (a + b - 1) / b; }
 
 // ---- kernel launchers (one per .hip file) -----------------------------------
-// Compacts, for each column block b < nblk, the rows x < nrows with
-// tab[x*ldtab + b] != 0 into lists[b*nrows + *] (ascending) and counts[b].
 // *flag = 0 unless every D_s (s < nset) equals its transpose bitwise on [0, nao)^2; the caller
 // sets *flag beforehand (nothing is written when all are symmetric)
 int launch_dm_symmetry(hipStream_t stream, const double *dms, size_t stride_ao, size_t nao, size_t nset, int *flag);
+// Compacts, for each column block b < nblk, the rows x < nrows with
+// tab[x*ldtab + b] != 0 into lists[b*nrows + *] (ascending) and counts[b].  With keys, also
+// keys[b*maxchunks + c] = the plan bucket (of kPlanBuckets over nrows) of the midpoint of the
+// c-th 64-entry chunk of the list, -1 past its end.
 int launch_build_lists(hipStream_t stream, const uint8_t *tab, size_t ldtab, size_t nrows, size_t nblk, int *lists,
-                       int *counts);
+                       int *counts, int *keys = nullptr, size_t maxchunks = 0);
+constexpr int kPlanBuckets = 1024;
 // rho[s*stride_grid + g] += sum over the 64-row chunks of g's block of partial[chunk][s][g]
 // (partial laid out [chunk][nset][ngrid]; chunk order fixed).
 int launch_rho_reduce(hipStream_t stream, double *rhos, const double *partial, const int *counts, size_t stride_grid,

@@ -27,10 +27,11 @@
 //   4. rho_reduce_kernel: rho[s][g] += sum over the chunks of g's block, in
 //      increasing chunk order -- deterministic, no float atomics.
 //
-// When lhs and rhs are one array (the density of one set of orbitals), dm_symmetry_kernel tests
-// every D_s for D_s == D_s^T, bitwise, before step 3; if all are, each chunk contracts only the
-// list positions from its own rows on, those past its rows twice: half the gathers and MFMAs.
-// The flag stays on the device (no host synchronisation); the chunk kernel reads it.
+// When lhs and rhs are one array (the density of one set of orbitals), the other workgroups of
+// the plan launch test every D_s for D_s == D_s^T, bitwise, beside the plan; if all are, each
+// chunk contracts only the list positions from its own rows on, those past its rows twice: half
+// the gathers and MFMAs.  The flag stays on the device (no host synchronisation); the chunk
+// kernel reads it.
 #include "internal.hpp"
 #include "rho_body.hpp"
 
@@ -38,7 +39,8 @@ namespace cfgmm {
 
 // Per grid block b: lists[b*nao + i] = i-th x with tab[x*ldtab + b] != 0, counts[b] = |L_b|.
 __global__ __launch_bounds__(256) void build_ao_lists(const uint8_t *__restrict__ tab, long ldtab, int nao, int nblk,
-                                                      int *__restrict__ lists, int *__restrict__ counts) {
+                                                      int *__restrict__ lists, int *__restrict__ counts,
+                                                      int *__restrict__ keys, int maxchunks) {
     const int b = blockIdx.x;
     if (b >= nblk) return;
     __shared__ int wave_cnt[256 / 64];
@@ -60,13 +62,25 @@ __global__ __launch_bounds__(256) void build_ao_lists(const uint8_t *__restrict_
         __syncthreads();
     }
     if (threadIdx.x == 0) counts[b] = base;
+    if (keys) {
+        __syncthreads();  // the list entries written by this workgroup
+        const int *__restrict__ lb = lists + (long)b * nao;
+        for (int c = threadIdx.x; c < maxchunks; c += 256) {
+            int k = -1;
+            if (c * 64 < base) {
+                const long mid = ((long)lb[c * 64] + lb[min(c * 64 + 64, base) - 1]) / 2;
+                k = (int)(mid * kPlanBuckets / nao);
+            }
+            keys[(long)b * maxchunks + c] = k;
+        }
+    }
 }
 
 int launch_build_lists(hipStream_t stream, const uint8_t *tab, size_t ldtab, size_t nrows, size_t nblk, int *lists,
-                       int *counts) {
+                       int *counts, int *keys, size_t maxchunks) {
     hipEvent_t t0 = trace_start(stream);
     hipLaunchKernelGGL(build_ao_lists, dim3((unsigned)nblk), dim3(256), 0, stream, tab, (long)ldtab, (int)nrows,
-                       (int)nblk, lists, counts);
+                       (int)nblk, lists, counts, keys, (int)maxchunks);
     trace_stop(stream, "build_ao_lists", t0);
     return hip_status(hipGetLastError(), "build_ao_lists launch");
 }
@@ -101,41 +115,46 @@ using rho::kThreads;
 using rho::kUC;
 
 constexpr int kPlanThreads = 1024;
-constexpr int kPlanBuckets = 1024;
 
 // Orders the (block, chunk) tasks of one evaluation by the AO range their rows cover (the
-// bucketed midpoint of the chunk's first and last row): tasks that run together on an XCD then
-// share rows of D in its L2, where block-major order reads each chunk's D rows from memory
-// (the columns of a chunk are a third of each row, spread over all its cache lines).
-// perm[0 .. *ntask) = slot b*maxchunks + c of the non-empty chunks, bucket-ordered; the order
-// inside a bucket follows atomics and varies from call to call, which the result does not
+// bucketed midpoint of the chunk's first and last row, keys from build_ao_lists): tasks that run
+// together on an XCD then share rows of D in its L2, where block-major order reads each chunk's D
+// rows from memory (the columns of a chunk are a third of each row, spread over all its cache
+// lines).  perm[0 .. *ntask) = slot b*maxchunks + c of the non-empty chunks, bucket-ordered; the
+// order inside a bucket follows atomics and varies from call to call, which the result does not
 // depend on (each task writes its own partial slot, reduced in fixed chunk order).
-// bounds[0..8] (general form) and bounds[9..17] (symmetric form): the sequence cut into eight
-// ranges of equal work, one per XCD; a task's work is rows x (contracted positions + 64).  The
-// general form's work is even enough along the sequence that equal counts serve it better.
-__global__ __launch_bounds__(kPlanThreads) void rho_plan_kernel(const int *__restrict__ lists,
-                                                                const int *__restrict__ counts, int nao, int nblk,
-                                                                int maxchunks, int *__restrict__ perm,
-                                                                int *__restrict__ ntask, int *__restrict__ sym,
-                                                                int *__restrict__ bounds) {
-    __shared__ int hist[kPlanBuckets];
-    __shared__ long wsum[2][kPlanThreads];
-    __shared__ int s_ntask;
+// bounds[0..8]: the sequence cut into eight ranges of equal work in the symmetric form, one per
+// XCD; a task's work is rows x (positions from its first row on + 64) (pwork[pos]).  The general
+// form's work is even along the sequence: it takes equal counts.
+// Workgroup 0 plans.  With dms, workgroups 1.. compare the 64x64 tile pairs (ti <= tj) of every
+// D_s with their transposes and write `epoch` to *symflag at a difference: after this kernel,
+// *symflag != epoch means every D_s is symmetric.
+__global__ __launch_bounds__(kPlanThreads) void rho_plan_kernel(
+    const int *__restrict__ keys, const int *__restrict__ counts, int nblk, int maxchunks, int *__restrict__ perm,
+    float *__restrict__ pwork, int *__restrict__ ntask, int *__restrict__ bounds, const double *__restrict__ dms,
+    long stride_ao, int nao, int npairs, int *__restrict__ symflag, unsigned epoch) {
     const int tid = threadIdx.x;
-    if (tid == 0) *sym = 1;  // cleared by dm_symmetry_kernel, which runs after this kernel
+    if (blockIdx.x > 0) {
+        __shared__ unsigned long long t[64][65];
+        const int w = blockIdx.x - 1, s = w / npairs;
+        const long k = w % npairs;
+        // upper-triangle index k = tj (tj + 1) / 2 + ti, 0 <= ti <= tj
+        long tj = (long)((sqrt(8.0 * (double)k + 1.0) - 1.0) / 2.0);
+        while (tj * (tj + 1) / 2 > k) --tj;
+        while ((tj + 1) * (tj + 2) / 2 <= k) ++tj;
+        const int ti = (int)(k - tj * (tj + 1) / 2);
+        if (rho::tile_pair_differs(dms + (long)s * stride_ao * stride_ao, stride_ao, nao, ti, (int)tj, t) && tid == 0)
+            __hip_atomic_store(symflag, (int)epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return;
+    }
+    __shared__ int hist[kPlanBuckets];
+    __shared__ double wsum[kPlanThreads];
+    __shared__ int s_ntask;
     const long nslot = (long)nblk * maxchunks;
-    auto key = [&](long sl) -> int {  // bucket of slot sl, or -1 for an empty chunk
-        const int b = (int)(sl / maxchunks), c = (int)(sl % maxchunks);
-        const int n = counts[b];
-        if (c * kUC >= n) return -1;
-        const int *__restrict__ lb = lists + (long)b * nao;
-        const long mid = ((long)lb[c * kUC] + lb[min(c * kUC + kUC, n) - 1]) / 2;
-        return (int)(mid * kPlanBuckets / nao);
-    };
     for (int i = tid; i < kPlanBuckets; i += kPlanThreads) hist[i] = 0;
     __syncthreads();
     for (long sl = tid; sl < nslot; sl += kPlanThreads) {
-        const int k = key(sl);
+        const int k = keys[sl];
         if (k >= 0) atomicAdd(&hist[k], 1);
     }
     __syncthreads();
@@ -158,59 +177,46 @@ __global__ __launch_bounds__(kPlanThreads) void rho_plan_kernel(const int *__res
     }
     __syncthreads();
     for (long sl = tid; sl < nslot; sl += kPlanThreads) {
-        const int k = key(sl);
-        if (k >= 0) perm[atomicAdd(&hist[k], 1)] = (int)sl;
+        const int k = keys[sl];
+        if (k < 0) continue;
+        const int pos = atomicAdd(&hist[k], 1);
+        perm[pos] = (int)sl;
+        const int b = (int)(sl / maxchunks), c = (int)(sl % maxchunks), cnt = counts[b];
+        pwork[pos] = (float)min(kUC, cnt - c * kUC) * (float)(cnt - c * kUC + kUC);
     }
-    __syncthreads();  // perm complete (the global writes of this workgroup are visible to it)
+    __syncthreads();  // pwork complete (the global writes of this workgroup are visible to it)
 
     // equal-work XCD ranges: each thread takes a contiguous piece of the sequence
     const int n = s_ntask;
     const int per = (n + kPlanThreads - 1) / kPlanThreads;
     const int p0 = min(n, tid * per), p1 = min(n, p0 + per);
-    auto work = [&](int pos, int form) -> long {
-        const int sl = perm[pos];
-        const int b = sl / maxchunks, c = sl % maxchunks, cnt = counts[b];
-        const long rows = min(kUC, cnt - c * kUC);
-        return rows * ((form ? cnt - c * kUC : cnt) + kUC);
-    };
-    long mine2[2] = {0, 0};
-    for (int pos = p0; pos < p1; ++pos) {
-        mine2[0] += work(pos, 0);
-        mine2[1] += work(pos, 1);
-    }
-    // inclusive scan of the per-thread sums (Hillis-Steele)
-    long inc2[2] = {mine2[0], mine2[1]};
-    wsum[0][tid] = inc2[0];
-    wsum[1][tid] = inc2[1];
+    double mine2 = 0.0;
+    for (int pos = p0; pos < p1; ++pos) mine2 += pwork[pos];
+    double inc2 = mine2;  // inclusive scan of the per-thread sums (Hillis-Steele)
+    wsum[tid] = inc2;
     for (int d = 1; d < kPlanThreads; d <<= 1) {
         __syncthreads();
-        const long a0 = tid >= d ? wsum[0][tid - d] : 0, a1 = tid >= d ? wsum[1][tid - d] : 0;
+        const double a0 = tid >= d ? wsum[tid - d] : 0.0;
         __syncthreads();
-        inc2[0] += a0;
-        inc2[1] += a1;
-        wsum[0][tid] = inc2[0];
-        wsum[1][tid] = inc2[1];
+        inc2 += a0;
+        wsum[tid] = inc2;
     }
     __syncthreads();
-    const long tot[2] = {wsum[0][kPlanThreads - 1], wsum[1][kPlanThreads - 1]};
-    if (tid < 2) {
-        bounds[9 * tid] = 0;
-        for (int x = 1; x <= 8; ++x) bounds[9 * tid + x] = n;
+    const double tot = wsum[kPlanThreads - 1];
+    if (tid == 0) {
+        bounds[0] = 0;
+        for (int x = 1; x <= 8; ++x) bounds[x] = n;
     }
     __syncthreads();
+    if (tot <= 0.0) return;
     // position pos goes to XCD floor(8 * (work before pos) / total); bounds[x] = first position of XCD >= x
-#pragma unroll
-    for (int form = 0; form < 2; ++form) {
-        if (tot[form] == 0) continue;
-        long before = inc2[form] - mine2[form];
-        // the XCD of the position before this thread's piece (position 0 starts XCD 0)
-        int prev = p0 > 0 && p0 < p1 ? (int)min(7L, (before - work(p0 - 1, form)) * 8 / tot[form]) : 0;
-        for (int pos = p0; pos < p1; ++pos) {
-            const int xcd = (int)min(7L, before * 8 / tot[form]);
-            for (int x = prev + 1; x <= xcd; ++x) bounds[9 * form + x] = pos;
-            prev = xcd;
-            before += work(pos, form);
-        }
+    double before = inc2 - mine2;
+    int prev = p0 > 0 && p0 < p1 ? (int)min(7.0, floor((before - pwork[p0 - 1]) * 8.0 / tot)) : 0;
+    for (int pos = p0; pos < p1; ++pos) {
+        const int xcd = (int)min(7.0, floor(before * 8.0 / tot));
+        for (int x = prev + 1; x <= xcd; ++x) bounds[x] = pos;
+        prev = xcd;
+        before += pwork[pos];
     }
 }
 
@@ -233,9 +239,9 @@ __global__ __launch_bounds__(kThreads, 2) void rho_chunk_kernel(
     const double *__restrict__ ao_rhs, long stride_ao, long stride_grid, int nao, int ngrid, int nset, int set0,
     const int *__restrict__ lists, const int *__restrict__ counts, int blksize, int nsub, int maxchunks,
     int ngroups, int ncomp, long lhs_comp_stride, long part_comp_stride, const int *__restrict__ perm,
-    const int *__restrict__ ntask, const int *__restrict__ sym, const int *__restrict__ bounds) {
+    const int *__restrict__ ntask, const int *__restrict__ sym, unsigned epoch, const int *__restrict__ bounds) {
     __shared__ __attribute__((aligned(16))) unsigned char smem[rho::chunk_lds_bytes<NS, NT>()];
-    const int symmetric = sym ? *sym : 0;
+    const bool symmetric = sym && *sym != (int)epoch;
     bool first = true;
     // one (block, chunk, group of NS density matrices, subtile) task
     auto run = [&](int blk, int chunk, int grp, int sub) {
@@ -270,7 +276,7 @@ __global__ __launch_bounds__(kThreads, 2) void rho_chunk_kernel(
         // (workgroup i runs on XCD i % 8) takes the planned positions [bounds[x], bounds[x+1]) --
         // equal work per XCD -- for every group and subtile, group-major; its workgroups stride
         // over them (several tasks each where short tasks are many)
-        const int b0 = bounds[9 + x], npos = bounds[9 + x + 1] - b0;
+        const int b0 = bounds[x], npos = bounds[x + 1] - b0;
         const long per = (long)npos * nsub, items = per * ngroups;
         for (long kk = k; kk < items; kk += gridDim.x / 8) {
             const int grp = (int)(kk / per);
@@ -305,12 +311,13 @@ template <int NS>
 int launch_ns(int nt, dim3 grid, hipStream_t stream, double *partial, const double *dms, const double *ao_lhs,
               const double *ao_rhs, long stride_ao, long stride_grid, int nao, int ngrid, int nset, int set0,
               const int *lists, const int *counts, int blksize, int nsub, int maxchunks, int ngroups, int ncomp,
-              long lhs_cs, long part_cs, const int *perm, const int *ntask, const int *sym, const int *bounds) {
+              long lhs_cs, long part_cs, const int *perm, const int *ntask, const int *sym, unsigned epoch,
+              const int *bounds) {
     hipEvent_t t0 = trace_start(stream);
 #define CFGMM_RHO_LAUNCH(NT_)                                                                                      \
     hipLaunchKernelGGL((rho_chunk_kernel<NS, NT_>), grid, dim3(kThreads), 0, stream, partial, dms, ao_lhs, ao_rhs, \
                        stride_ao, stride_grid, nao, ngrid, nset, set0, lists, counts, blksize, nsub, maxchunks, \
-                       ngroups, ncomp, lhs_cs, part_cs, perm, ntask, sym, bounds)
+                       ngroups, ncomp, lhs_cs, part_cs, perm, ntask, sym, epoch, bounds)
     switch (nt) {
         case 1: CFGMM_RHO_LAUNCH(1); break;
         case 2: CFGMM_RHO_LAUNCH(2); break;
@@ -364,12 +371,11 @@ int launch_rho_multi(hipStream_t stream, double *rhos, size_t rho_cs, const doub
     const int nsub = (int)(blksize / (16 * nt));
 
     // workspace: [lists nblk*nao int][counts nblk int][perm nblk*maxchunks int][ntask int][sym int]
-    //            [bounds 18 int][pad]
+    //            [bounds 9 int][keys nblk*maxchunks int][pwork nblk*maxchunks float][pad]
     //            [partial ncomp*maxchunks*nset*ngrid double]
-    const size_t lists_bytes = sizeof(int) * nblk * nao;
-    const size_t counts_bytes = sizeof(int) * nblk;
-    const size_t perm_bytes = sizeof(int) * (nblk * maxchunks + 2 + 18);
-    const size_t part_off = div_up(lists_bytes + counts_bytes + perm_bytes, 256) * 256;
+    const size_t nslot = nblk * maxchunks;
+    const size_t head_bytes = sizeof(int) * (nblk * nao + nblk + nslot + 2 + 9 + nslot) + sizeof(float) * nslot;
+    const size_t part_off = div_up(head_bytes, 256) * 256;
     const size_t part_cs = maxchunks * nset * ngrid;  // doubles per component
     const size_t part_bytes = sizeof(double) * part_cs * ncomp;
     void *ws = nullptr;
@@ -378,27 +384,29 @@ int launch_rho_multi(hipStream_t stream, double *rhos, size_t rho_cs, const doub
     int *lists = static_cast<int *>(ws);
     int *counts = lists + nblk * nao;
     int *perm = counts + nblk;
-    int *ntask = perm + nblk * maxchunks;
+    int *ntask = perm + nslot;
     int *sym = ntask + 1;
     int *bounds = sym + 1;
+    int *keys = bounds + 9;
+    float *pwork = reinterpret_cast<float *>(keys + nslot);
     double *partial = reinterpret_cast<double *>(static_cast<char *>(ws) + part_off);
 
-    st = launch_build_lists(stream, tab, ldtab, nao, nblk, lists, counts);
-    if (st != CFGMM_OK) return st;
     const bool ordered = tune_get(CFGMM_TUNE_RHO_ORDER) == 0;
+    st = launch_build_lists(stream, tab, ldtab, nao, nblk, lists, counts, ordered ? keys : nullptr, maxchunks);
+    if (st != CFGMM_OK) return st;
+    // the symmetric form: one set of orbitals on both sides and every D_s symmetric, tested on the
+    // device by the plan launch's other workgroups (a flag against this call's epoch: no clearing)
+    const size_t nt64 = div_up(nao, 64), npairs = nt64 * (nt64 + 1) / 2;
+    const bool try_sym = ordered && ncomp == 1 && ao_lhs == ao_rhs && tune_get(CFGMM_TUNE_RHO_SYMMETRIC) == 0 &&
+                         npairs * nset + 1 <= (size_t)INT32_MAX;
+    const unsigned epoch = try_sym ? workspace_epoch(stream) : 0u;
     if (ordered) {
         hipEvent_t t0 = trace_start(stream);
-        hipLaunchKernelGGL(rho_plan_kernel, dim3(1), dim3(kPlanThreads), 0, stream, lists, counts, (int)nao,
-                           (int)nblk, (int)maxchunks, perm, ntask, sym, bounds);
+        hipLaunchKernelGGL(rho_plan_kernel, dim3((unsigned)(1 + (try_sym ? npairs * nset : 0))), dim3(kPlanThreads),
+                           0, stream, keys, counts, (int)nblk, (int)maxchunks, perm, pwork, ntask, bounds, dms,
+                           (long)stride_ao, (int)nao, (int)npairs, sym, epoch);
         trace_stop(stream, "rho_plan_kernel", t0);
         st = hip_status(hipGetLastError(), "rho_plan_kernel launch");
-        if (st != CFGMM_OK) return st;
-    }
-    // the symmetric form: one set of orbitals on both sides, every D_s tested on the device
-    const bool try_sym = ordered && ncomp == 1 && ao_lhs == ao_rhs && tune_get(CFGMM_TUNE_RHO_SYMMETRIC) == 0 &&
-                         div_up(nao, 64) <= 65535;
-    if (try_sym) {
-        st = launch_dm_symmetry(stream, dms, stride_ao, nao, nset, sym);
         if (st != CFGMM_OK) return st;
     }
     const int *symp = try_sym ? sym : nullptr;
@@ -415,10 +423,10 @@ int launch_rho_multi(hipStream_t stream, double *rhos, size_t rho_cs, const doub
         const dim3 grid((unsigned)(ordered ? div_up(slots, 8) * 8 : slots), 1, 1);
         const int *pp = ordered ? perm : nullptr;
         switch (ns) {
-            case 1: return launch_ns<1>(nt, grid, stream, partial, dms, ao_lhs, ao_rhs, sa, sg, (int)nao, (int)ngrid, (int)nset, (int)set0, lists, counts, (int)blksize, nsub, (int)maxchunks, (int)ngroups, (int)ncomp, (long)lhs_cs, (long)part_cs, pp, ntask, symp, bounds);
-            case 2: return launch_ns<2>(nt, grid, stream, partial, dms, ao_lhs, ao_rhs, sa, sg, (int)nao, (int)ngrid, (int)nset, (int)set0, lists, counts, (int)blksize, nsub, (int)maxchunks, (int)ngroups, (int)ncomp, (long)lhs_cs, (long)part_cs, pp, ntask, symp, bounds);
-            case 3: return launch_ns<3>(nt, grid, stream, partial, dms, ao_lhs, ao_rhs, sa, sg, (int)nao, (int)ngrid, (int)nset, (int)set0, lists, counts, (int)blksize, nsub, (int)maxchunks, (int)ngroups, (int)ncomp, (long)lhs_cs, (long)part_cs, pp, ntask, symp, bounds);
-            default: return launch_ns<4>(nt, grid, stream, partial, dms, ao_lhs, ao_rhs, sa, sg, (int)nao, (int)ngrid, (int)nset, (int)set0, lists, counts, (int)blksize, nsub, (int)maxchunks, (int)ngroups, (int)ncomp, (long)lhs_cs, (long)part_cs, pp, ntask, symp, bounds);
+            case 1: return launch_ns<1>(nt, grid, stream, partial, dms, ao_lhs, ao_rhs, sa, sg, (int)nao, (int)ngrid, (int)nset, (int)set0, lists, counts, (int)blksize, nsub, (int)maxchunks, (int)ngroups, (int)ncomp, (long)lhs_cs, (long)part_cs, pp, ntask, symp, epoch, bounds);
+            case 2: return launch_ns<2>(nt, grid, stream, partial, dms, ao_lhs, ao_rhs, sa, sg, (int)nao, (int)ngrid, (int)nset, (int)set0, lists, counts, (int)blksize, nsub, (int)maxchunks, (int)ngroups, (int)ncomp, (long)lhs_cs, (long)part_cs, pp, ntask, symp, epoch, bounds);
+            case 3: return launch_ns<3>(nt, grid, stream, partial, dms, ao_lhs, ao_rhs, sa, sg, (int)nao, (int)ngrid, (int)nset, (int)set0, lists, counts, (int)blksize, nsub, (int)maxchunks, (int)ngroups, (int)ncomp, (long)lhs_cs, (long)part_cs, pp, ntask, symp, epoch, bounds);
+            default: return launch_ns<4>(nt, grid, stream, partial, dms, ao_lhs, ao_rhs, sa, sg, (int)nao, (int)ngrid, (int)nset, (int)set0, lists, counts, (int)blksize, nsub, (int)maxchunks, (int)ngroups, (int)ncomp, (long)lhs_cs, (long)part_cs, pp, ntask, symp, epoch, bounds);
         }
     };
     st = launch_groups(NS, 0, full);

@@ -271,20 +271,20 @@ __device__ inline void rho_chunk_body(const RhoChunk &task, unsigned char *__res
 
 
 // Whether the 64x64 tiles (ti, tj) and (tj, ti) of the nao x nao matrix d (row stride ld) are
-// not each other's transpose, bitwise; the same answer on every thread of the workgroup (256
-// threads, all of which must call it).  t: LDS scratch of 64 x 65 words.
+// not each other's transpose, bitwise; the same answer on every thread of the workgroup (a
+// multiple of 64 threads up to 4096, all of which must call it).  t: LDS scratch of 64 x 65 words.
 __device__ inline bool tile_pair_differs(const double *__restrict__ d, long ld, int nao, int ti, int tj,
                                          unsigned long long (*t)[65]) {
-    const int c = threadIdx.x & 63, r0 = threadIdx.x >> 6;
+    const int c = threadIdx.x & 63, r0 = threadIdx.x >> 6, dr = blockDim.x >> 6;
     const int i0 = ti * 64, j0 = tj * 64;
     __syncthreads();  // an earlier call's readers of t are done
-    for (int r = r0; r < 64; r += 4) {
+    for (int r = r0; r < 64; r += dr) {
         const int i = i0 + r, j = j0 + c;
         t[r][c] = i < nao && j < nao ? (unsigned long long)__double_as_longlong(d[(long)i * ld + j]) : 0ull;
     }
     __syncthreads();
     bool differ = false;
-    for (int r = r0; r < 64; r += 4) {  // row j0 + r, column i0 + c of d against t[c][r]
+    for (int r = r0; r < 64; r += dr) {  // row j0 + r, column i0 + c of d against t[c][r]
         const int j = j0 + r, i = i0 + c;
         if (i < nao && j < nao) differ |= (unsigned long long)__double_as_longlong(d[(long)j * ld + i]) != t[c][r];
     }
