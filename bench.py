@@ -67,7 +67,7 @@ RESIDENT_SLICES = 5
 # Bytes moved past L2 by one rho_chunk_kernel launch on the default workload, from two rocprofv3
 # --pmc passes (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, factors from a 256 MiB device copy):
 # profiles/rho_chunk_kernel_fp64_nao2228_ngrid16384_nset3.txt.  Only valid for that workload.
-RHO_DEFAULT_TRAFFIC_BYTES = 2953547 * 1024 * 2.000 + 4167 * 1024 * 1.000
+RHO_DEFAULT_TRAFFIC_BYTES = 1377975 * 1024 * 2.000 + 4167 * 1024 * 1.000  # the symmetric form
 
 
 def log(msg: str) -> None:
@@ -748,8 +748,20 @@ def main():
     r = bench_rho(p, args.steps, args.warmup)
     evals_per_s = args.steps / (r["gpu_ms"] * 1e-3)
     ms_per_step = r["gpu_ms"] / args.steps
-    flops = p.flops
+    # one buffer on both sides and bitwise-symmetric D: the library takes the upper-triangle form,
+    # whose algorithmic flops are the ones the roofline counts
+    symmetric = all(np.array_equal(d, d.T) for d in p.dms)
+    flops = p.flops_symmetric if symmetric else p.flops
     kern_tflops = flops / (r["kernel_avg_ms"] * 1e-3) / 1e12
+    log("[bench] rho in the general form (CFGMM_TUNE_RHO_SYMMETRIC = 1), for reference")
+    cfgmm.tune_set(cfgmm.TUNE_RHO_SYMMETRIC, 1)
+    rg = bench_rho(p, max(20, args.steps // 4), 5)
+    cfgmm.tune_set(cfgmm.TUNE_RHO_SYMMETRIC, 0)
+    general = {"ms_per_step": round(rg["gpu_ms"] / max(20, args.steps // 4), 5),
+               "kernel_avg_ms": round(rg["kernel_avg_ms"], 5),
+               "kernel_tflops": round(p.flops / (rg["kernel_avg_ms"] * 1e-3) / 1e12, 3),
+               "masked_gflop_per_eval": round(p.flops / 1e9, 3),
+               "rel_diff_vs_symmetric": rel_fro(rg["rho_once"], r["rho_once"])}
 
     out = {
         "metric": METRIC,
@@ -771,8 +783,12 @@ def main():
                    "nao": p.nao, "ngrid": p.ngrid, "nset": p.nset, "blksize": p.blksize,
                    "mask_density": round(float(p.shell_tab.mean()), 4),
                    "skipped_tile_fraction": round(p.skipped_fraction(), 4),
-                   "masked_gflop_per_eval": round(flops / 1e9, 3),
+                   "masked_gflop_per_eval": round(p.flops / 1e9, 3),
+                   "form": "symmetric (upper triangle per block; D == D^T checked on the device)" if symmetric
+                           else "general",
+                   "form_gflop_per_eval": round(flops / 1e9, 3),
                    "wall_s_timed_region": round(r["wall_s"], 4)},
+        "general_form": general,
         "latency_ms": {"p50": round(float(np.percentile(r["per_step_ms"], 50)), 5),
                        "p99": round(float(np.percentile(r["per_step_ms"], 99)), 5)},
         "roofline": {"bound": "mfma", "kernel": "rho_chunk_kernel", "achieved": round(kern_tflops, 3),

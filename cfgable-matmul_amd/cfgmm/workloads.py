@@ -112,6 +112,20 @@ class RhoProblem:
         return float(self.nset * np.sum(npts * (2.0 * n_b * n_b + 2.0 * n_b)))
 
     @property
+    def flops_symmetric(self) -> float:
+        """Masked flops of the upper-triangle form (D symmetric, lhs == rhs): each 64-row chunk c of
+        a block's list takes the positions from its own first row on, sum_s sum_b npts_b *
+        sum_c (2 rows_c (n_b - 64 c) + 2 rows_c)."""
+        npts = np.full(self.nblk, float(self.blksize))
+        npts[-1] = self.ngrid - self.blksize * (self.nblk - 1)
+        tot = 0.0
+        for nb, npt in zip(self.counts().astype(np.int64), npts):
+            c = np.arange((nb + 63) // 64)
+            rows = np.minimum(64, nb - 64 * c).astype(np.float64)
+            tot += npt * float(np.sum(2.0 * rows * (nb - 64 * c) + 2.0 * rows))
+        return float(self.nset) * tot
+
+    @property
     def dense_flops(self) -> float:
         return float(self.nset) * self.ngrid * (2.0 * self.nao * self.nao + 2.0 * self.nao)
 
