@@ -67,7 +67,7 @@ RESIDENT_SLICES = 5
 # Bytes moved past L2 by one rho_chunk_kernel launch on the default workload, from two rocprofv3
 # --pmc passes (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, factors from a 256 MiB device copy):
 # profiles/rho_chunk_kernel_fp64_nao2228_ngrid16384_nset3.txt.  Only valid for that workload.
-RHO_DEFAULT_TRAFFIC_BYTES = 1377975 * 1024 * 2.000 + 4167 * 1024 * 1.000  # the symmetric form
+RHO_DEFAULT_TRAFFIC_BYTES = 1375554 * 1024 * 2.000 + 4167 * 1024 * 1.000  # the symmetric form
 
 
 def log(msg: str) -> None:
