@@ -12,6 +12,9 @@
 //      row operand in registers, the next tile's column operand loaded during the current
 //      tile's MFMAs); S_b lives in a workspace of sum_b kpad_b^2 doubles (kpad_b = |L_b|
 //      rounded up to 64).  The strips of one block run back to back on one XCD.
+//      With lhs and rhs one array S_b is symmetric: only the tiles (ti, tj >= ti) are computed,
+//      each off-diagonal one stored twice, as itself and transposed (the four stores of a lane
+//      group fill whole 128-byte lines of the transposed rows, merged in L2).
 //   2. vxc_reduce_kernel: each wave owns one row u of V over a 256-column tile and streams, for
 //      the blocks in increasing order where u is active, the contiguous segment of row
 //      pos_b(u) of S_b whose columns L_b[k] fall in the tile, adding it into an LDS copy of
@@ -79,7 +82,7 @@ __global__ __launch_bounds__(kThreads, 3) void vxc_block_kernel(
     const int4 *__restrict__ tasks, const long *__restrict__ s_off, const int *__restrict__ kpads,
     const int *__restrict__ lists, const int *__restrict__ counts, const double *__restrict__ lhs,
     const double *__restrict__ rhs, const double *__restrict__ w, long stride_grid, int nao, int ngrid, int blksize,
-    double *__restrict__ S) {
+    int sym, double *__restrict__ S) {
     // One 64-point x 64-row tile [point][row] (33 KB, so that three workgroups fit a CU); the
     // general path splits it into two 32-point tiles, A and B of a pass.
     __shared__ double smem[kKC * (kT + 1)];
@@ -153,6 +156,11 @@ __global__ __launch_bounds__(kThreads, 3) void vxc_block_kernel(
 #pragma unroll
             for (int r = 0; r < 4; ++r) soff[a][r] = (unsigned)(((wr + a * 16 + lk + 4 * r) * kpad + wc + lrow) * 8);
         char *const srow = (char *)(Sb + (long)ti * kT * kpad);
+        // symmetric form: per-lane offsets (bytes) of the transposed element, (j, i) for the (i, j)
+        // a lane holds, from the transposed tile's corner; + a*128 + r*32 per element
+        unsigned moff[2];
+#pragma unroll
+        for (int c = 0; c < 2; ++c) moff[c] = (unsigned)(((wc + c * 16 + lrow) * kpad + wr + lk) * 8);
         // kq = 4-point steps to run; the full-block call is inlined with kq = 16 (no guards)
         auto mfma_tile = [&](int kq, f64x4 (&acc)[2][2]) {
 #pragma unroll
@@ -182,6 +190,15 @@ __global__ __launch_bounds__(kThreads, 3) void vxc_block_kernel(
                 for (int c = 0; c < 2; ++c)
 #pragma unroll
                     for (int r = 0; r < 4; ++r) *(double *)(st + soff[a][r] + c * 128) = acc[a][c][r];
+            if (sym && tj0 + jt > ti) {  // the transposed tile (tj, ti)
+                char *const mt = (char *)(Sb + (long)(tj0 + jt) * kT * kpad + (long)ti * kT);
+#pragma unroll
+                for (int a = 0; a < 2; ++a)
+#pragma unroll
+                    for (int c = 0; c < 2; ++c)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) *(double *)(mt + moff[c] + a * 128 + r * 32) = acc[a][c][r];
+            }
             if (jt + 1 < ntj) {
                 __syncthreads();  // every wave has read tile jt
                 put16(rg, rows, false);
@@ -243,6 +260,7 @@ __global__ __launch_bounds__(kThreads, 3) void vxc_block_kernel(
                         const int i = ti * kT + wr + a * 16 + lk + 4 * r;  // D[l/16 + 4r][l%16]
                         const int j = tj * kT + wc + c * 16 + lrow;
                         Sb[(long)i * kpad + j] = acc[a][c][r];  // i, j < kpad: the padding is written too
+                        if (sym && tj > ti) Sb[(long)j * kpad + i] = acc[a][c][r];
                     }
                     acc[a][c] = f64x4{0.0, 0.0, 0.0, 0.0};
                 }
@@ -352,7 +370,9 @@ int launch_vxc(hipStream_t stream, double *V, size_t ldv, const double *ao_lhs, 
     std::vector<int> hc(nblk);
     CFGMM_HIP_TRY(hipMemcpyAsync(hc.data(), counts, sizeof(int) * nblk, hipMemcpyDeviceToHost, stream));
     CFGMM_HIP_TRY(hipStreamSynchronize(stream));
-    // S_b offsets and the strip tasks, block-major (block, tile row, strip)
+    // S_b offsets and the strip tasks, block-major (block, tile row, strip); one array on both
+    // sides makes S_b symmetric
+    const int sym = ao_lhs == ao_rhs && tune_get(CFGMM_TUNE_RHO_SYMMETRIC) == 0 ? 1 : 0;
     std::vector<long> soff(nblk);
     std::vector<int> kp(nblk);
     std::vector<int4> tasks;
@@ -362,12 +382,15 @@ int launch_vxc(hipStream_t stream, double *V, size_t ldv, const double *ao_lhs, 
         soff[b] = (long)s_elems;
         s_elems += (size_t)kp[b] * kp[b];
         const int nt = kp[b] / kT;
-        const int ns = (nt + kStrip - 1) / kStrip;  // strips per tile row, of near-equal length
-        for (int i = 0; i < nt; ++i)
+        for (int i = 0; i < nt; ++i) {
+            // tiles j0 .. nt-1 of row i (the symmetric form: j0 = i), strips of near-equal length
+            const int first = sym ? i : 0, len = nt - first;
+            const int ns = (len + kStrip - 1) / kStrip;
             for (int q = 0; q < ns; ++q) {
-                const int j0 = q * nt / ns, j1 = (q + 1) * nt / ns;
+                const int j0 = first + q * len / ns, j1 = first + (q + 1) * len / ns;
                 tasks.push_back(int4{(int)b, i, j0, j1 - j0});
             }
+        }
     }
     if (tasks.size() > (size_t)INT32_MAX) return fail(CFGMM_EINVAL, "vxc: too many tiles");
     const size_t o_soff = head;
@@ -403,7 +426,7 @@ int launch_vxc(hipStream_t stream, double *V, size_t ldv, const double *ao_lhs, 
     if (!tasks.empty())
         hipLaunchKernelGGL(vxc_block_kernel, dim3((unsigned)tasks.size()), dim3(kThreads), 0, stream, d_tasks, d_soff,
                            d_kp, lists, counts, ao_lhs, ao_rhs, w, (long)stride_grid, (int)nao, (int)ngrid,
-                           (int)blksize, S);
+                           (int)blksize, sym, S);
     trace_stop(stream, "vxc_block_kernel", t0);
     st = hip_status(hipGetLastError(), "vxc_block_kernel launch");
     if (st != CFGMM_OK) return st;

@@ -349,7 +349,9 @@ enum cfgmm_tunable {
                                         (default), 1 = block-major */
     CFGMM_TUNE_RHO_SYMMETRIC = 9,    /* FP64 rho: 0 = when ao_lhs == ao_rhs, test every D_s for
                                         D_s == D_s^T on the device and, if so, contract only the
-                                        upper triangle of each block (default); 1 = never */
+                                        upper triangle of each block (default); 1 = never.
+                                        Also the Vxc contraction: with ao_lhs == ao_rhs only
+                                        the upper tiles of each block product are computed */
     CFGMM_TUNE_COUNT = 10
 };
 int cfgmm_tune_set(int param, int value);

@@ -26,20 +26,49 @@ def vxc_numpy(lhs, rhs, w, tab, nao, ngrid, blksize):
     return V
 
 
-def run(lhs, rhs, w, tab, nao, ngrid, blksize, ldv=None, V0=None):
+def run(lhs, rhs, w, tab, nao, ngrid, blksize, ldv=None, V0=None, same=False):
+    """same: lhs and rhs in one device buffer (the symmetric form: upper tiles of S_b only)."""
     ldv = nao if ldv is None else ldv
     stride = lhs.shape[1]
     V0 = np.zeros(nao * ldv) if V0 is None else V0
     s = cfgmm.Stream()
     bufs = [cfgmm.DeviceBuffer.from_array(np.ascontiguousarray(x)) for x in (V0, lhs, rhs, tab)]
+    if same:
+        bufs[2].free()
+        bufs[2] = bufs[1]
     d_w = None if w is None else cfgmm.DeviceBuffer.from_array(w)
     cfgmm.aogridao2mat_dev(s, bufs[0], ldv, bufs[1], bufs[2], d_w, stride, nao, ngrid, bufs[3], tab.shape[1], blksize)
     s.synchronize()
     out = bufs[0].to_array(np.float64, (nao, ldv))
-    for b in bufs + ([d_w] if d_w is not None else []):
+    for b in {id(b): b for b in bufs}.values():
         b.free()
+    if d_w is not None:
+        d_w.free()
     s.destroy()
     return out
+
+
+@pytest.mark.parametrize("blksize,ngrid", [(64, 64 * 30), (64, 1001), (96, 1000), (16, 1000)])
+def test_vxc_symmetric_form(blksize, ngrid):
+    """One buffer as lhs and rhs: only the upper tiles of each S_b are computed and mirrored.
+    numpy to 1e-12, the general form (separate buffers) to 1e-14, V0 accumulated into, the padding
+    of V untouched, bitwise repeatable; blocks of 64 points, a partial one (41 points), blocks of
+    96 and 16."""
+    p = W.rho_problem(seed=blksize + ngrid, nshell=230, nao=700, ngrid=ngrid, nset=1, blksize=blksize, density=0.4)
+    rng = np.random.default_rng(5)
+    w = rng.uniform(0.1, 2.0, p.ngrid)
+    tab = np.ascontiguousarray(p.tab)
+    ldv = p.nao + 3
+    V0 = rng.standard_normal(p.nao * ldv)
+    sym = run(p.phi, p.phi, w, tab, p.nao, p.ngrid, p.blksize, ldv=ldv, V0=V0.copy(), same=True)
+    gen = run(p.phi, p.phi, w, tab, p.nao, p.ngrid, p.blksize, ldv=ldv, V0=V0.copy())
+    ref = vxc_numpy(p.phi, p.phi, w, tab, p.nao, p.ngrid, p.blksize)
+    V0m = V0.reshape(p.nao, ldv)
+    assert rel_fro(sym[:, :p.nao] - V0m[:, :p.nao], ref) <= 1e-12
+    assert rel_fro(sym[:, :p.nao] - V0m[:, :p.nao], gen[:, :p.nao] - V0m[:, :p.nao]) <= 1e-14
+    assert np.array_equal(sym[:, p.nao:], V0m[:, p.nao:])
+    again = run(p.phi, p.phi, w, tab, p.nao, p.ngrid, p.blksize, ldv=ldv, V0=V0.copy(), same=True)
+    assert np.array_equal(sym, again)
 
 
 def test_vxc_generated_molecule():
