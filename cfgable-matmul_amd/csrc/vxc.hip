@@ -12,14 +12,16 @@
 //      row operand in registers, the next tile's column operand loaded during the current
 //      tile's MFMAs); S_b lives in a workspace of sum_b kpad_b^2 doubles (kpad_b = |L_b|
 //      rounded up to 64).  The strips of one block run back to back on one XCD.
-//      With lhs and rhs one array S_b is symmetric: only the tiles (ti, tj >= ti) are computed,
-//      each off-diagonal one stored twice, as itself and transposed (the four stores of a lane
-//      group fill whole 128-byte lines of the transposed rows, merged in L2).
+//      With lhs and rhs one array S_b and V's increment X are symmetric: only the tiles
+//      (ti, tj >= ti) of S_b are computed and stored.
 //   2. vxc_reduce_kernel: each wave owns one row u of V over a 256-column tile and streams, for
 //      the blocks in increasing order where u is active, the contiguous segment of row
 //      pos_b(u) of S_b whose columns L_b[k] fall in the tile, adding it into an LDS copy of
 //      the row (pos_b(x) = index of x in L_b).  Every element of S_b is read once, by full
-//      lanes; no float atomics, so the result does not depend on scheduling.
+//      lanes; no float atomics, so the result does not depend on scheduling.  Symmetric form:
+//      the row takes only columns v >= u (list positions from pos_b(u) on, all in stored tiles),
+//      adds them into V and writes those v > u into X; vxc_mirror_kernel then adds X^T into the
+//      lower triangle of V through LDS tiles.
 #include "internal.hpp"
 
 #include <vector>
@@ -82,7 +84,7 @@ __global__ __launch_bounds__(kThreads, 3) void vxc_block_kernel(
     const int4 *__restrict__ tasks, const long *__restrict__ s_off, const int *__restrict__ kpads,
     const int *__restrict__ lists, const int *__restrict__ counts, const double *__restrict__ lhs,
     const double *__restrict__ rhs, const double *__restrict__ w, long stride_grid, int nao, int ngrid, int blksize,
-    int sym, double *__restrict__ S) {
+    double *__restrict__ S) {
     // One 64-point x 64-row tile [point][row] (33 KB, so that three workgroups fit a CU); the
     // general path splits it into two 32-point tiles, A and B of a pass.
     __shared__ double smem[kKC * (kT + 1)];
@@ -156,11 +158,6 @@ __global__ __launch_bounds__(kThreads, 3) void vxc_block_kernel(
 #pragma unroll
             for (int r = 0; r < 4; ++r) soff[a][r] = (unsigned)(((wr + a * 16 + lk + 4 * r) * kpad + wc + lrow) * 8);
         char *const srow = (char *)(Sb + (long)ti * kT * kpad);
-        // symmetric form: per-lane offsets (bytes) of the transposed element, (j, i) for the (i, j)
-        // a lane holds, from the transposed tile's corner; + a*128 + r*32 per element
-        unsigned moff[2];
-#pragma unroll
-        for (int c = 0; c < 2; ++c) moff[c] = (unsigned)(((wc + c * 16 + lrow) * kpad + wr + lk) * 8);
         // kq = 4-point steps to run; the full-block call is inlined with kq = 16 (no guards)
         auto mfma_tile = [&](int kq, f64x4 (&acc)[2][2]) {
 #pragma unroll
@@ -190,15 +187,6 @@ __global__ __launch_bounds__(kThreads, 3) void vxc_block_kernel(
                 for (int c = 0; c < 2; ++c)
 #pragma unroll
                     for (int r = 0; r < 4; ++r) *(double *)(st + soff[a][r] + c * 128) = acc[a][c][r];
-            if (sym && tj0 + jt > ti) {  // the transposed tile (tj, ti)
-                char *const mt = (char *)(Sb + (long)(tj0 + jt) * kT * kpad + (long)ti * kT);
-#pragma unroll
-                for (int a = 0; a < 2; ++a)
-#pragma unroll
-                    for (int c = 0; c < 2; ++c)
-#pragma unroll
-                        for (int r = 0; r < 4; ++r) *(double *)(mt + moff[c] + a * 128 + r * 32) = acc[a][c][r];
-            }
             if (jt + 1 < ntj) {
                 __syncthreads();  // every wave has read tile jt
                 put16(rg, rows, false);
@@ -260,7 +248,6 @@ __global__ __launch_bounds__(kThreads, 3) void vxc_block_kernel(
                         const int i = ti * kT + wr + a * 16 + lk + 4 * r;  // D[l/16 + 4r][l%16]
                         const int j = tj * kT + wc + c * 16 + lrow;
                         Sb[(long)i * kpad + j] = acc[a][c][r];  // i, j < kpad: the padding is written too
-                        if (sym && tj > ti) Sb[(long)j * kpad + i] = acc[a][c][r];
                     }
                     acc[a][c] = f64x4{0.0, 0.0, 0.0, 0.0};
                 }
@@ -275,12 +262,14 @@ __global__ __launch_bounds__(kThreads) void vxc_reduce_kernel(const int *__restr
                                                                const long *__restrict__ s_off,
                                                                const int *__restrict__ kpads, int nao, int nblk,
                                                                int nct, const double *__restrict__ S,
-                                                               double *__restrict__ V, long ldv) {
+                                                               double *__restrict__ V, long ldv, int sym,
+                                                               double *__restrict__ X) {
     __shared__ double Vt[kRW][kRC];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int ct = blockIdx.y, v0 = ct * kRC, u = blockIdx.x * kRW + wave;
     double *__restrict__ row = Vt[wave];  // private to this wave: no barriers
     if (u >= nao) return;
+    if (sym && v0 + kRC <= u) return;  // below the diagonal: vxc_mirror_kernel
 #pragma unroll
     for (int c = 0; c < kSeg; ++c) row[c * 64 + lane] = 0.0;
     for (int c0 = 0; c0 < nblk; c0 += 64) {
@@ -289,7 +278,9 @@ __global__ __launch_bounds__(kThreads) void vxc_reduce_kernel(const int *__restr
         const bool ok = c0 + lane < nblk;
         const int bq = ok ? c0 + lane : 0;
         int iu = pos[(long)bq * nao + u];
-        const int klo = kr[(long)bq * (nct + 1) + ct], khi = kr[(long)bq * (nct + 1) + ct + 1];
+        const int khi = kr[(long)bq * (nct + 1) + ct + 1];
+        // symmetric form: columns v >= u only, the list positions from u's own on
+        const int klo = sym ? max(kr[(long)bq * (nct + 1) + ct], iu) : kr[(long)bq * (nct + 1) + ct];
         const long so = s_off[bq];
         const int kp = kpads[bq];
         iu = ok ? iu : -1;
@@ -343,7 +334,27 @@ __global__ __launch_bounds__(kThreads) void vxc_reduce_kernel(const int *__restr
 #pragma unroll
     for (int c = 0; c < kSeg; ++c) {
         const int v = v0 + c * 64 + lane;
-        if (v < nao) vrow[v] += row[c * 64 + lane];
+        if (v < nao && (!sym || v >= u)) vrow[v] += row[c * 64 + lane];
+        if (sym && v < nao && v > u) X[(long)u * nao + v] = row[c * 64 + lane];
+    }
+}
+
+// Symmetric form: V[v][u] += X[u][v] for u < v.  Workgroup (J, I), J <= I: the tile of rows
+// u in J, columns v in I of X through LDS into rows v in I, columns u in J of V.
+__global__ __launch_bounds__(kThreads) void vxc_mirror_kernel(const double *__restrict__ X, double *__restrict__ V,
+                                                              long ldv, int nao) {
+    const int J = blockIdx.x, I = blockIdx.y;
+    if (J > I) return;
+    __shared__ double t[kT][kT + 1];
+    const int c = threadIdx.x & 63, r0 = threadIdx.x >> 6;
+    for (int r = r0; r < kT; r += kThreads / 64) {
+        const int u = J * kT + r, v = I * kT + c;
+        t[r][c] = (u < nao && v < nao && v > u) ? X[(long)u * nao + v] : 0.0;
+    }
+    __syncthreads();
+    for (int r = r0; r < kT; r += kThreads / 64) {
+        const int v = I * kT + r, u = J * kT + c;
+        if (u < nao && v < nao && v > u) V[(long)v * ldv + u] += t[c][r];
     }
 }
 
@@ -397,8 +408,9 @@ int launch_vxc(hipStream_t stream, double *V, size_t ldv, const double *ao_lhs, 
     const size_t o_kp = o_soff + al(sizeof(long) * nblk);
     const size_t o_tasks = o_kp + al(sizeof(int) * nblk);
     const size_t o_S = o_tasks + al(sizeof(int4) * std::max<size_t>(tasks.size(), 1));
+    const size_t o_X = o_S + al(sizeof(double) * std::max<size_t>(s_elems, 1));  // symmetric form: X, nao^2
     void *const ws_lists = ws;
-    st = workspace_get(stream, o_S + sizeof(double) * std::max<size_t>(s_elems, 1), &ws);  // may move: re-derive
+    st = workspace_get(stream, o_X + (sym ? sizeof(double) * nao * nao : 0), &ws);  // may move: re-derive
     if (st != CFGMM_OK) return st;
     char *base = static_cast<char *>(ws);
     lists = reinterpret_cast<int *>(base);
@@ -409,6 +421,7 @@ int launch_vxc(hipStream_t stream, double *V, size_t ldv, const double *ao_lhs, 
     int *d_kp = reinterpret_cast<int *>(base + o_kp);
     int4 *d_tasks = reinterpret_cast<int4 *>(base + o_tasks);
     double *S = reinterpret_cast<double *>(base + o_S);
+    double *X = sym ? reinterpret_cast<double *>(base + o_X) : nullptr;
     if (ws != ws_lists) {  // a grown workspace no longer holds the lists: build them again
         st = launch_build_lists(stream, tab, ldtab, nao, nblk, lists, counts);
         if (st != CFGMM_OK) return st;
@@ -426,13 +439,17 @@ int launch_vxc(hipStream_t stream, double *V, size_t ldv, const double *ao_lhs, 
     if (!tasks.empty())
         hipLaunchKernelGGL(vxc_block_kernel, dim3((unsigned)tasks.size()), dim3(kThreads), 0, stream, d_tasks, d_soff,
                            d_kp, lists, counts, ao_lhs, ao_rhs, w, (long)stride_grid, (int)nao, (int)ngrid,
-                           (int)blksize, sym, S);
+                           (int)blksize, S);
     trace_stop(stream, "vxc_block_kernel", t0);
     st = hip_status(hipGetLastError(), "vxc_block_kernel launch");
     if (st != CFGMM_OK) return st;
     t0 = trace_start(stream);
     hipLaunchKernelGGL(vxc_reduce_kernel, dim3((unsigned)div_up(nao, kRW), (unsigned)nct), dim3(kThreads), 0, stream,
-                       pos, lists, kr, d_soff, d_kp, (int)nao, (int)nblk, (int)nct, S, V, (long)ldv);
+                       pos, lists, kr, d_soff, d_kp, (int)nao, (int)nblk, (int)nct, S, V, (long)ldv, sym, X);
+    if (sym) {
+        const unsigned nt64 = (unsigned)div_up(nao, kT);
+        hipLaunchKernelGGL(vxc_mirror_kernel, dim3(nt64, nt64), dim3(kThreads), 0, stream, X, V, (long)ldv, (int)nao);
+    }
     trace_stop(stream, "vxc_reduce_kernel", t0);
     st = hip_status(hipGetLastError(), "vxc_reduce_kernel launch");
     if (st != CFGMM_OK) return st;
