@@ -22,7 +22,9 @@
 #include "rho_body.hpp"
 
 #include <algorithm>
+#include <map>
 #include <numeric>
+#include <tuple>
 #include <vector>
 
 namespace cfgmm {
@@ -38,17 +40,26 @@ struct BatchProblem {
     long stride_ao, stride_grid, ldtab;
     int nao, ngrid, nset, blksize, nblk, nsub, nt;
     long lists_off, counts_off, partial_off;
-    int sym;  // 1: lhs is rhs; cleared by batch_symmetry unless every D_s is symmetric
+    int sym;  // index + 1 of the problem's (D, stride, nao, nset) in the symmetry table; 0: lhs is not rhs
 };
 
-// Workgroup (k, problem): the k-th pair (ti <= tj) of 64x64 tiles of the problem's D_s, every s;
-// clears the problem's sym flag at a difference.
-__global__ __launch_bounds__(256) void batch_symmetry(BatchProblem *__restrict__ probs) {
-    BatchProblem &p = probs[blockIdx.y];
+// One distinct (D, stride_ao, nao, nset) of the batch whose problems have lhs == rhs; flag starts
+// at 1 and batch_symmetry clears it unless every D_s is symmetric.
+struct SymCheck {
+    const double *dms;
+    long stride_ao;
+    int nao, nset;
+    int flag;
+};
+
+// Workgroup (k, entry): the k-th pair (ti <= tj) of 64x64 tiles of the entry's D_s, every s;
+// clears the entry's flag at a difference.
+__global__ __launch_bounds__(256) void batch_symmetry(SymCheck *__restrict__ checks) {
+    SymCheck &p = checks[blockIdx.y];
     // read once for the workgroup (another workgroup may clear it meanwhile): all threads leave
     // together or none does -- the comparison below has barriers
     __shared__ int go;
-    if (threadIdx.x == 0) go = __hip_atomic_load(&p.sym, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x == 0) go = __hip_atomic_load(&p.flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
     if (!go) return;
     const long k = blockIdx.x;
@@ -61,7 +72,7 @@ __global__ __launch_bounds__(256) void batch_symmetry(BatchProblem *__restrict__
     __shared__ unsigned long long t[64][65];
     for (int s = 0; s < p.nset; ++s) {
         if (rho::tile_pair_differs(p.dms + (long)s * p.stride_ao * p.stride_ao, p.stride_ao, p.nao, ti, (int)tj, t)) {
-            if (threadIdx.x == 0) __hip_atomic_store(&p.sym, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (threadIdx.x == 0) __hip_atomic_store(&p.flag, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             return;  // the answer is the workgroup's: every thread leaves together
         }
     }
@@ -228,7 +239,8 @@ __global__ __launch_bounds__(kThreads) void batch_rho_kernel(const BatchProblem 
                                                              const int4 *__restrict__ tasks,
                                                              const int *__restrict__ ntasks, int *__restrict__ head,
                                                              const int *__restrict__ lists,
-                                                             const int *__restrict__ counts, double *__restrict__ partial) {
+                                                             const int *__restrict__ counts, double *__restrict__ partial,
+                                                             const SymCheck *__restrict__ checks) {
     __shared__ __attribute__((aligned(16))) unsigned char smem[rho::chunk_lds_bytes<NSMAX, 4>()];
     __shared__ int cur;
     const int n = *ntasks;
@@ -271,7 +283,7 @@ __global__ __launch_bounds__(kThreads) void batch_rho_kernel(const BatchProblem 
         t.cnt = counts[p.counts_off + w.y];
         t.chunk = w.z & 0xffff;
         t.g0 = w.y * p.blksize + (w.z >> 16) * p.nt * 16;
-        const bool sym = p.sym != 0;
+        const bool sym = p.sym > 0 && checks[p.sym - 1].flag != 0;
         if (t.g0 >= p.ngrid) continue;  // subtile past the end of a partial last block (uniform)
         const int ns = min(NSMAX, p.nset - w.w);
         if (NSMAX >= 4 && ns == 4)
@@ -324,7 +336,8 @@ int launch_rho_batched(hipStream_t stream, const cfgmm_rho_problem *problems, si
     std::vector<BatchProblem> bp;
     bp.reserve(nprob);
     long lists_n = 0, counts_n = 0, partial_n = 0, tasks_max = 0, max_pairs = 0;
-    bool any_sym = false;
+    std::vector<SymCheck> checks;
+    std::map<std::tuple<const double *, size_t, size_t, size_t>, int> sym_index;
     int max_nblk = 0;
     long max_elems = 0;
     for (size_t oi : order) {
@@ -351,9 +364,17 @@ int launch_rho_batched(hipStream_t stream, const cfgmm_rho_problem *problems, si
         p.lists_off = lists_n;
         p.counts_off = counts_n;
         p.partial_off = partial_n;
-        p.sym = q.ao_lhs == q.ao_rhs && tune_get(CFGMM_TUNE_RHO_SYMMETRIC) == 0 ? 1 : 0;
-        any_sym |= p.sym != 0;
-        max_pairs = std::max(max_pairs, (long)div_up(q.nao, 64) * ((long)div_up(q.nao, 64) + 1) / 2);
+        if (q.ao_lhs == q.ao_rhs && tune_get(CFGMM_TUNE_RHO_SYMMETRIC) == 0) {
+            // problems sharing D (and its used extent) share one test
+            const auto key = std::make_tuple(q.dms, q.stride_ao, q.nao, q.nset);
+            auto it = sym_index.find(key);
+            if (it == sym_index.end()) {
+                it = sym_index.emplace(key, (int)checks.size()).first;
+                checks.push_back(SymCheck{q.dms, (long)q.stride_ao, (int)q.nao, (int)q.nset, 1});
+                max_pairs = std::max(max_pairs, (long)div_up(q.nao, 64) * ((long)div_up(q.nao, 64) + 1) / 2);
+            }
+            p.sym = it->second + 1;
+        }
         const long maxchunks = (long)div_up(q.nao, kUC);
         if (maxchunks > 0xffff || p.nsub > 0x7fff) return fail(CFGMM_EINVAL, "batched rho: problem too large");
         lists_n += (long)p.nblk * p.nao;
@@ -377,7 +398,8 @@ int launch_rho_batched(hipStream_t stream, const cfgmm_rho_problem *problems, si
     const size_t o_toff = o_cnum + al(sizeof(int) * np);
     const size_t o_ctr = o_toff + al(sizeof(int) * np);
     void *ws = nullptr;
-    int st = workspace_get(stream, o_ctr + 9 * sizeof(int), &ws);
+    const size_t o_chk = o_ctr + al(9 * sizeof(int));
+    int st = workspace_get(stream, o_chk + sizeof(SymCheck) * std::max<size_t>(checks.size(), 1), &ws);
     if (st != CFGMM_OK) return st;
     char *base = static_cast<char *>(ws);
     BatchProblem *d_probs = reinterpret_cast<BatchProblem *>(base);
@@ -391,6 +413,10 @@ int launch_rho_batched(hipStream_t stream, const cfgmm_rho_problem *problems, si
 
     // Pageable source: the copy is complete when hipMemcpyAsync returns, so `bp` may go out of scope.
     CFGMM_HIP_TRY(hipMemcpyAsync(d_probs, bp.data(), sizeof(BatchProblem) * np, hipMemcpyHostToDevice, stream));
+    SymCheck *d_checks = reinterpret_cast<SymCheck *>(base + o_chk);
+    if (!checks.empty())
+        CFGMM_HIP_TRY(hipMemcpyAsync(d_checks, checks.data(), sizeof(SymCheck) * checks.size(), hipMemcpyHostToDevice,
+                                     stream));
     hipEvent_t t0 = trace_start(stream);
     hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(64), 0, stream, ctr, 9);
     hipLaunchKernelGGL(batch_build_lists, dim3((unsigned)max_nblk, (unsigned)np), dim3(256), 0, stream, d_probs,
@@ -399,8 +425,9 @@ int launch_rho_batched(hipStream_t stream, const cfgmm_rho_problem *problems, si
     hipLaunchKernelGGL(batch_scan_offsets, dim3(1), dim3(1024), 0, stream, d_probs, np, nsmax, cnum, toff, ctr);
     hipLaunchKernelGGL(batch_fill_tasks, dim3((unsigned)np), dim3(kFillThreads), 0, stream, d_probs, lists, counts,
                        nsmax, cnum, toff, tasks);
-    if (any_sym)  // the symmetric form where lhs is rhs and every D_s is symmetric
-        hipLaunchKernelGGL(batch_symmetry, dim3((unsigned)max_pairs, (unsigned)np), dim3(256), 0, stream, d_probs);
+    if (!checks.empty())  // the symmetric form where lhs is rhs and every D_s is symmetric
+        hipLaunchKernelGGL(batch_symmetry, dim3((unsigned)max_pairs, (unsigned)checks.size()), dim3(256), 0, stream,
+                           d_checks);
     trace_stop(stream, "batch_prepare", t0);
     st = hip_status(hipGetLastError(), "batched rho prepare launch");
     if (st != CFGMM_OK) return st;
@@ -409,13 +436,13 @@ int launch_rho_batched(hipStream_t stream, const cfgmm_rho_problem *problems, si
     t0 = trace_start(stream);
     if (nsmax >= 4)
         hipLaunchKernelGGL((batch_rho_kernel<4>), dim3(grid), dim3(kThreads), 0, stream, d_probs, tasks, ctr, ctr + 1,
-                           lists, counts, partial);
+                           lists, counts, partial, d_checks);
     else if (nsmax == 2)
         hipLaunchKernelGGL((batch_rho_kernel<2>), dim3(grid), dim3(kThreads), 0, stream, d_probs, tasks, ctr, ctr + 1,
-                           lists, counts, partial);
+                           lists, counts, partial, d_checks);
     else
         hipLaunchKernelGGL((batch_rho_kernel<1>), dim3(grid), dim3(kThreads), 0, stream, d_probs, tasks, ctr, ctr + 1,
-                           lists, counts, partial);
+                           lists, counts, partial, d_checks);
     trace_stop(stream, "batch_rho_kernel", t0);
     st = hip_status(hipGetLastError(), "batch_rho_kernel launch");
     if (st != CFGMM_OK) return st;
