@@ -158,6 +158,7 @@ int host_staging_get(hipStream_t stream, size_t bytes, void **out) {
     CFGMM_HIP_TRY(hipGetDevice(&dev));
     std::lock_guard<std::mutex> lock(g_ws_mutex);
     Workspace &ws = g_ws[std::make_pair(dev, stream)];
+    if (ws.host_done) CFGMM_HIP_TRY(hipEventSynchronize(ws.host_done));  // earlier copies have read it
     if (ws.host_bytes < bytes) {
         if (ws.host) {
             CFGMM_HIP_TRY(hipStreamSynchronize(stream));
@@ -173,6 +174,15 @@ int host_staging_get(hipStream_t stream, size_t bytes, void **out) {
     }
     *out = ws.host;
     return CFGMM_OK;
+}
+
+int host_staging_release(hipStream_t stream) {
+    int dev = 0;
+    CFGMM_HIP_TRY(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(g_ws_mutex);
+    Workspace &ws = g_ws[std::make_pair(dev, stream)];
+    if (!ws.host_done) CFGMM_HIP_TRY(hipEventCreateWithFlags(&ws.host_done, hipEventDisableTiming));
+    return hip_status(hipEventRecord(ws.host_done, stream), "host staging: event record");
 }
 
 // ---------------------------------------------------------------------------
@@ -852,6 +862,7 @@ int cfgmm_stream_destroy(void *stream) {
                 (void)hipStreamSynchronize(static_cast<hipStream_t>(stream));
                 if (it->second.ptr) (void)hipFree(it->second.ptr);
                 if (it->second.host) (void)hipHostFree(it->second.host);
+                if (it->second.host_done) (void)hipEventDestroy(it->second.host_done);
                 g_ws.erase(it);
             }
         }

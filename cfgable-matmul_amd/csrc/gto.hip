@@ -302,6 +302,8 @@ int launch_gto(hipStream_t stream, double *phi, size_t stride_grid, uint8_t *tab
     auto *d_exps = reinterpret_cast<double *>(base + o_exps);
     auto *d_coefs = reinterpret_cast<double *>(base + o_coefs);
     CFGMM_HIP_TRY(hipMemcpyAsync(base, h, ntab ? bytes : sizeof(ShellGroup) * ngrp, hipMemcpyHostToDevice, stream));
+    st = host_staging_release(stream);  // the next call waits for this copy before refilling
+    if (st != CFGMM_OK) return st;
     hipEvent_t t0 = trace_start(stream);
     const dim3 grid((unsigned)nblk, (unsigned)ngrp);
     if (blksize <= 64)
@@ -311,9 +313,7 @@ int launch_gto(hipStream_t stream, double *phi, size_t stride_grid, uint8_t *tab
         hipLaunchKernelGGL(gto_eval_kernel<true>, grid, dim3(64), 0, stream, d_groups, d_exps, d_coefs, coords,
                            (int)ngrid, (int)blksize, cutoff, phi, (long)stride_grid, tab, (long)ldtab);
     trace_stop(stream, "gto_eval_kernel", t0);
-    st = hip_status(hipGetLastError(), "gto_eval_kernel launch");
-    if (st != CFGMM_OK) return st;
-    return hip_status(hipStreamSynchronize(stream), "gto: sync");  // the staging is free again on return
+    return hip_status(hipGetLastError(), "gto_eval_kernel launch");
 }
 
 }  // namespace cfgmm

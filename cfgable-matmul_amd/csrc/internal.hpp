@@ -36,16 +36,18 @@ struct Workspace {
     size_t bytes = 0;
     void *host = nullptr;  // pinned host staging for tables copied to the device
     size_t host_bytes = 0;
+    hipEvent_t host_done = nullptr;  // recorded after the last copy out of `host`
     unsigned epoch = 0;
 };
 int workspace_get(hipStream_t stream, size_t bytes, void **out);
 // A new value per call on the stream (wraps after 2^32): a flag in the workspace written with
 // the current value during a call was written by that call.
 unsigned workspace_epoch(hipStream_t stream);
-// Pinned host memory of the stream, for host tables that a call copies to the device.  The
-// caller fills it only after every copy that read it has completed (a call that uses it
-// synchronises the stream before returning).
+// Pinned host memory of the stream, for host tables that a call copies to the device.  It waits
+// until the copies enqueued before the last host_staging_release have read it; the caller fills
+// it, enqueues its copies, then calls host_staging_release.
 int host_staging_get(hipStream_t stream, size_t bytes, void **out);
+int host_staging_release(hipStream_t stream);
 
 // ---- tracing -----------------------------------------------------------------
 // trace_start returns nullptr unless tracing is on; trace_stop(nullptr) is a no-op.

@@ -202,7 +202,8 @@ int cfgmm_aogridao2mat_dev(void *stream, double *V, size_t ldv, const double *ao
  * d = coords[g] - R (coords: device, ngrid x 3).  The mask: tab[(ao_off+c)*ldtab + b] = 1 when
  * max over the points of block b and the components of the shell of |phi| > cutoff, else 0, and
  * then phi is zeroed on that block.  Rows of AOs owned by no shell, and columns g >= ngrid, are
- * left untouched.  Synchronises the stream before returning (the shell tables are staged). */
+ * left untouched.  Asynchronous on the stream: the host tables (shells, exps, coefs) are copied
+ * into pinned staging before the call returns, so the caller may reuse them at once. */
 typedef struct cfgmm_gto_shell {
     double center[3];
     int l, nprim, prim_off, ao_off;
