@@ -757,8 +757,10 @@ def main():
     kern_tflops = flops / (r["kernel_avg_ms"] * 1e-3) / 1e12
     log("[bench] rho in the general form (CFGMM_TUNE_RHO_SYMMETRIC = 1), for reference")
     cfgmm.tune_set(cfgmm.TUNE_RHO_SYMMETRIC, 1)
-    rg = bench_rho(p, max(20, args.steps // 4), 5)
-    cfgmm.tune_set(cfgmm.TUNE_RHO_SYMMETRIC, 0)
+    try:
+        rg = bench_rho(p, max(20, args.steps // 4), 5)
+    finally:
+        cfgmm.tune_set(cfgmm.TUNE_RHO_SYMMETRIC, 0)
     general = {"ms_per_step": round(rg["gpu_ms"] / max(20, args.steps // 4), 5),
                "kernel_avg_ms": round(rg["kernel_avg_ms"], 5),
                "kernel_tflops": round(p.flops / (rg["kernel_avg_ms"] * 1e-3) / 1e12, 3),

@@ -272,3 +272,22 @@ def test_rho_generated_workload_small():
     ref = W.rho_reference_dense(p)
     assert rel_fro(rhos.reshape(p.nset, p.ngrid), ref) <= 1e-13
     assert np.all(rhos >= -1e-12 * np.max(np.abs(rhos)))  # PSD D, lhs == rhs
+
+
+def test_symmetric_form_flop_count():
+    """The flop count bench.py prices the symmetric rho form at: brute force over each block's
+    64-row chunks (rows of chunk c times the list positions from its own first row on), equal to
+    the general count when every block fits one chunk, and within (1/2, 1] of it otherwise."""
+    p = W.rho_problem(seed=2, nshell=120, nao=400, ngrid=64 * 9 + 20, nset=2, density=0.4)
+    brute = 0.0
+    for b in range(p.nblk):
+        n = int(p.tab[:, b].sum())
+        npts = min(p.blksize, p.ngrid - b * p.blksize)
+        for c in range(0, n, 64):
+            rows = min(64, n - c)
+            brute += npts * (2.0 * rows * (n - c) + 2.0 * rows)
+    brute *= p.nset
+    assert p.flops_symmetric == pytest.approx(brute, rel=1e-15)
+    assert 0.5 * p.flops < p.flops_symmetric < p.flops
+    q = W.rho_problem(seed=2, nshell=30, nao=60, ngrid=64 * 4, nset=1, density=0.5)
+    assert q.flops_symmetric == pytest.approx(q.flops, rel=1e-15)
