@@ -337,3 +337,44 @@ def test_rho_symmetric_form(blksize, nset):
         assert np.array_equal(_rho_dev(p, p.dms, p.phi, p.phi, same_buffer=False), gen)
     finally:
         cfgmm.tune_set(cfgmm.TUNE_RHO_SYMMETRIC, old)
+
+
+def test_rho_symmetric_form_strided_padding():
+    """The symmetric form with D rows strided past nao (asymmetric garbage in the padding, which the
+    test on the device must ignore) and phi and rho rows strided past ngrid: the oracle to 1e-12,
+    the general form to 1e-13, the rho padding untouched."""
+    p = W.rho_problem(seed=33, nshell=200, nao=650, ngrid=64 * 17 + 9, nset=2, density=0.35)
+    sa, sg = p.nao + 5, p.ngrid + 11
+    rng = np.random.default_rng(8)
+    D = rng.standard_normal((p.nset, sa, sa))  # asymmetric padding
+    D[:, :p.nao, :p.nao] = p.dms
+    phi = np.zeros((p.nao, sg))
+    phi[:, :p.ngrid] = p.phi
+    phi[:, p.ngrid:] = 1e30
+    r0 = np.full(p.nset * sg, 0.5)
+    ref = r0.reshape(p.nset, sg)[:, :p.ngrid].copy()
+    ref += _oracle_rho(p).reshape(p.nset, p.ngrid)
+    s = cfgmm.Stream()
+    d_d = cfgmm.DeviceBuffer.from_array(D)
+    d_phi = cfgmm.DeviceBuffer.from_array(phi)
+    d_t = cfgmm.DeviceBuffer.from_array(np.ascontiguousarray(p.tab))
+    outs = []
+    old = cfgmm.tune_get(cfgmm.TUNE_RHO_SYMMETRIC)
+    try:
+        for off in (0, 1):
+            cfgmm.tune_set(cfgmm.TUNE_RHO_SYMMETRIC, off)
+            d_r = cfgmm.DeviceBuffer.from_array(r0)
+            cfgmm.aodmao2rho_dev(s, d_r, d_d, d_phi, d_phi, sa, sg, p.nao, p.ngrid, p.nset, d_t, p.nblk, p.blksize)
+            s.synchronize()
+            outs.append(d_r.to_array(np.float64, (p.nset, sg)))
+            d_r.free()
+    finally:
+        cfgmm.tune_set(cfgmm.TUNE_RHO_SYMMETRIC, old)
+    for b in (d_d, d_phi, d_t):
+        b.free()
+    s.destroy()
+    sym, gen = outs
+    assert rel_fro(sym[:, :p.ngrid] - 0.5, ref - 0.5) <= TOL
+    assert rel_fro(sym[:, :p.ngrid] - 0.5, gen[:, :p.ngrid] - 0.5) <= 1e-13
+    assert not np.array_equal(sym, gen)  # the symmetric form ran
+    assert np.all(sym[:, p.ngrid:] == 0.5)
