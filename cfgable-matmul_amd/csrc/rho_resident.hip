@@ -657,7 +657,7 @@ int resident_create(hipStream_t stream, const double *dms, const double *ao_lhs,
         st = hip_status(hipMemcpyAsync(h->d_tasks, tasks.data(), sizeof(int4) * tasks.size(), hipMemcpyHostToDevice,
                                        stream),
                         "resident rho: tasks H2D");
-    if (st == CFGMM_OK) {
+    if (st == CFGMM_OK && max_kpad > 0) {  // an all-empty mask has no tiles to fill
         switch (S) {
             case 2: st = prep_launch<2>(stream, h, dms, ao_lhs, ao_rhs, stride_ao, stride_grid, lists, max_kpad); break;
             case 3: st = prep_launch<3>(stream, h, dms, ao_lhs, ao_rhs, stride_ao, stride_grid, lists, max_kpad); break;

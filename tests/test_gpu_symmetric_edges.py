@@ -1,0 +1,75 @@
+"""The symmetric forms (one buffer as lhs and rhs, D_s symmetric) over small and ragged shapes:
+nao below one 64-row tile, ngrid below one block, one block, empty and full masks, nset up to 5,
+every blksize kind.  Per call, batched and resident evaluations against the oracle (1e-12 FP64,
+1e-10 split precision); batched bitwise equal to per call."""
+import numpy as np
+import pytest
+
+import cfgmm
+import oracle
+from helpers import rel_fro
+
+pytestmark = pytest.mark.gpu
+
+CASES = [  # (nao, ngrid, blksize, nset, density)
+    (1, 17, 16, 1, 1.0), (5, 64, 64, 2, 0.6), (63, 65, 64, 1, 0.5), (64, 1, 16, 3, 0.7), (65, 200, 48, 5, 0.4),
+    (130, 130, 128, 2, 0.9), (200, 640, 64, 4, 0.0), (257, 333, 16, 3, 1.0), (300, 1000, 96, 2, 0.3),
+    (700, 256, 64, 1, 0.55),
+]
+
+
+def _case(nao, ngrid, blksize, nset, density, seed):
+    rng = np.random.default_rng(seed)
+    nblk = -(-ngrid // blksize)
+    tab = (rng.random((nao, nblk)) < density).astype(np.uint8)
+    phi = rng.standard_normal((nao, ngrid))
+    C = rng.standard_normal((nset, nao, 7))
+    dms = np.einsum("sik,sjk->sij", C, C)
+    dms = (dms + dms.transpose(0, 2, 1)) / 2  # bitwise symmetric
+    r0 = rng.standard_normal(nset * ngrid)
+    ref = r0.copy()
+    assert oracle.aodmao2rho(ref, dms.ravel(), phi.ravel(), phi.ravel(), nao, ngrid, nao, ngrid, nset, tab, nblk,
+                             blksize) == 0
+    return tab, phi, dms, r0, ref, nblk
+
+
+@pytest.mark.parametrize("i", range(len(CASES)))
+def test_symmetric_forms_small_shapes(i):
+    nao, ngrid, blksize, nset, density = CASES[i]
+    tab, phi, dms, r0, ref, nblk = _case(nao, ngrid, blksize, nset, density, seed=100 + i)
+    s = cfgmm.Stream()
+    d_d = cfgmm.DeviceBuffer.from_array(dms)
+    d_phi = cfgmm.DeviceBuffer.from_array(phi)
+    d_t = cfgmm.DeviceBuffer.from_array(tab)
+    # per call
+    d_r = cfgmm.DeviceBuffer.from_array(r0)
+    cfgmm.aodmao2rho_dev(s, d_r, d_d, d_phi, d_phi, nao, ngrid, nao, ngrid, nset, d_t, nblk, blksize)
+    s.synchronize()
+    single = d_r.to_array(np.float64, r0.shape)
+    # batched (the same problem twice, one buffer each)
+    d_b = [cfgmm.DeviceBuffer.from_array(r0) for _ in range(2)]
+    probs = [cfgmm.RhoProblem(b.ptr, d_d.ptr, d_phi.ptr, d_phi.ptr, nao, ngrid, nao, ngrid, nset, d_t.ptr, nblk,
+                              blksize) for b in d_b]
+    cfgmm.aodmao2rho_batched_dev(s, probs)
+    s.synchronize()
+    batched = [b.to_array(np.float64, r0.shape) for b in d_b]
+    # resident, five slices
+    d_z = cfgmm.DeviceBuffer.from_array(r0)
+    h = cfgmm.RhoResident(s, d_d, d_phi, d_phi, nao, ngrid, nao, ngrid, nset, d_t, nblk, blksize,
+                          cfgmm.PREC_OZAKI_INT8, 5)
+    h.eval(s, d_z, ngrid)
+    s.synchronize()
+    res = d_z.to_array(np.float64, r0.shape)
+    h.destroy()
+    for b in [d_d, d_phi, d_t, d_r, d_z] + d_b:
+        b.free()
+    s.destroy()
+    inc = ref - r0
+    if not tab.any():  # empty mask: nothing added
+        for out in [single, res] + batched:
+            assert np.array_equal(out, r0)
+        return
+    assert rel_fro(single - r0, inc) <= 1e-12
+    for b in batched:
+        assert np.array_equal(b, single)
+    assert rel_fro(res - r0, inc) <= 1e-10
