@@ -73,3 +73,42 @@ def test_symmetric_forms_small_shapes(i):
     for b in batched:
         assert np.array_equal(b, single)
     assert rel_fro(res - r0, inc) <= 1e-10
+
+
+@pytest.mark.parametrize("i", range(len(CASES)))
+def test_vxc_symmetric_small_shapes(i):
+    """The AO-grid-AO contraction with one buffer as lhs and rhs (upper tiles of S_b, the mirror
+    into V's lower triangle) against numpy, V padded and accumulated into."""
+    nao, ngrid, blksize, _, density = CASES[i]
+    rng = np.random.default_rng(200 + i)
+    nblk = -(-ngrid // blksize)
+    tab = (rng.random((nao, nblk)) < density).astype(np.uint8)
+    phi = rng.standard_normal((nao, ngrid))
+    w = rng.uniform(0.1, 2.0, ngrid)
+    ldv = nao + 2
+    V0 = rng.standard_normal(nao * ldv)
+    ref = np.zeros((nao, nao))
+    for b in range(nblk):
+        cols = slice(b * blksize, min((b + 1) * blksize, ngrid))
+        on = tab[:, b].astype(bool)
+        A = np.where(on[:, None], phi[:, cols] * w[cols], 0.0)
+        B = np.where(on[:, None], phi[:, cols], 0.0)
+        ref += A @ B.T
+    s = cfgmm.Stream()
+    d_v = cfgmm.DeviceBuffer.from_array(V0)
+    d_phi = cfgmm.DeviceBuffer.from_array(phi)
+    d_w = cfgmm.DeviceBuffer.from_array(w)
+    d_t = cfgmm.DeviceBuffer.from_array(tab)
+    cfgmm.aogridao2mat_dev(s, d_v, ldv, d_phi, d_phi, d_w, ngrid, nao, ngrid, d_t, nblk, blksize)
+    s.synchronize()
+    got = d_v.to_array(np.float64, (nao, ldv))
+    for b in (d_v, d_phi, d_w, d_t):
+        b.free()
+    s.destroy()
+    V0m = V0.reshape(nao, ldv)
+    assert np.array_equal(got[:, nao:], V0m[:, nao:])
+    inc = got[:, :nao] - V0m[:, :nao]
+    if not tab.any():
+        assert np.array_equal(got, V0m)
+        return
+    assert rel_fro(inc, ref) <= 1e-12
