@@ -1,5 +1,5 @@
-"""The symmetric forms (one buffer as lhs and rhs, D_s symmetric) over small and ragged shapes:
-nao below one 64-row tile, ngrid below one block, one block, empty and full masks, nset up to 5,
+"""The symmetric forms (one buffer as lhs and rhs, D_s symmetric) and the general forms over
+small and ragged shapes: nao below one 64-row tile, ngrid below one block, one block, empty and full masks, nset up to 5,
 every blksize kind.  Per call, batched and resident evaluations against the oracle (1e-12 FP64,
 1e-10 split precision); batched bitwise equal to per call."""
 import numpy as np
@@ -112,3 +112,49 @@ def test_vxc_symmetric_small_shapes(i):
         assert np.array_equal(got, V0m)
         return
     assert rel_fro(inc, ref) <= 1e-12
+
+
+@pytest.mark.parametrize("i", range(len(CASES)))
+def test_general_forms_small_shapes(i):
+    """The same shapes in the general form: lhs and rhs different buffers, D not symmetric."""
+    nao, ngrid, blksize, nset, density = CASES[i]
+    rng = np.random.default_rng(300 + i)
+    nblk = -(-ngrid // blksize)
+    tab = (rng.random((nao, nblk)) < density).astype(np.uint8)
+    lhs = rng.standard_normal((nao, ngrid))
+    rhs = rng.standard_normal((nao, ngrid))
+    dms = rng.standard_normal((nset, nao, nao))
+    r0 = rng.standard_normal(nset * ngrid)
+    ref = r0.copy()
+    assert oracle.aodmao2rho(ref, dms.ravel(), lhs.ravel(), rhs.ravel(), nao, ngrid, nao, ngrid, nset, tab, nblk,
+                             blksize) == 0
+    s = cfgmm.Stream()
+    d_d = cfgmm.DeviceBuffer.from_array(dms)
+    d_l = cfgmm.DeviceBuffer.from_array(lhs)
+    d_rr = cfgmm.DeviceBuffer.from_array(rhs)
+    d_t = cfgmm.DeviceBuffer.from_array(tab)
+    d_r = cfgmm.DeviceBuffer.from_array(r0)
+    cfgmm.aodmao2rho_dev(s, d_r, d_d, d_l, d_rr, nao, ngrid, nao, ngrid, nset, d_t, nblk, blksize)
+    d_b = cfgmm.DeviceBuffer.from_array(r0)
+    cfgmm.aodmao2rho_batched_dev(s, [cfgmm.RhoProblem(d_b.ptr, d_d.ptr, d_l.ptr, d_rr.ptr, nao, ngrid, nao, ngrid, nset,
+                                                      d_t.ptr, nblk, blksize)])
+    d_z = cfgmm.DeviceBuffer.from_array(r0)
+    h = cfgmm.RhoResident(s, d_d, d_l, d_rr, nao, ngrid, nao, ngrid, nset, d_t, nblk, blksize,
+                          cfgmm.PREC_OZAKI_INT8, 6)
+    h.eval(s, d_z, ngrid)
+    s.synchronize()
+    single = d_r.to_array(np.float64, r0.shape)
+    batched = d_b.to_array(np.float64, r0.shape)
+    res = d_z.to_array(np.float64, r0.shape)
+    h.destroy()
+    for b in (d_d, d_l, d_rr, d_t, d_r, d_b, d_z):
+        b.free()
+    s.destroy()
+    if not tab.any():
+        for out in (single, batched, res):
+            assert np.array_equal(out, r0)
+        return
+    inc = ref - r0
+    assert rel_fro(single - r0, inc) <= 1e-12
+    assert np.array_equal(batched, single)
+    assert rel_fro(res - r0, inc) <= 1e-10
