@@ -13,6 +13,11 @@ Modes:
   vxc            V += phi w phi^T over the same mask (vxc_block_kernel, vxc_reduce_kernel)
   gto            AO tabulation of workloads.gto_workload (700 shells, 1890 AOs, 16384 points)
   copy           a 256 MiB device-to-device copy (byte-counter calibration)
+  batched        512 uneven rho problems over 32 generated molecules in one FP64 batched launch
+                 (workloads.batched_problems; rho_batched kernels)
+  batched_resident<S> the same 512 problems as leading ranges of the 32 molecules' resident
+                 handles (S int8 slices; the handles are created before the repetitions) in one
+                 cfgmm_rho_resident_eval_batched call
 
 Each pass of a hardware-counter profiler is a separate process running this
 module; the inputs are generated, uploaded once and then the operation runs
@@ -83,6 +88,37 @@ def main(argv: list[str]) -> int:
         for _ in range(reps):
             cfgmm.eval_gto_dev(s, d_phi, gw["ngrid"], d_tab, gw["nblk"], shells, gw["exps"], gw["coefs"], d_xyz,
                                gw["ngrid"], gw["nao"], gw["blksize"], gw["cutoff"])
+    elif mode == "batched" or mode.startswith("batched_resident"):
+        pool = W.molecule_pool(n=32, seed=100)
+        probs = W.batched_problems(pool, n=512, seed=7)
+        dev = [(cfgmm.DeviceBuffer.from_array(m.prob.dms), cfgmm.DeviceBuffer.from_array(m.prob.phi),
+                cfgmm.DeviceBuffer.from_array(np.ascontiguousarray(m.prob.tab))) for m in pool]
+        # rho rows strided by the molecule's grid, as its phi rows are
+        sizes = [(q.nset - 1) * pool[q.mol].prob.ngrid + q.ngrid for q in probs]
+        offs = np.concatenate([[0], np.cumsum(sizes)])
+        out = cfgmm.DeviceBuffer(8 * int(offs[-1]))
+        out.zero()
+        if mode == "batched":
+            descs = []
+            for i, q in enumerate(probs):
+                pm = pool[q.mol].prob
+                d, phi, tab = dev[q.mol]
+                descs.append(cfgmm.RhoProblem(out.ptr + 8 * int(offs[i]), d.ptr, phi.ptr, phi.ptr, pm.nao, pm.ngrid,
+                                              pm.nao, q.ngrid, q.nset, tab.ptr, pm.nblk, pm.blksize))
+            arr = (cfgmm.RhoProblem * len(descs))(*descs)
+            for _ in range(reps):
+                cfgmm.aodmao2rho_batched_dev(s, arr, cfgmm.PREC_FP64, 0)
+        else:
+            nslices = int(mode[len("batched_resident"):])
+            handles = []
+            for m, (d, phi, tab) in zip(pool, dev):
+                pm = m.prob
+                handles.append(cfgmm.RhoResident(s, d, phi, phi, pm.nao, pm.ngrid, pm.nao, pm.ngrid, pm.nset, tab,
+                                                 pm.nblk, pm.blksize, cfgmm.PREC_OZAKI_INT8, nslices))
+            evals = [(handles[q.mol], out.ptr + 8 * int(offs[i]), pool[q.mol].prob.ngrid, q.ngrid, q.nset)
+                     for i, q in enumerate(probs)]
+            for _ in range(reps):
+                cfgmm.rho_resident_eval_batched(s, evals)
     elif mode == "copy":
         nbytes = 256 << 20
         a = cfgmm.DeviceBuffer(nbytes)
