@@ -85,6 +85,26 @@ def rel_fro(x, ref) -> float:
     return float(np.linalg.norm(x - ref) / den) if den > 0 else float(np.linalg.norm(x))
 
 
+def traced_kernel_ms(fn, stream, names, reps: int):
+    """Average hipEvent duration of the named kernels over `reps` calls of fn, in a pass of its
+    own with library tracing on (the timed steps run with tracing off)."""
+    stream.synchronize()
+    cfgmm.trace_reset()
+    cfgmm.trace_enable(True)
+    try:
+        for _ in range(reps):
+            fn()
+        stream.synchronize()
+    finally:
+        cfgmm.trace_enable(False)
+    out = {}
+    for name in names:
+        tot, n = cfgmm.trace_read(name)
+        out[name] = tot / n if n else float("nan")
+    cfgmm.trace_reset()
+    return out
+
+
 def bench_rho(p: W.RhoProblem, steps: int, warmup: int, precision=cfgmm.PREC_FP64, nslices=0):
     s = cfgmm.Stream()
     d_r = cfgmm.DeviceBuffer(8 * p.nset * p.ngrid)
@@ -106,8 +126,6 @@ def bench_rho(p: W.RhoProblem, steps: int, warmup: int, precision=cfgmm.PREC_FP6
     s.synchronize()
 
     evs = [cfgmm.Event() for _ in range(steps + 1)]
-    cfgmm.trace_reset()
-    cfgmm.trace_enable(True)
     cfgmm.check(cfgmm.lib().cfgmm_device_synchronize())
     t0 = time.perf_counter()
     evs[0].record(s)
@@ -116,18 +134,16 @@ def bench_rho(p: W.RhoProblem, steps: int, warmup: int, precision=cfgmm.PREC_FP6
         evs[i + 1].record(s)
     s.synchronize()
     wall = time.perf_counter() - t0
-    cfgmm.trace_enable(False)
     per_step = np.array([evs[i].elapsed_ms(evs[i + 1]) for i in range(steps)])
-    kern_ms, kern_n = cfgmm.trace_read("rho_chunk_kernel")
     total_gpu_ms = evs[0].elapsed_ms(evs[-1])
-    cfgmm.trace_reset()
+    kern = traced_kernel_ms(step, s, ["rho_chunk_kernel"], min(steps, 50))
     for e in evs:
         e.destroy()
     for b in (d_r, d_d, d_phi, d_tab):
         b.free()
     s.destroy()
     return dict(rho_once=rho_once, wall_s=wall, gpu_ms=total_gpu_ms, per_step_ms=per_step,
-                kernel_avg_ms=kern_ms / max(kern_n, 1), kernel_launches=kern_n)
+                kernel_avg_ms=kern["rho_chunk_kernel"])
 
 
 def bench_resident(p: W.RhoProblem, steps: int, nslices: int = RESIDENT_SLICES):
@@ -149,17 +165,14 @@ def bench_resident(p: W.RhoProblem, steps: int, nslices: int = RESIDENT_SLICES):
     for _ in range(5):
         h.eval(s, d_r, p.ngrid)
     e0, e1 = cfgmm.Event(), cfgmm.Event()
-    cfgmm.trace_reset()
-    cfgmm.trace_enable(True)
     e0.record(s)
     for _ in range(steps):
         h.eval(s, d_r, p.ngrid)
     e1.record(s)
     s.synchronize()
-    cfgmm.trace_enable(False)
-    km, kn = cfgmm.trace_read("rho_resident_kernel")
-    cfgmm.trace_reset()
     ms = e0.elapsed_ms(e1) / steps
+    km = traced_kernel_ms(lambda: h.eval(s, d_r, p.ngrid), s, ["rho_resident_kernel"], min(steps, 50))
+    km = km["rho_resident_kernel"]
     nbytes = h.bytes
     h.destroy()
     for b in (d_r, d_d, d_phi, d_tab):
@@ -167,7 +180,7 @@ def bench_resident(p: W.RhoProblem, steps: int, nslices: int = RESIDENT_SLICES):
     e0.destroy()
     e1.destroy()
     s.destroy()
-    return dict(ms=ms, kernel_ms=km / max(kn, 1), create_ms=create_ms, bytes=nbytes, rho=rho)
+    return dict(ms=ms, kernel_ms=km, create_ms=create_ms, bytes=nbytes, rho=rho)
 
 
 def bench_multi(p: W.RhoProblem, ncomp: int = 4, reps: int = 20):
@@ -262,17 +275,12 @@ def bench_gto(p: W.RhoProblem, reps: int = 10):
         cfgmm.eval_gto_dev(s, d_phi, p.ngrid, d_tab, nblk, shells, exps, coefs, d_xyz, p.ngrid, nao, p.blksize, cutoff)
     tab_once()
     s.synchronize()
-    cfgmm.trace_reset()
-    cfgmm.trace_enable(True)
     t0 = time.perf_counter()
     for _ in range(reps):
         tab_once()  # asynchronous: the shell tables are staged through pinned memory
     s.synchronize()
     ms = (time.perf_counter() - t0) / reps * 1e3
-    cfgmm.trace_enable(False)
-    kern_sum, kern_n = cfgmm.trace_read("gto_eval_kernel")
-    kern_ms = kern_sum / max(kern_n, 1)
-    cfgmm.trace_reset()
+    kern_ms = traced_kernel_ms(tab_once, s, ["gto_eval_kernel"], reps)["gto_eval_kernel"]
     tab = d_tab.to_array(np.uint8, (nao, nblk))
     # rho on the tabulated operands (D of the basis size, on the device)
     C = rng.standard_normal((p.nset, nao, 100))
@@ -313,23 +321,19 @@ def bench_dgemm(g: W.GemmProblem, reps: int, warmup: int = 2):
         cfgmm.dgemm_dev(s, dC, dA, dB, g.m, g.n, g.k, g.lda, g.ldb, g.ldc, g.transa, g.transb)
     s.synchronize()
     e0, e1 = cfgmm.Event(), cfgmm.Event()
-    cfgmm.trace_reset()
-    cfgmm.trace_enable(True)
     e0.record(s)
     for _ in range(reps):
         cfgmm.dgemm_dev(s, dC, dA, dB, g.m, g.n, g.k, g.lda, g.ldb, g.ldc, g.transa, g.transb)
     e1.record(s)
     s.synchronize()
-    cfgmm.trace_enable(False)
     step_ms = e0.elapsed_ms(e1) / reps
-    kern_ms, kern_n = cfgmm.trace_read("dgemm_kernel")
-    cfgmm.trace_reset()
+    kavg = traced_kernel_ms(lambda: cfgmm.dgemm_dev(s, dC, dA, dB, g.m, g.n, g.k, g.lda, g.ldb, g.ldc, g.transa,
+                                                    g.transb), s, ["dgemm_kernel"], reps)["dgemm_kernel"]
     for x in (dA, dB, dC):
         x.free()
     e0.destroy()
     e1.destroy()
     s.destroy()
-    kavg = kern_ms / max(kern_n, 1)
     tf_step = g.flops / (step_ms * 1e-3) / 1e12
     tf_kern = g.flops / (kavg * 1e-3) / 1e12
     return dict(shape=f"{g.m}x{g.n}x{g.k}", trans=("T" if g.transa else "N") + ("T" if g.transb else "N"),
@@ -790,6 +794,8 @@ def main():
     finally:
         cfgmm.tune_set(cfgmm.TUNE_RHO_SYMMETRIC, 0)
     general = {"ms_per_step": round(rg["gpu_ms"] / max(20, args.steps // 4), 5),
+               "evals_per_s": round(max(20, args.steps // 4) / (rg["gpu_ms"] * 1e-3), 3),
+               "vs_baseline": round(max(20, args.steps // 4) / (rg["gpu_ms"] * 1e-3) / README_RHO_EVALS_PER_S, 2),
                "kernel_avg_ms": round(rg["kernel_avg_ms"], 5),
                "kernel_tflops": round(p.flops / (rg["kernel_avg_ms"] * 1e-3) / 1e12, 3),
                "masked_gflop_per_eval": round(p.flops / 1e9, 3),
@@ -807,7 +813,9 @@ def main():
         "scaling": "weak",
         "vs_baseline": round(evals_per_s / README_RHO_EVALS_PER_S, 2),
         "vs_baseline_ref": "README.md:51 masked rho (nao 2228, ngrid 16384, 30 %, 3 DMs) 120 ms on a 16-core Ryzen "
-                           "7945HX = 8.33 evals/s",
+                           "7945HX = 8.33 evals/s; `value` is the symmetric form (D == D^T and one phi on both "
+                           "sides, 26.3 of the 48.4 masked GFLOP), so the ratio includes that 1.84x flop "
+                           "saving; general_form.vs_baseline is the same computation as the reference's",
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "masked AO-DM-AO rho, FP64: 700 shells (s/p/d/f 40/30/20/10 %) -> nao 2228, "

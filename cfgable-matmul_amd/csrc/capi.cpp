@@ -120,14 +120,22 @@ int device_cu_count() {
 // ---------------------------------------------------------------------------
 // per-stream workspace
 // ---------------------------------------------------------------------------
+// g_ws_mutex guards the map only (its nodes never move); each Workspace's own mutex is held while
+// that stream is synchronized or its buffers change, so a call waiting on one stream's work does
+// not hold up calls on other streams.
 static std::mutex g_ws_mutex;
 static std::map<std::pair<int, hipStream_t>, Workspace> g_ws;
+
+static Workspace &workspace_of(int dev, hipStream_t stream) {
+    std::lock_guard<std::mutex> lock(g_ws_mutex);
+    return g_ws[std::make_pair(dev, stream)];
+}
 
 int workspace_get(hipStream_t stream, size_t bytes, void **out) {
     int dev = 0;
     CFGMM_HIP_TRY(hipGetDevice(&dev));
-    std::lock_guard<std::mutex> lock(g_ws_mutex);
-    Workspace &ws = g_ws[std::make_pair(dev, stream)];
+    Workspace &ws = workspace_of(dev, stream);
+    std::lock_guard<std::mutex> lock(ws.mu);
     if (ws.bytes < bytes) {
         if (ws.ptr) {
             // kernels already enqueued on this stream may still read the old buffer
@@ -149,15 +157,16 @@ int workspace_get(hipStream_t stream, size_t bytes, void **out) {
 unsigned workspace_epoch(hipStream_t stream) {
     int dev = 0;
     (void)hipGetDevice(&dev);
-    std::lock_guard<std::mutex> lock(g_ws_mutex);
-    return ++g_ws[std::make_pair(dev, stream)].epoch;
+    Workspace &ws = workspace_of(dev, stream);
+    std::lock_guard<std::mutex> lock(ws.mu);
+    return ++ws.epoch;
 }
 
 int host_staging_get(hipStream_t stream, size_t bytes, void **out) {
     int dev = 0;
     CFGMM_HIP_TRY(hipGetDevice(&dev));
-    std::lock_guard<std::mutex> lock(g_ws_mutex);
-    Workspace &ws = g_ws[std::make_pair(dev, stream)];
+    Workspace &ws = workspace_of(dev, stream);
+    std::lock_guard<std::mutex> lock(ws.mu);
     if (ws.host_done) CFGMM_HIP_TRY(hipEventSynchronize(ws.host_done));  // earlier copies have read it
     if (ws.host_bytes < bytes) {
         if (ws.host) {
@@ -179,8 +188,8 @@ int host_staging_get(hipStream_t stream, size_t bytes, void **out) {
 int host_staging_release(hipStream_t stream) {
     int dev = 0;
     CFGMM_HIP_TRY(hipGetDevice(&dev));
-    std::lock_guard<std::mutex> lock(g_ws_mutex);
-    Workspace &ws = g_ws[std::make_pair(dev, stream)];
+    Workspace &ws = workspace_of(dev, stream);
+    std::lock_guard<std::mutex> lock(ws.mu);
     if (!ws.host_done) CFGMM_HIP_TRY(hipEventCreateWithFlags(&ws.host_done, hipEventDisableTiming));
     return hip_status(hipEventRecord(ws.host_done, stream), "host staging: event record");
 }
@@ -859,6 +868,7 @@ int cfgmm_stream_destroy(void *stream) {
             std::lock_guard<std::mutex> lock(g_ws_mutex);
             auto it = g_ws.find(std::make_pair(dev, static_cast<hipStream_t>(stream)));
             if (it != g_ws.end()) {
+                // a stream being destroyed has no calls in flight on other threads
                 (void)hipStreamSynchronize(static_cast<hipStream_t>(stream));
                 if (it->second.ptr) (void)hipFree(it->second.ptr);
                 if (it->second.host) (void)hipHostFree(it->second.host);

@@ -5,6 +5,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <mutex>
 #include <string>
 
 #include "../../include/cfgmm.h"
@@ -32,6 +33,7 @@ int hip_status(hipError_t e, const char *what);
 // from the workspace bound to their stream; growing it synchronizes that
 // stream first so that no in-flight kernel still reads the old buffer.
 struct Workspace {
+    std::mutex mu;  // held while this stream's buffers are synchronized, grown or handed out
     void *ptr = nullptr;
     size_t bytes = 0;
     void *host = nullptr;  // pinned host staging for tables copied to the device

@@ -8,17 +8,20 @@
 //   * each row i of op(A) is scaled by 2^-ea[i] and each column j of op(B) by
 //     2^-eb[j], with ea/eb the binary exponents of the row / column maxima, so
 //     that every scaled entry lies in (-1, 1);
-//   * a scaled entry x is cut into S int8 slices by exact f64 arithmetic,
-//       x = sum_{s=1..S} q_s 2^(-7 s) + r,   q_s = trunc(2^7 * rest_{s-1}),
-//     |q_s| <= 127, |r| < 2^(-7 S);
-//   * A B ~= diag(2^ea) [ sum_{s+t <= S+1} 2^(-7(s+t)) Q_s P_t ] diag(2^eb):
+//   * a scaled entry x is cut into S balanced int8 digits of radix 254:
+//       v_0 = 127 x,  q_s = rint(v_s),  v_{s+1} = 254 (v_s - q_s),
+//       x = sum_{s<S} q_s / (127 * 254^s) + r,   |q_s| <= 127,  |r| <= 254^-S
+//     (v_0 lies in (-127, 127) and every later v is 254 times a remainder in
+//     [-1/2, 1/2], so each int8 digit carries about 8 bits);
+//   * A B ~= diag(2^ea) [ sum_{s+t < S} Q_s P_t / (127^2 254^(s+t)) ] diag(2^eb):
 //     every slice product Q_s P_t is computed exactly with
 //     v_mfma_i32_16x16x64_i8 (int32 accumulation; products of one level
 //     s+t share an accumulator, exact while S * 127^2 * K < 2^31), and the
-//     levels are converted to f64, scaled by powers of two (exact) and summed
-//     in f64 from the smallest level up.
-// The dropped terms (s+t > S+1) and the slice residuals are O(2^(-7S)) relative
-// to |A||B|; S = 6 gives about 1e-12 relative error on well-scaled data.
+//     levels are converted to f64, weighted and summed in f64 from the
+//     smallest level up.
+// Per product term the dropped levels (s+t >= S) and the digit residuals are
+// at most (S + 1.01) 254^-S in scaled units; S = 5 gives about 1e-11 relative
+// error on well-scaled data, S = 6 about 1e-13.
 // Rows or columns containing Inf/NaN produce NaN in the rows/columns of C they touch.
 #include "internal.hpp"
 
@@ -94,8 +97,15 @@ __global__ void row_exponent_kernel(const unsigned long long *__restrict__ rowma
     if (i < rows) ex[i] = row_exponent(rowmax[i]);
 }
 
-// q[s][r][c] (int8, rows_pad x cols_pad per slice, c contiguous) = s-th 7-bit
-// slice of x(r, c) * 2^-ex[r]; zero outside X.
+// Weight of the digit products of level l = s + t: 1 / (127^2 254^l).
+__device__ __host__ constexpr double level_weight(int l) {
+    double w = 1.0 / (127.0 * 127.0);
+    for (int i = 0; i < l; ++i) w /= 254.0;
+    return w;
+}
+
+// q[s][r][c] (int8, rows_pad x cols_pad per slice, c contiguous) = s-th balanced
+// digit of x(r, c) * 2^-ex[r]; zero outside X.
 template <int S>
 __global__ __launch_bounds__(kThreads) void slice_kernel(const double *__restrict__ x, long rs, long cs, int rows,
                                                          int cols, const int *__restrict__ ex, int8_t *__restrict__ q,
@@ -115,12 +125,11 @@ __global__ __launch_bounds__(kThreads) void slice_kernel(const double *__restric
         for (int w = 0; w < 4; ++w) packed[s][w] = 0;
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
-        double rest = finite ? ldexp(t[r][chunk * 16 + j], -e) : 0.0;  // |rest| < 1
+        double v = finite ? ldexp(t[r][chunk * 16 + j], -e) * 127.0 : 0.0;  // |v| < 127
 #pragma unroll
         for (int s = 0; s < S; ++s) {
-            const double v = rest * 128.0;
-            const double qv = trunc(v);
-            rest = v - qv;  // exact
+            const double qv = rint(v);  // |qv| <= 127
+            v = (v - qv) * 254.0;       // v - qv is exact, |v - qv| <= 1/2
             packed[s][j / 4] |= ((int)qv & 0xff) << (8 * (j % 4));
         }
     }
@@ -163,7 +172,7 @@ __global__ __launch_bounds__(kThreads, 1) void ozaki_gemm_kernel(double *__restr
                                                                  long n_pad, long k_pad, int k0, int k1,
                                                                  const int *__restrict__ ea,
                                                                  const int *__restrict__ eb, int group_m) {
-    constexpr int LEV = S;  // levels s + t = 2 .. S + 1
+    constexpr int LEV = S;  // levels s + t = 0 .. S - 1
     __shared__ __attribute__((aligned(16))) int8_t As[2][S][kOzBM * kRowBytes];
     __shared__ __attribute__((aligned(16))) int8_t Bs[2][S][kOzBN * kRowBytes];
 
@@ -244,7 +253,7 @@ __global__ __launch_bounds__(kThreads, 1) void ozaki_gemm_kernel(double *__restr
         __syncthreads();
     }
 
-    // Recombination: C += 2^(ea+eb) * sum_l 2^(-7(l+2)) acc[l], smallest level first.
+    // Recombination: C += 2^(ea+eb) * sum_l acc[l] / (127^2 254^l), smallest level first.
     // i32 16x16 output layout: lane l, element r holds D[4*(l/16) + r][l%16].
 #pragma unroll
     for (int rt = 0; rt < 4; ++rt)
@@ -257,7 +266,7 @@ __global__ __launch_bounds__(kThreads, 1) void ozaki_gemm_kernel(double *__restr
                 if (row < m && col < n) {
                     double v = 0.0;
 #pragma unroll
-                    for (int l = LEV - 1; l >= 0; --l) v += ldexp((double)acc[l][rt][ct][r], -7 * (l + 2));
+                    for (int l = LEV - 1; l >= 0; --l) v += (double)acc[l][rt][ct][r] * level_weight(l);
                     const int e1 = ea[row], e2 = eb[col];
                     v = (e1 == kExpNonFinite || e2 == kExpNonFinite) ? __builtin_nan("") : ldexp(v, e1 + e2);
                     c[(long)row * ldc + col] += v;

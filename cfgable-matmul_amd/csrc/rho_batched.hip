@@ -41,6 +41,8 @@ struct BatchProblem {
     int nao, ngrid, nset, blksize, nblk, nsub, nt;
     long lists_off, counts_off, partial_off;
     int sym;  // index + 1 of the problem's (D, stride, nao, nset) in the symmetry table; 0: lhs is not rhs
+    int wave;  // reduction pass: problems whose rho ranges overlap add into rho in call order
+    int call;  // index of the problem in the caller's array
 };
 
 // One distinct (D, stride_ao, nao, nset) of the batch whose problems have lhs == rhs; flag starts
@@ -298,8 +300,9 @@ __global__ __launch_bounds__(kThreads) void batch_rho_kernel(const BatchProblem 
 }
 
 __global__ void batch_reduce(const BatchProblem *__restrict__ probs, const int *__restrict__ counts,
-                             const double *__restrict__ partial) {
+                             const double *__restrict__ partial, int wave) {
     const BatchProblem p = probs[blockIdx.y];
+    if (p.wave != wave) return;
     const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (long)p.nset * p.ngrid) return;
     const int s = (int)(idx / p.ngrid), g = (int)(idx % p.ngrid);
@@ -308,6 +311,71 @@ __global__ void batch_reduce(const BatchProblem *__restrict__ probs, const int *
     double v = 0.0;
     for (int ch = 0; ch < nch; ++ch) v += part[((long)ch * p.nset + s) * p.ngrid + g];
     p.rhos[(long)s * p.stride_grid + g] += v;
+}
+
+// Reduction passes of the problems (`index` = position in the call, 0..n-1): problem i goes
+// one pass after the last earlier problem whose rho range
+// [rhos, rhos + (nset-1)*stride_grid + ngrid) overlaps its own, so overlapping problems add in
+// call order and every other problem shares pass 0.  Returns the number of passes.
+struct RhoSpan {
+    uintptr_t lo, hi;
+    int index;
+};
+
+int assign_reduce_waves_spans(std::vector<RhoSpan> &spans, std::vector<int> &wave_of) {
+    const size_t n = spans.size();
+    wave_of.assign(n, 0);
+    std::vector<RhoSpan> by_lo(spans);
+    std::sort(by_lo.begin(), by_lo.end(), [](const RhoSpan &a, const RhoSpan &b) {
+        return a.lo != b.lo ? a.lo < b.lo : a.index < b.index;
+    });
+    bool any = false;
+    uintptr_t reach = 0;
+    for (size_t i = 0; i < n; ++i) {
+        if (i > 0 && by_lo[i].lo < reach) any = true;
+        reach = std::max(reach, by_lo[i].hi);
+    }
+    if (!any) return 1;
+    // prefix maximum of hi over by_lo: a scan to the left stops once no earlier span reaches lo
+    std::vector<uintptr_t> pmax(n);
+    std::vector<size_t> pos(n);
+    for (size_t i = 0; i < n; ++i) {
+        pmax[i] = std::max(i ? pmax[i - 1] : 0, by_lo[i].hi);
+        pos[by_lo[i].index] = i;
+    }
+    int nwaves = 1;
+    for (size_t idx = 0; idx < n; ++idx) {  // call order: earlier problems have their pass
+        const size_t p = pos[idx];
+        const RhoSpan me = by_lo[p];
+        int w = 0;
+        for (size_t j = p; j-- > 0 && pmax[j] > me.lo;)
+            if (by_lo[j].hi > me.lo && (size_t)by_lo[j].index < idx) w = std::max(w, wave_of[by_lo[j].index] + 1);
+        for (size_t j = p + 1; j < n && by_lo[j].lo < me.hi; ++j)
+            if ((size_t)by_lo[j].index < idx) w = std::max(w, wave_of[by_lo[j].index] + 1);
+        wave_of[idx] = w;
+        nwaves = std::max(nwaves, w + 1);
+    }
+    return nwaves;
+}
+
+int assign_reduce_waves(std::vector<BatchProblem> &bp) {
+    // bp is in launch order (largest first); rank[i] = position of bp[i] in call order
+    const size_t n = bp.size();
+    std::vector<size_t> by_call(n), rank(n);
+    std::iota(by_call.begin(), by_call.end(), 0);
+    std::sort(by_call.begin(), by_call.end(), [&](size_t x, size_t y) { return bp[x].call < bp[y].call; });
+    for (size_t r = 0; r < n; ++r) rank[by_call[r]] = r;
+    std::vector<RhoSpan> spans(n);
+    for (size_t i = 0; i < n; ++i) {
+        const BatchProblem &p = bp[i];
+        const uintptr_t lo = reinterpret_cast<uintptr_t>(p.rhos);
+        spans[i] = RhoSpan{lo, lo + sizeof(double) * ((uintptr_t)(p.nset - 1) * p.stride_grid + p.ngrid),
+                           (int)rank[i]};
+    }
+    std::vector<int> wave_of;
+    const int nwaves = assign_reduce_waves_spans(spans, wave_of);
+    for (size_t i = 0; i < n; ++i) bp[i].wave = wave_of[rank[i]];
+    return nwaves;
 }
 
 int tile_nt(size_t blksize) {
@@ -346,6 +414,7 @@ int launch_rho_batched(hipStream_t stream, const cfgmm_rho_problem *problems, si
         if (q.nao > INT32_MAX || q.ngrid > INT32_MAX || q.nset > 65535)
             return fail(CFGMM_EINVAL, "batched rho: problem size out of range");
         BatchProblem p{};
+        p.call = (int)oi;
         p.rhos = q.rhos;
         p.dms = q.dms;
         p.ao_lhs = q.ao_lhs;
@@ -387,6 +456,7 @@ int launch_rho_batched(hipStream_t stream, const cfgmm_rho_problem *problems, si
     }
     if (bp.empty()) return CFGMM_OK;
     if (tasks_max > INT32_MAX) return fail(CFGMM_EINVAL, "batched rho: too many tasks for one launch");
+    const int nwaves = assign_reduce_waves(bp);
     const int np = (int)bp.size();
 
     auto al = [](size_t x) { return div_up(x, 256) * 256; };
@@ -446,9 +516,12 @@ int launch_rho_batched(hipStream_t stream, const cfgmm_rho_problem *problems, si
     trace_stop(stream, "batch_rho_kernel", t0);
     st = hip_status(hipGetLastError(), "batch_rho_kernel launch");
     if (st != CFGMM_OK) return st;
+    // one pass per wave, in wave order: problems that add into the same rho elements do so in the
+    // order of the call, as separate calls would
     t0 = trace_start(stream);
-    hipLaunchKernelGGL(batch_reduce, dim3((unsigned)div_up(max_elems, 256), (unsigned)np), dim3(256), 0, stream,
-                       d_probs, counts, partial);
+    for (int w = 0; w < nwaves; ++w)
+        hipLaunchKernelGGL(batch_reduce, dim3((unsigned)div_up(max_elems, 256), (unsigned)np), dim3(256), 0, stream,
+                           d_probs, counts, partial, w);
     trace_stop(stream, "batch_reduce", t0);
     return hip_status(hipGetLastError(), "batch_reduce launch");
 }

@@ -127,35 +127,63 @@ struct Server {
         return b;
     }
 
-    // Uploads a molecule (and, in split precision, builds its tiles) on `stream`; synchronous.
-    int make_resident(ServerMolecule &m, hipStream_t stream) {
-        if (m.resident) return CFGMM_OK;
-        const size_t base = molecule_bytes(m);
-        if (!make_room(base, m.id)) return fail(CFGMM_EOOM, "server: molecule does not fit the HBM budget");
+    // Frees whatever device memory a molecule that is not (yet) resident holds, after the work
+    // on `stream` that may still use it; leaves the molecule clean for a later upload.
+    void discard(ServerMolecule &m, hipStream_t stream) {
+        (void)hipStreamSynchronize(stream);
+        if (m.tiles) (void)resident_destroy(m.tiles);
+        if (m.d_dms) (void)hipFree(m.d_dms);
+        if (m.d_phi) (void)hipFree(m.d_phi);
+        if (m.d_tab) (void)hipFree(m.d_tab);
+        m.tiles = nullptr;
+        m.d_dms = m.d_phi = m.d_tab = nullptr;
+        m.bytes = 0;
+        m.resident = false;
+    }
+
+    // Allocations, copies and (split precision) tiles of a molecule, into m's fields.
+    int upload(ServerMolecule &m, hipStream_t stream) {
         CFGMM_HIP_TRY(hipMalloc(&m.d_dms, sizeof(double) * m.nset * m.nao * m.nao));
         CFGMM_HIP_TRY(hipMalloc(&m.d_phi, sizeof(double) * m.nao * m.ngrid));
         CFGMM_HIP_TRY(hipMalloc(&m.d_tab, std::max<size_t>(m.nao * m.nblk, 1)));
-        m.resident = true;
-        m.bytes = base;
-        held += base;
         CFGMM_HIP_TRY(hipMemcpyAsync(m.d_dms, m.h_dms, sizeof(double) * m.nset * m.nao * m.nao, hipMemcpyHostToDevice,
                                      stream));
         CFGMM_HIP_TRY(
             hipMemcpyAsync(m.d_phi, m.h_phi, sizeof(double) * m.nao * m.ngrid, hipMemcpyHostToDevice, stream));
         CFGMM_HIP_TRY(hipMemcpyAsync(m.d_tab, m.h_tab, m.nao * m.nblk, hipMemcpyHostToDevice, stream));
         if (cfg.rho_precision == CFGMM_PREC_OZAKI_INT8) {
-            int st = resident_create(stream, static_cast<double *>(m.d_dms), static_cast<double *>(m.d_phi),
-                                     static_cast<double *>(m.d_phi), m.nao, m.ngrid, m.nao, m.ngrid, m.nset,
-                                     static_cast<uint8_t *>(m.d_tab), m.nblk, m.blksize, cfg.rho_nslices, &m.tiles);
+            ResidentRho *tiles = nullptr;
+            const int st = resident_create(stream, static_cast<double *>(m.d_dms), static_cast<double *>(m.d_phi),
+                                           static_cast<double *>(m.d_phi), m.nao, m.ngrid, m.nao, m.ngrid, m.nset,
+                                           static_cast<uint8_t *>(m.d_tab), m.nblk, m.blksize, cfg.rho_nslices,
+                                           &tiles);
             if (st != CFGMM_OK) return st;
-            const size_t tb = resident_bytes(m.tiles);
-            m.bytes += tb;
-            held += tb;
-            // the tiles replace D for evaluations; keep D and phi (small next to the tiles) for re-creation
-            if (held > cfg.hbm_budget_bytes && !make_room(0, m.id))
-                return fail(CFGMM_EOOM, "server: molecule tiles do not fit the HBM budget");
+            m.tiles = tiles;
         }
-        CFGMM_HIP_TRY(hipStreamSynchronize(stream));
+        return hip_status(hipStreamSynchronize(stream), "server: molecule upload");
+    }
+
+    // Uploads a molecule (and, in split precision, builds its tiles) on `stream`; synchronous.
+    // The molecule is marked resident and charged to the budget only once everything succeeded;
+    // on any failure what was allocated is freed and the molecule stays as it was.
+    int make_resident(ServerMolecule &m, hipStream_t stream) {
+        if (m.resident) return CFGMM_OK;
+        const size_t base = molecule_bytes(m);
+        if (!make_room(base, m.id)) return fail(CFGMM_EOOM, "server: molecule does not fit the HBM budget");
+        const int st = upload(m, stream);
+        if (st != CFGMM_OK) {
+            const std::string msg = cfgmm_last_error();  // discard() may overwrite it
+            discard(m, stream);
+            return fail(st, msg);
+        }
+        // the tiles replace D for evaluations; D and phi (small next to the tiles) stay for re-creation
+        m.bytes = base + (m.tiles ? resident_bytes(m.tiles) : 0);
+        m.resident = true;
+        held += m.bytes;
+        if (held > cfg.hbm_budget_bytes && !make_room(0, m.id)) {
+            evict(m);
+            return fail(CFGMM_EOOM, "server: molecule tiles do not fit the HBM budget");
+        }
         stats.uploads++;
         peak = std::max(peak, held);
         return CFGMM_OK;
@@ -472,6 +500,7 @@ int server_run(Server *sv, const cfgmm_request *reqs, size_t n, cfgmm_request_re
             if (st == CFGMM_OK) st = hip_status(hipEventRecord(sl.done, s), "server: event record");
             if (st != CFGMM_OK) {
                 status = st;
+                release(sl);  // the failed batch's buffers and its budget charge
                 break;
             }
             for (size_t r : batch) {
@@ -494,9 +523,12 @@ int server_run(Server *sv, const cfgmm_request *reqs, size_t n, cfgmm_request_re
             }
         }
         if (!progress) {
-            // nothing to do until a completion or the next arrival
+            // nothing to do until a completion or the next arrival: sleep only while no slot is
+            // busy (a completion is then seen at once) and the next arrival is far off
             const double wait = next < n ? reqs[order[next]].arrival_s - seconds_since(t0) : 0.0;
-            if (wait > 200e-6 && pending.empty())
+            bool any_busy = false;
+            for (auto &sl : sv->slots) any_busy = any_busy || sl.busy;
+            if (!any_busy && wait > 200e-6 && pending.empty())
                 std::this_thread::sleep_for(std::chrono::microseconds(100));
             else
                 std::this_thread::yield();

@@ -129,3 +129,46 @@ def test_tunables_validate():
         cfgmm.tune_set(cfgmm.TUNE_BATCH_WGS_PER_CU, 0)
     with pytest.raises(cfgmm.CfgmmError):
         cfgmm.tune_set(99, 1)
+
+
+def test_batched_overlapping_outputs_add_in_call_order():
+    """Problems whose rho ranges overlap (three problems into one buffer, a fourth offset by half a
+    row into it, a fifth elsewhere) add into rho in call order: bitwise the result of separate
+    calls one after another."""
+    probs = _problems(5, seed=23)
+    # the same grid for all: every problem writes its nset rows of ngrid into a shared buffer
+    ng = min(p.ngrid for p in probs)
+    probs = [W.rho_problem(seed=900 + i, nshell=40, nao=120, ngrid=ng, nset=p.nset, blksize=p.blksize,
+                           density=0.4) for i, p in enumerate(probs)]
+    rng = np.random.default_rng(4)
+    nrow = max(p.nset for p in probs)
+    shared0 = rng.standard_normal(nrow * ng + ng)
+    other0 = rng.standard_normal(nrow * ng)
+    # (buffer, offset in elements) per problem
+    where = [("shared", 0), ("other", 0), ("shared", 0), ("shared", ng // 2), ("shared", 0)]
+    s = cfgmm.Stream()
+    ups = [Uploaded(p, np.zeros(p.nset * p.ngrid)) for p in probs]
+
+    def run(batched):
+        bufs = {"shared": cfgmm.DeviceBuffer.from_array(shared0), "other": cfgmm.DeviceBuffer.from_array(other0)}
+        descs = []
+        for u, (name, off) in zip(ups, where):
+            d = u.desc()
+            d.rhos = bufs[name].ptr + 8 * off
+            descs.append(d)
+        if batched:
+            cfgmm.aodmao2rho_batched_dev(s, descs)
+        else:
+            for d in descs:
+                cfgmm.aodmao2rho_batched_dev(s, [d])
+        s.synchronize()
+        out = {k: b.to_array(np.float64, (b.nbytes // 8,)) for k, b in bufs.items()}
+        for b in bufs.values():
+            b.free()
+        return out
+
+    sep, bat = run(False), run(True)
+    assert np.array_equal(sep["shared"], bat["shared"])
+    assert np.array_equal(sep["other"], bat["other"])
+    assert not np.array_equal(sep["shared"], shared0)
+    s.destroy()

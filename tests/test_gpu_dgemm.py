@@ -129,3 +129,41 @@ def test_dgemm_small_plans(tile, splits, ta, tb):
     R = c_ref.reshape(301, ldc)
     assert rel_fro(G[:, :277], R[:, :277]) <= TOL
     assert np.array_equal(G[:, 277:], c.reshape(301, ldc)[:, 277:])
+
+
+def test_dgemm_large_nn_full_size():
+    """C(3527x9583) += A(3527x6581) B(6581x9583), NN, uniform[-1, 1) seed 0 (README.md:11-17): the
+    plan takes whole 128x128 tiles plus a K-split tail reduced in a fixed order."""
+    g = W.dgemm_large_nn(seed=0)
+    c_ref = np.zeros(g.m * g.ldc)
+    oracle.matmul(c_ref, g.a, g.b, g.m, g.n, g.k, g.lda, g.ldb, g.ldc, g.transa, g.transb)
+    s = cfgmm.Stream()
+    dA, dB = cfgmm.DeviceBuffer.from_array(g.a), cfgmm.DeviceBuffer.from_array(g.b)
+    dC = cfgmm.DeviceBuffer(8 * g.m * g.ldc)
+    dC.zero()
+    cfgmm.dgemm_dev(s, dC, dA, dB, g.m, g.n, g.k, g.lda, g.ldb, g.ldc, g.transa, g.transb)
+    s.synchronize()
+    c = dC.to_array(np.float64, (g.m * g.ldc,))
+    s.destroy()
+    err = rel_fro(c, c_ref)
+    print(f"dgemm 3527x9583x6581 NN rel_fro = {err:.3e}", flush=True)
+    assert err <= TOL
+    A, B = g.dense()
+    assert rel_fro(c.reshape(g.m, g.n), A @ B) <= TOL
+
+
+# The reference's own GEMM tests (src/impl_matmul.rs:250-350): a[x] = sin(x), b[x] = cos(x) over
+# the flat index, m = 3527, n = 9583, k = 6581, through matmul_anyway_full.
+@pytest.mark.parametrize("ta,tb", [(False, False), (False, True), (True, False)], ids=["NN", "NT", "TN"])
+def test_dgemm_reference_harness_sincos(ta, tb):
+    m, n, k = 3527, 9583, 6581
+    lda = m if ta else k
+    ldb = k if tb else n
+    a, b = W.sin_cos_operands((k if ta else m) * lda, (n if tb else k) * ldb)
+    c_ref = np.zeros(m * n)
+    oracle.matmul(c_ref, a, b, m, n, k, lda, ldb, n, ta, tb)
+    c = np.zeros(m * n)
+    cfgmm.matmul_anyway_full(c, a, b, m, n, k, lda, ldb, n, ta, tb)
+    err = rel_fro(c, c_ref)
+    print(f"dgemm sin/cos {m}x{n}x{k} trans={ta}{tb} rel_fro = {err:.3e}", flush=True)
+    assert err <= TOL

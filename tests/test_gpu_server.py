@@ -120,3 +120,38 @@ def test_server_rejects_bad_requests(pool):
         sv.run([bad])
     assert e.value.status == cfgmm.EINVAL_DIMS
     sv.destroy()
+
+
+def test_server_failed_upload_leaves_molecule_clean():
+    """A split-precision server whose budget holds molecule 0's D, phi and mask but not its tiles:
+    the request on it fails with EOOM after the tiles are built, the molecule is freed and left
+    non-resident with nothing charged, and the same server then serves the small molecule 1
+    with the result it gives under a large budget, and fails molecule 0 again the same way."""
+    pool = [W.Molecule(0, W.rho_problem(seed=31, nshell=100, nao=300, ngrid=64 * 48, nset=2, density=0.6)),
+            W.Molecule(1, W.rho_problem(seed=32, nshell=20, nao=60, ngrid=64 * 4, nset=2, density=0.5))]
+
+    def held_after_warm(prec, mid):
+        sv = make_server(pool, prec)
+        sv.warm([mid])
+        held = sv.stats()["held_bytes"]
+        sv.destroy()
+        return held
+    base0 = held_after_warm(cfgmm.PREC_FP64, 0)
+    tiles0 = held_after_warm(cfgmm.PREC_OZAKI_INT8, 0) - base0
+    full1 = held_after_warm(cfgmm.PREC_OZAKI_INT8, 1)
+    budget = base0 + tiles0 // 2
+    assert tiles0 > (8 << 20) and full1 + (4 << 20) < budget  # the sizes this test relies on
+    r0 = cfgmm.Request(0, 0, pool[0].prob.ngrid, 2, 0, 0, 0, 0, 0.0, None)
+    r1 = cfgmm.Request(0, 1, pool[1].prob.ngrid, 2, 0, 0, 0, 0, 0.0, None)
+    ref = make_server(pool, cfgmm.PREC_OZAKI_INT8)
+    want1 = ref.run([r1])[0].checksum
+    ref.destroy()
+    sv = make_server(pool, cfgmm.PREC_OZAKI_INT8, budget=budget)
+    for _ in range(2):
+        with pytest.raises(cfgmm.CfgmmError) as e:
+            sv.run([r0])
+        assert e.value.status == cfgmm.EOOM
+        assert sv.stats()["held_bytes"] == 0
+        res = sv.run([r1])
+        assert res[0].status == 0 and res[0].checksum == want1
+    sv.destroy()
