@@ -72,6 +72,7 @@ _SIGNATURES = {
                                        _I),
     "cfgmm_aodmao2rho_anyway": ([_VP, _SZ, _VP, _SZ, _VP, _SZ, _VP, _SZ] + [_SZ] * 5 + [_VP, _SZ, _SZ], _I),
     "cfgmm_aodmao2rho_dev": ([_VP, _VP, _VP, _VP, _VP] + [_SZ] * 5 + [_VP, _SZ, _SZ], _I),
+    "cfgmm_aodmao2rho_dev_noalloc": ([_VP, _VP, _VP, _VP, _VP] + [_SZ] * 5 + [_VP, _SZ, _SZ], _I),
     "cfgmm_aodmao2rho_ex_dev": ([_VP, _VP, _VP, _VP, _VP] + [_SZ] * 5 + [_VP, _SZ, _SZ, _I, _I], _I),
     "cfgmm_aodmao2rho_multi_dev": ([_VP, _VP, _SZ, _VP, _VP, _SZ, _SZ, _VP] + [_SZ] * 5 + [_VP, _SZ, _SZ], _I),
     "cfgmm_aogridao2mat_dev": ([_VP, _VP, _SZ, _VP, _VP, _VP] + [_SZ] * 3 + [_VP, _SZ, _SZ], _I),

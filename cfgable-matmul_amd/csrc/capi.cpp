@@ -52,9 +52,13 @@ struct TraceRec {
 static std::atomic<bool> g_trace_on{false};
 static std::mutex g_trace_mutex;
 static std::vector<TraceRec> g_trace;
+static thread_local int t_trace_off = 0;
+
+TraceOff::TraceOff() { ++t_trace_off; }
+TraceOff::~TraceOff() { --t_trace_off; }
 
 hipEvent_t trace_start(hipStream_t stream) {
-    if (!g_trace_on.load(std::memory_order_relaxed)) return nullptr;
+    if (t_trace_off || !g_trace_on.load(std::memory_order_relaxed)) return nullptr;
     hipEvent_t e = nullptr;
     if (hipEventCreate(&e) != hipSuccess) return nullptr;
     if (hipEventRecord(e, stream) != hipSuccess) {
@@ -577,6 +581,19 @@ int cfgmm_aodmao2rho_dev(void *stream, double *rhos, const double *dms, const do
                          const uint8_t *tab, size_t ldtab, size_t blksize) {
     return rho_dev_common(stream, rhos, dms, ao_lhs, ao_rhs, stride_ao, stride_grid, nao, ngrid, nset, tab, ldtab,
                           blksize, CFGMM_PREC_FP64, 0);
+}
+
+int cfgmm_aodmao2rho_dev_noalloc(void *stream, double *rhos, const double *dms, const double *ao_lhs,
+                                 const double *ao_rhs, size_t stride_ao, size_t stride_grid, size_t nao, size_t ngrid,
+                                 size_t nset, const uint8_t *tab, size_t ldtab, size_t blksize) {
+    clear_error();
+    RhoExtents ext;
+    CFGMM_TRY(rho_extents(stride_ao, stride_grid, nao, ngrid, nset, ldtab, blksize, &ext));
+    if (nao && ngrid && nset && (!rhos || !dms || !ao_lhs || !ao_rhs || !tab))
+        return fail(CFGMM_EINVAL, "null device pointer");
+    CFGMM_TRY(check_device());
+    return launch_rho_noalloc(static_cast<hipStream_t>(stream), rhos, dms, ao_lhs, ao_rhs, stride_ao, stride_grid,
+                              nao, ngrid, nset, tab, ldtab, blksize);
 }
 
 int cfgmm_aodmao2rho_multi_dev(void *stream, double *rhos, size_t rho_comp_stride, const double *dms,

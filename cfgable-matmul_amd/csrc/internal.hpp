@@ -55,6 +55,11 @@ int host_staging_release(hipStream_t stream);
 // trace_start returns nullptr unless tracing is on; trace_stop(nullptr) is a no-op.
 hipEvent_t trace_start(hipStream_t stream);
 void trace_stop(hipStream_t stream, const char *name, hipEvent_t start);
+// While one exists on a thread, trace_start returns nullptr there (no events are created).
+struct TraceOff {
+    TraceOff();
+    ~TraceOff();
+};
 
 // ---- tunables (cfgmm_tune_set / cfgmm_tune_get) ----------------------------------
 int tune_get(int param);
@@ -92,9 +97,21 @@ int launch_sgemm_non0tab(hipStream_t stream, float *c, const float *a, const flo
 int launch_rho(hipStream_t stream, double *rhos, const double *dms, const double *ao_lhs, const double *ao_rhs,
                size_t stride_ao, size_t stride_grid, size_t nao, size_t ngrid, size_t nset, const uint8_t *tab,
                size_t ldtab, size_t blksize);
+// Caller-provided scratch of one rho evaluation (instead of the stream's workspace); `epoch`
+// tags the symmetry flag the evaluation writes in it.
+struct RhoScratch {
+    void *ptr;
+    size_t bytes;
+    unsigned epoch;
+};
 int launch_rho_multi(hipStream_t stream, double *rhos, size_t rho_cs, const double *dms, const double *ao_lhs,
                      size_t lhs_cs, size_t ncomp, const double *ao_rhs, size_t stride_ao, size_t stride_grid,
-                     size_t nao, size_t ngrid, size_t nset, const uint8_t *tab, size_t ldtab, size_t blksize);
+                     size_t nao, size_t ngrid, size_t nset, const uint8_t *tab, size_t ldtab, size_t blksize,
+                     const RhoScratch *scratch = nullptr);
+// launch_rho with its scratch in the library's fixed device array (cfgmm_aodmao2rho_dev_noalloc)
+int launch_rho_noalloc(hipStream_t stream, double *rhos, const double *dms, const double *ao_lhs,
+                       const double *ao_rhs, size_t stride_ao, size_t stride_grid, size_t nao, size_t ngrid,
+                       size_t nset, const uint8_t *tab, size_t ldtab, size_t blksize);
 int launch_dgemm_ozaki(hipStream_t stream, double *c, const double *a, const double *b, size_t m, size_t n, size_t k,
                        size_t lda, size_t ldb, size_t ldc, bool transa, bool transb, int nslices);
 int launch_rho_batched(hipStream_t stream, const cfgmm_rho_problem *problems, size_t nprob);

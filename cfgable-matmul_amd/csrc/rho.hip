@@ -35,6 +35,11 @@
 #include "internal.hpp"
 #include "rho_body.hpp"
 
+#include <atomic>
+#include <map>
+#include <mutex>
+#include <string>
+
 namespace cfgmm {
 
 // Per grid block b: lists[b*nao + i] = i-th x with tab[x*ldtab + b] != 0, counts[b] = |L_b|.
@@ -361,7 +366,8 @@ int launch_rho(hipStream_t stream, double *rhos, const double *dms, const double
 // ncomp evaluations sharing D, rhs and the mask: rho_c = rhos + c*rho_cs, lhs_c = ao_lhs + c*lhs_cs.
 int launch_rho_multi(hipStream_t stream, double *rhos, size_t rho_cs, const double *dms, const double *ao_lhs,
                      size_t lhs_cs, size_t ncomp, const double *ao_rhs, size_t stride_ao, size_t stride_grid,
-                     size_t nao, size_t ngrid, size_t nset, const uint8_t *tab, size_t ldtab, size_t blksize) {
+                     size_t nao, size_t ngrid, size_t nset, const uint8_t *tab, size_t ldtab, size_t blksize,
+                     const RhoScratch *scratch) {
     if (nao == 0 || ngrid == 0 || nset == 0 || ncomp == 0) return CFGMM_OK;
     if (ncomp > 16) return fail(CFGMM_EINVAL, "rho: at most 16 left-hand components");
     if (nao > INT32_MAX || ngrid > INT32_MAX || nset > 65535) return fail(CFGMM_EINVAL, "rho: size out of range");
@@ -379,8 +385,17 @@ int launch_rho_multi(hipStream_t stream, double *rhos, size_t rho_cs, const doub
     const size_t part_cs = maxchunks * nset * ngrid;  // doubles per component
     const size_t part_bytes = sizeof(double) * part_cs * ncomp;
     void *ws = nullptr;
-    int st = workspace_get(stream, part_off + part_bytes, &ws);
-    if (st != CFGMM_OK) return st;
+    int st = CFGMM_OK;
+    if (scratch) {
+        if (part_off + part_bytes > scratch->bytes)
+            return fail(CFGMM_EINVAL, "rho: this problem needs " + std::to_string(part_off + part_bytes) +
+                                          " bytes of scratch; the allocation-free entry point has " +
+                                          std::to_string(scratch->bytes));
+        ws = scratch->ptr;
+    } else {
+        st = workspace_get(stream, part_off + part_bytes, &ws);
+        if (st != CFGMM_OK) return st;
+    }
     int *lists = static_cast<int *>(ws);
     int *counts = lists + nblk * nao;
     int *perm = counts + nblk;
@@ -399,7 +414,7 @@ int launch_rho_multi(hipStream_t stream, double *rhos, size_t rho_cs, const doub
     const size_t nt64 = div_up(nao, 64), npairs = nt64 * (nt64 + 1) / 2;
     const bool try_sym = ordered && ncomp == 1 && ao_lhs == ao_rhs && tune_get(CFGMM_TUNE_RHO_SYMMETRIC) == 0 &&
                          npairs * nset + 1 <= (size_t)INT32_MAX;
-    const unsigned epoch = try_sym ? workspace_epoch(stream) : 0u;
+    const unsigned epoch = try_sym ? (scratch ? scratch->epoch : workspace_epoch(stream)) : 0u;
     if (ordered) {
         hipEvent_t t0 = trace_start(stream);
         hipLaunchKernelGGL(rho_plan_kernel, dim3((unsigned)(1 + (try_sym ? npairs * nset : 0))), dim3(kPlanThreads),
@@ -441,6 +456,49 @@ int launch_rho_multi(hipStream_t stream, double *rhos, size_t rho_cs, const doub
         if (st != CFGMM_OK) return st;
     }
     return CFGMM_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Allocation-free evaluation (cfgmm_aodmao2rho_dev_noalloc): the AO lists, plan and chunk
+// partials of launch_rho live in `scratch`, a file-scope device array, instead of the stream's
+// workspace.  Each GPU holds its own copy at its own address; in host code the name `scratch`
+// only identifies the array, so its device address comes from hipGetSymbolAddress (once per
+// device, with that device current) and only that address is passed to kernels.  Every call on
+// a GPU shares the array: calls that overlap in time on one GPU race.  Nothing in it needs
+// clearing: each evaluation rewrites the lists, plan and partials it reads, and the symmetry
+// flag is tagged with a per-call epoch.
+constexpr size_t kRhoScratchBytes = CFGMM_RHO_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kRhoScratchBytes];
+
+static std::mutex g_scratch_mutex;
+static std::map<int, void *> g_scratch_addr;  // device -> address of its `scratch`
+static std::atomic<unsigned> g_scratch_epoch{0};
+
+static int scratch_address(void **out) {
+    int dev = 0;
+    CFGMM_HIP_TRY(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    auto it = g_scratch_addr.find(dev);
+    if (it == g_scratch_addr.end()) {
+        void *p = nullptr;
+        CFGMM_HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(CFGMM_EHIP, "hipGetSymbolAddress(scratch) returned no address");
+        it = g_scratch_addr.emplace(dev, p).first;
+    }
+    *out = it->second;
+    return CFGMM_OK;
+}
+
+int launch_rho_noalloc(hipStream_t stream, double *rhos, const double *dms, const double *ao_lhs,
+                       const double *ao_rhs, size_t stride_ao, size_t stride_grid, size_t nao, size_t ngrid,
+                       size_t nset, const uint8_t *tab, size_t ldtab, size_t blksize) {
+    void *p = nullptr;
+    const int st = scratch_address(&p);
+    if (st != CFGMM_OK) return st;
+    const TraceOff no_trace;  // library tracing would create events
+    const RhoScratch sc{p, kRhoScratchBytes, ++g_scratch_epoch};
+    return launch_rho_multi(stream, rhos, 0, dms, ao_lhs, 0, 1, ao_rhs, stride_ao, stride_grid, nao, ngrid, nset,
+                            tab, ldtab, blksize, &sc);
 }
 
 }  // namespace cfgmm

@@ -167,6 +167,22 @@ int cfgmm_aodmao2rho_dev(void *stream, double *rhos, const double *dms, const do
                          size_t stride_ao, size_t stride_grid, size_t nao, size_t ngrid, size_t nset,
                          const uint8_t *tab, size_t ldtab, size_t blksize);
 
+/* Allocation-free form of cfgmm_aodmao2rho_dev.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments and result as cfgmm_aodmao2rho_dev
+ * (FP64, the same kernels); it calls no hipMalloc or hipFree and creates no events.  Its
+ * scratch -- the compacted AO lists, the task plan and the chunk partials -- is a device array
+ * of CFGMM_RHO_NOALLOC_SCRATCH_BYTES in the library, one per GPU, held until the process ends.
+ * A problem needs about 4*nblk*(nao + 3*ceil(nao/64)) + 8*ceil(nao/64)*nset*ngrid bytes
+ * (nblk = ceil(ngrid/blksize); 16 MB for nao 2228, ngrid 16384, nset 3); a larger one returns
+ * CFGMM_EINVAL with the needed size in cfgmm_last_error().  Every call on a GPU shares that
+ * GPU's array: calls that overlap in time on one GPU (other streams or threads) race, and the
+ * caller must serialize them (one stream, or events between them). */
+#define CFGMM_RHO_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int cfgmm_aodmao2rho_dev_noalloc(void *stream, double *rhos, const double *dms, const double *ao_lhs,
+                                 const double *ao_rhs, size_t stride_ao, size_t stride_grid, size_t nao, size_t ngrid,
+                                 size_t nset, const uint8_t *tab, size_t ldtab, size_t blksize);
+
 /* Split-precision rho (no counterpart in the reference). */
 int cfgmm_aodmao2rho_ex_dev(void *stream, double *rhos, const double *dms, const double *ao_lhs,
                             const double *ao_rhs, size_t stride_ao, size_t stride_grid, size_t nao, size_t ngrid,
