@@ -417,12 +417,20 @@ static int rho_host(double *rhos, size_t rhos_len, const double *dms, size_t dms
     return stage_out(rhos, dr, ext.rhos);
 }
 
+// Arguments of a device-pointer GEMM; the device is checked by the caller after every argument.
 static int check_dev_gemm(const double *c, const double *a, const double *b, size_t m, size_t n, size_t k,
                           size_t lda, size_t ldb, size_t ldc, int transa, int transb) {
     GemmExtents ext;
     CFGMM_TRY(gemm_extents(m, n, k, lda, ldb, ldc, transa, transb, &ext));
     if (m && n && k && (!c || !a || !b)) return fail(CFGMM_EINVAL, "null device pointer");
-    return check_device();
+    return CFGMM_OK;
+}
+
+static int check_precision(int precision, int nslices) {
+    if (precision == CFGMM_PREC_FP64) return CFGMM_OK;
+    if (precision != CFGMM_PREC_OZAKI_INT8) return fail(CFGMM_EINVAL, "unknown precision mode " + std::to_string(precision));
+    if (nslices < 2 || nslices > 6) return fail(CFGMM_EINVAL, "nslices must be in [2, 6]");
+    return CFGMM_OK;
 }
 
 }  // namespace cfgmm
@@ -457,6 +465,7 @@ int cfgmm_dgemm_dev(void *stream, double *c, const double *a, const double *b, s
                     size_t lda, size_t ldb, size_t ldc, int transa, int transb) {
     clear_error();
     CFGMM_TRY(check_dev_gemm(c, a, b, m, n, k, lda, ldb, ldc, transa, transb));
+    CFGMM_TRY(check_device());
     return launch_dgemm(static_cast<hipStream_t>(stream), c, a, b, m, n, k, lda, ldb, ldc, transa != 0, transb != 0);
 }
 
@@ -464,14 +473,12 @@ int cfgmm_dgemm_ex_dev(void *stream, double *c, const double *a, const double *b
                        size_t lda, size_t ldb, size_t ldc, int transa, int transb, int precision, int nslices) {
     clear_error();
     CFGMM_TRY(check_dev_gemm(c, a, b, m, n, k, lda, ldb, ldc, transa, transb));
+    CFGMM_TRY(check_precision(precision, nslices));
+    CFGMM_TRY(check_device());
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (precision == CFGMM_PREC_FP64)
         return launch_dgemm(s, c, a, b, m, n, k, lda, ldb, ldc, transa != 0, transb != 0);
-    if (precision == CFGMM_PREC_OZAKI_INT8) {
-        if (nslices < 2 || nslices > 6) return fail(CFGMM_EINVAL, "nslices must be in [2, 6]");
-        return launch_dgemm_ozaki(s, c, a, b, m, n, k, lda, ldb, ldc, transa != 0, transb != 0, nslices);
-    }
-    return fail(CFGMM_EINVAL, "unknown precision mode " + std::to_string(precision));
+    return launch_dgemm_ozaki(s, c, a, b, m, n, k, lda, ldb, ldc, transa != 0, transb != 0, nslices);
 }
 
 // ---- masked GEMM -------------------------------------------------------------
@@ -502,6 +509,7 @@ static int masked_dev_common(void *stream, double *c, const double *a, const dou
         CFGMM_TRY(check_ld("ldtab", ldtab, div_up(n, blksize), true));
         if (!tab) return fail(CFGMM_EINVAL, "tab is null");
     }
+    CFGMM_TRY(check_device());
     return launch_dgemm_non0tab(static_cast<hipStream_t>(stream), c, a, b, m, n, k, lda, ldb, ldc, transa != 0,
                                 transb != 0, tab, ldtab, blksize);
 }
@@ -535,6 +543,7 @@ int cfgmm_sgemm_non0tab_dev(void *stream, float *c, const float *a, const float 
         CFGMM_TRY(check_ld("ldtab", ldtab, div_up(n, blksize), true));
         if (!tab) return fail(CFGMM_EINVAL, "tab is null");
     }
+    CFGMM_TRY(check_device());
     return launch_sgemm_non0tab(static_cast<hipStream_t>(stream), c, a, b, m, n, k, lda, ldb, ldc, transa != 0,
                                 transb != 0, tab, ldtab, blksize);
 }
@@ -564,16 +573,13 @@ static int rho_dev_common(void *stream, double *rhos, const double *dms, const d
     CFGMM_TRY(rho_extents(stride_ao, stride_grid, nao, ngrid, nset, ldtab, blksize, &ext));
     if (nao && ngrid && nset && (!rhos || !dms || !ao_lhs || !ao_rhs || !tab))
         return fail(CFGMM_EINVAL, "null device pointer");
+    CFGMM_TRY(check_precision(precision, nslices));
     CFGMM_TRY(check_device());
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (precision == CFGMM_PREC_FP64)
         return launch_rho(s, rhos, dms, ao_lhs, ao_rhs, stride_ao, stride_grid, nao, ngrid, nset, tab, ldtab, blksize);
-    if (precision == CFGMM_PREC_OZAKI_INT8) {
-        if (nslices < 2 || nslices > 6) return fail(CFGMM_EINVAL, "nslices must be in [2, 6]");
-        return launch_rho_ozaki(s, rhos, dms, ao_lhs, ao_rhs, stride_ao, stride_grid, nao, ngrid, nset, tab, ldtab,
-                                blksize, nslices);
-    }
-    return fail(CFGMM_EINVAL, "unknown precision mode " + std::to_string(precision));
+    return launch_rho_ozaki(s, rhos, dms, ao_lhs, ao_rhs, stride_ao, stride_grid, nao, ngrid, nset, tab, ldtab,
+                            blksize, nslices);
 }
 
 int cfgmm_aodmao2rho_dev(void *stream, double *rhos, const double *dms, const double *ao_lhs, const double *ao_rhs,
@@ -669,11 +675,11 @@ int cfgmm_aodmao2rho_batched_dev(void *stream, const cfgmm_rho_problem *problems
         if (q.nao && q.ngrid && q.nset && (!q.rhos || !q.dms || !q.ao_lhs || !q.ao_rhs || !q.tab))
             return fail(CFGMM_EINVAL, "problem " + std::to_string(i) + ": null device pointer");
     }
+    CFGMM_TRY(check_precision(precision, nslices));
     CFGMM_TRY(check_device());
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (precision == CFGMM_PREC_FP64) return launch_rho_batched(s, problems, nproblems);
-    if (precision == CFGMM_PREC_OZAKI_INT8) {
-        if (nslices < 2 || nslices > 6) return fail(CFGMM_EINVAL, "nslices must be in [2, 6]");
+    {
         for (size_t i = 0; i < nproblems; ++i) {
             const cfgmm_rho_problem &q = problems[i];
             CFGMM_TRY(launch_rho_ozaki(s, q.rhos, q.dms, q.ao_lhs, q.ao_rhs, q.stride_ao, q.stride_grid, q.nao,
@@ -681,7 +687,6 @@ int cfgmm_aodmao2rho_batched_dev(void *stream, const cfgmm_rho_problem *problems
         }
         return CFGMM_OK;
     }
-    return fail(CFGMM_EINVAL, "unknown precision mode " + std::to_string(precision));
 }
 
 // ---- resident rho operands -------------------------------------------------------

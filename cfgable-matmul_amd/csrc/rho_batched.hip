@@ -19,6 +19,7 @@
 // Every task writes its own partial slot, so the result does not depend on
 // which workgroup ran which task: it is bitwise the same as a per-problem call.
 #include "internal.hpp"
+#include "reduce_waves.hpp"
 #include "rho_body.hpp"
 
 #include <algorithm>
@@ -311,51 +312,6 @@ __global__ void batch_reduce(const BatchProblem *__restrict__ probs, const int *
     double v = 0.0;
     for (int ch = 0; ch < nch; ++ch) v += part[((long)ch * p.nset + s) * p.ngrid + g];
     p.rhos[(long)s * p.stride_grid + g] += v;
-}
-
-// Reduction passes of the problems (`index` = position in the call, 0..n-1): problem i goes
-// one pass after the last earlier problem whose rho range
-// [rhos, rhos + (nset-1)*stride_grid + ngrid) overlaps its own, so overlapping problems add in
-// call order and every other problem shares pass 0.  Returns the number of passes.
-struct RhoSpan {
-    uintptr_t lo, hi;
-    int index;
-};
-
-int assign_reduce_waves_spans(std::vector<RhoSpan> &spans, std::vector<int> &wave_of) {
-    const size_t n = spans.size();
-    wave_of.assign(n, 0);
-    std::vector<RhoSpan> by_lo(spans);
-    std::sort(by_lo.begin(), by_lo.end(), [](const RhoSpan &a, const RhoSpan &b) {
-        return a.lo != b.lo ? a.lo < b.lo : a.index < b.index;
-    });
-    bool any = false;
-    uintptr_t reach = 0;
-    for (size_t i = 0; i < n; ++i) {
-        if (i > 0 && by_lo[i].lo < reach) any = true;
-        reach = std::max(reach, by_lo[i].hi);
-    }
-    if (!any) return 1;
-    // prefix maximum of hi over by_lo: a scan to the left stops once no earlier span reaches lo
-    std::vector<uintptr_t> pmax(n);
-    std::vector<size_t> pos(n);
-    for (size_t i = 0; i < n; ++i) {
-        pmax[i] = std::max(i ? pmax[i - 1] : 0, by_lo[i].hi);
-        pos[by_lo[i].index] = i;
-    }
-    int nwaves = 1;
-    for (size_t idx = 0; idx < n; ++idx) {  // call order: earlier problems have their pass
-        const size_t p = pos[idx];
-        const RhoSpan me = by_lo[p];
-        int w = 0;
-        for (size_t j = p; j-- > 0 && pmax[j] > me.lo;)
-            if (by_lo[j].hi > me.lo && (size_t)by_lo[j].index < idx) w = std::max(w, wave_of[by_lo[j].index] + 1);
-        for (size_t j = p + 1; j < n && by_lo[j].lo < me.hi; ++j)
-            if ((size_t)by_lo[j].index < idx) w = std::max(w, wave_of[by_lo[j].index] + 1);
-        wave_of[idx] = w;
-        nwaves = std::max(nwaves, w + 1);
-    }
-    return nwaves;
 }
 
 int assign_reduce_waves(std::vector<BatchProblem> &bp) {
