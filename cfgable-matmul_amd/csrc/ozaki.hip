@@ -35,7 +35,6 @@ typedef long long i64x2 __attribute__((ext_vector_type(2)));
 
 constexpr int kThreads = 256;
 constexpr int kXcds = 8;
-constexpr int kOzBK = 64;      // K per int8 MFMA (16x16x64)
 constexpr int kOzKMax = 16384; // K per launch: 6 * 127^2 * 16384 < 2^31 keeps every level exact
 constexpr int kTile = 64;      // split-kernel tile (rows x cols)
 constexpr int kExpNonFinite = 0x7fffffff;
@@ -104,12 +103,37 @@ __device__ __host__ constexpr double level_weight(int l) {
     return w;
 }
 
-// q[s][r][c] (int8, rows_pad x cols_pad per slice, c contiguous) = s-th balanced
-// digit of x(r, c) * 2^-ex[r]; zero outside X.
+// Slice planes in the GEMM's LDS order.  An operand X (rows_pad x k_pad: the rows of op(A), or
+// the columns of op(B)) is stored as 4 KiB blocks, one per (plane s, 128-row tile t, 32-byte k
+// step q), at byte ((s * ntiles + t) * nks + q) * 4096; inside a block the 16-byte chunk holding
+// row rr, k bytes [16 h, 16 h + 16) sits at chunk index 2 rr + (h ^ ((rr >> 3) & 1)).  A stage of
+// the GEMM is then one contiguous copy per block into LDS, and the swizzle spreads the fragment
+// reads of 16 consecutive rows over all LDS banks.
+constexpr int kGT = 128;               // GEMM tile rows / columns
+constexpr int kGK = 32;                // k bytes per step (v_mfma_i32_32x32x32_i8)
+constexpr int kBlock = kGT * kGK;      // bytes of one (plane, tile, k step) block
+constexpr int kGThreads = 512;         // 8 waves: 2 x 4 wave tiles of 64 x 32
+
+// 16 bytes per lane from global memory straight into LDS at dst + 16 * lane (dst uniform across
+// the wave; global_load_lds_dwordx4).  Device-only: a kernel template that names the builtin in
+// its host pass loses its launch stub.
+__device__ __attribute__((always_inline)) inline void copy16_to_lds(const void *src, void *dst) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void *)dst, 16, 0, 0);
+#endif
+}
+
+__host__ __device__ inline long block_offset(int s, int t, int q, int ntiles, int nks) {
+    return (((long)s * ntiles + t) * nks + q) * kBlock;
+}
+__device__ inline int chunk_in_block(int rr, int h) { return 2 * rr + (h ^ ((rr >> 3) & 1)); }
+
+// Digit planes of X: the s-th balanced digit of x(r, c) * 2^-ex[r] in the block layout above;
+// zero outside X.  One workgroup per 64 x 64 tile of X; each thread cuts 16 consecutive k.
 template <int S>
 __global__ __launch_bounds__(kThreads) void slice_kernel(const double *__restrict__ x, long rs, long cs, int rows,
                                                          int cols, const int *__restrict__ ex, int8_t *__restrict__ q,
-                                                         long rows_pad, long cols_pad) {
+                                                         int ntiles, int nks) {
     __shared__ double t[kTile][kTile + 1];
     const int r0 = blockIdx.y * kTile, c0 = blockIdx.x * kTile;
     load_tile(t, x, rs, cs, rows, cols, r0, c0);
@@ -133,18 +157,20 @@ __global__ __launch_bounds__(kThreads) void slice_kernel(const double *__restric
             packed[s][j / 4] |= ((int)qv & 0xff) << (8 * (j % 4));
         }
     }
+    const int kc = (c0 + chunk * 16) >> 4;  // 16-byte k chunk
+    const long off = block_offset(0, gr / kGT, kc >> 1, ntiles, nks) + 16L * chunk_in_block(gr % kGT, kc & 1);
+    const long plane = (long)ntiles * nks * kBlock;
 #pragma unroll
     for (int s = 0; s < S; ++s) {
         i32x4 v = {packed[s][0], packed[s][1], packed[s][2], packed[s][3]};
-        *reinterpret_cast<i32x4 *>(q + ((long)s * rows_pad + gr) * cols_pad + c0 + chunk * 16) = v;
+        *reinterpret_cast<i32x4 *>(q + s * plane + off) = v;
     }
 }
 
 // ---------------------------------------------------------------------------
 // int8 slice GEMM with per-level int32 accumulators and f64 recombination
 // ---------------------------------------------------------------------------
-constexpr int kOzBM = 128, kOzBN = 64;  // workgroup tile: 2 x 2 waves of 64 x 32
-constexpr int kRowBytes = kOzBK;         // LDS row: 64 int8 of K, 16-byte chunks XOR-swizzled
+typedef int i32x16 __attribute__((ext_vector_type(16)));
 
 __device__ inline void raster_of(int L, int tiles_m, int tiles_n, int group_m, int &tm, int &tn) {
     const int in_group = group_m * tiles_n;
@@ -161,126 +187,151 @@ __device__ inline int xcd_remap(int pid, int total) {
     return pid < full ? (pid % kXcds) * (total / kXcds) + pid / kXcds : pid;
 }
 
-// byte offset of 16-byte chunk `ch` of LDS row `row` (swizzle spreads the 16
-// rows a ds_read_b128 phase touches over distinct banks)
-__device__ inline int swz(int row, int ch) { return row * kRowBytes + ((ch ^ ((row >> 2) & 3)) << 4); }
-
+// One 128 x 128 tile of C per workgroup of 8 waves; wave (wm, wn) owns rows wm*64.., columns
+// wn*32.. as two 32 x 32 v_mfma_i32_32x32x32_i8 tiles per level.  A stage (32 k bytes of every
+// digit plane of the A and B tiles, S * 8 KiB) goes from global memory straight into LDS
+// (global_load_lds_dwordx4: no registers held), three stages in flight: stage t+2 is issued
+// while stage t is computed.  Wave w copies, per plane, the 1 KiB part (w & 3) of the A (w < 4)
+// or B block.  Lane l of an MFMA operand holds row / column l % 32, k bytes 16 (l / 32) ..;
+// output element r of lane l is row 8 (r / 4) + 4 (l / 32) + r % 4, column l % 32.
 template <int S>
-__global__ __launch_bounds__(kThreads, 1) void ozaki_gemm_kernel(double *__restrict__ c, long ldc, int m, int n,
-                                                                 const int8_t *__restrict__ qa,
-                                                                 const int8_t *__restrict__ qb, long m_pad,
-                                                                 long n_pad, long k_pad, int k0, int k1,
-                                                                 const int *__restrict__ ea,
-                                                                 const int *__restrict__ eb, int group_m) {
-    constexpr int LEV = S;  // levels s + t = 0 .. S - 1
-    __shared__ __attribute__((aligned(16))) int8_t As[2][S][kOzBM * kRowBytes];
-    __shared__ __attribute__((aligned(16))) int8_t Bs[2][S][kOzBN * kRowBytes];
+__global__ __launch_bounds__(kGThreads, 1) void ozaki_gemm_kernel(
+    double *__restrict__ c, long ldc, int m, int n, const int8_t *__restrict__ qa, const int8_t *__restrict__ qb,
+    int ntiles_a, int ntiles_b, int nks, int q0, int q1, const int *__restrict__ ea, const int *__restrict__ eb,
+    int group_m) {
+    // one __shared__ object per stage, so that the compiler's wait for a stage's copies does not
+    // also wait for the copies into the other two
+    __shared__ __attribute__((aligned(16))) int8_t St0[2][S][kBlock];
+    __shared__ __attribute__((aligned(16))) int8_t St1[2][S][kBlock];
+    __shared__ __attribute__((aligned(16))) int8_t St2[2][S][kBlock];
 
-    const int tiles_m = (m + kOzBM - 1) / kOzBM, tiles_n = (n + kOzBN - 1) / kOzBN;
+    const int tiles_m = (m + kGT - 1) / kGT, tiles_n = (n + kGT - 1) / kGT;
     int tm, tn;
     raster_of(xcd_remap(blockIdx.x, gridDim.x), tiles_m, tiles_n, group_m, tm, tn);
-    const int m0 = tm * kOzBM, n0 = tn * kOzBN;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 2, wn = wave & 3;
 
-    const int tid = threadIdx.x;
-    const int lane = tid & 63, wave = tid >> 6;
-    const int wm = wave >> 1, wn = wave & 1;  // wave tile: rows wm*64.., cols wn*32..
-    const int lrow = lane & 15, lk = lane >> 4;
-
-    // global -> register staging: per slice, A tile = 512 chunks (2 per thread), B tile = 256 (1 per thread)
-    i32x4 ra[S][2], rb[S];
-    auto load_tiles = [&](int kk) {
-#pragma unroll
-        for (int s = 0; s < S; ++s) {
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const int e = tid + kThreads * h, row = e >> 2, ch = e & 3;
-                ra[s][h] = *reinterpret_cast<const i32x4 *>(qa + ((long)s * m_pad + m0 + row) * k_pad + kk + ch * 16);
-            }
-            const int row = tid >> 2, ch = tid & 3;
-            rb[s] = *reinterpret_cast<const i32x4 *>(qb + ((long)s * n_pad + n0 + row) * k_pad + kk + ch * 16);
-        }
-    };
-    auto store_tiles = [&](int buf) {
-#pragma unroll
-        for (int s = 0; s < S; ++s) {
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const int e = tid + kThreads * h, row = e >> 2, ch = e & 3;
-                *reinterpret_cast<i32x4 *>(&As[buf][s][swz(row, ch)]) = ra[s][h];
-            }
-            *reinterpret_cast<i32x4 *>(&Bs[buf][s][swz(tid >> 2, tid & 3)]) = rb[s];
-        }
-    };
-
-    i32x4 acc[LEV][4][2];
-#pragma unroll
-    for (int l = 0; l < LEV; ++l)
-#pragma unroll
-        for (int rt = 0; rt < 4; ++rt)
-#pragma unroll
-            for (int ct = 0; ct < 2; ++ct) acc[l][rt][ct] = i32x4{0, 0, 0, 0};
-
-    const int ntiles = (k1 - k0) / kOzBK;
-    load_tiles(k0);
-    store_tiles(0);
-    __syncthreads();
-    for (int t = 0; t < ntiles; ++t) {
-        const int cur = t & 1;
-        if (t + 1 < ntiles) load_tiles(k0 + (t + 1) * kOzBK);
-        // B fragments of every slice; A fragments one slice at a time.
-        i32x4 fb[S][2];
+    // this wave's copies: operand (wave >> 2), part (wave & 3) of each plane's block
+    const int op = wave >> 2, part = wave & 3;
+    const int8_t *__restrict__ src =
+        (op == 0 ? qa + block_offset(0, tm, 0, ntiles_a, nks) : qb + block_offset(0, tn, 0, ntiles_b, nks)) +
+        part * 1024 + lane * 16;
+    const long plane = (long)(op == 0 ? ntiles_a : ntiles_b) * nks * kBlock;
+    auto issue = [&](int q, int8_t (*st)[S][kBlock]) {
 #pragma unroll
         for (int s = 0; s < S; ++s)
+            copy16_to_lds(src + s * plane + (long)q * kBlock, &st[op][s][part * 1024]);
+    };
+
+    i32x16 acc[S][2];
 #pragma unroll
-            for (int ct = 0; ct < 2; ++ct)
-                fb[s][ct] = *reinterpret_cast<const i32x4 *>(&Bs[cur][s][swz(wn * 32 + ct * 16 + lrow, lk)]);
+    for (int l = 0; l < S; ++l)
+#pragma unroll
+        for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+            for (int e = 0; e < 16; ++e) acc[l][rt][e] = 0;
+
+    // fragment offsets (bytes) inside a block: rows wm*64 + rt*32 + lane%32 of A, columns
+    // wn*32 + lane%32 of B, chunk lane/32
+    const int h = lane >> 5;
+    int a_off[2];
+#pragma unroll
+    for (int rt = 0; rt < 2; ++rt) {
+        const int rr = wm * 64 + rt * 32 + (lane & 31);
+        a_off[rt] = 16 * chunk_in_block(rr, h);
+    }
+    const int b_off = 16 * chunk_in_block(wn * 32 + (lane & 31), h);
+    auto compute = [&](const int8_t (*st)[S][kBlock]) {
+        i32x4 fb[S];
+#pragma unroll
+        for (int sb = 0; sb < S; ++sb) fb[sb] = *reinterpret_cast<const i32x4 *>(&st[1][sb][b_off]);
 #pragma unroll
         for (int sa = 0; sa < S; ++sa) {
-            i32x4 fa[4];
+            i32x4 fa[2];
 #pragma unroll
-            for (int rt = 0; rt < 4; ++rt)
-                fa[rt] = *reinterpret_cast<const i32x4 *>(&As[cur][sa][swz(wm * 64 + rt * 16 + lrow, lk)]);
+            for (int rt = 0; rt < 2; ++rt) fa[rt] = *reinterpret_cast<const i32x4 *>(&st[0][sa][a_off[rt]]);
 #pragma unroll
-            for (int sb = 0; sb + sa < S; ++sb)  // level sa + sb (0-based) <= S - 1
+            for (int sb = 0; sb + sa < S; ++sb)
 #pragma unroll
-                for (int rt = 0; rt < 4; ++rt)
-#pragma unroll
-                    for (int ct = 0; ct < 2; ++ct)
-                        acc[sa + sb][rt][ct] =
-                            __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[rt], fb[sb][ct], acc[sa + sb][rt][ct], 0, 0, 0);
+                for (int rt = 0; rt < 2; ++rt)
+                    acc[sa + sb][rt] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[rt], fb[sb], acc[sa + sb][rt], 0, 0, 0);
         }
-        if (t + 1 < ntiles) store_tiles(cur ^ 1);
-        __syncthreads();
+    };
+    // Before stage t is read: this wave's copies of stage t have landed (S newer copies, those of
+    // stage t+1, may still be in flight), then the barrier makes that true of every wave -- and
+    // tells each wave that all have finished reading stage t-1, whose buffer the copies of stage
+    // t+2 refill.  s_waitcnt vmcnt(S): vmcnt in bits 3:0, expcnt and lgkmcnt left at their maxima.
+    auto sync = [&]() {
+        __builtin_amdgcn_s_waitcnt(S | (0x7 << 4) | (0xF << 8));
+        asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+    };
+    // Past the last step the copies re-read the last step into an idle buffer (never computed), so
+    // that the number of copies in flight, and the wait above, is the same every step.
+    const int nsteps = q1 - q0, last = q1 - 1;
+    issue(q0, St0);
+    issue(min(q0 + 1, last), St1);
+    int t = 0;
+    for (; t + 3 <= nsteps; t += 3) {
+        sync();
+        issue(min(q0 + t + 2, last), St2);
+        compute(St0);
+        sync();
+        issue(min(q0 + t + 3, last), St0);
+        compute(St1);
+        sync();
+        issue(min(q0 + t + 4, last), St1);
+        compute(St2);
     }
+    if (t < nsteps) {
+        sync();
+        issue(last, St2);
+        compute(St0);
+    }
+    if (t + 1 < nsteps) {
+        sync();
+        compute(St1);
+    }
+    __builtin_amdgcn_s_waitcnt(0);  // no copy into LDS outlives the workgroup
 
-    // Recombination: C += 2^(ea+eb) * sum_l acc[l] / (127^2 254^l), smallest level first.
-    // i32 16x16 output layout: lane l, element r holds D[4*(l/16) + r][l%16].
+    // Recombination: C += 2^(ea+eb) * sum_l acc[l] / (127^2 254^l), smallest level first.  The
+    // loads of a row tile's exponents and C values are issued together (clamped in-bounds
+    // addresses), so that the workgroup waits for one load latency, not one per element.
+    const int m0 = tm * kGT, n0 = tn * kGT;
+    const int col = n0 + wn * 32 + (lane & 31);
+    const bool col_ok = col < n;
+    const int colc = col_ok ? col : n - 1;
+    const int e2 = eb[colc];
 #pragma unroll
-    for (int rt = 0; rt < 4; ++rt)
+    for (int rt = 0; rt < 2; ++rt) {
+        int e1[16];
+        double cv[16];
 #pragma unroll
-        for (int ct = 0; ct < 2; ++ct)
+        for (int r = 0; r < 16; ++r) {
+            const int row = m0 + wm * 64 + rt * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
+            const int rowc = row < m ? row : m - 1;
+            e1[r] = ea[rowc];
+            cv[r] = c[(long)rowc * ldc + colc];
+        }
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int row = m0 + wm * 64 + rt * 16 + 4 * lk + r;
-                const int col = n0 + wn * 32 + ct * 16 + lrow;
-                if (row < m && col < n) {
-                    double v = 0.0;
+        for (int r = 0; r < 16; ++r) {
+            const int row = m0 + wm * 64 + rt * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
+            double v = 0.0;
 #pragma unroll
-                    for (int l = LEV - 1; l >= 0; --l) v += (double)acc[l][rt][ct][r] * level_weight(l);
-                    const int e1 = ea[row], e2 = eb[col];
-                    v = (e1 == kExpNonFinite || e2 == kExpNonFinite) ? __builtin_nan("") : ldexp(v, e1 + e2);
-                    c[(long)row * ldc + col] += v;
-                }
-            }
+            for (int l = S - 1; l >= 0; --l) v += (double)acc[l][rt][r] * level_weight(l);
+            v = (e1[r] == kExpNonFinite || e2 == kExpNonFinite) ? __builtin_nan("") : ldexp(v, e1[r] + e2);
+            if (row < m && col_ok) c[(long)row * ldc + col] = cv[r] + v;
+        }
+    }
 }
 
 template <int S>
 int run_ozaki(hipStream_t stream, double *c, const double *a, const double *b, int m, int n, int k, long lda, long ldb,
               long ldc, bool ta, bool tb) {
-    const long m_pad = (long)div_up(m, kOzBM) * kOzBM;
-    const long n_pad = (long)div_up(n, kOzBN) * kOzBN;
-    const long k_pad = (long)div_up(k, kOzBK) * kOzBK;
-    // workspace: qa [S][m_pad][k_pad], qb [S][n_pad][k_pad], rowmax (m+n) u64, exponents (m+n) int
+    const long m_pad = (long)div_up(m, kGT) * kGT;
+    const long n_pad = (long)div_up(n, kGT) * kGT;
+    const long k_pad = (long)div_up(k, kTile) * kTile;  // the slicing grid's k tile; a multiple of kGK
+    const int ntiles_a = (int)(m_pad / kGT), ntiles_b = (int)(n_pad / kGT), nks = (int)(k_pad / kGK);
+    // workspace: qa [S][m_pad x k_pad], qb [S][n_pad x k_pad] (block layout), rowmax (m+n) u64, exponents (m+n) int
     const size_t qa_bytes = (size_t)S * m_pad * k_pad, qb_bytes = (size_t)S * n_pad * k_pad;
     const size_t off_qb = div_up(qa_bytes, 256) * 256;
     const size_t off_max = off_qb + div_up(qb_bytes, 256) * 256;
@@ -307,19 +358,20 @@ int run_ozaki(hipStream_t stream, double *c, const double *a, const double *b, i
     hipLaunchKernelGGL(row_exponent_kernel, dim3((unsigned)div_up(m + n, 256)), dim3(256), 0, stream, rmax, ex,
                        m + n);
     hipLaunchKernelGGL((slice_kernel<S>), dim3((unsigned)(k_pad / kTile), (unsigned)(m_pad / kTile)), dim3(kThreads),
-                       0, stream, a, a_rs, a_cs, m, k, ex, qa, m_pad, k_pad);
+                       0, stream, a, a_rs, a_cs, m, k, ex, qa, ntiles_a, nks);
     hipLaunchKernelGGL((slice_kernel<S>), dim3((unsigned)(k_pad / kTile), (unsigned)(n_pad / kTile)), dim3(kThreads),
-                       0, stream, b, b_rs, b_cs, n, k, ex + m, qb, n_pad, k_pad);
+                       0, stream, b, b_rs, b_cs, n, k, ex + m, qb, ntiles_b, nks);
     trace_stop(stream, "ozaki_split", t0);
     st = hip_status(hipGetLastError(), "ozaki split launch");
     if (st != CFGMM_OK) return st;
 
-    const int tiles = (int)((m_pad / kOzBM) * (n_pad / kOzBN));
-    for (long k0 = 0; k0 < k_pad; k0 += kOzKMax) {
-        const long k1 = std::min<long>(k_pad, k0 + kOzKMax);
+    const int tiles = ntiles_a * ntiles_b;
+    const int ks_max = kOzKMax / kGK;  // k steps per launch that keep every level sum exact
+    for (int q0 = 0; q0 < nks; q0 += ks_max) {
+        const int q1 = std::min(nks, q0 + ks_max);
         t0 = trace_start(stream);
-        hipLaunchKernelGGL((ozaki_gemm_kernel<S>), dim3((unsigned)tiles), dim3(kThreads), 0, stream, c, ldc, m, n, qa,
-                           qb, m_pad, n_pad, k_pad, (int)k0, (int)k1, ex, ex + m, 4);
+        hipLaunchKernelGGL((ozaki_gemm_kernel<S>), dim3((unsigned)tiles), dim3(kGThreads), 0, stream, c, ldc, m, n, qa,
+                           qb, ntiles_a, ntiles_b, nks, q0, q1, ex, ex + m, 4);
         trace_stop(stream, "ozaki_gemm_kernel", t0);
         st = hip_status(hipGetLastError(), "ozaki_gemm_kernel launch");
         if (st != CFGMM_OK) return st;
