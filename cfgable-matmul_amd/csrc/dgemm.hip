@@ -165,18 +165,30 @@ __global__ __launch_bounds__(kThreads, 2) void dgemm_kernel(double *__restrict__
         __syncthreads();
     }
 
-    // Epilogue: C += acc (one owner per element), or store the partial tile.
+    // Epilogue: C += acc (one owner per element), or store the partial tile.  A fragment row's C
+    // values are loaded together from clamped, in-bounds addresses, so that the workgroup waits
+    // for one load latency per fragment row instead of one per element.
     if (pid < dp_tiles) {
 #pragma unroll
-        for (int mi = 0; mi < FM; ++mi)
+        for (int mi = 0; mi < FM; ++mi) {
+            double cv[FN][4];
+#pragma unroll
+            for (int ni = 0; ni < FN; ++ni)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = min(m0 + wm * WM + mi * 16 + lk + 4 * r, m - 1);
+                    const int col = min(n0 + wn * WN + ni * 16 + lrow, n - 1);
+                    cv[ni][r] = c[(long)row * ldc + col];
+                }
 #pragma unroll
             for (int ni = 0; ni < FN; ++ni)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int row = m0 + wm * WM + mi * 16 + lk + 4 * r;
                     const int col = n0 + wn * WN + ni * 16 + lrow;
-                    if (row < m && col < n) c[(long)row * ldc + col] += acc[mi][ni][r];
+                    if (row < m && col < n) c[(long)row * ldc + col] = cv[ni][r] + acc[mi][ni][r];
                 }
+        }
     } else {
         double *__restrict__ pt = partial + ((long)(pid - dp_tiles)) * (BM * BN);  // [tail tile][split][BM][BN]
 #pragma unroll

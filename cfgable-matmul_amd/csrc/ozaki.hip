@@ -40,12 +40,20 @@ constexpr int kTile = 64;      // split-kernel tile (rows x cols)
 constexpr int kExpNonFinite = 0x7fffffff;
 
 // ---------------------------------------------------------------------------
-// splitting: X is a logical rows x cols matrix, x(r, c) = x[r*rs + c*cs]
+// splitting: X is a logical rows x cols matrix, x(r, c) = x[r*rs + c*cs]; the two operands of
+// a call (the rows of op(A) and the columns of op(B)) are sliced by the same launches,
+// blockIdx.z selecting the operand.
 // ---------------------------------------------------------------------------
-__global__ void zero_u64_kernel(unsigned long long *__restrict__ p, int n) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0ull;
-}
+struct SliceOperand {
+    const double *x;
+    long rs, cs;
+    int rows;
+    int ntiles;                 // 128-row tiles of the digit planes
+    unsigned long long *pmax;   // [rows][nkc] partial row maxima
+    int *ex;                    // [rows] row exponents
+    int8_t *q;                  // digit planes
+};
+constexpr int kMaxChunk = 512;  // k columns per partial row maximum
 
 // Loads the 64x64 tile (r0, c0) of X into LDS, coalesced along whichever of
 // the two directions is contiguous in memory; entries outside X are zero.
@@ -62,22 +70,64 @@ __device__ inline void load_tile(double (*t)[kTile + 1], const double *__restric
     }
 }
 
-// rowmax[r] = max(rowmax[r], max_c |x(r, c)|), as the bit pattern of a
-// non-negative double (whose integer order equals its numeric order; NaN
-// patterns sort above Inf).
-__global__ __launch_bounds__(kThreads) void row_absmax_kernel(const double *__restrict__ x, long rs, long cs, int rows,
-                                                              int cols, unsigned long long *__restrict__ rowmax) {
-    __shared__ double t[kTile][kTile + 1];
-    const int r0 = blockIdx.y * kTile, c0 = blockIdx.x * kTile;
-    load_tile(t, x, rs, cs, rows, cols, r0, c0);
-    __syncthreads();
-    if (threadIdx.x < kTile && r0 + (int)threadIdx.x < rows) {
-        unsigned long long m = 0ull;
-        for (int c = 0; c < kTile; ++c) {
-            const unsigned long long bits = (unsigned long long)__double_as_longlong(fabs(t[threadIdx.x][c]));
-            m = bits > m ? bits : m;
+__device__ inline unsigned long long absbits(double v) {
+    return (unsigned long long)__double_as_longlong(fabs(v));
+}
+
+// pmax[r][kc] = max over columns [kc*512, kc*512 + 512) of |x(r, c)|, as the bit pattern of a
+// non-negative double (whose integer order is its numeric order; NaN patterns sort above Inf).
+// Workgroup (kc, 64-row block, operand); each thread keeps the maxima of the rows it loads and
+// the workgroup combines them -- no atomics, no zeroing.
+__global__ __launch_bounds__(kThreads) void row_absmax_kernel(SliceOperand opa, SliceOperand opb, int cols, int nkc) {
+    const SliceOperand &o = blockIdx.z == 0 ? opa : opb;
+    const int r0 = blockIdx.y * kTile, c0 = blockIdx.x * kMaxChunk;
+    if (r0 >= o.rows) return;
+    __shared__ unsigned long long red[kThreads / 64][kTile];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int c1 = min(cols, c0 + kMaxChunk);
+    if (o.cs == 1) {
+        // k contiguous: a wave reads 64 consecutive columns of rows wave, wave+4, ...
+        unsigned long long m[kTile / 4];
+#pragma unroll
+        for (int i = 0; i < kTile / 4; ++i) m[i] = 0ull;
+        for (int c = c0 + lane; c < c1; c += 64)
+#pragma unroll
+            for (int i = 0; i < kTile / 4; ++i) {
+                const int r = r0 + wave + 4 * i;
+                if (r < o.rows) {
+                    const unsigned long long b = absbits(o.x[(long)r * o.rs + c]);
+                    m[i] = b > m[i] ? b : m[i];
+                }
+            }
+#pragma unroll
+        for (int i = 0; i < kTile / 4; ++i) {
+            unsigned long long v = m[i];
+#pragma unroll
+            for (int d = 32; d >= 1; d >>= 1) {
+                const unsigned long long u = __shfl_xor(v, d);
+                v = u > v ? u : v;
+            }
+            if (lane == 0) red[0][wave + 4 * i] = v;
         }
-        atomicMax(rowmax + r0 + threadIdx.x, m);
+        __syncthreads();
+        if (tid < kTile && r0 + tid < o.rows) o.pmax[(long)(r0 + tid) * nkc + blockIdx.x] = red[0][tid];
+    } else {
+        // rows contiguous: lane = row, the waves split the columns
+        unsigned long long m = 0ull;
+        const int r = r0 + lane;
+        if (r < o.rows)
+            for (int c = c0 + wave; c < c1; c += kThreads / 64) {
+                const unsigned long long b = absbits(o.x[(long)r * o.rs + (long)c * o.cs]);
+                m = b > m ? b : m;
+            }
+        red[wave][lane] = m;
+        __syncthreads();
+        if (tid < kTile && r0 + tid < o.rows) {
+            unsigned long long v = red[0][tid];
+#pragma unroll
+            for (int w = 1; w < kThreads / 64; ++w) v = red[w][tid] > v ? red[w][tid] : v;
+            o.pmax[(long)(r0 + tid) * nkc + blockIdx.x] = v;
+        }
     }
 }
 
@@ -91,9 +141,19 @@ __device__ inline int row_exponent(unsigned long long maxbits) {
     return e;
 }
 
-__global__ void row_exponent_kernel(const unsigned long long *__restrict__ rowmax, int *__restrict__ ex, int rows) {
+// ex[r] from the partial maxima of row r, both operands (rows of op(A) first).
+__global__ void row_exponent_kernel(SliceOperand opa, SliceOperand opb, int nkc) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < rows) ex[i] = row_exponent(rowmax[i]);
+    const bool b = i >= opa.rows;
+    const SliceOperand &o = b ? opb : opa;
+    const int r = b ? i - opa.rows : i;
+    if (r >= o.rows) return;
+    unsigned long long v = 0ull;
+    for (int kc = 0; kc < nkc; ++kc) {
+        const unsigned long long u = o.pmax[(long)r * nkc + kc];
+        v = u > v ? u : v;
+    }
+    o.ex[r] = row_exponent(v);
 }
 
 // Weight of the digit products of level l = s + t: 1 / (127^2 254^l).
@@ -131,12 +191,15 @@ __device__ inline int chunk_in_block(int rr, int h) { return 2 * rr + (h ^ ((rr 
 // Digit planes of X: the s-th balanced digit of x(r, c) * 2^-ex[r] in the block layout above;
 // zero outside X.  One workgroup per 64 x 64 tile of X; each thread cuts 16 consecutive k.
 template <int S>
-__global__ __launch_bounds__(kThreads) void slice_kernel(const double *__restrict__ x, long rs, long cs, int rows,
-                                                         int cols, const int *__restrict__ ex, int8_t *__restrict__ q,
-                                                         int ntiles, int nks) {
+__global__ __launch_bounds__(kThreads) void slice_kernel(SliceOperand opa, SliceOperand opb, int cols, int nks) {
+    const SliceOperand &o = blockIdx.z == 0 ? opa : opb;
+    const int rows = o.rows, ntiles = o.ntiles;
+    const int *__restrict__ ex = o.ex;
+    int8_t *__restrict__ q = o.q;
     __shared__ double t[kTile][kTile + 1];
     const int r0 = blockIdx.y * kTile, c0 = blockIdx.x * kTile;
-    load_tile(t, x, rs, cs, rows, cols, r0, c0);
+    if (r0 >= ntiles * kGT) return;  // past this operand's padded rows (uniform)
+    load_tile(t, o.x, o.rs, o.cs, rows, cols, r0, c0);
     __syncthreads();
     const int r = threadIdx.x / 4, chunk = threadIdx.x % 4;  // 16 consecutive columns per thread
     const int gr = r0 + r;
@@ -331,36 +394,33 @@ int run_ozaki(hipStream_t stream, double *c, const double *a, const double *b, i
     const long n_pad = (long)div_up(n, kGT) * kGT;
     const long k_pad = (long)div_up(k, kTile) * kTile;  // the slicing grid's k tile; a multiple of kGK
     const int ntiles_a = (int)(m_pad / kGT), ntiles_b = (int)(n_pad / kGT), nks = (int)(k_pad / kGK);
-    // workspace: qa [S][m_pad x k_pad], qb [S][n_pad x k_pad] (block layout), rowmax (m+n) u64, exponents (m+n) int
+    // workspace: qa [S][m_pad x k_pad], qb [S][n_pad x k_pad] (block layout), partial row maxima
+    // (m+n) x nkc u64, exponents (m+n) int
+    const int nkc = (int)div_up(k, kMaxChunk);
     const size_t qa_bytes = (size_t)S * m_pad * k_pad, qb_bytes = (size_t)S * n_pad * k_pad;
     const size_t off_qb = div_up(qa_bytes, 256) * 256;
     const size_t off_max = off_qb + div_up(qb_bytes, 256) * 256;
-    const size_t off_ex = off_max + div_up(sizeof(unsigned long long) * (m + n), 256) * 256;
+    const size_t off_ex = off_max + div_up(sizeof(unsigned long long) * (m + n) * nkc, 256) * 256;
     const size_t total = off_ex + sizeof(int) * (m + n);
     void *ws = nullptr;
     int st = workspace_get(stream, total, &ws);
     if (st != CFGMM_OK) return st;
     char *base = static_cast<char *>(ws);
+    unsigned long long *pmax = reinterpret_cast<unsigned long long *>(base + off_max);
+    int *ex = reinterpret_cast<int *>(base + off_ex);
     int8_t *qa = reinterpret_cast<int8_t *>(base);
     int8_t *qb = reinterpret_cast<int8_t *>(base + off_qb);
-    unsigned long long *rmax = reinterpret_cast<unsigned long long *>(base + off_max);
-    int *ex = reinterpret_cast<int *>(base + off_ex);
 
     // op(A) rows: (i, p) -> a[i*lda + p] or a[p*lda + i]; op(B)^T rows: (j, p) -> b[p*ldb + j] or b[j*ldb + p]
-    const long a_rs = ta ? 1 : lda, a_cs = ta ? lda : 1;
-    const long b_rs = tb ? ldb : 1, b_cs = tb ? 1 : ldb;
+    const SliceOperand oa{a, ta ? 1 : lda, ta ? lda : 1, m, ntiles_a, pmax, ex, qa};
+    const SliceOperand ob{b, tb ? ldb : 1, tb ? 1 : ldb, n, ntiles_b, pmax + (size_t)m * nkc, ex + m, qb};
+    const unsigned row_blocks = (unsigned)(std::max(m_pad, n_pad) / kTile);
     hipEvent_t t0 = trace_start(stream);
-    hipLaunchKernelGGL(zero_u64_kernel, dim3((unsigned)div_up(m + n, 256)), dim3(256), 0, stream, rmax, m + n);
-    hipLaunchKernelGGL(row_absmax_kernel, dim3((unsigned)div_up(k, kTile), (unsigned)div_up(m, kTile)),
-                       dim3(kThreads), 0, stream, a, a_rs, a_cs, m, k, rmax);
-    hipLaunchKernelGGL(row_absmax_kernel, dim3((unsigned)div_up(k, kTile), (unsigned)div_up(n, kTile)),
-                       dim3(kThreads), 0, stream, b, b_rs, b_cs, n, k, rmax + m);
-    hipLaunchKernelGGL(row_exponent_kernel, dim3((unsigned)div_up(m + n, 256)), dim3(256), 0, stream, rmax, ex,
-                       m + n);
-    hipLaunchKernelGGL((slice_kernel<S>), dim3((unsigned)(k_pad / kTile), (unsigned)(m_pad / kTile)), dim3(kThreads),
-                       0, stream, a, a_rs, a_cs, m, k, ex, qa, ntiles_a, nks);
-    hipLaunchKernelGGL((slice_kernel<S>), dim3((unsigned)(k_pad / kTile), (unsigned)(n_pad / kTile)), dim3(kThreads),
-                       0, stream, b, b_rs, b_cs, n, k, ex + m, qb, ntiles_b, nks);
+    hipLaunchKernelGGL(row_absmax_kernel, dim3((unsigned)nkc, row_blocks, 2), dim3(kThreads), 0, stream, oa, ob, k,
+                       nkc);
+    hipLaunchKernelGGL(row_exponent_kernel, dim3((unsigned)div_up(m + n, 256)), dim3(256), 0, stream, oa, ob, nkc);
+    hipLaunchKernelGGL((slice_kernel<S>), dim3((unsigned)(k_pad / kTile), row_blocks, 2), dim3(kThreads), 0, stream,
+                       oa, ob, k, nks);
     trace_stop(stream, "ozaki_split", t0);
     st = hip_status(hipGetLastError(), "ozaki split launch");
     if (st != CFGMM_OK) return st;
