@@ -19,7 +19,10 @@
 //   Rq[b][p](c, i)     p-th slice of rhs[L_b[i]][b*blksize + c] * 2^-fg[g],
 //                      fg = exponent of the column maximum over the block's AOs;
 //                      stored [i/64][c][i%64] -- one 64-k step of all columns of
-//                      the block is blksize*64 contiguous bytes;
+//                      the block is blksize*64 contiguous bytes -- with the four
+//                      16-byte chunks of each 64-byte column row permuted (chunk
+//                      j at j ^ ((c >> 2) & 3)), the order the evaluation reads
+//                      them in from LDS, so that a stage is a plain copy;
 //   Lc[b][i][c]        lhs[L_b[i]][b*blksize + c] in f64 (zero padded).
 //
 // An evaluation then streams Dq once: each workgroup owns 64 compacted rows of
@@ -110,6 +113,21 @@ __device__ inline void slice4(const double x[4], const int e[4], unsigned w[S]) 
             w[p] |= ((unsigned)(int)t & 0xffu) << (8 * q);
         }
     }
+}
+
+// Byte offset of 16-byte chunk `ch` of the 64-byte row `row` (a column of a B tile) in LDS: the
+// chunks are XOR-permuted against bank conflicts of the ds_read_b128 fragment reads.
+__device__ inline int swz(int row, int ch) { return row * kK + ((ch ^ ((row >> 2) & 3)) << 4); }
+// The same permutation inside one row: position of byte k (0..63) of column c's row.
+__device__ inline int swz_in_row(int c, int k) { return (((k >> 4) ^ ((c >> 2) & 3)) << 4) | (k & 15); }
+
+// 16 bytes per lane from global memory straight into LDS at dst + 16 * lane (dst uniform across
+// the wave; global_load_lds_dwordx4).  Device-only: a kernel template that names the builtin in
+// its host pass loses its launch stub.
+__device__ __attribute__((always_inline)) inline void copy16_to_lds(const void *src, void *dst) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void *)dst, 16, 0, 0);
+#endif
 }
 
 // One wave per compacted row i of block b and set s: ed and the S slice planes of the row.
@@ -208,7 +226,8 @@ __global__ __launch_bounds__(kThreads) void prep_phi_kernel(unsigned char *__res
         slice4<S>(x, ee, w);
 #pragma unroll
         for (int p = 0; p < S; ++p)
-            *reinterpret_cast<unsigned *>(rq + p * plane + ((long)(i >> 6) * blksize + c) * kK + (i & 63)) = w[p];
+            *reinterpret_cast<unsigned *>(rq + p * plane + ((long)(i >> 6) * blksize + c) * kK + swz_in_row(c, i & 63)) =
+                w[p];
     }
     // Lc[i][c]
     double *__restrict__ lc = reinterpret_cast<double *>(base + bd.l_off);
@@ -218,8 +237,7 @@ __global__ __launch_bounds__(kThreads) void prep_phi_kernel(unsigned char *__res
     }
 }
 
-// byte offset of 16-byte chunk `ch` of a 64-byte LDS row (XOR swizzle against ds_read_b128 conflicts)
-__device__ inline int swz(int row, int ch) { return row * kK + ((ch ^ ((row >> 2) & 3)) << 4); }
+
 
 // Task t of an evaluation: tasks[t] = {block, row tile, column subtile}.  Workgroup i takes task
 // start_x + i/8 with x = i%8, so consecutive tasks (the tiles of one block, which share its Rq)
@@ -238,9 +256,11 @@ __device__ __attribute__((always_inline)) inline void resident_tile(
     double *__restrict__ partial, const unsigned char *__restrict__ base, const BlockDesc *__restrict__ blocks,
     int4 task, int s, const int *__restrict__ fg, int blksize, int ngrid, int nset) {
     constexpr int W = NT * 16;
-    constexpr int kBChunks = S * W * (kK / 16);  // 16-byte chunks of one B tile
-    constexpr int kBPer = (kBChunks + kThreads - 1) / kThreads;
-    __shared__ __attribute__((aligned(16))) int8_t Bs[2][S][W * kK];
+    // the two B stages are __shared__ objects of their own: the compiler's wait before reading one
+    // then does not also wait for the copies into the other
+    __shared__ __attribute__((aligned(16))) int8_t Bs0[S][W * kK];
+    __shared__ __attribute__((aligned(16))) int8_t Bs1[S][W * kK];
+    __shared__ __attribute__((aligned(16))) int8_t Bs2[DEPTH >= 2 ? S : 1][W * kK];
     __shared__ double red[kThreads / 64][W];
 
     const int b = task.x, sub = task.z;
@@ -286,42 +306,34 @@ __device__ __attribute__((always_inline)) inline void resident_tile(
     };
     zero_acc();
 
-    i32x4 fa0[S], fa1[S], fa2[S], breg[kBPer];
+    i32x4 fa0[S], fa1[S], fa2[S];
     auto load_a = [&](int t, i32x4 *dst) {
         const int8_t *__restrict__ ar = (t < n1 ? arow1 : arow2) + (long)a_of(t) * 1024;
 #pragma unroll
         for (int p = 0; p < S; ++p)  // each A tile is read once per evaluation: stream it past the caches
             dst[p] = __builtin_nontemporal_load(reinterpret_cast<const i32x4 *>(ar + p * dplane));
     };
-    // B tile of step t: S planes x W columns x 64 k, contiguous per plane
-    auto load_b = [&](int t) {
-        const int ks = ks_of(t);
+    // B stage of step t: per plane, the W columns x 64 k of its k step are NT KiB contiguous in Rq,
+    // already in LDS order; copied straight into LDS, 1 KiB per wave instruction.  Every wave issues
+    // the same number of copies (the last part repeated where S*NT is not a multiple of 4), so that
+    // the wait below counts the same on every wave.
+    constexpr int kBParts = S * NT, kBPerWave = (kBParts + 3) / 4;
+    auto issue_b = [&](int t, int8_t (*dst)[W * kK]) {
+        const long ks = ks_of(t);
 #pragma unroll
-        for (int e = 0; e < kBPer; ++e) {
-            const int q = tid + e * kThreads;
-            if (kBChunks % kThreads == 0 || q < kBChunks) {
-                const int p = q / (W * 4), rem = q % (W * 4);
-                breg[e] = *reinterpret_cast<const i32x4 *>(rq + p * rplane + (long)ks * blksize * kK + rem * 16);
-            }
+        for (int i = 0; i < kBPerWave; ++i) {
+            const int j = min(wave + 4 * i, kBParts - 1);
+            const int p = j / NT, part = j % NT;
+            copy16_to_lds(rq + p * rplane + ks * blksize * kK + part * 1024 + lane * 16, &dst[p][part * 1024]);
         }
     };
-    auto store_b = [&](int buf) {
-#pragma unroll
-        for (int e = 0; e < kBPer; ++e) {
-            const int q = tid + e * kThreads;
-            if (kBChunks % kThreads == 0 || q < kBChunks) {
-                const int p = q / (W * 4), rem = q % (W * 4), c = rem >> 2, ch = rem & 3;
-                *reinterpret_cast<i32x4 *>(&Bs[buf][p][swz(c, ch)]) = breg[e];
-            }
-        }
-    };
-    auto compute = [&](int buf, const i32x4 *fa) {
+    auto compute = [&](const int8_t (*bs)[W * kK], const i32x4 *fa) {
 #pragma unroll
         for (int sb = 0; sb < S; ++sb) {
             i32x4 fb[NT];
 #pragma unroll
             for (int nt = 0; nt < NT; ++nt)
-                fb[nt] = *reinterpret_cast<const i32x4 *>(&Bs[buf][sb][swz(nt * 16 + lrow, lk)]);
+                fb[nt] = *reinterpret_cast<const i32x4 *>(&bs[sb][swz(nt * 16 + lrow, lk)]);
 #pragma unroll
             for (int sa = 0; sa + sb < S; ++sa)
 #pragma unroll
@@ -359,7 +371,8 @@ __device__ __attribute__((always_inline)) inline void resident_tile(
             part += __shfl_xor(part, 32);
             if (lk == 0) red[wave][nt * 16 + lrow] = part;
         }
-        __syncthreads();
+        // an LDS-only barrier: __syncthreads() would also wait for the prefetches in flight
+        asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
         for (int c = tid; c < W; c += kThreads) {
             const int g = gblk + c0 + c;
             if (g < ngrid) {
@@ -371,26 +384,37 @@ __device__ __attribute__((always_inline)) inline void resident_tile(
         }
     };
 
-    // A fragments are prefetched DEPTH k steps ahead (DEPTH 1: two register sets; DEPTH 2: a ring
-    // of three; the roles static in a loop unrolled by two or three); the B tile one step ahead
-    // through LDS.  A prefetch index is clamped instead of guarded (a guard makes the compiler wait for
-    // every load at its join): past the last step it re-reads that step, and the values are not
-    // used.  In the symmetric form the int32 sums of a tile are doubled before its diagonal step.
+    // Both operands are prefetched DEPTH k steps ahead: the B stage into one of DEPTH+1 LDS buffers,
+    // the A fragments into one of DEPTH+1 register sets, their roles static in a loop unrolled by
+    // DEPTH+1 (no copy of a register set, which would wait for its prefetch, and each buffer a
+    // fixed object, so that the compiler's wait before the LDS reads of one does not also wait
+    // for the copies into the others).  A prefetch index is clamped instead of guarded (a guard
+    // makes the compiler wait for every load at its join): past the last step it re-reads that
+    // step into an idle buffer, and the values are not used.  Each step issues its B copies, then
+    // its A loads: at the start of step t the loads of steps t+1 .. t+DEPTH-1 may stay in flight
+    // (vmcnt in bits 3:0, expcnt and lgkmcnt left at their maxima), and the barrier makes the
+    // arrival of B(t) true of every wave and tells each that all have finished reading the
+    // buffer the copies of step t+DEPTH refill.  In the symmetric form the int32 sums of a tile are
+    // doubled before its diagonal step.
     const int last = nsteps - 1;
-    auto step = [&](int t, const i32x4 *fa_cur, i32x4 *fa_ahead) {
-        const int buf = t & 1;
-        store_b(buf);
-        __syncthreads();
-        load_b(min(t + 1, last));
-        load_a(min(t + DEPTH, last), fa_ahead);
-        __builtin_amdgcn_sched_barrier(0);  // keep the prefetch ahead of the MFMAs (not sunk to its use)
-        if (sym && (t == n1 - 1 || t == last)) {  // the steps past the diagonal count twice
+    constexpr int kVmAhead = (DEPTH - 1) * (kBPerWave + S);  // loads of the steps that may stay in flight
+    static_assert(kVmAhead < 64, "vmcnt is a 6-bit count");
+    auto step = [&](int t, const i32x4 *fa_cur, i32x4 *fa_new, const int8_t (*bcur)[W * kK],
+                    int8_t (*bnew)[W * kK]) {
+        // the steps past the diagonal count twice: doubled here, not between the loads and the
+        // LDS reads, where the join of the branch would make the compiler wait for every load
+        if (sym && (t == n1 - 1 || t == last)) {
 #pragma unroll
             for (int l = 0; l < S; ++l)
 #pragma unroll
                 for (int nt = 0; nt < NT; ++nt) acc[l][nt] <<= 1;
         }
-        compute(buf, fa_cur);
+        __builtin_amdgcn_s_waitcnt((kVmAhead & 15) | ((kVmAhead >> 4) << 14) | (0x7 << 4) | (0xF << 8));
+        asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+        issue_b(min(t + DEPTH, last), bnew);
+        load_a(min(t + DEPTH, last), fa_new);
+        __builtin_amdgcn_sched_barrier(0);  // keep the prefetch ahead of the MFMAs (not sunk to its use)
+        compute(bcur, fa_cur);
     };
     auto after = [&](int t) {
         if (t == n1 - 1 && n2 > 0) {  // first tile of a pair done; the pipeline runs on into the second
@@ -398,46 +422,47 @@ __device__ __attribute__((always_inline)) inline void resident_tile(
             zero_acc();
         }
     };
-    load_b(0);
+    issue_b(0, Bs0);
     load_a(0, fa0);
     if constexpr (DEPTH == 1) {
-        // unrolled by two, the register sets swapping roles (no copy that would wait for the prefetch)
         int t = 0;
         for (; t + 2 <= nsteps; t += 2) {
-            step(t, fa0, fa1);
+            step(t, fa0, fa1, Bs0, Bs1);
             after(t);
-            step(t + 1, fa1, fa0);
+            step(t + 1, fa1, fa0, Bs1, Bs0);
             after(t + 1);
         }
         if (t < nsteps) {
-            step(t, fa0, fa1);
+            step(t, fa0, fa1, Bs0, Bs1);
             after(t);
         }
     } else {
+        issue_b(min(1, last), Bs1);
         load_a(min(1, last), fa1);
         int t = 0;
         for (; t + 3 <= nsteps; t += 3) {
-            step(t, fa0, fa2);
+            step(t, fa0, fa2, Bs0, Bs2);
             after(t);
-            step(t + 1, fa1, fa0);
+            step(t + 1, fa1, fa0, Bs1, Bs0);
             after(t + 1);
-            step(t + 2, fa2, fa1);
+            step(t + 2, fa2, fa1, Bs2, Bs1);
             after(t + 2);
         }
         if (t < nsteps) {
-            step(t, fa0, fa2);
+            step(t, fa0, fa2, Bs0, Bs2);
             after(t);
         }
         if (t + 1 < nsteps) {
-            step(t + 1, fa1, fa0);
+            step(t + 1, fa1, fa0, Bs1, Bs0);
             after(t + 1);
         }
     }
+    __builtin_amdgcn_s_waitcnt(0);  // no copy into LDS outlives the task
     epilogue(n2 > 0 ? T2 : T1);
 }
 
 template <int S, int NT, int DEPTH>
-__global__ __launch_bounds__(kThreads, DEPTH == 1 ? 2 : 1) void rho_resident_kernel(
+__global__ __launch_bounds__(kThreads, 2) void rho_resident_kernel(
     double *__restrict__ partial, const unsigned char *__restrict__ base, const BlockDesc *__restrict__ blocks,
     const int4 *__restrict__ tasks, int ntasks, const int *__restrict__ fg, int blksize, int ngrid, int nset) {
     // (task, density matrix) pairs, task-major: the sets of a block's tasks, which share its Rq and
@@ -461,7 +486,7 @@ struct EvalDesc {
 // Every (task, density matrix) of every evaluation in one grid.  The XCD-contiguous order runs
 // over the whole batch, evaluations laid out one after another, so the tiles of a block still
 // meet on one XCD.
-template <int S, int NT>
+template <int S, int NT, int DEPTH>
 __global__ __launch_bounds__(kThreads, 2) void rho_resident_batched_kernel(const EvalDesc *__restrict__ evals,
                                                                            int nevals) {
     const int L = xcd_task(blockIdx.x, gridDim.x);
@@ -473,8 +498,8 @@ __global__ __launch_bounds__(kThreads, 2) void rho_resident_batched_kernel(const
     }
     const EvalDesc e = evals[lo];
     const int t = L - e.wg_begin;  // task-major (task, density matrix), as in the single evaluation
-    resident_tile<S, NT, 1>(e.partial, e.base, e.blocks, e.tasks[t / e.nset], t % e.nset, e.fg, e.blksize, e.ngrid,
-                            e.nset);
+    resident_tile<S, NT, DEPTH>(e.partial, e.base, e.blocks, e.tasks[t / e.nset], t % e.nset, e.fg, e.blksize,
+                                e.ngrid, e.nset);
 }
 
 // rho[s*stride + g] += sum over the 64-row tiles of g's block of partial[tile][s][g], per evaluation.
@@ -801,9 +826,16 @@ int resident_eval_batched(hipStream_t stream, const ResidentEvalReq *reqs, size_
         CFGMM_HIP_TRY(hipMemcpyAsync(base, d.data(), sizeof(EvalDesc) * d.size(), hipMemcpyHostToDevice, stream));
         const EvalDesc *dd = reinterpret_cast<const EvalDesc *>(base);
         hipEvent_t t0 = trace_start(stream);
-#define CFGMM_RESB(S_, NT_)                                                                                  \
-    hipLaunchKernelGGL((rho_resident_batched_kernel<S_, NT_>), dim3((unsigned)wg), dim3(kThreads), 0, stream, dd, \
-                       (int)d.size())
+        const bool deep = tune_get(CFGMM_TUNE_RESIDENT_DEPTH) >= 2;
+#define CFGMM_RESB(S_, NT_)                                                                                     \
+    do {                                                                                                         \
+        if (deep)                                                                                                \
+            hipLaunchKernelGGL((rho_resident_batched_kernel<S_, NT_, 2>), dim3((unsigned)wg), dim3(kThreads), 0, \
+                               stream, dd, (int)d.size());                                                       \
+        else                                                                                                     \
+            hipLaunchKernelGGL((rho_resident_batched_kernel<S_, NT_, 1>), dim3((unsigned)wg), dim3(kThreads), 0, \
+                               stream, dd, (int)d.size());                                                       \
+    } while (0)
 #define CFGMM_RESB_NT(S_)               \
     switch (nt) {                       \
         case 1: CFGMM_RESB(S_, 1); break; \
