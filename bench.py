@@ -64,6 +64,8 @@ README_RHO_EVALS_PER_S = 1000.0 / 120.0  # README.md:51, 16-core Ryzen 7945HX
 # int8 slices of the resident split-precision rho (balanced digits, about 8 bits each): the fewest
 # that keep rho within 1e-10 of the FP64 oracle on the workloads here (five: ~1e-11)
 RESIDENT_SLICES = 5
+# int8 slices of the split-precision GEMM (balanced digits): within 1e-10 of FP64 on uniform operands
+GEMM_SPLIT_SLICES = 5
 # Bytes moved past L2 by one rho_chunk_kernel launch on the default workload, from two rocprofv3
 # --pmc passes (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, factors from a 256 MiB device copy):
 # profiles/rho_chunk_kernel_fp64_nao2228_ngrid16384_nset3.txt.  Only valid for that workload.
@@ -311,35 +313,112 @@ def bench_gto(p: W.RhoProblem, reps: int = 10):
                     "achieved_GBs: written bytes over the gto_eval_kernel hipEvent average"}
 
 
-def bench_dgemm(g: W.GemmProblem, reps: int, warmup: int = 2):
+def bench_dgemm(g: W.GemmProblem, reps: int, warmup: int = 2, precision=cfgmm.PREC_FP64, nslices=0,
+                ref_c=None):
+    """ms per call over `reps` back-to-back calls (tracing off), then the kernel average from a traced
+    pass; for split precision the kernel is ozaki_gemm_kernel (one launch per slice-pair wave) and the
+    operand preparation (three launches) is reported beside it."""
     s = cfgmm.Stream()
     dA = cfgmm.DeviceBuffer.from_array(g.a)
     dB = cfgmm.DeviceBuffer.from_array(g.b)
     dC = cfgmm.DeviceBuffer(8 * g.m * g.ldc)
     dC.zero()
+
+    def call():
+        cfgmm.dgemm_dev(s, dC, dA, dB, g.m, g.n, g.k, g.lda, g.ldb, g.ldc, g.transa, g.transb, precision, nslices)
+    err = None
+    if ref_c is not None:
+        call()
+        s.synchronize()
+        err = rel_fro(dC.to_array(np.float64, (g.m * g.ldc,)), ref_c)
     for _ in range(warmup):
-        cfgmm.dgemm_dev(s, dC, dA, dB, g.m, g.n, g.k, g.lda, g.ldb, g.ldc, g.transa, g.transb)
+        call()
     s.synchronize()
     e0, e1 = cfgmm.Event(), cfgmm.Event()
     e0.record(s)
     for _ in range(reps):
-        cfgmm.dgemm_dev(s, dC, dA, dB, g.m, g.n, g.k, g.lda, g.ldb, g.ldc, g.transa, g.transb)
+        call()
     e1.record(s)
     s.synchronize()
     step_ms = e0.elapsed_ms(e1) / reps
-    kavg = traced_kernel_ms(lambda: cfgmm.dgemm_dev(s, dC, dA, dB, g.m, g.n, g.k, g.lda, g.ldb, g.ldc, g.transa,
-                                                    g.transb), s, ["dgemm_kernel"], reps)["dgemm_kernel"]
+    split = precision != cfgmm.PREC_FP64
+    names = ["ozaki_gemm_kernel", "ozaki_split"] if split else ["dgemm_kernel"]
+    tr = traced_kernel_ms(call, s, names, reps)
     for x in (dA, dB, dC):
         x.free()
     e0.destroy()
     e1.destroy()
     s.destroy()
     tf_step = g.flops / (step_ms * 1e-3) / 1e12
-    tf_kern = g.flops / (kavg * 1e-3) / 1e12
-    return dict(shape=f"{g.m}x{g.n}x{g.k}", trans=("T" if g.transa else "N") + ("T" if g.transb else "N"),
-                ms_per_call=round(step_ms, 4), tflops=round(tf_step, 2),
-                frac_fp64_peak=round(tf_step / FP64_MFMA_PEAK_TFLOPS, 4),
-                dgemm_kernel_avg_ms=round(kavg, 4), kernel_tflops=round(tf_kern, 2))
+    out = dict(shape=f"{g.m}x{g.n}x{g.k}", trans=("T" if g.transa else "N") + ("T" if g.transb else "N"),
+               ms_per_call=round(step_ms, 4), tflops=round(tf_step, 2),
+               frac_fp64_peak=round(tf_step / FP64_MFMA_PEAK_TFLOPS, 4))
+    if split:
+        out.update(slices=nslices, tflops_fp64_equivalent=out.pop("tflops"),
+                   ozaki_gemm_kernel_avg_ms=round(tr["ozaki_gemm_kernel"], 4),
+                   operand_prep_ms=round(tr["ozaki_split"], 4))
+        del out["frac_fp64_peak"]
+    else:
+        out.update(dgemm_kernel_avg_ms=round(tr["dgemm_kernel"], 4),
+                   kernel_tflops=round(g.flops / (tr["dgemm_kernel"] * 1e-3) / 1e12, 2))
+    if err is not None:
+        out["rel_err_vs_fp64_gpu"] = err
+    return out
+
+
+def masked_gemm_flops(tab: np.ndarray, m: int, n: int, blksize: int) -> float:
+    """2 m sum_b width_b |K_b|: the multiply-adds over the rows of B that block b's mask keeps."""
+    nblk = tab.shape[1]
+    width = np.minimum(blksize, n - blksize * np.arange(nblk))
+    return 2.0 * m * float(np.dot(width, tab.astype(bool).sum(axis=0)))
+
+
+def bench_masked_gemm(reps: int = 10):
+    """The reference's masked-GEMM test (src/impl_matmul_non0tab.rs:329-377): m = k = 2228, n = 16384,
+    a = sin, b = cos with rows zeroed under a 30 % periodic (row, block) mask, BLKSIZE 48; the roofline
+    counts the masked flops only."""
+    m = k = 2228
+    n, bs = 16384, 48
+    nblk = -(-n // bs)
+    a, b = W.sin_cos_operands(m * k, k * n)
+    tab = W.periodic_mask(k, nblk, W.MASK_30)
+    W.zero_under_mask(b, tab, bs, n)
+    s = cfgmm.Stream()
+    dA, dB = cfgmm.DeviceBuffer.from_array(a), cfgmm.DeviceBuffer.from_array(b)
+    dT = cfgmm.DeviceBuffer.from_array(tab)
+    dC = cfgmm.DeviceBuffer(8 * m * n)
+
+    def call():
+        cfgmm.dgemm_non0tab_dev(s, dC, dA, dB, m, n, k, k, n, n, False, False, dT, nblk, bs)
+    dC.zero()
+    call()
+    s.synchronize()
+    c = dC.to_array(np.float64, (m, n))
+    err = rel_fro(c, a.reshape(m, k) @ b.reshape(k, n))  # B is zero under the mask: the dense product
+    call()
+    s.synchronize()
+    e0, e1 = cfgmm.Event(), cfgmm.Event()
+    e0.record(s)
+    for _ in range(reps):
+        call()
+    e1.record(s)
+    s.synchronize()
+    ms = e0.elapsed_ms(e1) / reps
+    kavg = traced_kernel_ms(call, s, ["masked_gemm_kernel"], reps)["masked_gemm_kernel"]
+    for x in (dA, dB, dT, dC):
+        x.free()
+    e0.destroy()
+    e1.destroy()
+    s.destroy()
+    flops = masked_gemm_flops(tab, m, n, bs)
+    tf = flops / (kavg * 1e-3) / 1e12
+    return {"shape": f"{m}x{n}x{k} NN", "blksize": bs, "mask": "30 % periodic over the flat (row, block) index",
+            "masked_gflop": round(flops / 1e9, 3), "dense_gflop": round(2.0 * m * n * k / 1e9, 3),
+            "ms_per_call": round(ms, 4), "masked_gemm_kernel_avg_ms": round(kavg, 4),
+            "roofline": {"bound": "mfma", "kernel": "masked_gemm_kernel", "achieved": round(tf, 3),
+                         "peak": FP64_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": round(tf / FP64_MFMA_PEAK_TFLOPS, 4)},
+            "rel_err_vs_numpy": err}
 
 
 def bench_dgemm_tuned(g: W.GemmProblem, reps: int):
@@ -388,9 +467,131 @@ def cpu_baseline_rho(p: W.RhoProblem, budget_s: float = 15.0):
                             "restatement of the reference's aodmao2rho_loop_parallel (MC=252, KC=512, MR=14, NR=16)")
 
 
+def batched_independent(probs, reps=3, oracle_samples=(0, 255, 511), tune=False, split_per_call=False):
+    """Config 4 on its own terms: len(probs) independent problems (each its own molecule, D, phi
+    and mask) as 1. sequential per-call FP64 launches, 2. one persistent batched FP64 launch
+    (bitwise equal to 1), 3. split precision over resident operands: every problem sliced into a
+    resident handle (creation timed), then one batched evaluation of all of them.  Errors of 3
+    against the FP64 batch, and of sampled problems against the oracle."""
+    import oracle
+    s = cfgmm.Stream()
+    dev = [(cfgmm.DeviceBuffer.from_array(p.dms), cfgmm.DeviceBuffer.from_array(p.phi),
+            cfgmm.DeviceBuffer.from_array(np.ascontiguousarray(p.tab))) for p in probs]
+    sizes = [p.nset * p.ngrid for p in probs]  # rho rows strided by the problem's own grid
+    offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    out = cfgmm.DeviceBuffer(8 * int(offs[-1]))
+    descs = [cfgmm.RhoProblem(out.ptr + 8 * int(offs[i]), d.ptr, phi.ptr, phi.ptr, p.nao, p.ngrid, p.nao, p.ngrid,
+                              p.nset, tab.ptr, p.nblk, p.blksize) for i, (p, (d, phi, tab)) in enumerate(zip(probs, dev))]
+    arr = (cfgmm.RhoProblem * len(descs))(*descs)
+    lib = cfgmm.lib()
+
+    def sequential():
+        for q in descs:
+            cfgmm.check(lib.cfgmm_aodmao2rho_dev(s.handle, q.rhos, q.dms, q.ao_lhs, q.ao_rhs, q.stride_ao,
+                                                 q.stride_grid, q.nao, q.ngrid, q.nset, q.tab, q.ldtab, q.blksize))
+
+    def sequential_split():
+        for q in descs:
+            cfgmm.check(lib.cfgmm_aodmao2rho_ex_dev(s.handle, q.rhos, q.dms, q.ao_lhs, q.ao_rhs, q.stride_ao,
+                                                    q.stride_grid, q.nao, q.ngrid, q.nset, q.tab, q.ldtab, q.blksize,
+                                                    cfgmm.PREC_OZAKI_INT8, RESIDENT_SLICES))
+
+    def batched():
+        cfgmm.aodmao2rho_batched_dev(s, arr, cfgmm.PREC_FP64, 0)
+
+    def timed(fn, n=reps):
+        fn()
+        s.synchronize()
+        e0, e1 = cfgmm.Event(), cfgmm.Event()
+        e0.record(s)
+        for _ in range(n):
+            fn()
+        e1.record(s)
+        s.synchronize()
+        return e0.elapsed_ms(e1) / n
+
+    def result(fn):
+        out.zero()
+        fn()
+        s.synchronize()
+        return out.to_array(np.float64, (int(offs[-1]),))
+
+    flops = sum(p.flops for p in probs)
+    res = {"problems": len(probs), "masked_gflop_total": round(flops / 1e9, 2)}
+    t_seq = timed(sequential)
+    r_seq = result(sequential)
+    t_bat = timed(batched)
+    r_bat = result(batched)
+    res.update(sequential_fp64_ms=round(t_seq, 3), batched_fp64_ms=round(t_bat, 3),
+               batched_fp64_problems_per_s=round(len(probs) / (t_bat * 1e-3), 1),
+               batched_fp64_tflops=round(flops / (t_bat * 1e-3) / 1e12, 2),
+               speedup_batched_vs_sequential=round(t_seq / t_bat, 2),
+               batched_bitwise_equal_sequential=bool(np.array_equal(r_seq, r_bat)))
+    if tune:
+        from cfgmm import autotune
+        best, table = autotune.tune_batched_rho(batched, s, reps=2)
+        autotune.save({"batched_rho": best})
+        autotune.apply("batched_rho")
+        t_tuned = timed(batched)
+        res.update(batched_fp64_tuned_ms=round(t_tuned, 3), tuned=best,
+                   autotune_table=[[c["wgs_per_cu"], c["max_sets"], round(ms, 3)] for c, ms in table])
+    if split_per_call:
+        t_sp = timed(sequential_split, n=1)
+        res["sequential_split_per_call_ms"] = round(t_sp, 3)
+        res["split_per_call_rel_err_vs_fp64"] = rel_fro(result(sequential_split), r_bat)
+    # resident split precision: creation of every handle (gather + slicing), then one batched evaluation
+    s.synchronize()
+    t0 = time.perf_counter()
+    handles = [cfgmm.RhoResident(s, d, phi, phi, p.nao, p.ngrid, p.nao, p.ngrid, p.nset, tab, p.nblk, p.blksize,
+                                 cfgmm.PREC_OZAKI_INT8, RESIDENT_SLICES) for p, (d, phi, tab) in zip(probs, dev)]
+    s.synchronize()
+    create_ms = (time.perf_counter() - t0) * 1e3
+    evals = [(h, out.ptr + 8 * int(offs[i]), p.ngrid, p.ngrid, p.nset) for i, (h, p) in enumerate(zip(handles, probs))]
+    t_res = timed(lambda: cfgmm.rho_resident_eval_batched(s, evals))
+    r_res = result(lambda: cfgmm.rho_resident_eval_batched(s, evals))
+    res["resident_split"] = {
+        "slices": RESIDENT_SLICES, "create_ms_all": round(create_ms, 1), "batched_eval_ms": round(t_res, 3),
+        "resident_GiB": round(sum(h.bytes for h in handles) / 2 ** 30, 2),
+        "speedup_eval_vs_batched_fp64": round(t_bat / t_res, 2),
+        "speedup_eval_vs_sequential_fp64": round(t_seq / t_res, 2),
+        "speedup_incl_creation_vs_batched_fp64": round(t_bat / (t_res + create_ms), 3),
+        "rel_err_vs_fp64": rel_fro(r_res, r_bat)}
+    for h in handles:
+        h.destroy()
+    errs = []
+    for i in oracle_samples:
+        if i >= len(probs):
+            continue
+        p = probs[i]
+        ref = np.zeros(sizes[i])
+        assert oracle.aodmao2rho(ref, p.dms.ravel(), p.phi.ravel(), p.phi.ravel(), p.nao, p.ngrid, p.nao, p.ngrid,
+                                 p.nset, np.ascontiguousarray(p.tab), p.nblk, p.blksize) == 0
+        errs.append([i, rel_fro(r_bat[offs[i]:offs[i + 1]], ref), rel_fro(r_res[offs[i]:offs[i + 1]], ref)])
+    res["oracle_rel_err_samples"] = errs  # [problem, FP64 batch, resident split]
+    for x in dev:
+        for b in x:
+            b.free()
+    out.free()
+    s.destroy()
+    return res
+
+
+CONFIG4_GENERATOR = ("workloads.independent_problems(n=512, seed=4000): problem i has its own molecule, D, phi and "
+                     "mask from rho_problem(seed=4000+i), with nao ~ U{300..2228} (shells ~ nao/3), ngrid = "
+                     "64 U{1..64}, nset ~ U{1..4}, density ~ U(0.1, 0.6) drawn from default_rng((4000, i))")
+
+
 def bench_batched(args):
     from cfgmm import autotune
-
+    log("[bench] config 4: generating 512 independent problems")
+    t0 = time.perf_counter()
+    indep = W.independent_problems(n=512, seed=4000)
+    log(f"[bench]   generated in {time.perf_counter() - t0:.1f} s")
+    log("[bench] config 4: sequential FP64, batched FP64 (autotuned), per-call split, resident split")
+    c4 = batched_independent(indep, tune=True, split_per_call=True)
+    c4["generator"] = CONFIG4_GENERATOR
+    del indep
+    log(f"[bench]   {c4}")
     log("[bench] generating 32 molecules and 512 problems")
     pool = W.molecule_pool(n=32, seed=100)
     probs = W.batched_problems(pool, n=512, seed=7)
@@ -501,13 +702,21 @@ def bench_batched(args):
                                  np.ascontiguousarray(p.tab), p.nblk, p.blksize) == 0
         errs.append(rel_fro(r_bat[offs[i]:offs[i + 1]], ref))
     best_ms = min(t_bat, t_tuned)
+    c4_best = min(c4["batched_fp64_ms"], c4.get("batched_fp64_tuned_ms", c4["batched_fp64_ms"]))
     line = {
-        "metric": METRIC, "value": round(len(probs) / (best_ms * 1e-3), 2), "unit": "rho problems/s",
-        "n_gpus": 1, "steps": 3, "warmup": 1, "ms_per_step": round(best_ms, 4), "higher_is_better": True,
+        "metric": METRIC, "value": round(512 / (c4_best * 1e-3), 2), "unit": "rho problems/s",
+        "n_gpus": 1, "steps": 3, "warmup": 1, "ms_per_step": round(c4_best, 4), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "512 independent masked rho problems (nao 300..2228, ngrid 64..4096, nset 1..4, "
-                               "density 10..60 %) drawn from 32 generated molecules", "problems": len(probs),
-                   "masked_gflop_total": round(flops / 1e9, 2)},
+                               "density 10..60 %), one persistent batched FP64 launch",
+                   "generator": CONFIG4_GENERATOR, "problems": 512,
+                   "masked_gflop_total": c4["masked_gflop_total"]},
+        "independent": c4,
+        "molecule_pool_variant": {
+            "workload": "512 problems drawn from 32 generated molecules (workloads.molecule_pool(n=32, seed=100), "
+                        "batched_problems(n=512, seed=7)): leading ranges of a molecule's grid and DMs, so each "
+                        "molecule's resident tiles serve about 16 problems",
+            "problems": len(probs), "masked_gflop_total": round(flops / 1e9, 2)},
         "sequential_fp64_ms": round(t_seq, 3), "batched_fp64_ms": round(t_bat, 3),
         "batched_fp64_tuned_ms": round(t_tuned, 3), "tuned": best,
         "autotune_table": [[c["wgs_per_cu"], c["max_sets"], round(ms, 3)] for c, ms in table],
@@ -530,14 +739,23 @@ def bench_serving(args):
     """Serving mix: Poisson arrivals, 70 % rho over 8 resident molecules / 30 % DGEMM, lambda sweep."""
     import oracle  # samples only
     log("[bench] generating the serving pool (8 molecules)")
-    pool = W.serving_pool(n=8)
+    t0 = time.perf_counter()
+    pool = W.serving_pool(n=8, seed=500)
+    log(f"[bench]   {time.perf_counter() - t0:.1f} s: " + ", ".join(f"nao {m.prob.nao} ngrid {m.prob.ngrid}" for m in pool))
     free, total = cfgmm.mem_get_info()
     budget = int(min(free * 0.85, 220 * 2 ** 30))
     lams = [50, 100, 200, 500, 1000, 2000]
     out = {"metric": METRIC, "unit": "rho evals/s", "n_gpus": 1, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": f"f64 (rho split precision: {RESIDENT_SLICES} int8 slices)", "data": "synthetic",
            "config": {"workload": "serving: Poisson arrivals, 70 % masked rho (8 molecules, nao 500..4000, ngrid "
-                                  "64..65536, nset 1..4, density 10..60 %), 30 % DGEMM m,n,k log-uniform 256..4096",
+                                  "4096..65536, nset 1..4, density 10..60 %), 30 % DGEMM m,n,k log-uniform 256..4096",
+                      "generator": "pool: workloads.serving_pool(n=8, seed=500): nao ~ U{500..4000}, ngrid = 64 "
+                                   "U{64..1024}, density ~ U(0.1, 0.6), 4 PSD DMs, every draw kept; requests: "
+                                   "workloads.serving_requests(pool, n, lambda, seed): rho on a uniform pool "
+                                   "molecule over its leading 64 U{64..nblk} points (4096 and up) with U{1..4} "
+                                   "DMs, else a DGEMM with m, n, k log-uniform in 256..4096",
+                      "split_policy": "a molecule whose split-precision tiles could exceed a quarter of the "
+                                      "budget is served by the FP64 kernels (stats rho_fp64_requests)",
                       "hbm_budget_GiB": round(budget / 2 ** 30, 1), "slots": 4, "max_batch": 16,
                       "molecules": [{"nao": m.prob.nao, "ngrid": m.prob.ngrid,
                                      "density": round(float(m.prob.shell_tab.mean()), 3)} for m in pool]}}
@@ -593,7 +811,9 @@ def bench_serving(args):
             for r in creq(reqs):
                 r.arrival_s = 0.0
                 alone.append(sv.run([r])[0].checksum)
-            pick = [i for i, r in enumerate(reqs) if r.kind == 0][:2] + [i for i, r in enumerate(reqs) if r.kind == 1][:1]
+            # oracle samples: the two cheapest rho requests (the CPU oracle's time grows as nao^2 ngrid) and a GEMM
+            rho_ix = sorted((i for i, r in enumerate(reqs) if r.kind == 0), key=lambda i: reqs[i].flops(pool))
+            pick = rho_ix[:2] + [i for i, r in enumerate(reqs) if r.kind == 1][:1]
             outs = [np.zeros(r.nset * r.ngrid if r.kind == 0 else r.m * r.n) if i in pick else None
                     for i, r in enumerate(reqs)]
             res = sv.run(creq(reqs, outs))
@@ -759,6 +979,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-dgemm", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--workload", default="rho", choices=["rho", "batched", "serving", "precision"])
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -903,9 +1124,40 @@ def main():
         dg["small_nt_tuned"] = bench_dgemm_tuned(W.dgemm_small_nt(seed=0), reps=50)
         log(f"[bench]   {dg['small_nt_tuned']}")
         log("[bench] dgemm 3527x9583x6581 NN")
-        dg["large_nn"] = bench_dgemm(W.dgemm_large_nn(seed=0), reps=10)
+        gl = W.dgemm_large_nn(seed=0)
+        dg["large_nn"] = bench_dgemm(gl, reps=10)
         log(f"[bench]   {dg['large_nn']}")
+        log(f"[bench] dgemm 3527x9583x6581 NN in split precision ({GEMM_SPLIT_SLICES} slices)")
+        s = cfgmm.Stream()
+        dA, dB = cfgmm.DeviceBuffer.from_array(gl.a), cfgmm.DeviceBuffer.from_array(gl.b)
+        dC = cfgmm.DeviceBuffer(8 * gl.m * gl.ldc)
+        dC.zero()
+        cfgmm.dgemm_dev(s, dC, dA, dB, gl.m, gl.n, gl.k, gl.lda, gl.ldb, gl.ldc, gl.transa, gl.transb)
+        s.synchronize()
+        c64 = dC.to_array(np.float64, (gl.m * gl.ldc,))
+        for x in (dA, dB, dC):
+            x.free()
+        s.destroy()
+        sp = bench_dgemm(gl, reps=10, precision=cfgmm.PREC_OZAKI_INT8, nslices=GEMM_SPLIT_SLICES, ref_c=c64)
+        sp["speedup_vs_fp64"] = round(dg["large_nn"]["ms_per_call"] / sp["ms_per_call"], 3)
+        dg["large_nn_split"] = sp
+        log(f"[bench]   {sp}")
         out["dgemm"] = dg
+        log("[bench] masked GEMM 2228x16384x2228, 30 % mask, BLKSIZE 48")
+        out["masked_gemm"] = bench_masked_gemm()
+        log(f"[bench]   {out['masked_gemm']}")
+
+    if not args.no_batched:
+        log("[bench] 512 independent rho problems: generating")
+        t0 = time.perf_counter()
+        indep = W.independent_problems(n=512, seed=4000)
+        log(f"[bench]   generated in {time.perf_counter() - t0:.1f} s; sequential FP64, batched FP64, "
+            "resident split (creation included)")
+        c4 = batched_independent(indep, reps=2)
+        c4["generator"] = CONFIG4_GENERATOR
+        del indep
+        out["batched_512"] = c4
+        log(f"[bench]   {c4}")
 
     print(json.dumps(out), flush=True)
 

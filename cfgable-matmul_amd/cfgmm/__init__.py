@@ -464,7 +464,7 @@ REQ_GEMM = 1
 class ServerConfig(ctypes.Structure):
     """cfgmm_server_config."""
     _fields_ = [("hbm_budget_bytes", _SZ), ("slots", _I), ("rho_precision", _I), ("rho_nslices", _I),
-                ("max_batch", _I)]
+                ("max_batch", _I), ("split_tile_cap_bytes", _SZ)]
 
 
 class Request(ctypes.Structure):
@@ -483,15 +483,18 @@ class ServerStats(ctypes.Structure):
     """cfgmm_server_stats."""
     _fields_ = [(k, ctypes.c_ulonglong) for k in ("rho_requests", "gemm_requests", "batches", "hits", "misses",
                                                    "uploads", "evictions")] + [("peak_bytes", _SZ),
-                                                                               ("held_bytes", _SZ)]
+                                                                               ("held_bytes", _SZ),
+                                                                               ("rho_fp64_requests",
+                                                                                ctypes.c_ulonglong)]
 
 
 class Server:
     """The serving loop (cfgmm_server_*): molecules are registered as host numpy arrays, which
     this object keeps alive; ``run`` replays a request list in real time."""
 
-    def __init__(self, hbm_budget_bytes, slots=4, rho_precision=PREC_FP64, rho_nslices=6, max_batch=16):
-        cfg = ServerConfig(hbm_budget_bytes, slots, rho_precision, rho_nslices, max_batch)
+    def __init__(self, hbm_budget_bytes, slots=4, rho_precision=PREC_FP64, rho_nslices=6, max_batch=16,
+                 split_tile_cap_bytes=0):
+        cfg = ServerConfig(hbm_budget_bytes, slots, rho_precision, rho_nslices, max_batch, split_tile_cap_bytes)
         h = _VP()
         check(lib().cfgmm_server_create(ctypes.byref(cfg), ctypes.byref(h)))
         self.handle = h.value

@@ -199,11 +199,15 @@ def rho_problem(seed=0, nshell=700, nao=2228, ngrid=16384, nset=3, blksize=64, d
     shell_tab = mind < cutoff
     tab = shell_tab[shell_of_ao].astype(np.uint8)
 
-    # phi[u][g] = N(0,1) exp(-0.3 r^2), zero under off bits
-    r2 = ((centers[:, None, :] - points[None, :, :]) ** 2).sum(-1)  # (nshell, ngrid)
+    # phi[u][g] = N(0,1) exp(-0.3 r^2), zero under off bits.  r^2 summed over x, y, z in that
+    # order (as a sum over a length-3 axis adds), and the damping applied per shell's rows.
+    r2 = (centers[:, 0, None] - points[None, :, 0]) ** 2
+    r2 += (centers[:, 1, None] - points[None, :, 1]) ** 2
+    r2 += (centers[:, 2, None] - points[None, :, 2]) ** 2  # (nshell, ngrid)
     damp = np.exp(-0.3 * r2)
     phi = rng.standard_normal((nao, ngrid))
-    phi *= damp[shell_of_ao]
+    for sh in range(nshell):
+        phi[ao_loc[sh]:ao_loc[sh + 1]] *= damp[sh]
     on = np.repeat(tab.astype(bool), blksize, axis=1)[:, :ngrid]
     phi[~on] = 0.0
 
@@ -349,10 +353,11 @@ def tile_bytes_estimate(p: "RhoProblem", nslices: int = 6) -> float:
 
 
 def serving_pool(n=8, seed=500, nao_range=(500, 4000), blocks_range=(64, 1024), density_range=(0.1, 0.6), nset=4,
-                 blksize=64, max_tile_bytes=40e9):
-    """Molecules for the serving mix: nao ~ U{nao_range} (shells ~ nao/3), ngrid = blksize * U{blocks_range},
-    mask density ~ U(density_range), nset PSD DMs.  A molecule whose 6-slice resident tiles would
-    exceed max_tile_bytes is redrawn (the serving loop's HBM budget must hold it)."""
+                 blksize=64, max_tile_bytes=None):
+    """Molecules for the serving mix: nao ~ U{nao_range} (shells ~ nao/3), ngrid = blksize * U{blocks_range}
+    (4096 .. 65536 points at the defaults), mask density ~ U(density_range), nset PSD DMs.  With
+    max_tile_bytes, a molecule whose 6-slice resident tiles would exceed it is redrawn; by default
+    every draw is kept."""
     rng = np.random.default_rng(seed)
     pool = []
     i = 0
@@ -363,17 +368,43 @@ def serving_pool(n=8, seed=500, nao_range=(500, 4000), blocks_range=(64, 1024), 
             nao += 1
         nblk = int(rng.integers(blocks_range[0], blocks_range[1] + 1))
         dens = float(rng.uniform(*density_range))
-        # cheap pre-check before generating: blocks of about dens * nao AOs
-        kp = np.ceil(dens * nao / 64.0) * 64.0
         i += 1
-        if nblk * (nset * 6 * kp * kp + blksize * kp * 14) > max_tile_bytes:
-            continue
+        if max_tile_bytes is not None:
+            # cheap pre-check before generating: blocks of about dens * nao AOs
+            kp = np.ceil(dens * nao / 64.0) * 64.0
+            if nblk * (nset * 6 * kp * kp + blksize * kp * 14) > max_tile_bytes:
+                continue
         p = rho_problem(seed=seed * 1000 + i, nshell=nshell, nao=nao, ngrid=nblk * blksize, nset=nset,
                         blksize=blksize, density=dens, nocc=min(200, nao))
-        if tile_bytes_estimate(p) > max_tile_bytes:
+        if max_tile_bytes is not None and tile_bytes_estimate(p) > max_tile_bytes:
             continue
         pool.append(Molecule(seed * 1000 + i, p))
     return pool
+
+
+def independent_problems(n=512, seed=4000, nao_range=(300, 2228), blocks_range=(1, 64), nset_range=(1, 4),
+                         density_range=(0.1, 0.6), blksize=64, threads=None):
+    """n independent masked rho problems, each its own molecule, D, phi and mask: problem i draws
+    nao ~ U{nao_range} (shells ~ nao/3, nao raised by one where the shell count's parity needs it),
+    ngrid = blksize * U{blocks_range}, nset ~ U{nset_range} and density ~ U(density_range) from
+    default_rng((seed, i)), and is then generated by rho_problem(seed=seed + i).  Problems are
+    generated on a thread pool; each depends only on its own seeds."""
+    import os
+    from concurrent.futures import ThreadPoolExecutor
+
+    def one(i):
+        r = np.random.default_rng((seed, i))
+        nao = int(r.integers(nao_range[0], nao_range[1] + 1))
+        nshell = max(1, nao // 3)
+        if (nao - nshell) % 2:
+            nao += 1
+        nblk = int(r.integers(blocks_range[0], blocks_range[1] + 1))
+        nset = int(r.integers(nset_range[0], nset_range[1] + 1))
+        dens = float(r.uniform(*density_range))
+        return rho_problem(seed=seed + i, nshell=nshell, nao=nao, ngrid=nblk * blksize, nset=nset, blksize=blksize,
+                           density=dens, nocc=min(200, nao))
+    with ThreadPoolExecutor(threads or min(16, os.cpu_count() or 1)) as ex:
+        return list(ex.map(one, range(n)))
 
 
 @dataclass
@@ -394,10 +425,10 @@ class ServedRequest:
         return PoolProblem(self.mol, self.ngrid, self.nset).flops(pool)
 
 
-def serving_requests(pool, n, rate, seed=11, rho_fraction=0.7, gemm_range=(256, 4096)):
+def serving_requests(pool, n, rate, seed=11, rho_fraction=0.7, gemm_range=(256, 4096), min_blocks=64):
     """n requests with Poisson arrivals at `rate` per second: rho (a pool molecule, leading
-    64*U{1..nblk} points, U{1..nset} DMs) with probability rho_fraction, else a DGEMM with
-    m, n, k = round(exp U(ln lo, ln hi))."""
+    blksize*U{min(min_blocks, nblk)..nblk} points -- 4096 and up at the defaults -- and U{1..nset}
+    DMs) with probability rho_fraction, else a DGEMM with m, n, k = round(exp U(ln lo, ln hi))."""
     rng = np.random.default_rng(seed)
     t = np.cumsum(rng.exponential(1.0 / rate, size=n))
     out = []
@@ -406,7 +437,7 @@ def serving_requests(pool, n, rate, seed=11, rho_fraction=0.7, gemm_range=(256, 
         if rng.uniform() < rho_fraction:
             mi = int(rng.integers(0, len(pool)))
             p = pool[mi].prob
-            nb = int(rng.integers(1, p.nblk + 1))
+            nb = int(rng.integers(min(min_blocks, p.nblk), p.nblk + 1))
             out.append(ServedRequest(0, float(t[i]), mol=mi, ngrid=nb * p.blksize,
                                      nset=int(rng.integers(1, p.nset + 1))))
         else:

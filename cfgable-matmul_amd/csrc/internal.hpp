@@ -134,6 +134,9 @@ struct ResidentEvalReq {
 };
 int resident_eval_batched(hipStream_t stream, const ResidentEvalReq *reqs, size_t n);
 size_t resident_bytes(const ResidentRho *h);
+// Device bytes a handle over this mask (host copy, nao x ldtab) takes at most.
+size_t resident_bytes_bound(const uint8_t *tab, size_t ldtab, size_t nao, size_t nblk, size_t nset, size_t blksize,
+                            int nslices);
 size_t resident_ngrid(const ResidentRho *h);
 size_t resident_nset(const ResidentRho *h);
 size_t resident_blksize(const ResidentRho *h);

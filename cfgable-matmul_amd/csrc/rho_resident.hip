@@ -712,6 +712,24 @@ int resident_destroy(ResidentRho *h) {
 }
 
 size_t resident_bytes(const ResidentRho *h) { return h ? h->bytes : 0; }
+
+size_t resident_bytes_bound(const uint8_t *tab, size_t ldtab, size_t nao, size_t nblk, size_t nset, size_t blksize,
+                            int nslices) {
+    // the layout of resident_create with every block in the general form (the symmetric form
+    // stores about half of each D plane), from the mask on the host
+    auto al = [](size_t x) { return div_up(x, 256) * 256; };
+    std::vector<size_t> counts(nblk, 0);
+    for (size_t a = 0; a < nao; ++a)
+        for (size_t b = 0; b < nblk; ++b) counts[b] += tab[a * ldtab + b] != 0;
+    size_t bytes = al(sizeof(BlockDesc) * nblk) + al(sizeof(int) * nblk) + al(sizeof(int) * nblk * blksize);
+    const size_t S = (size_t)nslices;
+    for (size_t b = 0; b < nblk; ++b) {
+        const size_t kpad = div_up(counts[b], (size_t)kK) * kK;
+        bytes += al(nset * S * kpad * kpad) + al(S * blksize * kpad) + al(sizeof(double) * kpad * blksize) +
+                 al(sizeof(int) * nset * kpad) + al(sizeof(int4) * (kpad / kRows) * div_up(blksize, (size_t)16));
+    }
+    return bytes;
+}
 size_t resident_ngrid(const ResidentRho *h) { return h->ngrid; }
 size_t resident_nset(const ResidentRho *h) { return h->nset; }
 size_t resident_blksize(const ResidentRho *h) { return h->blksize; }

@@ -46,6 +46,9 @@ struct ServerMolecule {
     const double *h_dms = nullptr, *h_phi = nullptr;
     const uint8_t *h_tab = nullptr;
     size_t nao = 0, ngrid = 0, nset = 0, blksize = 0, nblk = 0;
+    // split precision: rho over resident tiles; false in FP64, and for a molecule whose tiles could
+    // take more than the configured cap (its requests then run in FP64 from D, phi and the mask)
+    bool split = false;
     // residency
     bool resident = false;
     void *d_dms = nullptr, *d_phi = nullptr, *d_tab = nullptr;
@@ -151,7 +154,7 @@ struct Server {
         CFGMM_HIP_TRY(
             hipMemcpyAsync(m.d_phi, m.h_phi, sizeof(double) * m.nao * m.ngrid, hipMemcpyHostToDevice, stream));
         CFGMM_HIP_TRY(hipMemcpyAsync(m.d_tab, m.h_tab, m.nao * m.nblk, hipMemcpyHostToDevice, stream));
-        if (cfg.rho_precision == CFGMM_PREC_OZAKI_INT8) {
+        if (m.split) {
             ResidentRho *tiles = nullptr;
             const int st = resident_create(stream, static_cast<double *>(m.d_dms), static_cast<double *>(m.d_phi),
                                            static_cast<double *>(m.d_phi), m.nao, m.ngrid, m.nao, m.ngrid, m.nset,
@@ -193,9 +196,8 @@ struct Server {
     // with the molecule's own grid as stride_grid (as the phi rows); the resident tiles take
     // any stride, so split-precision results are compact.
     size_t rho_ld(const cfgmm_request &r) const {
-        if (cfg.rho_precision == CFGMM_PREC_OZAKI_INT8) return r.ngrid;
         auto it = mols.find(r.molecule);
-        return it == mols.end() ? r.ngrid : it->second.ngrid;
+        return it == mols.end() || it->second.split ? r.ngrid : it->second.ngrid;
     }
     size_t rho_elems(const cfgmm_request &r) const { return (r.nset - 1) * rho_ld(r) + r.ngrid; }
 
@@ -271,6 +273,9 @@ int server_add_molecule(Server *sv, int id, const double *dms, const double *phi
     m.nset = nset;
     m.blksize = blksize;
     m.nblk = div_up(ngrid, blksize);
+    const size_t cap = sv->cfg.split_tile_cap_bytes ? sv->cfg.split_tile_cap_bytes : sv->cfg.hbm_budget_bytes / 4;
+    m.split = sv->cfg.rho_precision == CFGMM_PREC_OZAKI_INT8 &&
+              resident_bytes_bound(tab, m.nblk, nao, m.nblk, nset, blksize, sv->cfg.rho_nslices) <= cap;
     sv->mols[id] = m;
     return CFGMM_OK;
 }
@@ -458,7 +463,8 @@ int server_run(Server *sv, const cfgmm_request *reqs, size_t n, cfgmm_request_re
                         sl.allocs.push_back(o);
                     st = launch_fill_zero(s, o, sizeof(double) * sv->rho_elems(q));
                     if (st != CFGMM_OK) break;
-                    if (sv->cfg.rho_precision == CFGMM_PREC_FP64) {
+                    if (!m.split) {
+                        sv->stats.rho_fp64_requests++;
                         cfgmm_rho_problem p{};
                         p.rhos = o;
                         p.dms = static_cast<double *>(m.d_dms);

@@ -314,9 +314,13 @@ typedef struct cfgmm_server_config {
     size_t hbm_budget_bytes; /* device memory for resident molecules + in-flight requests */
     int slots;               /* concurrent streams, 1..16 */
     int rho_precision;       /* CFGMM_PREC_FP64 (per-call kernels over resident inputs) or
-                                CFGMM_PREC_OZAKI_INT8 (resident per-block tiles, see 2d) */
+                                CFGMM_PREC_OZAKI_INT8 (resident per-block tiles, see 2d; a molecule whose
+                                tiles could take more than split_tile_cap_bytes is served by the FP64
+                                kernels instead, from its resident D, phi and mask) */
     int rho_nslices;         /* split precision: slices, 2..6 */
     int max_batch;           /* rho requests dispatched together on one slot, >= 1 */
+    size_t split_tile_cap_bytes; /* split precision: largest tile set built for one molecule;
+                                    0 = a quarter of hbm_budget_bytes */
 } cfgmm_server_config;
 
 enum cfgmm_request_kind { CFGMM_REQ_RHO = 0, CFGMM_REQ_GEMM = 1 };
@@ -344,6 +348,7 @@ typedef struct cfgmm_request_result {
 typedef struct cfgmm_server_stats {
     unsigned long long rho_requests, gemm_requests, batches, hits, misses, uploads, evictions;
     size_t peak_bytes, held_bytes;
+    unsigned long long rho_fp64_requests; /* rho requests run by the FP64 kernels (all of them in FP64) */
 } cfgmm_server_stats;
 
 int cfgmm_server_create(const cfgmm_server_config *cfg, cfgmm_server **out);

@@ -28,7 +28,7 @@ def pool():
 
 
 def make_requests(pool, n, rate, seed):
-    return W.serving_requests(pool, n, rate, seed=seed, gemm_range=(48, 400))
+    return W.serving_requests(pool, n, rate, seed=seed, gemm_range=(48, 400), min_blocks=1)
 
 
 def to_ctypes(reqs, outs=None):
@@ -39,8 +39,9 @@ def to_ctypes(reqs, outs=None):
     return out
 
 
-def make_server(pool, precision, budget=64 << 30, slots=4):
-    sv = cfgmm.Server(budget, slots=slots, rho_precision=precision, rho_nslices=6, max_batch=8)
+def make_server(pool, precision, budget=64 << 30, slots=4, tile_cap=0):
+    sv = cfgmm.Server(budget, slots=slots, rho_precision=precision, rho_nslices=6, max_batch=8,
+                      split_tile_cap_bytes=tile_cap)
     for i, m in enumerate(pool):
         p = m.prob
         sv.add_molecule(i, p.dms, p.phi, np.ascontiguousarray(p.tab), p.blksize)
@@ -123,7 +124,8 @@ def test_server_rejects_bad_requests(pool):
 
 
 def test_server_failed_upload_leaves_molecule_clean():
-    """A split-precision server whose budget holds molecule 0's D, phi and mask but not its tiles:
+    """A split-precision server whose tile cap admits molecule 0's tiles but whose budget holds only
+    its D, phi and mask:
     the request on it fails with EOOM after the tiles are built, the molecule is freed and left
     non-resident with nothing charged, and the same server then serves the small molecule 1
     with the result it gives under a large budget, and fails molecule 0 again the same way."""
@@ -146,7 +148,7 @@ def test_server_failed_upload_leaves_molecule_clean():
     ref = make_server(pool, cfgmm.PREC_OZAKI_INT8)
     want1 = ref.run([r1])[0].checksum
     ref.destroy()
-    sv = make_server(pool, cfgmm.PREC_OZAKI_INT8, budget=budget)
+    sv = make_server(pool, cfgmm.PREC_OZAKI_INT8, budget=budget, tile_cap=budget)  # tiles built, then too big
     for _ in range(2):
         with pytest.raises(cfgmm.CfgmmError) as e:
             sv.run([r0])
@@ -155,3 +157,37 @@ def test_server_failed_upload_leaves_molecule_clean():
         res = sv.run([r1])
         assert res[0].status == 0 and res[0].checksum == want1
     sv.destroy()
+
+
+def test_server_tile_cap_serves_large_molecules_in_fp64():
+    """Split precision with a tile cap between the two molecules' tile sizes: requests on the large
+    molecule run through the FP64 kernels (the same checksums as an FP64 server), those on the
+    small one over resident tiles (the same checksums as a split server without a cap)."""
+    pool = [W.Molecule(0, W.rho_problem(seed=41, nshell=100, nao=300, ngrid=64 * 8, nset=3, density=0.3)),
+            W.Molecule(1, W.rho_problem(seed=42, nshell=200, nao=600, ngrid=64 * 40, nset=3, density=0.5))]
+    reqs = [r for r in make_requests(pool, 40, rate=3000.0, seed=13) if r.kind == 0]
+    held = []
+    for i in range(len(pool)):
+        sv = make_server(pool, cfgmm.PREC_OZAKI_INT8)
+        base = make_server(pool, cfgmm.PREC_FP64)
+        sv.warm([i])
+        base.warm([i])
+        held.append(sv.stats()["held_bytes"] - base.stats()["held_bytes"])
+        sv.destroy()
+        base.destroy()
+    cap = 2 * held[0]  # the cap is checked against a bound that counts whole D planes (at most 2x the tiles)
+    assert cap < held[1], held  # the sizes this test relies on
+    want = {}
+    for prec in (cfgmm.PREC_FP64, cfgmm.PREC_OZAKI_INT8):
+        sv = make_server(pool, prec)
+        want[prec] = [sv.run([r])[0].checksum for r in to_ctypes(reqs)]
+        sv.destroy()
+    sv = make_server(pool, cfgmm.PREC_OZAKI_INT8, tile_cap=cap)
+    res = sv.run(to_ctypes(reqs))
+    stats = sv.stats()
+    sv.destroy()
+    assert all(res[i].status == 0 for i in range(len(reqs)))
+    for i, r in enumerate(reqs):
+        assert res[i].checksum == want[cfgmm.PREC_FP64 if r.mol == 1 else cfgmm.PREC_OZAKI_INT8][i], (i, r.mol)
+    assert stats["rho_fp64_requests"] == sum(r.mol == 1 for r in reqs) > 0
+    assert 0 < sum(r.mol == 0 for r in reqs)
