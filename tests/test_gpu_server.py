@@ -148,7 +148,7 @@ def test_server_failed_upload_leaves_molecule_clean():
     ref = make_server(pool, cfgmm.PREC_OZAKI_INT8)
     want1 = ref.run([r1])[0].checksum
     ref.destroy()
-    sv = make_server(pool, cfgmm.PREC_OZAKI_INT8, budget=budget, tile_cap=budget)  # tiles built, then too big
+    sv = make_server(pool, cfgmm.PREC_OZAKI_INT8, budget=budget, tile_cap=1 << 40)  # tiles built, then too big
     for _ in range(2):
         with pytest.raises(cfgmm.CfgmmError) as e:
             sv.run([r0])
