@@ -22,6 +22,8 @@
 namespace cfgmm {
 namespace {
 
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+
 constexpr int kThreads = 256;
 constexpr int kXcds = 8;  // MI355X: 8 XCDs, workgroups are dealt round-robin across them
 
@@ -46,89 +48,164 @@ __device__ inline int xcd_remap(int pid, int total) {
     return pid < full ? (pid % kXcds) * (total / kXcds) + pid / kXcds : pid;
 }
 
+// 16 bytes per lane from global memory straight into LDS at dst + 16 * lane (dst uniform across
+// the wave; global_load_lds_dwordx4).  Device-only: a kernel template that names the builtin in
+// its host pass loses its launch stub.
+__device__ __attribute__((always_inline)) inline void copy16_to_lds(const void *src, void *dst) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void *)dst, 16, 0, 0);
+#endif
+}
+
+// LDS layouts of one BK = 16 slice of an operand tile of T = BM (= BN) rows or columns.
+//   K-contiguous (A not transposed, B transposed: the 16 k of a row are 128 contiguous bytes):
+//     row r is 8 chunks of two k; chunk c sits in slot c ^ ((r >> 1) & 7) of the row's 128
+//     bytes, so that the 16 rows one ds_read_b128 phase reads cover all 64 banks once.
+//   MN-contiguous (A transposed, B not transposed: a k row of T values is contiguous): copy
+//     units of 1 KiB (1024 / (8 T) k rows each) at a pitch of 1024 + PAD bytes, which puts the
+//     k rows 4 apart that one ds_read_b64 phase reads on disjoint banks.
+// Both are what one wave instruction of global_load_lds_dwordx4 writes: 1 KiB, lane i at 16 i.
+template <int T>
+struct SliceLayout {
+    static constexpr int kc_bytes = T * 128;
+    static constexpr int rows_per_unit = 1024 / (T * 8);  // k rows per 1 KiB unit: 1 (T 128), 2 (T 64)
+    static constexpr int pad = T == 128 ? 32 : 64;
+    static constexpr int mn_bytes = (16 / rows_per_unit) * (1024 + pad);
+    __device__ static int kc_offset(int r, int kq) {  // byte of element (row r, k kq)
+        return r * 128 + ((((kq >> 1) ^ (r >> 1)) & 7) << 4) + ((kq & 1) << 3);
+    }
+    __device__ static int mn_offset(int kq, int col) {  // byte of element (k kq, column col)
+        return (kq / rows_per_unit) * (1024 + pad) + (kq % rows_per_unit) * (T * 8) + col * 8;
+    }
+};
+
 // Work decomposition of one launch: workgroups pid < dp_tiles each own a whole
 // C tile (the first dp_tiles tiles of the raster order); the remaining tiles
 // are split into `splits` K ranges of `kchunk`, one workgroup per (tile,
 // range), whose partial tiles go to `partial` and are summed in a fixed order
 // by tail_reduce_kernel.
-template <int BM, int BN, int BK, int WM, int WN, bool TA, bool TB>
+//
+// Each k step of 16: the waves copy the next step's A and B slices straight into the idle LDS
+// buffers (global_load_lds_dwordx4, 1 KiB per wave instruction), then run the MFMAs of the
+// current one; one s_waitcnt vmcnt(0) + barrier per step makes the copies of every wave visible
+// and the idle buffers free.  The MFMAs take the k of a step in the order k = 4q + kk (lane group
+// q, MFMA kk), the same permutation for A and B, so a lane's four A (or B) values of a
+// K-contiguous operand are 32 contiguous bytes: two ds_read_b128.  Row (column) indices past the
+// matrix are clamped for K-contiguous operands (their results are never stored); an MN-contiguous
+// operand whose tile crosses the matrix edge, and the last k step when k is not a multiple of 16,
+// go through registers with zero fill instead.
+template <int BM, int WM, bool TA, bool TB>
 __global__ __launch_bounds__(kThreads, 2) void dgemm_kernel(double *__restrict__ c, const double *__restrict__ a,
                                                             const double *__restrict__ b, int m, int n, int k,
                                                             long lda, long ldb, long ldc, int dp_tiles, int splits,
                                                             int kchunk, int group_m, double *__restrict__ partial) {
+    constexpr int BN = BM, WN = WM, BK = 16;
     constexpr int WAVES_N = BN / WN;
     static_assert((BM / WM) * WAVES_N == kThreads / 64, "four waves per workgroup");
     constexpr int FM = WM / 16, FN = WN / 16;
-    // Pad the LDS rows only where the global->LDS copy transposes (store
-    // stride = row length), to spread those stores over the banks.
-    constexpr int LDA_S = BM + (TA ? 0 : 1);
-    constexpr int LDB_S = BN + (TB ? 1 : 0);
-    constexpr int A_PER_T = BM * BK / kThreads;
-    constexpr int B_PER_T = BN * BK / kThreads;
-    static_assert(A_PER_T * kThreads == BM * BK && B_PER_T * kThreads == BN * BK, "tile / thread split");
-
-    __shared__ double As[2][BK][LDA_S];
-    __shared__ double Bs[2][BK][LDB_S];
+    using L = SliceLayout<BM>;
+    constexpr int A_BYTES = TA ? L::mn_bytes : L::kc_bytes;
+    constexpr int B_BYTES = TB ? L::kc_bytes : L::mn_bytes;
+    // the two stages are __shared__ objects of their own: the compiler's wait before reading one
+    // then does not also wait for the copies into the other
+    __shared__ __attribute__((aligned(16))) unsigned char As0[A_BYTES];
+    __shared__ __attribute__((aligned(16))) unsigned char As1[A_BYTES];
+    __shared__ __attribute__((aligned(16))) unsigned char Bs0[B_BYTES];
+    __shared__ __attribute__((aligned(16))) unsigned char Bs1[B_BYTES];
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
     const int wm = wave / WAVES_N, wn = wave % WAVES_N;
+    const int lrow = lane & 15, lk = lane >> 4;
     const int tiles_m = (m + BM - 1) / BM, tiles_n = (n + BN - 1) / BN;
     const int pid = blockIdx.x;
-    int L, z = 0, kbeg = 0, kend = k;
+    int Lt, z = 0, kbeg = 0, kend = k;
     if (pid < dp_tiles) {
-        L = xcd_remap(pid, dp_tiles);
+        Lt = xcd_remap(pid, dp_tiles);
     } else {
         const int q = pid - dp_tiles;
-        L = dp_tiles + q / splits;
+        Lt = dp_tiles + q / splits;
         z = q % splits;
         kbeg = z * kchunk;
         kend = min(k, kbeg + kchunk);
     }
     int tm, tn;
-    raster_of(L, tiles_m, tiles_n, group_m, tm, tn);
+    raster_of(Lt, tiles_m, tiles_n, group_m, tm, tn);
     const int m0 = tm * BM, n0 = tn * BN;
-    const int ntiles = kend > kbeg ? (kend - kbeg + BK - 1) / BK : 0;
+    const int klen = kend > kbeg ? kend - kbeg : 0;
+    const int nfull = klen / BK;
+    const bool ktail = klen % BK != 0;
 
-    double ra[A_PER_T], rb[B_PER_T];
-
-    auto load_tiles = [&](int k0) {
+    // ---- direct copies of k step t (full steps only) ----
+    // K-contiguous operand x (row-major rows of length >= k): BM rows x 8 chunks, BM / 32 wave
+    // instructions per wave; instruction u covers rows 8u .. 8u+7
+    auto issue_kc = [&](const double *__restrict__ x, long ld, int row0, int nrows, int k0, unsigned char *dst) {
 #pragma unroll
-        for (int e = 0; e < A_PER_T; ++e) {
-            const int idx = tid + kThreads * e;
-            const int p = TA ? idx / BM : idx % BK;
-            const int i = TA ? idx % BM : idx / BK;
-            const int gi = m0 + i, gp = k0 + p;
-            double v = 0.0;
-            if (gi < m && gp < kend) v = TA ? a[(long)gp * lda + gi] : a[(long)gi * lda + gp];
-            ra[e] = v;
+        for (int e = 0; e < BM / 32; ++e) {
+            const int u = wave * (BM / 32) + e;
+            const int P = u * 64 + lane;
+            const int r = P >> 3, slot = P & 7;
+            const int ch = (slot ^ (r >> 1)) & 7;
+            const int gr = min(row0 + r, nrows - 1);
+            copy16_to_lds(x + (long)gr * ld + k0 + 2 * ch, dst + u * 1024);
         }
+    };
+    // MN-contiguous operand x (k rows of stride ld): 16 / rows_per_unit units, a quarter per wave
+    auto issue_mn = [&](const double *__restrict__ x, long ld, int col0, int k0, unsigned char *dst) {
+        constexpr int UNITS = 16 / L::rows_per_unit;
 #pragma unroll
-        for (int e = 0; e < B_PER_T; ++e) {
+        for (int e = 0; e < UNITS / 4; ++e) {
+            const int u = wave * (UNITS / 4) + e;
+            const int byte = lane * 16;
+            const int kr = u * L::rows_per_unit + byte / (BM * 8);
+            const int col = col0 + (byte % (BM * 8)) / 8;
+            copy16_to_lds(x + (long)(k0 + kr) * ld + col, dst + u * (1024 + L::pad));
+        }
+    };
+    auto issue = [&](int t, unsigned char *as, unsigned char *bs) {
+        const int k0 = kbeg + t * BK;
+        if (TA) issue_mn(a, lda, m0, k0, as);
+        else issue_kc(a, lda, m0, m, k0, as);
+        if (TB) issue_kc(b, ldb, n0, n, k0, bs);
+        else issue_mn(b, ldb, n0, k0, bs);
+    };
+
+    // ---- register path (zero fill): element e of a thread covers one (row or k row, k or column) ----
+    constexpr int PER_T = BM * BK / kThreads;
+    double ra[PER_T], rb[PER_T];
+    auto load_regs = [&](int k0) {
+#pragma unroll
+        for (int e = 0; e < PER_T; ++e) {
             const int idx = tid + kThreads * e;
-            const int p = TB ? idx % BK : idx / BN;
-            const int j = TB ? idx / BK : idx % BN;
-            const int gj = n0 + j, gp = k0 + p;
             double v = 0.0;
-            if (gj < n && gp < kend) v = TB ? b[(long)gj * ldb + gp] : b[(long)gp * ldb + gj];
+            if (TA) {  // a[p][i]: k row idx / BM, column idx % BM
+                const int p = k0 + idx / BM, i = m0 + idx % BM;
+                if (p < kend && i < m) v = a[(long)p * lda + i];
+            } else {   // a[i][p]: row idx / 16, k idx % 16
+                const int i = m0 + idx / BK, p = k0 + idx % BK;
+                if (p < kend && i < m) v = a[(long)i * lda + p];
+            }
+            ra[e] = v;
+            v = 0.0;
+            if (TB) {  // b[j][p]
+                const int j = n0 + idx / BK, p = k0 + idx % BK;
+                if (p < kend && j < n) v = b[(long)j * ldb + p];
+            } else {   // b[p][j]
+                const int p = k0 + idx / BN, j = n0 + idx % BN;
+                if (p < kend && j < n) v = b[(long)p * ldb + j];
+            }
             rb[e] = v;
         }
     };
-    auto store_tiles = [&](int buf) {
+    auto store_regs = [&](unsigned char *as, unsigned char *bs) {
 #pragma unroll
-        for (int e = 0; e < A_PER_T; ++e) {
+        for (int e = 0; e < PER_T; ++e) {
             const int idx = tid + kThreads * e;
-            const int p = TA ? idx / BM : idx % BK;
-            const int i = TA ? idx % BM : idx / BK;
-            As[buf][p][i] = ra[e];
-        }
-#pragma unroll
-        for (int e = 0; e < B_PER_T; ++e) {
-            const int idx = tid + kThreads * e;
-            const int p = TB ? idx % BK : idx / BN;
-            const int j = TB ? idx / BK : idx % BN;
-            Bs[buf][p][j] = rb[e];
+            const int ao = TA ? L::mn_offset(idx / BM, idx % BM) : L::kc_offset(idx / BK, idx % BK);
+            const int bo = TB ? L::kc_offset(idx / BK, idx % BK) : L::mn_offset(idx / BN, idx % BN);
+            *reinterpret_cast<double *>(as + ao) = ra[e];
+            *reinterpret_cast<double *>(bs + bo) = rb[e];
         }
     };
 
@@ -138,31 +215,92 @@ __global__ __launch_bounds__(kThreads, 2) void dgemm_kernel(double *__restrict__
 #pragma unroll
         for (int ni = 0; ni < FN; ++ni) acc[mi][ni] = f64x4{0.0, 0.0, 0.0, 0.0};
 
-    if (ntiles > 0) {
-        load_tiles(kbeg);
-        store_tiles(0);
-    }
-    __syncthreads();
-
-    const int lrow = lane & 15, lk = lane >> 4;
-    for (int t = 0; t < ntiles; ++t) {
-        const int cur = t & 1;
-        if (t + 1 < ntiles) load_tiles(kbeg + (t + 1) * BK);
+    // ---- one k step from LDS: lane group q supplies k = 4q + kk to MFMA kk; two halves of two kk
+    // each, so that only half the fragments are live at a time ----
+    auto compute = [&](const unsigned char *as, const unsigned char *bs) {
 #pragma unroll
-        for (int kk = 0; kk < BK / 4; ++kk) {
-            double fa[FM], fb[FN];
+        for (int h = 0; h < 2; ++h) {
+            f64x2 fa[FM], fb[FN];
 #pragma unroll
-            for (int mi = 0; mi < FM; ++mi) fa[mi] = As[cur][kk * 4 + lk][wm * WM + mi * 16 + lrow];
+            for (int mi = 0; mi < FM; ++mi) {
+                const int r = wm * WM + mi * 16 + lrow;
+                if (TA) {
+                    fa[mi][0] = *reinterpret_cast<const double *>(as + L::mn_offset(4 * lk + 2 * h, r));
+                    fa[mi][1] = *reinterpret_cast<const double *>(as + L::mn_offset(4 * lk + 2 * h + 1, r));
+                } else {
+                    fa[mi] = *reinterpret_cast<const f64x2 *>(as + L::kc_offset(r, 4 * lk + 2 * h));
+                }
+            }
 #pragma unroll
-            for (int ni = 0; ni < FN; ++ni) fb[ni] = Bs[cur][kk * 4 + lk][wn * WN + ni * 16 + lrow];
+            for (int ni = 0; ni < FN; ++ni) {
+                const int col = wn * WN + ni * 16 + lrow;
+                if (TB) {
+                    fb[ni] = *reinterpret_cast<const f64x2 *>(bs + L::kc_offset(col, 4 * lk + 2 * h));
+                } else {
+                    fb[ni][0] = *reinterpret_cast<const double *>(bs + L::mn_offset(4 * lk + 2 * h, col));
+                    fb[ni][1] = *reinterpret_cast<const double *>(bs + L::mn_offset(4 * lk + 2 * h + 1, col));
+                }
+            }
 #pragma unroll
-            for (int mi = 0; mi < FM; ++mi)
+            for (int kk = 0; kk < 2; ++kk)
 #pragma unroll
-                for (int ni = 0; ni < FN; ++ni)
-                    acc[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[mi], fb[ni], acc[mi][ni], 0, 0, 0);
+                for (int mi = 0; mi < FM; ++mi)
+#pragma unroll
+                    for (int ni = 0; ni < FN; ++ni)
+                        acc[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[mi][kk], fb[ni][kk], acc[mi][ni], 0, 0, 0);
         }
-        if (t + 1 < ntiles) store_tiles(cur ^ 1);
-        __syncthreads();
+    };
+    // every wave's copies landed (vmcnt(0): expcnt and lgkmcnt left at their maxima) and every
+    // wave is done with the buffers read before; an LDS-only barrier, since __syncthreads()
+    // would add a fence
+    auto sync = [&]() {
+        __builtin_amdgcn_s_waitcnt((0x7 << 4) | (0xF << 8));
+        asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+    };
+
+    // an MN-contiguous operand tile that crosses the matrix edge cannot be copied in 16-byte pairs
+    const bool direct = (!TA || m0 + BM <= m) && (TB || n0 + BN <= n);
+    if (direct) {
+        if (nfull > 0) {
+            const int last = nfull - 1;
+            issue(0, As0, Bs0);
+            int t = 0;
+            // the prefetch index is clamped rather than guarded (a guard makes the compiler wait
+            // for every copy at its join): past the last step it re-copies that step into the idle
+            // buffers, whose contents are not used
+            for (; t + 2 <= nfull; t += 2) {
+                sync();
+                issue(min(t + 1, last), As1, Bs1);
+                compute(As0, Bs0);
+                sync();
+                issue(min(t + 2, last), As0, Bs0);
+                compute(As1, Bs1);
+            }
+            if (t < nfull) {
+                sync();
+                issue(last, As1, Bs1);
+                compute(As0, Bs0);
+            }
+            sync();  // no copy outlives the loop; the buffers are free
+        }
+        if (ktail) {
+            load_regs(kbeg + nfull * BK);
+            store_regs(As0, Bs0);
+            __syncthreads();
+            compute(As0, Bs0);
+        }
+    } else {
+        // registers for every step (edge tiles only: the loads are not overlapped with the MFMAs,
+        // which keeps the staging registers out of the MFMA loop); the two buffers alternate, so
+        // one barrier per step separates the writes of a buffer from the reads two steps before
+        const int nsteps = nfull + (ktail ? 1 : 0);
+        for (int t = 0; t < nsteps; ++t) {
+            unsigned char *as = (t & 1) ? As1 : As0, *bs = (t & 1) ? Bs1 : Bs0;
+            load_regs(kbeg + t * BK);
+            store_regs(as, bs);
+            __syncthreads();
+            compute(as, bs);
+        }
     }
 
     // Epilogue: C += acc (one owner per element), or store the partial tile.  A fragment row's C
@@ -296,10 +434,10 @@ DgemmPlan plan_dgemm(int m, int n, int k, int cus) {
     return p;
 }
 
-template <int BM, int BN, int WM, int WN>
+template <int BM, int WM>
 int launch_plan(hipStream_t stream, const DgemmPlan &pl, double *c, const double *a, const double *b, int m, int n,
                 int k, long lda, long ldb, long ldc, bool ta, bool tb) {
-    constexpr int BK = 16;
+    constexpr int BN = BM;
     const int group_m = tune_get(CFGMM_TUNE_DGEMM_GROUP_M);
     const int tail_tiles = pl.tiles - pl.dp_tiles;
     double *partial = nullptr;
@@ -312,7 +450,7 @@ int launch_plan(hipStream_t stream, const DgemmPlan &pl, double *c, const double
     dim3 grid((unsigned)(pl.dp_tiles + tail_tiles * pl.splits)), block(kThreads);
     hipEvent_t t0 = trace_start(stream);
 #define CFGMM_DGEMM_LAUNCH(TA_, TB_)                                                                          \
-    hipLaunchKernelGGL((dgemm_kernel<BM, BN, BK, WM, WN, TA_, TB_>), grid, block, 0, stream, c, a, b, m, n, k, \
+    hipLaunchKernelGGL((dgemm_kernel<BM, WM, TA_, TB_>), grid, block, 0, stream, c, a, b, m, n, k,             \
                        lda, ldb, ldc, pl.dp_tiles, pl.splits, pl.kchunk, group_m, partial)
     if (!ta && !tb)
         CFGMM_DGEMM_LAUNCH(false, false);
@@ -343,8 +481,8 @@ int launch_dgemm(hipStream_t stream, double *c, const double *a, const double *b
     const int im = (int)m, in = (int)n, ik = (int)k;
     const DgemmPlan pl = plan_dgemm(im, in, ik, device_cu_count());
     if (pl.bm == 128)
-        return launch_plan<128, 128, 64, 64>(stream, pl, c, a, b, im, in, ik, lda, ldb, ldc, transa, transb);
-    return launch_plan<64, 64, 32, 32>(stream, pl, c, a, b, im, in, ik, lda, ldb, ldc, transa, transb);
+        return launch_plan<128, 64>(stream, pl, c, a, b, im, in, ik, lda, ldb, ldc, transa, transb);
+    return launch_plan<64, 32>(stream, pl, c, a, b, im, in, ik, lda, ldb, ldc, transa, transb);
 }
 
 }  // namespace cfgmm
