@@ -9,6 +9,9 @@ Modes:
                  one-time preparation runs before the repetitions)
   dgemm_large    FP64 DGEMM 3527x9583x6581 NN (dgemm_kernel)
   dgemm_small    FP64 DGEMM 1031x977x1543 NT
+  dgemm_small_tuned  the same with the autotuned small plan (64x64 tiles, 6 K ranges)
+  masked_gemm    the reference's masked GEMM test: 2228x16384x2228 NN, sin/cos, 30 % periodic mask,
+                 BLKSIZE 48 (masked_gemm_kernel)
   gemm_split<S>  the large GEMM in split precision (ozaki_gemm_kernel)
   vxc            V += phi w phi^T over the same mask (vxc_block_kernel, vxc_reduce_kernel)
   gto            AO tabulation of workloads.gto_workload (700 shells, 1890 AOs, 16384 points)
@@ -18,6 +21,10 @@ Modes:
   batched_resident<S> the same 512 problems as leading ranges of the 32 molecules' resident
                  handles (S int8 slices; the handles are created before the repetitions) in one
                  cfgmm_rho_resident_eval_batched call
+  indep          512 independent rho problems (workloads.independent_problems(n=512, seed=4000)) in
+                 one FP64 batched launch
+  indep_resident<S> the same problems, each sliced into its own resident handle (before the
+                 repetitions), in one cfgmm_rho_resident_eval_batched call
 
 Each pass of a hardware-counter profiler is a separate process running this
 module; the inputs are generated, uploaded once and then the operation runs
@@ -59,8 +66,11 @@ def main(argv: list[str]) -> int:
                               p.blksize, cfgmm.PREC_OZAKI_INT8, int(mode[len("rho_resident"):]))
         for _ in range(reps):
             h.eval(s, d_r, p.ngrid)
-    elif mode in ("dgemm_large", "dgemm_small") or mode.startswith("gemm_split"):
-        g = W.dgemm_small_nt() if mode == "dgemm_small" else W.dgemm_large_nn()
+    elif mode in ("dgemm_large", "dgemm_small", "dgemm_small_tuned") or mode.startswith("gemm_split"):
+        g = W.dgemm_small_nt() if mode.startswith("dgemm_small") else W.dgemm_large_nn()
+        if mode == "dgemm_small_tuned":
+            cfgmm.tune_set(cfgmm.TUNE_DGEMM_SMALL_TILE, 64)
+            cfgmm.tune_set(cfgmm.TUNE_DGEMM_SMALL_SPLITS, 6)
         prec, ns = (cfgmm.PREC_FP64, 0) if mode.startswith("dgemm") else (cfgmm.PREC_OZAKI_INT8,
                                                                            int(mode[len("gemm_split"):]))
         d_a = cfgmm.DeviceBuffer.from_array(g.a)
@@ -69,6 +79,40 @@ def main(argv: list[str]) -> int:
         d_c.zero()
         for _ in range(reps):
             cfgmm.dgemm_dev(s, d_c, d_a, d_b, g.m, g.n, g.k, g.lda, g.ldb, g.ldc, g.transa, g.transb, prec, ns)
+    elif mode == "masked_gemm":
+        m = k = 2228
+        n, bs = 16384, 48
+        nblk = -(-n // bs)
+        a, b = W.sin_cos_operands(m * k, k * n)
+        tab = W.periodic_mask(k, nblk, W.MASK_30)
+        W.zero_under_mask(b, tab, bs, n)
+        d_a, d_b = cfgmm.DeviceBuffer.from_array(a), cfgmm.DeviceBuffer.from_array(b)
+        d_t = cfgmm.DeviceBuffer.from_array(tab)
+        d_c = cfgmm.DeviceBuffer(8 * m * n)
+        d_c.zero()
+        for _ in range(reps):
+            cfgmm.dgemm_non0tab_dev(s, d_c, d_a, d_b, m, n, k, k, n, n, False, False, d_t, nblk, bs)
+    elif mode == "indep" or mode.startswith("indep_resident"):
+        probs = W.independent_problems(n=512, seed=4000)
+        dev = [(cfgmm.DeviceBuffer.from_array(q.dms), cfgmm.DeviceBuffer.from_array(q.phi),
+                cfgmm.DeviceBuffer.from_array(np.ascontiguousarray(q.tab))) for q in probs]
+        offs = np.concatenate([[0], np.cumsum([q.nset * q.ngrid for q in probs])])
+        out = cfgmm.DeviceBuffer(8 * int(offs[-1]))
+        out.zero()
+        if mode == "indep":
+            arr = (cfgmm.RhoProblem * len(probs))(*[
+                cfgmm.RhoProblem(out.ptr + 8 * int(offs[i]), d.ptr, phi.ptr, phi.ptr, q.nao, q.ngrid, q.nao, q.ngrid,
+                                 q.nset, tab.ptr, q.nblk, q.blksize) for i, (q, (d, phi, tab)) in enumerate(zip(probs, dev))])
+            for _ in range(reps):
+                cfgmm.aodmao2rho_batched_dev(s, arr, cfgmm.PREC_FP64, 0)
+        else:
+            nslices = int(mode[len("indep_resident"):])
+            handles = [cfgmm.RhoResident(s, d, phi, phi, q.nao, q.ngrid, q.nao, q.ngrid, q.nset, tab, q.nblk,
+                                         q.blksize, cfgmm.PREC_OZAKI_INT8, nslices) for q, (d, phi, tab) in zip(probs, dev)]
+            evals = [(h, out.ptr + 8 * int(offs[i]), q.ngrid, q.ngrid, q.nset)
+                     for i, (h, q) in enumerate(zip(handles, probs))]
+            for _ in range(reps):
+                cfgmm.rho_resident_eval_batched(s, evals)
     elif mode == "vxc":
         p = W.rho_problem(seed=0)
         w = np.random.default_rng(6).uniform(0.1, 2.0, p.ngrid)

@@ -352,8 +352,15 @@ __global__ void tail_reduce_kernel(double *__restrict__ c, const double *__restr
     const int row = tm * BM + e / BN, col = tn * BN + e % BN;
     if (row >= m || col >= n) return;
     const double *__restrict__ p = partial + (long)t * splits * (BM * BN) + e;
+    // eight loads in flight at a time (clamped indices, no branch around a load), summed in z order
     double s = 0.0;
-    for (int z = 0; z < splits; ++z) s += p[(long)z * (BM * BN)];
+    for (int z0 = 0; z0 < splits; z0 += 8) {
+        double v[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v[i] = p[(long)min(z0 + i, splits - 1) * (BM * BN)];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) s = z0 + i < splits ? s + v[i] : s;
+    }
     c[(long)row * ldc + col] += s;
 }
 

@@ -99,8 +99,17 @@ __global__ void rho_reduce_kernel(double *__restrict__ rhos, const double *__res
     if (idx >= (long)nset * ngrid) return;
     const int s = (int)(idx / ngrid), g = (int)(idx % ngrid);
     const int nch = (counts[g / blksize] + 63) / 64;
+    const double *__restrict__ p = partial + (long)s * ngrid + g;
+    const long step = (long)nset * ngrid;
+    // eight loads in flight at a time (clamped indices, no branch around a load), summed in chunk order
     double v = 0.0;
-    for (int ch = 0; ch < nch; ++ch) v += partial[((long)ch * nset + s) * ngrid + g];
+    for (int c0 = 0; c0 < nch; c0 += 8) {
+        double x[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) x[i] = p[min(c0 + i, nch - 1) * step];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v = c0 + i < nch ? v + x[i] : v;
+    }
     rhos[(long)s * stride_grid + g] += v;
 }
 

@@ -308,9 +308,17 @@ __global__ void batch_reduce(const BatchProblem *__restrict__ probs, const int *
     if (idx >= (long)p.nset * p.ngrid) return;
     const int s = (int)(idx / p.ngrid), g = (int)(idx % p.ngrid);
     const int nch = (counts[p.counts_off + g / p.blksize] + kUC - 1) / kUC;
-    const double *__restrict__ part = partial + p.partial_off;
+    const double *__restrict__ part = partial + p.partial_off + (long)s * p.ngrid + g;
+    const long step = (long)p.nset * p.ngrid;
+    // eight loads in flight at a time (clamped indices, no branch around a load), summed in chunk order
     double v = 0.0;
-    for (int ch = 0; ch < nch; ++ch) v += part[((long)ch * p.nset + s) * p.ngrid + g];
+    for (int c0 = 0; c0 < nch; c0 += 8) {
+        double x[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) x[i] = part[min(c0 + i, nch - 1) * step];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v = c0 + i < nch ? v + x[i] : v;
+    }
     p.rhos[(long)s * p.stride_grid + g] += v;
 }
 

@@ -461,14 +461,24 @@ __device__ __attribute__((always_inline)) inline void resident_tile(
     epilogue(n2 > 0 ? T2 : T1);
 }
 
+// With the handle's order (every grid block evaluated): workgroup i takes item i / 8 of XCD
+// i % 8's list of (task, density matrix) items (the XCD it runs on); items of density matrices
+// past the evaluated ones and the padding exit at once.  Without it (leading blocks only): the
+// first ntasks tasks, task-major, in contiguous equal-count shares per XCD.
 template <int S, int NT, int DEPTH>
 __global__ __launch_bounds__(kThreads, 2) void rho_resident_kernel(
     double *__restrict__ partial, const unsigned char *__restrict__ base, const BlockDesc *__restrict__ blocks,
-    const int4 *__restrict__ tasks, int ntasks, const int *__restrict__ fg, int blksize, int ngrid, int nset) {
-    // (task, density matrix) pairs, task-major: the sets of a block's tasks, which share its Rq and
-    // Lc, sit next to each other in the XCD-contiguous order
-    const int idx = xcd_task(blockIdx.x, ntasks * nset);
-    resident_tile<S, NT, DEPTH>(partial, base, blocks, tasks[idx / nset], idx % nset, fg, blksize, ngrid, nset);
+    const int4 *__restrict__ tasks, int ntasks, const int2 *__restrict__ order, int order_len,
+    const int *__restrict__ fg, int blksize, int ngrid, int nset) {
+    int2 item;
+    if (order) {
+        item = order[(blockIdx.x % 8) * order_len + blockIdx.x / 8];
+        if (item.x < 0 || item.y >= nset) return;
+    } else {
+        const int idx = xcd_task(blockIdx.x, ntasks * nset);
+        item = int2{idx / nset, idx % nset};
+    }
+    resident_tile<S, NT, DEPTH>(partial, base, blocks, tasks[item.x], item.y, fg, blksize, ngrid, nset);
 }
 
 // One evaluation of a batch (device table).
@@ -509,8 +519,17 @@ __global__ void resident_batched_reduce(const EvalDesc *__restrict__ evals) {
     if (idx >= (long)e.nset * e.ngrid) return;
     const int s = (int)(idx / e.ngrid), g = (int)(idx % e.ngrid);
     const int nch = (e.counts[g / e.blksize] + kRows - 1) / kRows;
+    const double *__restrict__ part = e.partial + (long)s * e.ngrid + g;
+    const long step = (long)e.nset * e.ngrid;
+    // eight loads in flight at a time (clamped indices, no branch around a load), summed in tile order
     double v = 0.0;
-    for (int ch = 0; ch < nch; ++ch) v += e.partial[((long)ch * e.nset + s) * e.ngrid + g];
+    for (int c0 = 0; c0 < nch; c0 += 8) {
+        double x[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) x[i] = part[min(c0 + i, nch - 1) * step];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v = c0 + i < nch ? v + x[i] : v;
+    }
     e.rhos[(long)s * e.stride_rho + g] += v;
 }
 
@@ -540,6 +559,8 @@ struct ResidentRho {
     int *d_fg = nullptr;
     int4 *d_tasks = nullptr;
     std::vector<int> task_prefix;  // tasks of blocks < b, for b = 0..nblk
+    int2 *d_order = nullptr;        // [8][order_len] (task, density matrix) items per XCD, (-1, 0) padded
+    int order_len = 0;
 };
 
 template <int S>
@@ -660,6 +681,49 @@ int resident_create(hipStream_t stream, const double *dms, const double *ao_lhs,
     off += al(sizeof(int) * nblk * blksize);
     const size_t o_tasks = off;
     off += al(sizeof(int4) * std::max<size_t>(tasks.size(), 1));
+    // Evaluation order of the (task, density matrix) items.  Workgroup i runs on XCD i % 8 and
+    // an XCD starts its workgroups in index order as slots free up; a contiguous equal-count share
+    // per XCD leaves the XCDs with unequal work (a block's tasks take nks + 1 k steps each) and
+    // ends on long tasks.  Whole blocks (their tasks share Rq and Lc in the XCD's L2) go to the
+    // XCD with the least work so far, largest first, and each XCD's items run longest first.
+    std::vector<int2> order;  // [8][order_len], copied to the device below
+    {
+        std::vector<std::vector<int2>> xcd_items(8);
+        auto steps_of = [&](const int4 &t) {  // the k steps resident_tile runs for task t
+            const int kp = blocks[t.x].kpad, nks = kp / kK, ntl = kp / kRows;
+            if (!t.w) return nks;
+            const int T2 = ntl - 1 - t.y;
+            return (nks - t.y) + (T2 > t.y ? nks - T2 : 0);
+        };
+        std::vector<int> steps(tasks.size());
+        for (size_t i = 0; i < tasks.size(); ++i) steps[i] = steps_of(tasks[i]);
+        std::vector<int> order_b;
+        std::vector<long> work_b(nblk, 0);
+        for (size_t b = 0; b < nblk; ++b) {
+            for (int i = h->task_prefix[b]; i < h->task_prefix[b + 1]; ++i) work_b[b] += (long)steps[i] * nset;
+            if (h->task_prefix[b + 1] > h->task_prefix[b]) order_b.push_back((int)b);
+        }
+        std::stable_sort(order_b.begin(), order_b.end(), [&](int x, int y) { return work_b[x] > work_b[y]; });
+        long load[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int b : order_b) {
+            const int x = (int)(std::min_element(load, load + 8) - load);
+            load[x] += work_b[b];
+            for (int i = h->task_prefix[b]; i < h->task_prefix[b + 1]; ++i)
+                for (size_t sset = 0; sset < nset; ++sset) xcd_items[x].push_back(int2{i, (int)sset});
+        }
+        for (auto &l : xcd_items) {
+            std::stable_sort(l.begin(), l.end(), [&](const int2 &u, const int2 &v) { return steps[u.x] > steps[v.x]; });
+            h->order_len = std::max(h->order_len, (int)l.size());
+        }
+        order.assign((size_t)8 * h->order_len, int2{-1, 0});
+        for (int x = 0; x < 8; ++x) std::copy(xcd_items[x].begin(), xcd_items[x].end(), order.begin() + (size_t)x * h->order_len);
+    }
+    if ((size_t)8 * h->order_len > INT32_MAX) {
+        delete h;
+        return fail(CFGMM_EINVAL, "resident rho: too many (task, density matrix) items");
+    }
+    const size_t o_order = off;
+    off += al(sizeof(int2) * 8 * std::max(h->order_len, 1));
     h->bytes = off;
 
     // 3. allocate, upload the tables, slice
@@ -673,6 +737,7 @@ int resident_create(hipStream_t stream, const double *dms, const double *ao_lhs,
     h->d_counts = reinterpret_cast<int *>(h->base + o_counts);
     h->d_fg = reinterpret_cast<int *>(h->base + o_fg);
     h->d_tasks = reinterpret_cast<int4 *>(h->base + o_tasks);
+    h->d_order = reinterpret_cast<int2 *>(h->base + o_order);
     st = hip_status(hipMemcpyAsync(h->d_blocks, blocks.data(), sizeof(BlockDesc) * nblk, hipMemcpyHostToDevice, stream),
                     "resident rho: table H2D");
     if (st == CFGMM_OK)
@@ -682,6 +747,10 @@ int resident_create(hipStream_t stream, const double *dms, const double *ao_lhs,
         st = hip_status(hipMemcpyAsync(h->d_tasks, tasks.data(), sizeof(int4) * tasks.size(), hipMemcpyHostToDevice,
                                        stream),
                         "resident rho: tasks H2D");
+    if (st == CFGMM_OK && !order.empty())
+        st = hip_status(hipMemcpyAsync(h->d_order, order.data(), sizeof(int2) * order.size(), hipMemcpyHostToDevice,
+                                       stream),
+                        "resident rho: order H2D");
     if (st == CFGMM_OK && max_kpad > 0) {  // an all-empty mask has no tiles to fill
         switch (S) {
             case 2: st = prep_launch<2>(stream, h, dms, ao_lhs, ao_rhs, stride_ao, stride_grid, lists, max_kpad); break;
@@ -736,13 +805,15 @@ size_t resident_blksize(const ResidentRho *h) { return h->blksize; }
 int resident_device(const ResidentRho *h) { return h->device; }
 
 template <int S>
-static int eval_launch(hipStream_t stream, const ResidentRho *h, double *partial, dim3 grid, int ntasks, int ngrid,
-                       int nset) {
+static int eval_launch(hipStream_t stream, const ResidentRho *h, double *partial, int ntasks, int ngrid, int nset) {
     hipEvent_t t0 = trace_start(stream);
     const int depth = tune_get(CFGMM_TUNE_RESIDENT_DEPTH);
+    const bool all = ntasks == h->task_prefix[h->nblk];
+    const dim3 grid((unsigned)(all ? 8 * h->order_len : ntasks * nset));
+    const int2 *order = all ? h->d_order : nullptr;
 #define CFGMM_RES_LAUNCH(NT_, D_)                                                                                    \
     hipLaunchKernelGGL((rho_resident_kernel<S, NT_, D_>), grid, dim3(kThreads), 0, stream, partial, h->base,          \
-                       h->d_blocks, h->d_tasks, ntasks, h->d_fg, (int)h->blksize, ngrid, nset)
+                       h->d_blocks, h->d_tasks, ntasks, order, h->order_len, h->d_fg, (int)h->blksize, ngrid, nset)
     if (depth >= 2) {
         switch (h->nt) {
             case 1: CFGMM_RES_LAUNCH(1, 2); break;
@@ -776,13 +847,13 @@ int resident_eval(hipStream_t stream, const ResidentRho *h, double *rhos, size_t
     if (st != CFGMM_OK) return st;
     double *partial = static_cast<double *>(ws);
     if ((size_t)ntasks * nset_used > INT32_MAX) return fail(CFGMM_EINVAL, "resident rho: too many workgroups");
-    const dim3 grid((unsigned)(ntasks * nset_used), 1, 1);
+    const int ng = (int)ngrid_used, ns = (int)nset_used;
     switch (h->nslices) {
-        case 2: st = eval_launch<2>(stream, h, partial, grid, ntasks, (int)ngrid_used, (int)nset_used); break;
-        case 3: st = eval_launch<3>(stream, h, partial, grid, ntasks, (int)ngrid_used, (int)nset_used); break;
-        case 4: st = eval_launch<4>(stream, h, partial, grid, ntasks, (int)ngrid_used, (int)nset_used); break;
-        case 5: st = eval_launch<5>(stream, h, partial, grid, ntasks, (int)ngrid_used, (int)nset_used); break;
-        default: st = eval_launch<6>(stream, h, partial, grid, ntasks, (int)ngrid_used, (int)nset_used); break;
+        case 2: st = eval_launch<2>(stream, h, partial, ntasks, ng, ns); break;
+        case 3: st = eval_launch<3>(stream, h, partial, ntasks, ng, ns); break;
+        case 4: st = eval_launch<4>(stream, h, partial, ntasks, ng, ns); break;
+        case 5: st = eval_launch<5>(stream, h, partial, ntasks, ng, ns); break;
+        default: st = eval_launch<6>(stream, h, partial, ntasks, ng, ns); break;
     }
     if (st != CFGMM_OK) return st;
     return launch_rho_reduce(stream, rhos, partial, h->d_counts, stride_rho, ngrid_used, nset_used, h->blksize);
