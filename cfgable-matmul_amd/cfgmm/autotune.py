@@ -10,7 +10,7 @@ Tunables:
                   task (1, 2, 4);
   DGEMM        -- tile rows per raster group (1, 2, 4, 8, 16); for shapes with
                   fewer 128x128 tiles than workgroup slots, the tile edge (64,
-                  128) and the number of K ranges per tile (1..8);
+                  128) and the number of K ranges per tile (1..12);
   FP64 rho     -- density matrices per workgroup (1..4).
 """
 from __future__ import annotations
@@ -70,7 +70,7 @@ def tune_dgemm_group(run_gemm, stream, groups=(1, 2, 4, 8, 16), reps=3):
     return best[0], results
 
 
-def tune_dgemm_small(run_gemm, stream, tiles=(64, 128), splits=(1, 2, 3, 4, 5, 6, 8), reps=20):
+def tune_dgemm_small(run_gemm, stream, tiles=(64, 128), splits=(1, 2, 3, 4, 5, 6, 7, 8, 10, 12), reps=20):
     """Tile edge and K ranges for a DGEMM too small to fill the chip with 128x128 tiles."""
     old = {p: cfgmm.tune_get(p) for p in (cfgmm.TUNE_DGEMM_SMALL_TILE, cfgmm.TUNE_DGEMM_SMALL_SPLITS)}
     results = []

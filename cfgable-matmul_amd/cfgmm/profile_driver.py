@@ -9,7 +9,8 @@ Modes:
                  one-time preparation runs before the repetitions)
   dgemm_large    FP64 DGEMM 3527x9583x6581 NN (dgemm_kernel)
   dgemm_small    FP64 DGEMM 1031x977x1543 NT
-  dgemm_small_tuned  the same with the autotuned small plan (64x64 tiles, 6 K ranges)
+  dgemm_small_tuned  the same with the autotuned small plan (the cached choice of class dgemm_small,
+                 else 64x64 tiles x 6 K ranges)
   masked_gemm    the reference's masked GEMM test: 2228x16384x2228 NN, sin/cos, 30 % periodic mask,
                  BLKSIZE 48 (masked_gemm_kernel)
   gemm_split<S>  the large GEMM in split precision (ozaki_gemm_kernel)
@@ -69,8 +70,10 @@ def main(argv: list[str]) -> int:
     elif mode in ("dgemm_large", "dgemm_small", "dgemm_small_tuned") or mode.startswith("gemm_split"):
         g = W.dgemm_small_nt() if mode.startswith("dgemm_small") else W.dgemm_large_nn()
         if mode == "dgemm_small_tuned":
-            cfgmm.tune_set(cfgmm.TUNE_DGEMM_SMALL_TILE, 64)
-            cfgmm.tune_set(cfgmm.TUNE_DGEMM_SMALL_SPLITS, 6)
+            from cfgmm import autotune
+            if not autotune.apply("dgemm_small"):  # no cached choice: the usual one
+                cfgmm.tune_set(cfgmm.TUNE_DGEMM_SMALL_TILE, 64)
+                cfgmm.tune_set(cfgmm.TUNE_DGEMM_SMALL_SPLITS, 6)
         prec, ns = (cfgmm.PREC_FP64, 0) if mode.startswith("dgemm") else (cfgmm.PREC_OZAKI_INT8,
                                                                            int(mode[len("gemm_split"):]))
         d_a = cfgmm.DeviceBuffer.from_array(g.a)
