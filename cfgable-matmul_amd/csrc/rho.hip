@@ -157,7 +157,8 @@ __global__ __launch_bounds__(kPlanThreads) void rho_plan_kernel(
         while (tj * (tj + 1) / 2 > k) --tj;
         while ((tj + 1) * (tj + 2) / 2 <= k) ++tj;
         const int ti = (int)(k - tj * (tj + 1) / 2);
-        if (rho::tile_pair_differs(dms + (long)s * stride_ao * stride_ao, stride_ao, nao, ti, (int)tj, t) && tid == 0)
+        if (rho::tile_pair_differs<kPlanThreads>(dms + (long)s * stride_ao * stride_ao, stride_ao, nao, ti, (int)tj, t) &&
+            tid == 0)
             __hip_atomic_store(symflag, (int)epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         return;
     }
@@ -242,7 +243,7 @@ __global__ __launch_bounds__(256) void dm_symmetry_kernel(const double *__restri
     const int tj = blockIdx.x, ti = blockIdx.y, s = blockIdx.z;
     if (ti > tj) return;
     __shared__ unsigned long long t[64][65];
-    if (rho::tile_pair_differs(dms + (long)s * stride_ao * stride_ao, stride_ao, nao, ti, tj, t) &&
+    if (rho::tile_pair_differs<256>(dms + (long)s * stride_ao * stride_ao, stride_ao, nao, ti, tj, t) &&
         threadIdx.x == 0)
         __hip_atomic_store(sym, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }

@@ -74,7 +74,7 @@ __global__ __launch_bounds__(256) void batch_symmetry(SymCheck *__restrict__ che
     if (tj * 64 >= p.nao) return;
     __shared__ unsigned long long t[64][65];
     for (int s = 0; s < p.nset; ++s) {
-        if (rho::tile_pair_differs(p.dms + (long)s * p.stride_ao * p.stride_ao, p.stride_ao, p.nao, ti, (int)tj, t)) {
+        if (rho::tile_pair_differs<256>(p.dms + (long)s * p.stride_ao * p.stride_ao, p.stride_ao, p.nao, ti, (int)tj, t)) {
             if (threadIdx.x == 0) __hip_atomic_store(&p.flag, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             return;  // the answer is the workgroup's: every thread leaves together
         }

@@ -271,22 +271,41 @@ __device__ inline void rho_chunk_body(const RhoChunk &task, unsigned char *__res
 
 
 // Whether the 64x64 tiles (ti, tj) and (tj, ti) of the nao x nao matrix d (row stride ld) are
-// not each other's transpose, bitwise; the same answer on every thread of the workgroup (a
-// multiple of 64 threads up to 4096, all of which must call it).  t: LDS scratch of 64 x 65 words.
+// not each other's transpose, bitwise; the same answer on every thread of the workgroup (NT
+// threads, a multiple of 64 dividing 4096, all of which must call it).  t: LDS scratch of 64 x 65
+// words.
+template <int NT>
 __device__ inline bool tile_pair_differs(const double *__restrict__ d, long ld, int nao, int ti, int tj,
                                          unsigned long long (*t)[65]) {
-    const int c = threadIdx.x & 63, r0 = threadIdx.x >> 6, dr = blockDim.x >> 6;
+    // NT threads: 64 / (NT / 64) rows each, all loads of a phase issued before any is used (clamped
+    // in-bounds addresses, values outside the matrix replaced afterwards)
+    constexpr int DR = NT / 64, PER = 64 / DR;
+    static_assert(DR * PER == 64 && DR * 64 == NT, "a multiple of 64 threads dividing 4096");
+    const int c = threadIdx.x & 63, r0 = threadIdx.x >> 6;
     const int i0 = ti * 64, j0 = tj * 64;
+    unsigned long long v[PER];
     __syncthreads();  // an earlier call's readers of t are done
-    for (int r = r0; r < 64; r += dr) {
-        const int i = i0 + r, j = j0 + c;
-        t[r][c] = i < nao && j < nao ? (unsigned long long)__double_as_longlong(d[(long)i * ld + j]) : 0ull;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int i = i0 + r0 + q * DR, j = j0 + c;
+        v[q] = (unsigned long long)__double_as_longlong(d[(long)min(i, nao - 1) * ld + min(j, nao - 1)]);
+    }
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int i = i0 + r0 + q * DR, j = j0 + c;
+        t[r0 + q * DR][c] = i < nao && j < nao ? v[q] : 0ull;
     }
     __syncthreads();
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {  // row j0 + r, column i0 + c of d against t[c][r]
+        const int j = j0 + r0 + q * DR, i = i0 + c;
+        v[q] = (unsigned long long)__double_as_longlong(d[(long)min(j, nao - 1) * ld + min(i, nao - 1)]);
+    }
     bool differ = false;
-    for (int r = r0; r < 64; r += dr) {  // row j0 + r, column i0 + c of d against t[c][r]
-        const int j = j0 + r, i = i0 + c;
-        if (i < nao && j < nao) differ |= (unsigned long long)__double_as_longlong(d[(long)j * ld + i]) != t[c][r];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int r = r0 + q * DR, j = j0 + r, i = i0 + c;
+        differ |= i < nao && j < nao && v[q] != t[c][r];
     }
     return __syncthreads_or(differ ? 1 : 0) != 0;
 }
