@@ -750,7 +750,9 @@ def bench_serving(args):
            "config": {"workload": "serving: Poisson arrivals, 70 % masked rho (8 molecules, nao 500..4000, ngrid "
                                   "4096..65536, nset 1..4, density 10..60 %), 30 % DGEMM m,n,k log-uniform 256..4096",
                       "generator": "pool: workloads.serving_pool(n=8, seed=500): nao ~ U{500..4000}, ngrid = 64 "
-                                   "U{64..1024}, density ~ U(0.1, 0.6), 4 PSD DMs, every draw kept; requests: "
+                                   "nblk with molecule k's nblk uniform in the k-th eighth of 64..1024 (the pool "
+                                   "spans 4096..65536 points), density ~ U(0.1, 0.6), 4 PSD DMs, every draw kept; "
+                                   "requests: "
                                    "workloads.serving_requests(pool, n, lambda, seed): rho on a uniform pool "
                                    "molecule over its leading 64 U{64..nblk} points (4096 and up) with U{1..4} "
                                    "DMs, else a DGEMM with m, n, k log-uniform in 256..4096",

@@ -353,32 +353,50 @@ def tile_bytes_estimate(p: "RhoProblem", nslices: int = 6) -> float:
 
 
 def serving_pool(n=8, seed=500, nao_range=(500, 4000), blocks_range=(64, 1024), density_range=(0.1, 0.6), nset=4,
-                 blksize=64, max_tile_bytes=None):
-    """Molecules for the serving mix: nao ~ U{nao_range} (shells ~ nao/3), ngrid = blksize * U{blocks_range}
+                 blksize=64, max_tile_bytes=None, threads=None, stratify=True):
+    """Molecules for the serving mix: nao ~ U{nao_range} (shells ~ nao/3), ngrid = blksize * nblk
     (4096 .. 65536 points at the defaults), mask density ~ U(density_range), nset PSD DMs.  With
-    max_tile_bytes, a molecule whose 6-slice resident tiles would exceed it is redrawn; by default
-    every draw is kept."""
+    stratify, molecule k draws nblk from the k-th of n equal parts of blocks_range (so the pool
+    spans the whole range); otherwise nblk ~ U{blocks_range}.  With max_tile_bytes, a molecule
+    whose 6-slice resident tiles would exceed it is redrawn; by default every draw is kept.  The
+    draws are sequential; the molecules of kept draws are generated on a thread pool (each depends
+    only on its own seed)."""
+    import os
+    from concurrent.futures import ThreadPoolExecutor
+
     rng = np.random.default_rng(seed)
+
+    def make(i, nao, nshell, nblk, dens):
+        return rho_problem(seed=seed * 1000 + i, nshell=nshell, nao=nao, ngrid=nblk * blksize, nset=nset,
+                           blksize=blksize, density=dens, nocc=min(200, nao))
     pool = []
     i = 0
     while len(pool) < n:
-        nao = int(rng.integers(nao_range[0], nao_range[1] + 1))
-        nshell = max(1, nao // 3)
-        if (nao - nshell) % 2:
-            nao += 1
-        nblk = int(rng.integers(blocks_range[0], blocks_range[1] + 1))
-        dens = float(rng.uniform(*density_range))
-        i += 1
-        if max_tile_bytes is not None:
-            # cheap pre-check before generating: blocks of about dens * nao AOs
-            kp = np.ceil(dens * nao / 64.0) * 64.0
-            if nblk * (nset * 6 * kp * kp + blksize * kp * 14) > max_tile_bytes:
+        draws = []
+        while len(pool) + len(draws) < n:
+            nao = int(rng.integers(nao_range[0], nao_range[1] + 1))
+            nshell = max(1, nao // 3)
+            if (nao - nshell) % 2:
+                nao += 1
+            lo, hi = blocks_range
+            if stratify:
+                kk, span = len(pool) + len(draws), hi - lo + 1
+                lo, hi = blocks_range[0] + span * kk // n, blocks_range[0] + span * (kk + 1) // n - 1
+            nblk = int(rng.integers(lo, hi + 1))
+            dens = float(rng.uniform(*density_range))
+            i += 1
+            if max_tile_bytes is not None:
+                # cheap pre-check before generating: blocks of about dens * nao AOs
+                kp = np.ceil(dens * nao / 64.0) * 64.0
+                if nblk * (nset * 6 * kp * kp + blksize * kp * 14) > max_tile_bytes:
+                    continue
+            draws.append((i, nao, nshell, nblk, dens))
+        with ThreadPoolExecutor(threads or min(16, os.cpu_count() or 1)) as ex:
+            probs = list(ex.map(lambda d: make(*d), draws))
+        for d, p in zip(draws, probs):
+            if max_tile_bytes is not None and tile_bytes_estimate(p) > max_tile_bytes:
                 continue
-        p = rho_problem(seed=seed * 1000 + i, nshell=nshell, nao=nao, ngrid=nblk * blksize, nset=nset,
-                        blksize=blksize, density=dens, nocc=min(200, nao))
-        if max_tile_bytes is not None and tile_bytes_estimate(p) > max_tile_bytes:
-            continue
-        pool.append(Molecule(seed * 1000 + i, p))
+            pool.append(Molecule(seed * 1000 + d[0], p))
     return pool
 
 

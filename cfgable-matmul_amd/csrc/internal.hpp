@@ -134,9 +134,10 @@ struct ResidentEvalReq {
 };
 int resident_eval_batched(hipStream_t stream, const ResidentEvalReq *reqs, size_t n);
 size_t resident_bytes(const ResidentRho *h);
-// Device bytes a handle over this mask (host copy, nao x ldtab) takes at most.
+// Device bytes a handle over this mask (host copy, nao x ldtab) takes at most; `symmetric`: every
+// D_s is symmetric and lhs is rhs (the upper-triangle form).
 size_t resident_bytes_bound(const uint8_t *tab, size_t ldtab, size_t nao, size_t nblk, size_t nset, size_t blksize,
-                            int nslices);
+                            int nslices, bool symmetric);
 size_t resident_ngrid(const ResidentRho *h);
 size_t resident_nset(const ResidentRho *h);
 size_t resident_blksize(const ResidentRho *h);

@@ -783,19 +783,25 @@ int resident_destroy(ResidentRho *h) {
 size_t resident_bytes(const ResidentRho *h) { return h ? h->bytes : 0; }
 
 size_t resident_bytes_bound(const uint8_t *tab, size_t ldtab, size_t nao, size_t nblk, size_t nset, size_t blksize,
-                            int nslices) {
-    // the layout of resident_create with every block in the general form (the symmetric form
-    // stores about half of each D plane), from the mask on the host
+                            int nslices, bool symmetric) {
+    // the layout of resident_create, from the mask on the host: the upper-triangle D planes when
+    // the caller knows every D_s to be symmetric (and lhs is rhs) and no block is too long for
+    // that form, else full planes
     auto al = [](size_t x) { return div_up(x, 256) * 256; };
     std::vector<size_t> counts(nblk, 0);
     for (size_t a = 0; a < nao; ++a)
         for (size_t b = 0; b < nblk; ++b) counts[b] += tab[a * ldtab + b] != 0;
-    size_t bytes = al(sizeof(BlockDesc) * nblk) + al(sizeof(int) * nblk) + al(sizeof(int) * nblk * blksize);
+    size_t max_kpad = 0;
+    for (size_t b = 0; b < nblk; ++b) max_kpad = std::max(max_kpad, div_up(counts[b], (size_t)kK) * kK);
+    const int sym = symmetric && max_kpad <= (size_t)kMaxKpadSym ? 1 : 0;
+    size_t bytes = al(sizeof(BlockDesc) * nblk) + al(sizeof(int) * nblk) + al(sizeof(int) * nblk * blksize) +
+                   al(sizeof(int2) * 8 * (1 + nblk * div_up(max_kpad, (size_t)kRows) * div_up(blksize, (size_t)16) * nset));
     const size_t S = (size_t)nslices;
     for (size_t b = 0; b < nblk; ++b) {
         const size_t kpad = div_up(counts[b], (size_t)kK) * kK;
-        bytes += al(nset * S * kpad * kpad) + al(S * blksize * kpad) + al(sizeof(double) * kpad * blksize) +
-                 al(sizeof(int) * nset * kpad) + al(sizeof(int4) * (kpad / kRows) * div_up(blksize, (size_t)16));
+        bytes += al(nset * S * (size_t)dq_plane_bytes((int)kpad, sym)) + al(S * blksize * kpad) +
+                 al(sizeof(double) * kpad * blksize) + al(sizeof(int) * nset * kpad) +
+                 al(sizeof(int4) * (kpad / kRows) * div_up(blksize, (size_t)16));
     }
     return bytes;
 }

@@ -273,9 +273,22 @@ int server_add_molecule(Server *sv, int id, const double *dms, const double *phi
     m.nset = nset;
     m.blksize = blksize;
     m.nblk = div_up(ngrid, blksize);
-    const size_t cap = sv->cfg.split_tile_cap_bytes ? sv->cfg.split_tile_cap_bytes : sv->cfg.hbm_budget_bytes / 4;
-    m.split = sv->cfg.rho_precision == CFGMM_PREC_OZAKI_INT8 &&
-              resident_bytes_bound(tab, m.nblk, nao, m.nblk, nset, blksize, sv->cfg.rho_nslices) <= cap;
+    if (sv->cfg.rho_precision == CFGMM_PREC_OZAKI_INT8) {
+        // the tiles' size: the upper-triangle form when every D_s is symmetric, bitwise (phi is on
+        // both sides), as resident_create will find on the device
+        bool sym = true;
+        for (size_t s = 0; s < nset && sym; ++s) {
+            const double *d = dms + s * nao * nao;
+            for (size_t i = 0; i < nao && sym; ++i)
+                for (size_t j = i + 1; j < nao; ++j)
+                    if (std::memcmp(d + i * nao + j, d + j * nao + i, sizeof(double)) != 0) {
+                        sym = false;
+                        break;
+                    }
+        }
+        const size_t cap = sv->cfg.split_tile_cap_bytes ? sv->cfg.split_tile_cap_bytes : sv->cfg.hbm_budget_bytes / 4;
+        m.split = resident_bytes_bound(tab, m.nblk, nao, m.nblk, nset, blksize, sv->cfg.rho_nslices, sym) <= cap;
+    }
     sv->mols[id] = m;
     return CFGMM_OK;
 }

@@ -175,7 +175,7 @@ def test_server_tile_cap_serves_large_molecules_in_fp64():
         held.append(sv.stats()["held_bytes"] - base.stats()["held_bytes"])
         sv.destroy()
         base.destroy()
-    cap = 2 * held[0]  # the cap is checked against a bound that counts whole D planes (at most 2x the tiles)
+    cap = 2 * held[0]  # between the two molecules' tile sizes (the server checks a bound of the tiles)
     assert cap < held[1], held  # the sizes this test relies on
     want = {}
     for prec in (cfgmm.PREC_FP64, cfgmm.PREC_OZAKI_INT8):
