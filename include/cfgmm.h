@@ -63,9 +63,12 @@ int cfgmm_version(void);
 /* ---- precision modes (split-precision variants) -------------------------- */
 enum cfgmm_precision {
     CFGMM_PREC_FP64 = 0,        /* plain FP64 MFMA */
-    CFGMM_PREC_OZAKI_INT8 = 1   /* Ozaki-scheme splitting of f64 into `nslices` int8 slices (7 bits each,
-                                   per-row / per-column power-of-two scaling), slice products on the int8
-                                   MFMA with exact int32 accumulation, FP64 recombination; nslices in [2, 6] */
+    CFGMM_PREC_OZAKI_INT8 = 1   /* Ozaki-scheme splitting of f64 into `nslices` int8 slices (per-row /
+                                   per-column power-of-two scaling), slice products on the int8 MFMA with
+                                   exact int32 accumulation, FP64 recombination; nslices in [2, 6].  GEMM and
+                                   resident rho: balanced radix-254 digits, about 8 bits per slice (5 slices
+                                   within 1e-10 on the tested workloads); per-call rho: 7-bit digits (6
+                                   slices for 1e-10) */
 };
 
 /* ======================================================================== */
@@ -183,7 +186,13 @@ int cfgmm_aodmao2rho_dev_noalloc(void *stream, double *rhos, const double *dms, 
                                  const double *ao_rhs, size_t stride_ao, size_t stride_grid, size_t nao, size_t ngrid,
                                  size_t nset, const uint8_t *tab, size_t ldtab, size_t blksize);
 
-/* Split-precision rho (no counterpart in the reference). */
+/* Split-precision rho (no counterpart in the reference).  Per call the split form gathers and
+ * slices D_s[L_b, L_b] and phi for every grid block before its int8 products, and that costs more
+ * than the FP64 evaluation saves: on the tested workloads it runs 4-7x slower than FP64 (config of
+ * 2228 AOs x 16384 points: 4.2 ms at 6 slices against 0.6 ms).  It is there for accuracy studies
+ * and as the split path of the batched call; for speed, evaluate in FP64 per call, or keep a
+ * resident handle (2d) when the same (D, phi, mask) is evaluated repeatedly (about 2x FP64 per
+ * evaluation).  6 slices keep 1e-10; 5 do not. */
 int cfgmm_aodmao2rho_ex_dev(void *stream, double *rhos, const double *dms, const double *ao_lhs,
                             const double *ao_rhs, size_t stride_ao, size_t stride_grid, size_t nao, size_t ngrid,
                             size_t nset, const uint8_t *tab, size_t ldtab, size_t blksize, int precision,
