@@ -342,8 +342,13 @@ def bench_dgemm(g: W.GemmProblem, reps: int, warmup: int = 2, precision=cfgmm.PR
     s.synchronize()
     step_ms = e0.elapsed_ms(e1) / reps
     split = precision != cfgmm.PREC_FP64
-    names = ["ozaki_gemm_kernel", "ozaki_split"] if split else ["dgemm_kernel"]
+    names = ["ozaki_gemm_kernel", "ozaki_split"] if split else ["dgemm_kernel", "dgemm_kgroup_kernel"]
     tr = traced_kernel_ms(call, s, names, reps)
+    if not split:  # one of the two kernels runs, by plan
+        kern = [v for v in (tr["dgemm_kernel"], tr["dgemm_kgroup_kernel"]) if v == v]
+        tr["dgemm_kernel"] = sum(kern) if kern else float("nan")
+        plan = "in-workgroup K split (dgemm_kgroup_kernel)" if tr["dgemm_kgroup_kernel"] == tr["dgemm_kgroup_kernel"] \
+            else "dgemm_kernel"
     for x in (dA, dB, dC):
         x.free()
     e0.destroy()
@@ -360,7 +365,7 @@ def bench_dgemm(g: W.GemmProblem, reps: int, warmup: int = 2, precision=cfgmm.PR
         del out["frac_fp64_peak"]
     else:
         out.update(dgemm_kernel_avg_ms=round(tr["dgemm_kernel"], 4),
-                   kernel_tflops=round(g.flops / (tr["dgemm_kernel"] * 1e-3) / 1e12, 2))
+                   kernel_tflops=round(g.flops / (tr["dgemm_kernel"] * 1e-3) / 1e12, 2), plan=plan)
     if err is not None:
         out["rel_err_vs_fp64_gpu"] = err
     return out
@@ -434,13 +439,13 @@ def bench_dgemm_tuned(g: W.GemmProblem, reps: int):
         x.free()
     s.destroy()
     autotune.save({"dgemm_small": best})
-    cfgmm.tune_set(cfgmm.TUNE_DGEMM_SMALL_TILE, best["small_tile"])
-    cfgmm.tune_set(cfgmm.TUNE_DGEMM_SMALL_SPLITS, best["small_splits"])
+    autotune.apply("dgemm_small")
     try:
         out = bench_dgemm(g, reps)
     finally:
         cfgmm.tune_set(cfgmm.TUNE_DGEMM_SMALL_TILE, 0)
         cfgmm.tune_set(cfgmm.TUNE_DGEMM_SMALL_SPLITS, 0)
+        cfgmm.tune_set(cfgmm.TUNE_DGEMM_KGROUPS, 0)
     out["tuned"] = best
     return out
 

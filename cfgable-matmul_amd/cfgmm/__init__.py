@@ -44,6 +44,7 @@ TUNE_RESIDENT_DEPTH = 6
 TUNE_RESIDENT_COLS = 7
 TUNE_RHO_ORDER = 8
 TUNE_RHO_SYMMETRIC = 9
+TUNE_DGEMM_KGROUPS = 10
 
 _lib = None
 

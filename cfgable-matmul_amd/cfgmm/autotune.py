@@ -9,8 +9,9 @@ Tunables:
   batched rho  -- persistent workgroups per CU (1..4) and density matrices per
                   task (1, 2, 4);
   DGEMM        -- tile rows per raster group (1, 2, 4, 8, 16); for shapes with
-                  fewer 128x128 tiles than workgroup slots, the tile edge (64,
-                  128) and the number of K ranges per tile (1..12);
+                  fewer 128x128 tiles than workgroup slots, the in-workgroup K
+                  split or the tile edge (64, 128) and the number of K ranges
+                  per tile (1..12);
   FP64 rho     -- density matrices per workgroup (1..4).
 """
 from __future__ import annotations
@@ -71,14 +72,18 @@ def tune_dgemm_group(run_gemm, stream, groups=(1, 2, 4, 8, 16), reps=3):
 
 
 def tune_dgemm_small(run_gemm, stream, tiles=(64, 128), splits=(1, 2, 3, 4, 5, 6, 7, 8, 10, 12), reps=20):
-    """Tile edge and K ranges for a DGEMM too small to fill the chip with 128x128 tiles."""
-    old = {p: cfgmm.tune_get(p) for p in (cfgmm.TUNE_DGEMM_SMALL_TILE, cfgmm.TUNE_DGEMM_SMALL_SPLITS)}
+    """Plan of a DGEMM too small to fill the chip with 128x128 tiles: the in-workgroup K split (where
+    it applies), or the tile edge and K ranges of the split over workgroups."""
+    params = (cfgmm.TUNE_DGEMM_KGROUPS, cfgmm.TUNE_DGEMM_SMALL_TILE, cfgmm.TUNE_DGEMM_SMALL_SPLITS)
+    old = {p: cfgmm.tune_get(p) for p in params}
+    cands = [{"kgroups": 0, "small_tile": 0, "small_splits": 0}] + \
+        [{"kgroups": 1, "small_tile": t, "small_splits": sp} for t, sp in itertools.product(tiles, splits)]
     results = []
     try:
-        for t, sp in itertools.product(tiles, splits):
-            cfgmm.tune_set(cfgmm.TUNE_DGEMM_SMALL_TILE, t)
-            cfgmm.tune_set(cfgmm.TUNE_DGEMM_SMALL_SPLITS, sp)
-            results.append(({"small_tile": t, "small_splits": sp}, _time(run_gemm, stream, reps, warmup=3)))
+        for cand in cands:
+            for p, key in zip(params, ("kgroups", "small_tile", "small_splits")):
+                cfgmm.tune_set(p, cand[key])
+            results.append((cand, _time(run_gemm, stream, reps, warmup=3)))
     finally:
         for p, v in old.items():
             cfgmm.tune_set(p, v)
@@ -135,6 +140,8 @@ def apply(workload_class: str, path=None) -> dict:
         cfgmm.tune_set(cfgmm.TUNE_DGEMM_SMALL_TILE, int(cfg["small_tile"]))
     if "small_splits" in cfg:
         cfgmm.tune_set(cfgmm.TUNE_DGEMM_SMALL_SPLITS, int(cfg["small_splits"]))
+    if "kgroups" in cfg:
+        cfgmm.tune_set(cfgmm.TUNE_DGEMM_KGROUPS, int(cfg["kgroups"]))
     if "rho_max_sets" in cfg:
         cfgmm.tune_set(cfgmm.TUNE_RHO_MAX_SETS, int(cfg["rho_max_sets"]))
     return cfg

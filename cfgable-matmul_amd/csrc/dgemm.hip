@@ -252,7 +252,9 @@ __global__ __launch_bounds__(kThreads, 2) void dgemm_kernel(double *__restrict__
     };
     // every wave's copies landed (vmcnt(0): expcnt and lgkmcnt left at their maxima) and every
     // wave is done with the buffers read before; an LDS-only barrier, since __syncthreads()
-    // would add a fence
+    // would add a fence.  (The scheduler sinks some of the step's MFMAs below the wait; pinning
+    // them above it with a scheduling barrier measured no faster here, the second workgroup on
+    // the CU covering the copies.)
     auto sync = [&]() {
         __builtin_amdgcn_s_waitcnt((0x7 << 4) | (0xF << 8));
         asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
@@ -336,6 +338,159 @@ __global__ __launch_bounds__(kThreads, 2) void dgemm_kernel(double *__restrict__
 #pragma unroll
                 for (int r = 0; r < 4; ++r)
                     pt[(wm * WM + mi * 16 + lk + 4 * r) * BN + wn * WN + ni * 16 + lrow] = acc[mi][ni][r];
+    }
+}
+
+// Small GEMMs with both operands K-contiguous (A not transposed, B transposed): one workgroup of
+// KG groups of four waves per BM x BN tile of C, at most one tile per CU.  Group g takes the k
+// steps KG i + g; each group copies its own A and B slices straight into its own pair of LDS
+// stages (the layout of the large kernel's K-contiguous operands), and every wave of a group owns
+// a 16-row strip of the tile.  The groups' sums are added in group order through LDS at the end:
+// no partial tiles in memory and no second launch, which the K split of the large kernel needs
+// for shapes with fewer tiles than workgroup slots.  The steps past the last KG-multiple (and the
+// k tail) go through registers with zero fill.
+__device__ inline int kc_off(int r, int kq) {
+    return r * 128 + ((((kq >> 1) ^ (r >> 1)) & 7) << 4) + ((kq & 1) << 3);
+}
+
+template <int BN, int KG>
+__global__ __launch_bounds__(256 * KG) void dgemm_kgroup_kernel(double *__restrict__ c, const double *__restrict__ a,
+                                                               const double *__restrict__ b, int m, int n, int k,
+                                                               long lda, long ldb, long ldc, int group_m) {
+    constexpr int BM = 64, BK = 16, NW = 4;  // four 16-row strips per group
+    constexpr int FN = BN / 16;
+    constexpr int A_BYTES = BM * 128, B_BYTES = BN * 128;
+    constexpr int UNITS = (BM + BN) / 8;  // 1 KiB copy instructions per slice pair
+    constexpr int PER_T = (BM + BN) * BK / 256;
+    static_assert(BN % 16 == 0 && (BM + BN) * BK % 256 == 0, "tile shape");
+    // the two stages are __shared__ objects of their own (see dgemm_kernel)
+    __shared__ __attribute__((aligned(16))) unsigned char S0[KG][A_BYTES + B_BYTES];
+    __shared__ __attribute__((aligned(16))) unsigned char S1[KG][A_BYTES + B_BYTES];
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int g = wave / NW, sw = wave % NW, gt = tid % 256;  // group, strip, thread in group
+    const int lrow = lane & 15, lk = lane >> 4;
+    const int tiles_m = (m + BM - 1) / BM, tiles_n = (n + BN - 1) / BN;
+    int tm, tn;  // each XCD takes a contiguous range of the raster order (the panels it shares stay in its L2)
+    raster_of(xcd_remap(blockIdx.x, tiles_m * tiles_n), tiles_m, tiles_n, group_m, tm, tn);
+    const int m0 = tm * BM, n0 = tn * BN;
+    const int nfull = k / BK;
+    const int R = nfull / KG;                    // rounds in which every group has a full step
+    const bool tail = R * KG * BK < k;           // one more round through registers
+
+    auto issue = [&](int t, unsigned char *dst) {
+        const int k0 = t * BK;
+        for (int u = sw; u < UNITS; u += NW) {
+            const bool isa = u < BM / 8;
+            const int uu = isa ? u : u - BM / 8;
+            const int P = uu * 64 + lane;
+            const int r = P >> 3, ch = ((P & 7) ^ (r >> 1)) & 7;
+            const double *src = isa ? a + (long)min(m0 + r, m - 1) * lda : b + (long)min(n0 + r, n - 1) * ldb;
+            copy16_to_lds(src + k0 + 2 * ch, dst + (isa ? 0 : A_BYTES) + uu * 1024);
+        }
+    };
+    f64x4 acc[FN];
+#pragma unroll
+    for (int ni = 0; ni < FN; ++ni) acc[ni] = f64x4{0.0, 0.0, 0.0, 0.0};
+    auto compute = [&](const unsigned char *st) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const f64x2 fa = *reinterpret_cast<const f64x2 *>(st + kc_off(sw * 16 + lrow, 4 * lk + 2 * h));
+            f64x2 fb[FN];
+#pragma unroll
+            for (int ni = 0; ni < FN; ++ni)
+                fb[ni] = *reinterpret_cast<const f64x2 *>(st + A_BYTES + kc_off(ni * 16 + lrow, 4 * lk + 2 * h));
+#pragma unroll
+            for (int kk = 0; kk < 2; ++kk)
+#pragma unroll
+                for (int ni = 0; ni < FN; ++ni)
+                    acc[ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[kk], fb[ni][kk], acc[ni], 0, 0, 0);
+        }
+    };
+    auto sync = [&]() {  // as in dgemm_kernel, the step's MFMAs pinned above the wait (one workgroup per CU)
+        __builtin_amdgcn_sched_barrier(0);
+        __builtin_amdgcn_s_waitcnt((0x7 << 4) | (0xF << 8));
+        asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+    };
+
+    if (R > 0) {
+        const int last = R - 1;
+        issue(g, S0[g]);
+        int i = 0;
+        // prefetch indices clamped rather than guarded (see dgemm_kernel)
+        for (; i + 2 <= R; i += 2) {
+            sync();
+            issue(min(i + 1, last) * KG + g, S1[g]);
+            compute(S0[g]);
+            sync();
+            issue(min(i + 2, last) * KG + g, S0[g]);
+            compute(S1[g]);
+        }
+        if (i < R) {
+            sync();
+            issue(last * KG + g, S1[g]);
+            compute(S0[g]);
+        }
+        sync();
+    }
+    if (tail) {
+        // step R KG + g of each group through registers, zero past k (and past the matrix)
+        const int k0 = (R * KG + g) * BK;
+        double v[PER_T];
+#pragma unroll
+        for (int e = 0; e < PER_T; ++e) {
+            const int idx = gt + 256 * e;
+            const bool isa = idx < BM * BK;
+            const int id2 = isa ? idx : idx - BM * BK;
+            const int r = id2 / BK, p = k0 + id2 % BK;
+            const int row = (isa ? m0 : n0) + r;
+            const bool ok = p < k && row < (isa ? m : n);
+            v[e] = ok ? (isa ? a[(long)row * lda + p] : b[(long)row * ldb + p]) : 0.0;
+        }
+#pragma unroll
+        for (int e = 0; e < PER_T; ++e) {
+            const int idx = gt + 256 * e;
+            const bool isa = idx < BM * BK;
+            const int id2 = isa ? idx : idx - BM * BK;
+            *reinterpret_cast<double *>(S0[g] + (isa ? 0 : A_BYTES) + kc_off(id2 / BK, id2 % BK)) = v[e];
+        }
+        __syncthreads();
+        compute(S0[g]);
+        __syncthreads();
+    }
+
+    // groups 1 .. KG-1 hand their sums to group 0 through LDS, one group at a time, in group order
+    double *red = reinterpret_cast<double *>(&S1[0][0]);  // NW x FN x 4 x 64 doubles
+    static_assert(NW * FN * 4 * 64 * 8 <= KG * (A_BYTES + B_BYTES), "reduction buffer");
+    for (int g2 = 1; g2 < KG; ++g2) {
+        if (g == g2) {
+#pragma unroll
+            for (int ni = 0; ni < FN; ++ni)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) red[((sw * FN + ni) * 4 + r) * 64 + lane] = acc[ni][r];
+        }
+        __syncthreads();
+        if (g == 0) {
+#pragma unroll
+            for (int ni = 0; ni < FN; ++ni)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) acc[ni][r] += red[((sw * FN + ni) * 4 + r) * 64 + lane];
+        }
+        __syncthreads();
+    }
+    if (g != 0) return;
+    // C += acc: a strip row's C values loaded together from clamped addresses, then stored
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int row = m0 + sw * 16 + lk + 4 * r;
+        double cv[FN];
+#pragma unroll
+        for (int ni = 0; ni < FN; ++ni) cv[ni] = c[(long)min(row, m - 1) * ldc + min(n0 + ni * 16 + lrow, n - 1)];
+#pragma unroll
+        for (int ni = 0; ni < FN; ++ni) {
+            const int col = n0 + ni * 16 + lrow;
+            if (row < m && col < n) c[(long)row * ldc + col] = cv[ni] + acc[ni][r];
+        }
     }
 }
 
@@ -486,7 +641,33 @@ int launch_dgemm(hipStream_t stream, double *c, const double *a, const double *b
     if (m == 0 || n == 0 || k == 0) return CFGMM_OK;
     if (m > INT32_MAX || n > INT32_MAX || k > INT32_MAX) return fail(CFGMM_EINVAL, "dgemm: dimension exceeds 2^31-1");
     const int im = (int)m, in = (int)n, ik = (int)k;
-    const DgemmPlan pl = plan_dgemm(im, in, ik, device_cu_count());
+    const int cus = device_cu_count();
+    // too few 128x128 tiles to fill the chip, both operands K-contiguous: the in-workgroup K split
+    // with the narrowest tile width that gives every tile a CU of its own, if the tiles then cover
+    // at least 3/4 of the CUs
+    if (!transa && transb && tune_get(CFGMM_TUNE_DGEMM_KGROUPS) == 0 &&
+        (long)div_up(m, 128) * div_up(n, 128) < (long)kSlots128 * cus) {
+        const long tm = (long)div_up(m, 64);
+        for (int bn : {64, 80, 96}) {
+            const long tiles = tm * (long)div_up(n, (size_t)bn);
+            if (tiles > cus) continue;
+            if (4 * tiles < 3L * cus) break;
+            const int group_m = tune_get(CFGMM_TUNE_DGEMM_GROUP_M);
+            hipEvent_t t0 = trace_start(stream);
+            if (bn == 64)
+                hipLaunchKernelGGL((dgemm_kgroup_kernel<64, 4>), dim3((unsigned)tiles), dim3(1024), 0, stream, c, a, b,
+                                   im, in, ik, (long)lda, (long)ldb, (long)ldc, group_m);
+            else if (bn == 80)
+                hipLaunchKernelGGL((dgemm_kgroup_kernel<80, 4>), dim3((unsigned)tiles), dim3(1024), 0, stream, c, a, b,
+                                   im, in, ik, (long)lda, (long)ldb, (long)ldc, group_m);
+            else
+                hipLaunchKernelGGL((dgemm_kgroup_kernel<96, 4>), dim3((unsigned)tiles), dim3(1024), 0, stream, c, a, b,
+                                   im, in, ik, (long)lda, (long)ldb, (long)ldc, group_m);
+            trace_stop(stream, "dgemm_kgroup_kernel", t0);
+            return hip_status(hipGetLastError(), "dgemm_kgroup_kernel launch");
+        }
+    }
+    const DgemmPlan pl = plan_dgemm(im, in, ik, cus);
     if (pl.bm == 128)
         return launch_plan<128, 64>(stream, pl, c, a, b, im, in, ik, lda, ldb, ldc, transa, transb);
     return launch_plan<64, 32>(stream, pl, c, a, b, im, in, ik, lda, ldb, ldc, transa, transb);

@@ -390,7 +390,12 @@ enum cfgmm_tunable {
                                         upper triangle of each block (default); 1 = never.
                                         Also the Vxc contraction: with ao_lhs == ao_rhs only
                                         the upper tiles of each block product are computed */
-    CFGMM_TUNE_COUNT = 10
+    CFGMM_TUNE_DGEMM_KGROUPS = 10,   /* DGEMM too small for 128x128 tiles, A not transposed and B
+                                        transposed: 0 = one workgroup per 64-row tile splitting K
+                                        among four groups of waves when the tiles then cover 3/4 or
+                                        more of the CUs (default), 1 = never (K split over
+                                        workgroups) */
+    CFGMM_TUNE_COUNT = 11
 };
 int cfgmm_tune_set(int param, int value);
 int cfgmm_tune_get(int param, int *value);

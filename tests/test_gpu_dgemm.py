@@ -167,3 +167,41 @@ def test_dgemm_reference_harness_sincos(ta, tb):
     err = rel_fro(c, c_ref)
     print(f"dgemm sin/cos {m}x{n}x{k} trans={ta}{tb} rel_fro = {err:.3e}", flush=True)
     assert err <= TOL
+
+
+# Shapes whose 64-row tiles of width 64, 80 or 96 give every tile a CU of its own (256 CUs) with
+# 3/4 of the CUs or more covered: the in-workgroup K split (dgemm_kgroup_kernel).  k covers whole
+# rounds of the four groups (64, 1543 = 24 rounds + a tail), leftover full steps only (224 = 3
+# rounds + 2 steps), and a tail round alone (37).
+@pytest.mark.parametrize("m,n", [(1000, 961), (1031, 1039), (1031, 1300)], ids=["bn64", "bn80", "bn96"])
+@pytest.mark.parametrize("k", [64, 1543, 224, 37])
+def test_dgemm_kgroup_plan(m, n, k):
+    def run(kgroups):
+        old = cfgmm.tune_get(cfgmm.TUNE_DGEMM_KGROUPS)
+        try:
+            cfgmm.tune_set(cfgmm.TUNE_DGEMM_KGROUPS, kgroups)
+            return _run_both(m, n, k, False, True, seed=m + n + k, pad=(2, 3, 1), api="dev")
+        finally:
+            cfgmm.tune_set(cfgmm.TUNE_DGEMM_KGROUPS, old)
+    c0, c_ref, c_kg, ldc = run(0)
+    _, _, c_split, _ = run(1)
+    G, R, S = (x.reshape(m, ldc) for x in (c_kg, c_ref, c_split))
+    assert rel_fro(G[:, :n], R[:, :n]) <= TOL
+    assert rel_fro(S[:, :n], R[:, :n]) <= TOL
+    assert np.array_equal(G[:, n:], c0.reshape(m, ldc)[:, n:])
+    # the fixed group order makes the result bitwise repeatable
+    _, _, c_kg2, _ = run(0)
+    assert np.array_equal(c_kg, c_kg2)
+
+
+def test_dgemm_kgroup_integer_exact():
+    """Small-integer operands through the in-workgroup K split: every partial sum and the group
+    reduction are exact, so the result is bitwise the integer product."""
+    rng = np.random.default_rng(9)
+    m, n, k = 1031, 977, 1543
+    a = rng.integers(-2, 3, size=m * k).astype(np.float64)
+    b = rng.integers(-2, 3, size=n * k).astype(np.float64)
+    c = rng.integers(-5, 6, size=m * n).astype(np.float64)
+    ref = c.reshape(m, n).astype(np.int64) + a.reshape(m, k).astype(np.int64) @ b.reshape(n, k).astype(np.int64).T
+    cfgmm.matmul_anyway_full(c, a, b, m, n, k, k, k, n, False, True)
+    assert np.array_equal(c.reshape(m, n).astype(np.int64), ref)
