@@ -14,6 +14,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <utility>
 #include <vector>
 
@@ -294,13 +295,103 @@ static int rho_extents(size_t stride_ao, size_t stride_grid, size_t nao, size_t 
 // host-pointer staging: one stream and a set of grow-only device buffers per
 // calling thread (kept until the process ends).
 // ---------------------------------------------------------------------------
+// Transfers of kPinMin bytes and up go through two pinned chunks of kPinChunk bytes: host threads
+// copy a chunk between the caller's (pageable) memory and one pinned chunk while the DMA engine
+// moves the other (about 52 GB/s on the host side measured, against 30 GB/s for pageable DMA).
+constexpr size_t kPinChunk = (size_t)32 << 20, kPinMin = (size_t)8 << 20;
 struct HostStage {
     int device = -1;
     hipStream_t stream = nullptr;
     void *buf[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     size_t cap[5] = {0, 0, 0, 0, 0};
+    void *pin[2] = {nullptr, nullptr};          // pinned chunks (hipHostMalloc)
+    hipEvent_t pin_done[2] = {nullptr, nullptr};  // recorded after the last DMA touching each chunk
+    bool pin_pending[2] = {false, false};
+    bool pin_failed = false;                    // no pinned memory: pageable transfers only
 };
 static thread_local HostStage t_stage;
+
+// memcpy split over up to 8 threads (the caller is one of them)
+static void parallel_memcpy(void *dst, const void *src, size_t n) {
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const unsigned nt = n < ((size_t)2 << 20) ? 1u : std::min(8u, hw);
+    std::vector<std::thread> th;
+    th.reserve(nt - 1);
+    for (unsigned t = 1; t < nt; ++t)
+        th.emplace_back([=] {
+            const size_t a = n * t / nt, b = n * (t + 1) / nt;
+            std::memcpy(static_cast<char *>(dst) + a, static_cast<const char *>(src) + a, b - a);
+        });
+    std::memcpy(dst, src, n / nt);
+    for (auto &x : th) x.join();
+}
+
+// true when the pinned chunks exist (allocated on first use)
+static bool pin_ready() {
+    if (t_stage.pin_failed) return false;
+    if (t_stage.pin[0]) return true;
+    for (int i = 0; i < 2; ++i) {
+        if (hipHostMalloc(&t_stage.pin[i], kPinChunk, hipHostMallocDefault) != hipSuccess ||
+            hipEventCreateWithFlags(&t_stage.pin_done[i], hipEventDisableTiming) != hipSuccess) {
+            t_stage.pin[i] = nullptr;
+            t_stage.pin_failed = true;
+            (void)hipGetLastError();
+            return false;
+        }
+    }
+    return true;
+}
+
+// waits until the DMA that last used pinned chunk i is done
+static int pin_wait(int i) {
+    if (t_stage.pin_pending[i]) CFGMM_HIP_TRY(hipEventSynchronize(t_stage.pin_done[i]));
+    t_stage.pin_pending[i] = false;
+    return CFGMM_OK;
+}
+
+static int copy_h2d(void *dev, const void *host, size_t bytes) {
+    if (bytes < kPinMin || !pin_ready())
+        return hip_status(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, t_stage.stream), "hipMemcpyAsync H2D");
+    for (size_t off = 0, i = 0; off < bytes; off += kPinChunk, ++i) {
+        const int c = (int)(i & 1);
+        const size_t n = std::min(kPinChunk, bytes - off);
+        CFGMM_TRY(pin_wait(c));
+        parallel_memcpy(t_stage.pin[c], static_cast<const char *>(host) + off, n);
+        CFGMM_HIP_TRY(hipMemcpyAsync(static_cast<char *>(dev) + off, t_stage.pin[c], n, hipMemcpyHostToDevice,
+                                     t_stage.stream));
+        CFGMM_HIP_TRY(hipEventRecord(t_stage.pin_done[c], t_stage.stream));
+        t_stage.pin_pending[c] = true;
+    }
+    return CFGMM_OK;
+}
+
+// device-to-host, synchronous: every byte is in `host` on return
+static int copy_d2h(void *host, const void *dev, size_t bytes) {
+    if (bytes < kPinMin || !pin_ready()) {
+        CFGMM_HIP_TRY(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, t_stage.stream));
+        return hip_status(hipStreamSynchronize(t_stage.stream), "hipStreamSynchronize");
+    }
+    // chunk i lands in pinned chunk i % 2 while the host copies chunk i - 1 out of the other
+    const size_t nchunks = div_up(bytes, kPinChunk);
+    for (size_t i = 0; i <= nchunks; ++i) {
+        if (i < nchunks) {
+            const int c = (int)(i & 1);
+            const size_t off = i * kPinChunk, n = std::min(kPinChunk, bytes - off);
+            CFGMM_TRY(pin_wait(c));  // an earlier DMA into or out of this chunk
+            CFGMM_HIP_TRY(hipMemcpyAsync(t_stage.pin[c], static_cast<const char *>(dev) + off, n,
+                                         hipMemcpyDeviceToHost, t_stage.stream));
+            CFGMM_HIP_TRY(hipEventRecord(t_stage.pin_done[c], t_stage.stream));
+            t_stage.pin_pending[c] = true;
+        }
+        if (i > 0) {
+            const int c = (int)((i - 1) & 1);
+            const size_t off = (i - 1) * kPinChunk, n = std::min(kPinChunk, bytes - off);
+            CFGMM_TRY(pin_wait(c));
+            parallel_memcpy(static_cast<char *>(host) + off, t_stage.pin[c], n);
+        }
+    }
+    return hip_status(hipStreamSynchronize(t_stage.stream), "hipStreamSynchronize");
+}
 
 static int stage_begin() {
     CFGMM_TRY(check_device());
@@ -334,16 +425,14 @@ template <typename T>
 static int stage_in(int slot, const T *host, size_t count, T **dev) {
     void *p = nullptr;
     CFGMM_TRY(stage_buf(slot, std::max<size_t>(count * sizeof(T), 256), &p));
-    if (count) CFGMM_HIP_TRY(hipMemcpyAsync(p, host, count * sizeof(T), hipMemcpyHostToDevice, t_stage.stream));
+    if (count) CFGMM_TRY(copy_h2d(p, host, count * sizeof(T)));
     *dev = static_cast<T *>(p);
     return CFGMM_OK;
 }
 
 static int stage_out(double *host, const double *dev, size_t count) {
-    if (count)
-        CFGMM_HIP_TRY(hipMemcpyAsync(host, dev, count * sizeof(double), hipMemcpyDeviceToHost, t_stage.stream));
-    CFGMM_HIP_TRY(hipStreamSynchronize(t_stage.stream));
-    return CFGMM_OK;
+    if (count) return copy_d2h(host, dev, count * sizeof(double));
+    return hip_status(hipStreamSynchronize(t_stage.stream), "hipStreamSynchronize");
 }
 
 // ---------------------------------------------------------------------------
