@@ -409,7 +409,8 @@ def bench_masked_gemm(reps: int = 10):
     e1.record(s)
     s.synchronize()
     ms = e0.elapsed_ms(e1) / reps
-    kavg = traced_kernel_ms(call, s, ["masked_gemm_kernel"], reps)["masked_gemm_kernel"]
+    traced = traced_kernel_ms(call, s, ["masked_gemm_kernel", "masked_prep_a"], reps)
+    kavg = traced["masked_gemm_kernel"]
     for x in (dA, dB, dT, dC):
         x.free()
     e0.destroy()
@@ -420,6 +421,7 @@ def bench_masked_gemm(reps: int = 10):
     return {"shape": f"{m}x{n}x{k} NN", "blksize": bs, "mask": "30 % periodic over the flat (row, block) index",
             "masked_gflop": round(flops / 1e9, 3), "dense_gflop": round(2.0 * m * n * k / 1e9, 3),
             "ms_per_call": round(ms, 4), "masked_gemm_kernel_avg_ms": round(kavg, 4),
+            "masked_prep_a_avg_ms": None if np.isnan(traced["masked_prep_a"]) else round(traced["masked_prep_a"], 4),
             "roofline": {"bound": "mfma", "kernel": "masked_gemm_kernel", "achieved": round(tf, 3),
                          "peak": FP64_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
                          "frac": round(tf / FP64_MFMA_PEAK_TFLOPS, 4)},

@@ -45,6 +45,7 @@ TUNE_RESIDENT_COLS = 7
 TUNE_RHO_ORDER = 8
 TUNE_RHO_SYMMETRIC = 9
 TUNE_DGEMM_KGROUPS = 10
+TUNE_MASKED_LDS = 11
 
 _lib = None
 

@@ -24,6 +24,8 @@
 // instruction and whose result rows are 4*(l/16)+r instead of l/16+4r.
 #include "internal.hpp"
 
+#include <type_traits>
+
 namespace cfgmm {
 
 // Zero sources for loads outside the problem or the compacted K list.
@@ -173,16 +175,261 @@ __global__ __launch_bounds__(kThreads, 2) void masked_gemm_kernel(T *__restrict_
         }
     }
 
+    // C += acc: a fragment row's C values are loaded together from clamped, in-bounds addresses and
+    // then stored where in range, so that the workgroup waits for one load latency per fragment
+    // row instead of one per element
 #pragma unroll
-    for (int mi = 0; mi < FM; ++mi)
+    for (int mi = 0; mi < FM; ++mi) {
+        T cv[NT][4];
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = min(m0 + wave * 32 + mi * 16 + MfmaOf<T>::row(lk, r), m - 1);
+                const int col = min(n0 + nt * 16 + lrow, n - 1);
+                cv[nt][r] = c[(long)row * ldc + col];
+            }
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int row = m0 + wave * 32 + mi * 16 + MfmaOf<T>::row(lk, r);
                 const int col = n0 + nt * 16 + lrow;
-                if (row < m && col < n) c[(long)row * ldc + col] += acc[mi][nt][r];
+                if (row < m && col < n) c[(long)row * ldc + col] = cv[nt][r] + acc[mi][nt][r];
             }
+    }
+}
+
+// ---- FP64 masked GEMM over LDS stages filled by direct copies (B not transposed, n even) ----
+//
+// op(A) is first written transposed into a k x ldt workspace (ldt = m rounded up to 128, the
+// padding zero), so that the listed k of a slice are whole rows of both operands: a workgroup's
+// slice is 16 rows of 128 A values (one 1 KiB global_load_lds_dwordx4 each) and 16 rows of W B
+// values (RPI rows per copy), copied straight into one of three LDS stages two slices ahead of the
+// MFMAs.  A copy writes lane i's 16 bytes at 16 i, so the layout is chosen on the source side: in
+// odd k rows the 128-byte halves of each 256 bytes are swapped (lane i loads chunk i ^ 8), which
+// puts the two k rows of one ds_read_b64 phase (lanes 0-31) on disjoint banks.  B rows of 48
+// values need no swap (the second row of a copy starts 384 bytes on), and B rows of 16 values sit
+// 128 bytes apart already.  A slice's copies are waited for with vmcnt (the next slice's stay
+// in flight); the last partial slice of a list segment goes through registers with zero fill.
+// The column block at the end of B reads clamped 16-byte chunks, exact for even n (a chunk never
+// straddles the last column) -- odd n keeps masked_gemm_kernel.
+
+__device__ __attribute__((always_inline)) inline void copy16_to_lds(const void *src, void *dst) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void *)dst, 16, 0, 0);
+#endif
+}
+
+// at[p][i] = op(A)[i][p] for i < m (0 for m <= i < ldt), p < k; 64 x 64 tiles
+template <bool TA>
+__global__ __launch_bounds__(256) void masked_prep_a(double *__restrict__ at, long ldt, const double *__restrict__ a,
+                                                     long lda, int m, int k) {
+    __shared__ double t[64][65];
+    const int p0 = blockIdx.y * 64, i0 = blockIdx.x * 64;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    double v[16];
+    if (TA) {  // a[p][i]: already k-major, copied with the padding
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int p = min(p0 + ty + 4 * q, k - 1), i = min(i0 + tx, m - 1);
+            v[q] = a[(long)p * lda + i];
+        }
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int p = p0 + ty + 4 * q, i = i0 + tx;
+            if (p < k) at[(long)p * ldt + i] = i < m ? v[q] : 0.0;
+        }
+    } else {  // a[i][p]: read along p, write along i through the LDS tile
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int i = min(i0 + ty + 4 * q, m - 1), p = min(p0 + tx, k - 1);
+            v[q] = a[(long)i * lda + p];
+        }
+#pragma unroll
+        for (int q = 0; q < 16; ++q) t[ty + 4 * q][tx] = i0 + ty + 4 * q < m ? v[q] : 0.0;
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int p = p0 + ty + 4 * q, i = i0 + tx;
+            if (p < k) at[(long)p * ldt + i] = t[tx][ty + 4 * q];
+        }
+    }
+}
+
+template <int NT>
+__global__ __launch_bounds__(kThreads, 2) void masked_lds_kernel(double *__restrict__ c, const double *__restrict__ at,
+                                                                 long ldt, const double *__restrict__ b, int m, int n,
+                                                                 long ldb, long ldc, int kmax,
+                                                                 const int *__restrict__ lists,
+                                                                 const int *__restrict__ counts, int blksize,
+                                                                 int nsub) {
+    constexpr int W = NT * 16, FM = 2;
+    constexpr int CPR = W / 2;                                           // 16-byte chunks per B row
+    constexpr int RPI = 64 / CPR >= 8 ? 8 : (64 / CPR >= 4 ? 4 : 2);   // B rows per copy
+    constexpr int BU = kBK / RPI;                                        // B copies per slice
+    constexpr int A_PITCH = kBM * 8, B_PITCH = 1024;                     // one copy per row / unit
+    constexpr int BSW = W % 32 == 0 ? 128 : 0;                           // B byte swap of odd k rows
+    constexpr int A_BYTES = kBK * A_PITCH, B_BYTES = BU * B_PITCH;
+    constexpr int A_PER_WAVE = kBK / 4, B_PER_WAVE = (BU + 3) / 4;
+    constexpr int PER_STEP = A_PER_WAVE + B_PER_WAVE;                    // copies per wave per slice
+    constexpr int SEG = 1024;
+    static_assert(RPI * CPR <= 64 && kBK % RPI == 0, "B copy shape");
+    static_assert(PER_STEP < 16, "vmcnt field");
+    __shared__ __attribute__((aligned(16))) unsigned char A0[A_BYTES];
+    __shared__ __attribute__((aligned(16))) unsigned char A1[A_BYTES];
+    __shared__ __attribute__((aligned(16))) unsigned char A2[A_BYTES];
+    __shared__ __attribute__((aligned(16))) unsigned char B0[B_BYTES];
+    __shared__ __attribute__((aligned(16))) unsigned char B1[B_BYTES];
+    __shared__ __attribute__((aligned(16))) unsigned char B2[B_BYTES];
+    __shared__ int vls[SEG];
+
+    const int cb = blockIdx.x / nsub, sub = blockIdx.x % nsub;
+    const int cnt = counts[cb];
+    if (cnt == 0) return;
+    const int *__restrict__ list = lists + (long)cb * kmax;
+    const int m0 = blockIdx.y * kBM;
+    const int n0 = cb * blksize + sub * W;
+    if (n0 >= n) return;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int lrow = lane & 15, lk = lane >> 4;
+
+    f64x4 acc[FM][NT];
+#pragma unroll
+    for (int mi = 0; mi < FM; ++mi)
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) acc[mi][nt] = f64x4{0.0, 0.0, 0.0, 0.0};
+
+    // B columns at n or past: a clamped chunk (n even: never straddling the last column)
+    const int bl = lane % (RPI * CPR);
+    const int bcol = min(n0 + 2 * ((bl % CPR) ^ (BSW && (bl / CPR) % 2 ? 8 : 0)), n - 2);
+    auto issue = [&](int t, unsigned char *as, unsigned char *bs) {
+        const int k0 = t * kBK;
+#pragma unroll
+        for (int e = 0; e < A_PER_WAVE; ++e) {
+            const int j = wave * A_PER_WAVE + e;
+            copy16_to_lds(at + (long)vls[k0 + j] * ldt + m0 + 2 * (lane ^ (j % 2 ? 8 : 0)), as + j * A_PITCH);
+        }
+#pragma unroll
+        for (int e = 0; e < B_PER_WAVE; ++e) {
+            const int u = min(wave + 4 * e, BU - 1);  // every wave issues as many (a repeated copy is harmless)
+            copy16_to_lds(b + (long)vls[k0 + u * RPI + bl / CPR] * ldb + bcol, bs + u * B_PITCH);
+        }
+    };
+    auto compute = [&](const unsigned char *as, const unsigned char *bs) {
+#pragma unroll
+        for (int kk = 0; kk < kBK / 4; ++kk) {
+            const int kq = kk * 4 + lk, odd = kq % 2;
+            double fa[FM], fb[NT];
+#pragma unroll
+            for (int mi = 0; mi < FM; ++mi)
+                fa[mi] = *reinterpret_cast<const double *>(
+                    as + kq * A_PITCH + (((wave * 32 + mi * 16 + lrow) * 8) ^ (odd << 7)));
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt)
+                fb[nt] = *reinterpret_cast<const double *>(bs + (kq / RPI) * B_PITCH + (kq % RPI) * (W * 8) +
+                                                           (((nt * 16 + lrow) * 8) ^ (odd ? BSW : 0)));
+#pragma unroll
+            for (int mi = 0; mi < FM; ++mi)
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt)
+                    acc[mi][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[mi], fb[nt], acc[mi][nt], 0, 0, 0);
+        }
+    };
+    // slice t's copies landed on every wave (the next slice's, PER_STEP per wave, may stay in flight)
+    // and every wave is done with the stage refilled next
+    auto sync = [&]() {
+        __builtin_amdgcn_s_waitcnt((PER_STEP & 15) | (0x7 << 4) | (0xF << 8));
+        asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+    };
+
+    for (int seg0 = 0; seg0 < cnt; seg0 += SEG) {
+        const int segn = min(SEG, cnt - seg0);
+        __syncthreads();  // the previous segment's readers of vls and the stages are done
+        for (int j = tid; j < segn; j += kThreads) vls[j] = list[seg0 + j];
+        __syncthreads();
+        const int nfull = segn / kBK;
+        if (nfull > 0) {
+            const int last = nfull - 1;
+            issue(0, A0, B0);
+            issue(min(1, last), A1, B1);
+            int t = 0;
+            for (; t + 3 <= nfull; t += 3) {
+                sync();
+                issue(min(t + 2, last), A2, B2);
+                compute(A0, B0);
+                sync();
+                issue(min(t + 3, last), A0, B0);
+                compute(A1, B1);
+                sync();
+                issue(min(t + 4, last), A1, B1);
+                compute(A2, B2);
+            }
+            if (t < nfull) {
+                sync();
+                issue(last, A2, B2);
+                compute(A0, B0);
+            }
+            if (t + 1 < nfull) {
+                sync();
+                issue(last, A0, B0);
+                compute(A1, B1);
+            }
+            __builtin_amdgcn_s_waitcnt((0x7 << 4) | (0xF << 8));  // no copy outlives the segment
+            asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+        }
+        if (nfull * kBK < segn) {
+            // the last partial slice: registers, rows past the list zero
+            const int k0 = nfull * kBK, nrem = segn - k0;
+            constexpr int A_PER_T = kBK * kBM / kThreads, B_PER_T = kBK * W / kThreads;
+            double va[A_PER_T], vb[B_PER_T];
+#pragma unroll
+            for (int e = 0; e < A_PER_T; ++e) {
+                const int idx = tid + kThreads * e, j = idx / kBM, i = idx % kBM;
+                va[e] = at[(long)vls[k0 + min(j, nrem - 1)] * ldt + m0 + i];
+            }
+#pragma unroll
+            for (int e = 0; e < B_PER_T; ++e) {
+                const int idx = tid + kThreads * e, j = idx / W, col = min(n0 + idx % W, n - 1);
+                vb[e] = b[(long)vls[k0 + min(j, nrem - 1)] * ldb + col];
+            }
+#pragma unroll
+            for (int e = 0; e < A_PER_T; ++e) {
+                const int idx = tid + kThreads * e, j = idx / kBM, i = idx % kBM;
+                *reinterpret_cast<double *>(A0 + j * A_PITCH + ((i * 8) ^ ((j % 2) << 7))) = j < nrem ? va[e] : 0.0;
+            }
+#pragma unroll
+            for (int e = 0; e < B_PER_T; ++e) {
+                const int idx = tid + kThreads * e, j = idx / W, col = idx % W;
+                *reinterpret_cast<double *>(B0 + (j / RPI) * B_PITCH + (j % RPI) * (W * 8) +
+                                            ((col * 8) ^ (j % 2 ? BSW : 0))) = j < nrem ? vb[e] : 0.0;
+            }
+            __syncthreads();
+            compute(A0, B0);
+        }
+    }
+
+    // C += acc, a fragment row's C values loaded together (as in masked_gemm_kernel)
+#pragma unroll
+    for (int mi = 0; mi < FM; ++mi) {
+        double cv[NT][4];
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = min(m0 + wave * 32 + mi * 16 + lk + 4 * r, m - 1);
+                const int col = min(n0 + nt * 16 + lrow, n - 1);
+                cv[nt][r] = c[(long)row * ldc + col];
+            }
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = m0 + wave * 32 + mi * 16 + lk + 4 * r;
+                const int col = n0 + nt * 16 + lrow;
+                if (row < m && col < n) c[(long)row * ldc + col] = cv[nt][r] + acc[mi][nt][r];
+            }
+    }
 }
 
 template <typename T, int NT>
@@ -219,9 +466,13 @@ int launch_gemm_non0tab(hipStream_t stream, T *c, const T *a, const T *b, size_t
     const size_t mtiles = div_up(m, kBM);
     if (nblk * nsub > (size_t)INT32_MAX || mtiles > 65535u) return fail(CFGMM_EINVAL, "masked dgemm: grid too large");
 
+    // FP64 with B not transposed and n even: op(A) transposed into the workspace, direct copies
+    const bool lds = std::is_same<T, double>::value && !transb && n % 2 == 0 && tune_get(CFGMM_TUNE_MASKED_LDS) == 0;
+    const size_t ldt = div_up(m, (size_t)kBM) * kBM;
     void *ws = nullptr;
     const size_t lists_bytes = sizeof(int) * nblk * k;
-    int st = workspace_get(stream, lists_bytes + sizeof(int) * nblk, &ws);
+    const size_t at_off = div_up(lists_bytes + sizeof(int) * nblk, (size_t)256) * 256;
+    int st = workspace_get(stream, lds ? at_off + sizeof(double) * k * ldt : lists_bytes + sizeof(int) * nblk, &ws);
     if (st != CFGMM_OK) return st;
     int *lists = static_cast<int *>(ws);
     int *counts = lists + nblk * k;
@@ -230,6 +481,35 @@ int launch_gemm_non0tab(hipStream_t stream, T *c, const T *a, const T *b, size_t
 
     dim3 grid((unsigned)(nblk * nsub), (unsigned)mtiles, 1);
     const int im = (int)m, in = (int)n, ik = (int)k, ib = (int)blksize;
+    if constexpr (std::is_same<T, double>::value) {
+        if (lds) {
+            if (div_up(k, (size_t)64) > 65535u) return fail(CFGMM_EINVAL, "masked dgemm: k too large");
+            double *at = reinterpret_cast<double *>(static_cast<char *>(ws) + at_off);
+            hipEvent_t t0 = trace_start(stream);
+            const dim3 pg((unsigned)(ldt / 64), (unsigned)div_up(k, (size_t)64));
+            if (transa)
+                hipLaunchKernelGGL((masked_prep_a<true>), pg, dim3(256), 0, stream, at, (long)ldt, a, (long)lda, im, ik);
+            else
+                hipLaunchKernelGGL((masked_prep_a<false>), pg, dim3(256), 0, stream, at, (long)ldt, a, (long)lda, im,
+                                   ik);
+            trace_stop(stream, "masked_prep_a", t0);
+            st = hip_status(hipGetLastError(), "masked_prep_a launch");
+            if (st != CFGMM_OK) return st;
+            t0 = trace_start(stream);
+#define CFGMM_MASKED_LDS(NT_)                                                                                     \
+    hipLaunchKernelGGL((masked_lds_kernel<NT_>), grid, dim3(kThreads), 0, stream, c, at, (long)ldt, b, im, in,   \
+                       (long)ldb, (long)ldc, ik, lists, counts, ib, nsub)
+            switch (nt) {
+                case 1: CFGMM_MASKED_LDS(1); break;
+                case 2: CFGMM_MASKED_LDS(2); break;
+                case 3: CFGMM_MASKED_LDS(3); break;
+                default: CFGMM_MASKED_LDS(4); break;
+            }
+#undef CFGMM_MASKED_LDS
+            trace_stop(stream, "masked_gemm_kernel", t0);
+            return hip_status(hipGetLastError(), "masked_lds_kernel launch");
+        }
+    }
     switch (nt) {
         case 1: return launch_nt<T, 1>(transa, transb, grid, stream, c, a, b, im, in, (long)lda, (long)ldb, (long)ldc, ik, lists, counts, ib, nsub);
         case 2: return launch_nt<T, 2>(transa, transb, grid, stream, c, a, b, im, in, (long)lda, (long)ldb, (long)ldc, ik, lists, counts, ib, nsub);

@@ -395,7 +395,10 @@ enum cfgmm_tunable {
                                         among four groups of waves when the tiles then cover 3/4 or
                                         more of the CUs (default), 1 = never (K split over
                                         workgroups) */
-    CFGMM_TUNE_COUNT = 11
+    CFGMM_TUNE_MASKED_LDS = 11,      /* FP64 masked GEMM with B not transposed and n even: 0 = op(A)
+                                        transposed into the workspace and slices copied straight
+                                        into LDS (default), 1 = register-staged slices */
+    CFGMM_TUNE_COUNT = 12
 };
 int cfgmm_tune_set(int param, int value);
 int cfgmm_tune_get(int param, int *value);

@@ -128,3 +128,49 @@ def test_masked_sgemm_f32(blksize, ta, tb):
     G, R = outs[0].astype(np.float64), c_ref.reshape(m, ldc)
     assert rel_fro(G[:, :n], R[:, :n]) <= 1e-5
     assert np.array_equal(outs[0][:, n:], c.reshape(m, ldc)[:, n:])
+
+
+@pytest.mark.parametrize("blksize", [16, 32, 48, 64])
+@pytest.mark.parametrize("ta", [False, True])
+@pytest.mark.parametrize("n", [998, 999])
+def test_masked_gemm_long_lists_both_paths(blksize, ta, n):
+    """B not transposed: even n takes the LDS-copy kernel (op(A) transposed into the workspace),
+    odd n the register-staged one.  A 90 % mask over k = 2600 gives lists of ~2340 entries (several
+    1024-entry segments, each ending in a partial slice); odd ldb, padded lda and ldc, a partial
+    last column block.  Checked against the oracle, against the register-staged kernel
+    (CFGMM_TUNE_MASKED_LDS = 1) and for bitwise repeatability."""
+    rng = np.random.default_rng(7000 + blksize + 2 * ta + n)
+    m, k = 200, 2600
+    lda = (m if ta else k) + 3
+    ldb, ldc = n + 3, n + 1
+    a = rng.uniform(-1, 1, size=(k if ta else m) * lda)
+    b = rng.uniform(-1, 1, size=k * ldb)
+    nblk = (n + blksize - 1) // blksize
+    tab = (rng.random((k, nblk)) < 0.9).astype(np.uint8)
+    tab[:, nblk // 2] = 0  # one empty list
+    c = rng.uniform(-1, 1, size=m * ldc)
+    c_ref = c.copy()
+    bk = np.ascontiguousarray(b.reshape(k, ldb)[:, :n]).ravel()
+    assert oracle.matmul_non0tab(c_ref, a, bk, m, n, k, lda, n, ldc, ta, False, tab, nblk, blksize) == 0
+    s = cfgmm.Stream()
+    dA, dB, dT = (cfgmm.DeviceBuffer.from_array(np.ascontiguousarray(x)) for x in (a, b, tab))
+    outs = []
+    old = cfgmm.tune_get(cfgmm.TUNE_MASKED_LDS)
+    try:
+        for lds_off in (0, 0, 1):
+            cfgmm.tune_set(cfgmm.TUNE_MASKED_LDS, lds_off)
+            dC = cfgmm.DeviceBuffer.from_array(c)
+            cfgmm.dgemm_non0tab_dev(s, dC, dA, dB, m, n, k, lda, ldb, ldc, ta, False, dT, nblk, blksize)
+            s.synchronize()
+            outs.append(dC.to_array(np.float64, (m, ldc)))
+            dC.free()
+    finally:
+        cfgmm.tune_set(cfgmm.TUNE_MASKED_LDS, old)
+    for x in (dA, dB, dT):
+        x.free()
+    s.destroy()
+    assert np.array_equal(outs[0], outs[1])
+    R = c_ref.reshape(m, ldc)
+    for G in (outs[0], outs[2]):
+        assert rel_fro(G[:, :n], R[:, :n]) <= TOL
+        assert np.array_equal(G[:, n:], c.reshape(m, ldc)[:, n:])
