@@ -35,6 +35,7 @@ __device__ const float kZerosMf[16] = {};
 namespace {
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
 
 template <typename T>
 struct MfmaOf;
@@ -58,6 +59,25 @@ struct MfmaOf<float> {
 };
 
 constexpr int kThreads = 256;
+constexpr int kXcds = 8;  // MI355X: 8 XCDs, each with its own L2 and 4 shader engines
+constexpr int kSes = 4;
+
+// Task (column subtile, row tile) of this workgroup on a 1-D grid of ncolt x (row tiles) tasks,
+// column subtile fastest.  The dispatcher deals workgroup i to XCD i % 8 and, within it, to shader
+// engine (i / 8) % 4, in order: when that engine has no free slot, the XCD's later workgroups wait
+// too.  So each (XCD, engine) pair takes a contiguous run of the task order, the four runs of an
+// XCD adjacent: every engine then gets the same mix of long and short lists whatever their
+// pattern over the column blocks (dealing consecutive tasks round-robin would put every 32nd
+// column block on one engine), and the workgroups of an XCD that run together share the op(A)
+// rows of one or two row tiles in its L2.
+__device__ inline void masked_task(int ncolt, int &colt, int &mt) {
+    const int pid = blockIdx.x, total = gridDim.x;
+    constexpr int kClasses = kXcds * kSes;
+    const int per = total / kClasses;
+    const int v = pid < per * kClasses ? ((pid % kXcds) * kSes + (pid / kXcds) % kSes) * per + pid / kClasses : pid;
+    colt = v % ncolt;
+    mt = v / ncolt;
+}
 constexpr int kBM = 128;  // rows per workgroup (4 waves x 32)
 constexpr int kBK = 16;   // compacted-K slice depth
 constexpr int kSeg = 2048;  // compacted-K list entries staged in LDS at a time
@@ -68,7 +88,7 @@ __global__ __launch_bounds__(kThreads, 2) void masked_gemm_kernel(T *__restrict_
                                                                   long lda, long ldb, long ldc, int kmax,
                                                                   const int *__restrict__ lists,
                                                                   const int *__restrict__ counts, int blksize,
-                                                                  int nsub) {
+                                                                  int nsub, int ncolt) {
     constexpr int W = NT * 16;
     constexpr int FM = 2;  // 32 rows per wave
     constexpr int LDA_S = kBM + (TA ? 0 : 1);
@@ -76,11 +96,13 @@ __global__ __launch_bounds__(kThreads, 2) void masked_gemm_kernel(T *__restrict_
     constexpr int B_PER_T = kBK * W / kThreads;
     static_assert(B_PER_T * kThreads == kBK * W, "B tile split");
 
-    const int cb = blockIdx.x / nsub, sub = blockIdx.x % nsub;
+    int colt, mt;
+    masked_task(ncolt, colt, mt);
+    const int cb = colt / nsub, sub = colt % nsub;
     const int cnt = counts[cb];
     if (cnt == 0) return;
     const int *__restrict__ list = lists + (long)cb * kmax;
-    const int m0 = blockIdx.y * kBM;
+    const int m0 = mt * kBM;
     const int n0 = cb * blksize + sub * W;
     if (n0 >= n) return;  // subtile past the end of a partial last column block
 
@@ -200,20 +222,13 @@ __global__ __launch_bounds__(kThreads, 2) void masked_gemm_kernel(T *__restrict_
     }
 }
 
-// ---- FP64 masked GEMM over LDS stages filled by direct copies (B not transposed, n even) ----
+// ---- FP64 masked GEMM with B not transposed and n even: direct copies into LDS ----
 //
-// op(A) is first written transposed into a k x ldt workspace (ldt = m rounded up to 128, the
-// padding zero), so that the listed k of a slice are whole rows of both operands: a workgroup's
-// slice is 16 rows of 128 A values (one 1 KiB global_load_lds_dwordx4 each) and 16 rows of W B
-// values (RPI rows per copy), copied straight into one of three LDS stages two slices ahead of the
-// MFMAs.  A copy writes lane i's 16 bytes at 16 i, so the layout is chosen on the source side: in
-// odd k rows the 128-byte halves of each 256 bytes are swapped (lane i loads chunk i ^ 8), which
-// puts the two k rows of one ds_read_b64 phase (lanes 0-31) on disjoint banks.  B rows of 48
-// values need no swap (the second row of a copy starts 384 bytes on), and B rows of 16 values sit
-// 128 bytes apart already.  A slice's copies are waited for with vmcnt (the next slice's stay
-// in flight); the last partial slice of a list segment goes through registers with zero fill.
-// The column block at the end of B reads clamped 16-byte chunks, exact for even n (a chunk never
-// straddles the last column) -- odd n keeps masked_gemm_kernel.
+// op(A) is first written transposed into a k x ldt workspace (ldt = m rounded up to the tile
+// rows, the padding zero), so that the listed k of a slice are whole rows of both operands and
+// both go from global memory straight into LDS (global_load_lds_dwordx4, 1 KiB per wave
+// instruction).  The column block at the end of B is read as clamped 16-byte chunks, exact for
+// even n (a chunk never straddles the last column); odd n keeps masked_gemm_kernel.
 
 __device__ __attribute__((always_inline)) inline void copy16_to_lds(const void *src, void *dst) {
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -257,42 +272,53 @@ __global__ __launch_bounds__(256) void masked_prep_a(double *__restrict__ at, lo
     }
 }
 
-template <int NT>
-__global__ __launch_bounds__(kThreads, 2) void masked_lds_kernel(double *__restrict__ c, const double *__restrict__ at,
-                                                                 long ldt, const double *__restrict__ b, int m, int n,
-                                                                 long ldb, long ldc, int kmax,
-                                                                 const int *__restrict__ lists,
-                                                                 const int *__restrict__ counts, int blksize,
-                                                                 int nsub) {
-    constexpr int W = NT * 16, FM = 2;
-    constexpr int CPR = W / 2;                                           // 16-byte chunks per B row
-    constexpr int RPI = 64 / CPR >= 8 ? 8 : (64 / CPR >= 4 ? 4 : 2);   // B rows per copy
-    constexpr int BU = kBK / RPI;                                        // B copies per slice
-    constexpr int A_PITCH = kBM * 8, B_PITCH = 1024;                     // one copy per row / unit
-    constexpr int BSW = W % 32 == 0 ? 128 : 0;                           // B byte swap of odd k rows
-    constexpr int A_BYTES = kBK * A_PITCH, B_BYTES = BU * B_PITCH;
-    constexpr int A_PER_WAVE = kBK / 4, B_PER_WAVE = (BU + 3) / 4;
-    constexpr int PER_STEP = A_PER_WAVE + B_PER_WAVE;                    // copies per wave per slice
-    constexpr int SEG = 1024;
-    static_assert(RPI * CPR <= 64 && kBK % RPI == 0, "B copy shape");
-    static_assert(PER_STEP < 16, "vmcnt field");
+// ---- FP64 masked GEMM, 64 x FM rows per workgroup, two LDS stages filled by direct copies ----
+//
+// Wave w owns rows 16 FM w .. 16 FM w + 16 FM - 1 of a 64 FM-row tile and all W columns: FM x NT
+// MFMAs per k step of 4, 4 FM NT per slice of 16 list entries between two barriers.  op(A) is read
+// from the transposed workspace (ldt a multiple of 64 FM), one 1 KiB copy per 128 values of a
+// listed k row; B's 16 listed rows are packed (W doubles per row, lane i of copy u writing chunk
+// 64 u + i).  Odd k rows swap the 128-byte halves of each 256 bytes (lane i loads chunk i ^ 8) for
+// A, and for B when a row is a multiple of 256 bytes, so that the two k rows of one ds_read_b64
+// phase fall on disjoint banks.  The list entries a wave copies next are loaded one slice ahead
+// straight from the compacted list (no LDS copy of the list); one s_waitcnt vmcnt(0) + barrier per
+// slice makes the copies of every wave visible and frees the other stage.  The last, partial
+// slice is copied with its entries clamped to the list's last one, and its rows past the list
+// are zeroed in LDS before its MFMAs.
+template <int NT, int FM>
+__global__ __launch_bounds__(kThreads, 2) void masked_pipe_kernel(double *__restrict__ c,
+                                                                  const double *__restrict__ at, long ldt,
+                                                                  const double *__restrict__ b, int m, int n,
+                                                                  long ldb, long ldc, int kmax,
+                                                                  const int *__restrict__ lists,
+                                                                  const int *__restrict__ counts, int blksize,
+                                                                  int nsub, int ncolt) {
+    constexpr int W = NT * 16, BMR = 64 * FM;
+    constexpr int A_ROW = BMR * 8, A_BYTES = kBK * A_ROW;  // one k row: FM / 2 copies
+    constexpr int ACP = A_ROW / 1024;
+    constexpr int B_ROW = W * 8, B_BYTES = kBK * B_ROW;    // packed: W / 8 copies
+    constexpr int BC = B_BYTES / 1024, B_PER_WAVE = (BC + 3) / 4;
+    constexpr int CPR = W / 2;                              // 16-byte chunks per B row
+    constexpr int BSW = B_ROW % 256 == 0 ? 128 : 0;
+    static_assert(ACP * 1024 == A_ROW && BC * 1024 == B_BYTES, "copy shape");
     __shared__ __attribute__((aligned(16))) unsigned char A0[A_BYTES];
     __shared__ __attribute__((aligned(16))) unsigned char A1[A_BYTES];
-    __shared__ __attribute__((aligned(16))) unsigned char A2[A_BYTES];
     __shared__ __attribute__((aligned(16))) unsigned char B0[B_BYTES];
     __shared__ __attribute__((aligned(16))) unsigned char B1[B_BYTES];
-    __shared__ __attribute__((aligned(16))) unsigned char B2[B_BYTES];
-    __shared__ int vls[SEG];
 
-    const int cb = blockIdx.x / nsub, sub = blockIdx.x % nsub;
+    int colt, mt;
+    masked_task(ncolt, colt, mt);
+    const int cb = colt / nsub, sub = colt % nsub;
     const int cnt = counts[cb];
     if (cnt == 0) return;
     const int *__restrict__ list = lists + (long)cb * kmax;
-    const int m0 = blockIdx.y * kBM;
+    const int m0 = mt * BMR;
     const int n0 = cb * blksize + sub * W;
     if (n0 >= n) return;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int lrow = lane & 15, lk = lane >> 4;
+    const int nslices = (cnt + kBK - 1) / kBK, nrem = cnt % kBK;
 
     f64x4 acc[FM][NT];
 #pragma unroll
@@ -300,21 +326,32 @@ __global__ __launch_bounds__(kThreads, 2) void masked_lds_kernel(double *__restr
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) acc[mi][nt] = f64x4{0.0, 0.0, 0.0, 0.0};
 
-    // B columns at n or past: a clamped chunk (n even: never straddling the last column)
-    const int bl = lane % (RPI * CPR);
-    const int bcol = min(n0 + 2 * ((bl % CPR) ^ (BSW && (bl / CPR) % 2 ? 8 : 0)), n - 2);
-    auto issue = [&](int t, unsigned char *as, unsigned char *bs) {
+    // this lane's B chunk of each of the wave's copies: k row and (clamped, even n) column
+    int brow[B_PER_WAVE], bcol[B_PER_WAVE];
+#pragma unroll
+    for (int e = 0; e < B_PER_WAVE; ++e) {
+        const int u = min(wave + 4 * e, BC - 1), P = u * 64 + lane;
+        brow[e] = P / CPR;
+        bcol[e] = min(n0 + 2 * ((P % CPR) ^ (BSW && brow[e] % 2 ? 8 : 0)), n - 2);
+    }
+    int ia[4], ib[B_PER_WAVE];  // list entries of the slice copied next
+    auto load_idx = [&](int t) {
         const int k0 = t * kBK;
 #pragma unroll
-        for (int e = 0; e < A_PER_WAVE; ++e) {
-            const int j = wave * A_PER_WAVE + e;
-            copy16_to_lds(at + (long)vls[k0 + j] * ldt + m0 + 2 * (lane ^ (j % 2 ? 8 : 0)), as + j * A_PITCH);
-        }
+        for (int jj = 0; jj < 4; ++jj) ia[jj] = list[min(k0 + 4 * wave + jj, cnt - 1)];
 #pragma unroll
-        for (int e = 0; e < B_PER_WAVE; ++e) {
-            const int u = min(wave + 4 * e, BU - 1);  // every wave issues as many (a repeated copy is harmless)
-            copy16_to_lds(b + (long)vls[k0 + u * RPI + bl / CPR] * ldb + bcol, bs + u * B_PITCH);
-        }
+        for (int e = 0; e < B_PER_WAVE; ++e) ib[e] = list[min(k0 + brow[e], cnt - 1)];
+    };
+    auto issue = [&](unsigned char *as, unsigned char *bs) {
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj)
+#pragma unroll
+            for (int h = 0; h < ACP; ++h)
+                copy16_to_lds(at + (long)ia[jj] * ldt + m0 + h * 128 + 2 * (lane ^ (jj % 2 ? 8 : 0)),
+                              as + (4 * wave + jj) * A_ROW + h * 1024);
+#pragma unroll
+        for (int e = 0; e < B_PER_WAVE; ++e)
+            if (wave + 4 * e < BC) copy16_to_lds(b + (long)ib[e] * ldb + bcol[e], bs + (wave + 4 * e) * 1024);
     };
     auto compute = [&](const unsigned char *as, const unsigned char *bs) {
 #pragma unroll
@@ -324,10 +361,10 @@ __global__ __launch_bounds__(kThreads, 2) void masked_lds_kernel(double *__restr
 #pragma unroll
             for (int mi = 0; mi < FM; ++mi)
                 fa[mi] = *reinterpret_cast<const double *>(
-                    as + kq * A_PITCH + (((wave * 32 + mi * 16 + lrow) * 8) ^ (odd << 7)));
+                    as + kq * A_ROW + (((wave * 16 * FM + mi * 16 + lrow) * 8) ^ (odd << 7)));
 #pragma unroll
             for (int nt = 0; nt < NT; ++nt)
-                fb[nt] = *reinterpret_cast<const double *>(bs + (kq / RPI) * B_PITCH + (kq % RPI) * (W * 8) +
+                fb[nt] = *reinterpret_cast<const double *>(bs + kq * B_ROW +
                                                            (((nt * 16 + lrow) * 8) ^ (odd ? BSW : 0)));
 #pragma unroll
             for (int mi = 0; mi < FM; ++mi)
@@ -336,77 +373,39 @@ __global__ __launch_bounds__(kThreads, 2) void masked_lds_kernel(double *__restr
                     acc[mi][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[mi], fb[nt], acc[mi][nt], 0, 0, 0);
         }
     };
-    // slice t's copies landed on every wave (the next slice's, PER_STEP per wave, may stay in flight)
-    // and every wave is done with the stage refilled next
+    // the copies (and list loads) of this wave landed, and every wave is past the same point
     auto sync = [&]() {
-        __builtin_amdgcn_s_waitcnt((PER_STEP & 15) | (0x7 << 4) | (0xF << 8));
+        __builtin_amdgcn_s_waitcnt((0x7 << 4) | (0xF << 8));
         asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
     };
-
-    for (int seg0 = 0; seg0 < cnt; seg0 += SEG) {
-        const int segn = min(SEG, cnt - seg0);
-        __syncthreads();  // the previous segment's readers of vls and the stages are done
-        for (int j = tid; j < segn; j += kThreads) vls[j] = list[seg0 + j];
+    // rows nrem..15 of the partial last slice (copies of the list's last entry) set to zero
+    auto zero_tail = [&](unsigned char *as, unsigned char *bs) {
+        for (int o = nrem * A_ROW + tid * 16; o < A_BYTES; o += kThreads * 16)
+            *reinterpret_cast<f64x2 *>(as + o) = f64x2{0.0, 0.0};
+        for (int o = nrem * B_ROW + tid * 16; o < B_BYTES; o += kThreads * 16)
+            *reinterpret_cast<f64x2 *>(bs + o) = f64x2{0.0, 0.0};
         __syncthreads();
-        const int nfull = segn / kBK;
-        if (nfull > 0) {
-            const int last = nfull - 1;
-            issue(0, A0, B0);
-            issue(min(1, last), A1, B1);
-            int t = 0;
-            for (; t + 3 <= nfull; t += 3) {
-                sync();
-                issue(min(t + 2, last), A2, B2);
-                compute(A0, B0);
-                sync();
-                issue(min(t + 3, last), A0, B0);
-                compute(A1, B1);
-                sync();
-                issue(min(t + 4, last), A1, B1);
-                compute(A2, B2);
-            }
-            if (t < nfull) {
-                sync();
-                issue(last, A2, B2);
-                compute(A0, B0);
-            }
-            if (t + 1 < nfull) {
-                sync();
-                issue(last, A0, B0);
-                compute(A1, B1);
-            }
-            __builtin_amdgcn_s_waitcnt((0x7 << 4) | (0xF << 8));  // no copy outlives the segment
-            asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+    };
+
+    load_idx(0);
+    issue(A0, B0);
+    if (nslices > 1) load_idx(1);
+    for (int t = 0; t < nslices; t += 2) {
+        sync();
+        if (t + 1 < nslices) {
+            issue(A1, B1);
+            if (t + 2 < nslices) load_idx(t + 2);
         }
-        if (nfull * kBK < segn) {
-            // the last partial slice: registers, rows past the list zero
-            const int k0 = nfull * kBK, nrem = segn - k0;
-            constexpr int A_PER_T = kBK * kBM / kThreads, B_PER_T = kBK * W / kThreads;
-            double va[A_PER_T], vb[B_PER_T];
-#pragma unroll
-            for (int e = 0; e < A_PER_T; ++e) {
-                const int idx = tid + kThreads * e, j = idx / kBM, i = idx % kBM;
-                va[e] = at[(long)vls[k0 + min(j, nrem - 1)] * ldt + m0 + i];
-            }
-#pragma unroll
-            for (int e = 0; e < B_PER_T; ++e) {
-                const int idx = tid + kThreads * e, j = idx / W, col = min(n0 + idx % W, n - 1);
-                vb[e] = b[(long)vls[k0 + min(j, nrem - 1)] * ldb + col];
-            }
-#pragma unroll
-            for (int e = 0; e < A_PER_T; ++e) {
-                const int idx = tid + kThreads * e, j = idx / kBM, i = idx % kBM;
-                *reinterpret_cast<double *>(A0 + j * A_PITCH + ((i * 8) ^ ((j % 2) << 7))) = j < nrem ? va[e] : 0.0;
-            }
-#pragma unroll
-            for (int e = 0; e < B_PER_T; ++e) {
-                const int idx = tid + kThreads * e, j = idx / W, col = idx % W;
-                *reinterpret_cast<double *>(B0 + (j / RPI) * B_PITCH + (j % RPI) * (W * 8) +
-                                            ((col * 8) ^ (j % 2 ? BSW : 0))) = j < nrem ? vb[e] : 0.0;
-            }
-            __syncthreads();
-            compute(A0, B0);
+        if (t == nslices - 1 && nrem) zero_tail(A0, B0);
+        compute(A0, B0);
+        if (t + 1 >= nslices) break;
+        sync();
+        if (t + 2 < nslices) {
+            issue(A0, B0);
+            if (t + 3 < nslices) load_idx(t + 3);
         }
+        if (t + 1 == nslices - 1 && nrem) zero_tail(A1, B1);
+        compute(A1, B1);
     }
 
     // C += acc, a fragment row's C values loaded together (as in masked_gemm_kernel)
@@ -417,7 +416,7 @@ __global__ __launch_bounds__(kThreads, 2) void masked_lds_kernel(double *__restr
         for (int nt = 0; nt < NT; ++nt)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int row = min(m0 + wave * 32 + mi * 16 + lk + 4 * r, m - 1);
+                const int row = min(m0 + wave * 16 * FM + mi * 16 + lk + 4 * r, m - 1);
                 const int col = min(n0 + nt * 16 + lrow, n - 1);
                 cv[nt][r] = c[(long)row * ldc + col];
             }
@@ -425,7 +424,7 @@ __global__ __launch_bounds__(kThreads, 2) void masked_lds_kernel(double *__restr
         for (int nt = 0; nt < NT; ++nt)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int row = m0 + wave * 32 + mi * 16 + lk + 4 * r;
+                const int row = m0 + wave * 16 * FM + mi * 16 + lk + 4 * r;
                 const int col = n0 + nt * 16 + lrow;
                 if (row < m && col < n) c[(long)row * ldc + col] = cv[nt][r] + acc[mi][nt][r];
             }
@@ -434,11 +433,11 @@ __global__ __launch_bounds__(kThreads, 2) void masked_lds_kernel(double *__restr
 
 template <typename T, int NT>
 int launch_nt(bool ta, bool tb, dim3 grid, hipStream_t stream, T *c, const T *a, const T *b, int m, int n, long lda,
-              long ldb, long ldc, int kmax, const int *lists, const int *counts, int blksize, int nsub) {
+              long ldb, long ldc, int kmax, const int *lists, const int *counts, int blksize, int nsub, int ncolt) {
     hipEvent_t t0 = trace_start(stream);
 #define CFGMM_MASKED(TA_, TB_)                                                                                      \
     hipLaunchKernelGGL((masked_gemm_kernel<T, NT, TA_, TB_>), grid, dim3(kThreads), 0, stream, c, a, b, m, n, lda, \
-                       ldb, ldc, kmax, lists, counts, blksize, nsub)
+                       ldb, ldc, kmax, lists, counts, blksize, nsub, ncolt)
     if (ta && tb)
         CFGMM_MASKED(true, true);
     else if (ta)
@@ -464,11 +463,16 @@ int launch_gemm_non0tab(hipStream_t stream, T *c, const T *a, const T *b, size_t
         if ((blksize / 16) % cand == 0) { nt = cand; break; }
     const int nsub = (int)(blksize / (16 * nt));
     const size_t mtiles = div_up(m, kBM);
-    if (nblk * nsub > (size_t)INT32_MAX || mtiles > 65535u) return fail(CFGMM_EINVAL, "masked dgemm: grid too large");
+    if (nblk * nsub * div_up(m, (size_t)64) > (size_t)INT32_MAX)
+        return fail(CFGMM_EINVAL, "masked dgemm: grid too large");
+    const int ncolt = (int)(nblk * nsub);
 
     // FP64 with B not transposed and n even: op(A) transposed into the workspace, direct copies
-    const bool lds = std::is_same<T, double>::value && !transb && n % 2 == 0 && tune_get(CFGMM_TUNE_MASKED_LDS) == 0;
-    const size_t ldt = div_up(m, (size_t)kBM) * kBM;
+    const int mode = tune_get(CFGMM_TUNE_MASKED_LDS);
+    const bool lds = std::is_same<T, double>::value && !transb && n % 2 == 0 && mode != 1;
+    const int fm = mode == 2 && nt < 4 ? 4 : 2;  // 64 fm rows per workgroup (fm 4 at NT 4 spills)
+    const size_t bmr = 64 * (size_t)fm;
+    const size_t ldt = div_up(m, bmr) * bmr;
     void *ws = nullptr;
     const size_t lists_bytes = sizeof(int) * nblk * k;
     const size_t at_off = div_up(lists_bytes + sizeof(int) * nblk, (size_t)256) * 256;
@@ -479,7 +483,7 @@ int launch_gemm_non0tab(hipStream_t stream, T *c, const T *a, const T *b, size_t
     st = launch_build_lists(stream, tab, ldtab, k, nblk, lists, counts);
     if (st != CFGMM_OK) return st;
 
-    dim3 grid((unsigned)(nblk * nsub), (unsigned)mtiles, 1);
+    dim3 grid((unsigned)(nblk * nsub * mtiles), 1, 1);
     const int im = (int)m, in = (int)n, ik = (int)k, ib = (int)blksize;
     if constexpr (std::is_same<T, double>::value) {
         if (lds) {
@@ -496,25 +500,34 @@ int launch_gemm_non0tab(hipStream_t stream, T *c, const T *a, const T *b, size_t
             st = hip_status(hipGetLastError(), "masked_prep_a launch");
             if (st != CFGMM_OK) return st;
             t0 = trace_start(stream);
-#define CFGMM_MASKED_LDS(NT_)                                                                                     \
-    hipLaunchKernelGGL((masked_lds_kernel<NT_>), grid, dim3(kThreads), 0, stream, c, at, (long)ldt, b, im, in,   \
-                       (long)ldb, (long)ldc, ik, lists, counts, ib, nsub)
-            switch (nt) {
-                case 1: CFGMM_MASKED_LDS(1); break;
-                case 2: CFGMM_MASKED_LDS(2); break;
-                case 3: CFGMM_MASKED_LDS(3); break;
-                default: CFGMM_MASKED_LDS(4); break;
+            const dim3 pgrid((unsigned)(nblk * nsub * (ldt / bmr)), 1, 1);
+#define CFGMM_MASKED_PIPE(NT_, FM_)                                                                              \
+    hipLaunchKernelGGL((masked_pipe_kernel<NT_, FM_>), pgrid, dim3(kThreads), 0, stream, c, at, (long)ldt, b, im, \
+                       in, (long)ldb, (long)ldc, ik, lists, counts, ib, nsub, ncolt)
+            if (fm == 4) {
+                switch (nt) {
+                    case 1: CFGMM_MASKED_PIPE(1, 4); break;
+                    case 2: CFGMM_MASKED_PIPE(2, 4); break;
+                    default: CFGMM_MASKED_PIPE(3, 4); break;
+                }
+            } else {
+                switch (nt) {
+                    case 1: CFGMM_MASKED_PIPE(1, 2); break;
+                    case 2: CFGMM_MASKED_PIPE(2, 2); break;
+                    case 3: CFGMM_MASKED_PIPE(3, 2); break;
+                    default: CFGMM_MASKED_PIPE(4, 2); break;
+                }
             }
-#undef CFGMM_MASKED_LDS
+#undef CFGMM_MASKED_PIPE
             trace_stop(stream, "masked_gemm_kernel", t0);
-            return hip_status(hipGetLastError(), "masked_lds_kernel launch");
+            return hip_status(hipGetLastError(), "masked_gemm_kernel launch");
         }
     }
     switch (nt) {
-        case 1: return launch_nt<T, 1>(transa, transb, grid, stream, c, a, b, im, in, (long)lda, (long)ldb, (long)ldc, ik, lists, counts, ib, nsub);
-        case 2: return launch_nt<T, 2>(transa, transb, grid, stream, c, a, b, im, in, (long)lda, (long)ldb, (long)ldc, ik, lists, counts, ib, nsub);
-        case 3: return launch_nt<T, 3>(transa, transb, grid, stream, c, a, b, im, in, (long)lda, (long)ldb, (long)ldc, ik, lists, counts, ib, nsub);
-        default: return launch_nt<T, 4>(transa, transb, grid, stream, c, a, b, im, in, (long)lda, (long)ldb, (long)ldc, ik, lists, counts, ib, nsub);
+        case 1: return launch_nt<T, 1>(transa, transb, grid, stream, c, a, b, im, in, (long)lda, (long)ldb, (long)ldc, ik, lists, counts, ib, nsub, ncolt);
+        case 2: return launch_nt<T, 2>(transa, transb, grid, stream, c, a, b, im, in, (long)lda, (long)ldb, (long)ldc, ik, lists, counts, ib, nsub, ncolt);
+        case 3: return launch_nt<T, 3>(transa, transb, grid, stream, c, a, b, im, in, (long)lda, (long)ldb, (long)ldc, ik, lists, counts, ib, nsub, ncolt);
+        default: return launch_nt<T, 4>(transa, transb, grid, stream, c, a, b, im, in, (long)lda, (long)ldb, (long)ldc, ik, lists, counts, ib, nsub, ncolt);
     }
 }
 

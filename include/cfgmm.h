@@ -396,8 +396,10 @@ enum cfgmm_tunable {
                                         more of the CUs (default), 1 = never (K split over
                                         workgroups) */
     CFGMM_TUNE_MASKED_LDS = 11,      /* FP64 masked GEMM with B not transposed and n even: 0 = op(A)
-                                        transposed into the workspace and slices copied straight
-                                        into LDS (default), 1 = register-staged slices */
+                                        transposed into the workspace, slices copied straight into
+                                        LDS, 128-row tiles (default); 2 = the same with 256-row
+                                        tiles (blksize not a multiple of 64); 1 = register-staged
+                                        slices, as for f32, transposed B and odd n */
     CFGMM_TUNE_COUNT = 12
 };
 int cfgmm_tune_set(int param, int value);

@@ -134,11 +134,12 @@ def test_masked_sgemm_f32(blksize, ta, tb):
 @pytest.mark.parametrize("ta", [False, True])
 @pytest.mark.parametrize("n", [998, 999])
 def test_masked_gemm_long_lists_both_paths(blksize, ta, n):
-    """B not transposed: even n takes the LDS-copy kernel (op(A) transposed into the workspace),
-    odd n the register-staged one.  A 90 % mask over k = 2600 gives lists of ~2340 entries (several
-    1024-entry segments, each ending in a partial slice); odd ldb, padded lda and ldc, a partial
-    last column block.  Checked against the oracle, against the register-staged kernel
-    (CFGMM_TUNE_MASKED_LDS = 1) and for bitwise repeatability."""
+    """B not transposed: even n takes the LDS-copy kernel (op(A) transposed into the workspace;
+    128-row tiles, and 256-row tiles under CFGMM_TUNE_MASKED_LDS = 2), odd n the register-staged
+    one.  A 90 % mask over k = 2600 gives lists of ~2340 entries (longer than the register kernel's
+    2048-entry LDS segments, ending in a partial slice) beside one empty list; odd ldb, padded lda
+    and ldc, a partial last column block.  Each path against the oracle, the default bitwise
+    repeatable."""
     rng = np.random.default_rng(7000 + blksize + 2 * ta + n)
     m, k = 200, 2600
     lda = (m if ta else k) + 3
@@ -157,8 +158,8 @@ def test_masked_gemm_long_lists_both_paths(blksize, ta, n):
     outs = []
     old = cfgmm.tune_get(cfgmm.TUNE_MASKED_LDS)
     try:
-        for lds_off in (0, 0, 1):
-            cfgmm.tune_set(cfgmm.TUNE_MASKED_LDS, lds_off)
+        for mode in (0, 0, 1, 2):
+            cfgmm.tune_set(cfgmm.TUNE_MASKED_LDS, mode)
             dC = cfgmm.DeviceBuffer.from_array(c)
             cfgmm.dgemm_non0tab_dev(s, dC, dA, dB, m, n, k, lda, ldb, ldc, ta, False, dT, nblk, blksize)
             s.synchronize()
@@ -171,6 +172,6 @@ def test_masked_gemm_long_lists_both_paths(blksize, ta, n):
     s.destroy()
     assert np.array_equal(outs[0], outs[1])
     R = c_ref.reshape(m, ldc)
-    for G in (outs[0], outs[2]):
+    for G in outs[1:]:
         assert rel_fro(G[:, :n], R[:, :n]) <= TOL
         assert np.array_equal(G[:, n:], c.reshape(m, ldc)[:, n:])
