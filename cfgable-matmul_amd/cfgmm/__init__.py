@@ -46,6 +46,7 @@ TUNE_RHO_ORDER = 8
 TUNE_RHO_SYMMETRIC = 9
 TUNE_DGEMM_KGROUPS = 10
 TUNE_MASKED_LDS = 11
+TUNE_RESIDENT_PERSISTENT = 12
 
 _lib = None
 

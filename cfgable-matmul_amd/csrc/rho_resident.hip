@@ -481,6 +481,235 @@ __global__ __launch_bounds__(kThreads, 2) void rho_resident_kernel(
     resident_tile<S, NT, DEPTH>(partial, base, blocks, tasks[item.x], item.y, fg, blksize, ngrid, nset);
 }
 
+// ---------------------------------------------------------------------------------------------
+// Persistent evaluation.  With one workgroup per item, every item starts from an empty pipeline
+// (its descriptors, then its first A and B loads, each a full memory latency) and ends with its
+// epilogue's loads; at about a dozen k steps per item those bubbles cost as much as the
+// streaming.  Here the grid is two workgroups per CU and each walks a fixed list of items (dealt
+// at creation, longest first, to the least loaded workgroup of the item's XCD) as one pipeline:
+// the prefetch of the next item's first step is issued during the last step of the current one.
+
+// One (task, density matrix) item: everything its k steps and epilogues address, in bytes from
+// the handle's base.  Tile q of the item is row tile T[q] (q = 1 only for the paired tile of the
+// symmetric form; n[1] = 0 otherwise).
+struct ResItem {
+    long a[2];    // wave 0's row of 1 KiB A tiles of tile T[q], density matrix s, plane 0
+    long rq;      // Rq plane 0 at the item's first column
+    long ed;      // ed of density matrix s
+    long lc;      // Lc of the block
+    int aw[2];    // bytes between consecutive waves' rows of A tiles
+    int n[2];     // k steps of each tile
+    int T[2];     // row tiles
+    int dplane;   // bytes between the slice planes of Dq
+    int rplane;   // bytes between the slice planes of Rq
+    int c0;       // first column of the item in its block
+    int b;        // grid block
+    int s;        // density matrix
+    int nrows;    // active AOs of the block: rows past them are padding
+    int sym;      // symmetric form: the k steps past the diagonal first, the diagonal step last
+    int pad;
+};
+static_assert(sizeof(ResItem) == 96, "ResItem is six 16-byte words");
+
+constexpr int kItemsLds = 32;  // item records staged in LDS at a time
+
+template <int S, int NT>
+__global__ __launch_bounds__(kThreads, 2) void rho_resident_persistent_kernel(
+    double *__restrict__ partial, const unsigned char *__restrict__ base, const ResItem *__restrict__ items,
+    const int *__restrict__ slot_begin, const int *__restrict__ fg, int blksize, int ngrid, int nset) {
+    constexpr int W = NT * 16;
+    __shared__ __attribute__((aligned(16))) int8_t Bs0[S][W * kK];
+    __shared__ __attribute__((aligned(16))) int8_t Bs1[S][W * kK];
+    __shared__ double red[kThreads / 64][W];
+    __shared__ __attribute__((aligned(16))) ResItem its[kItemsLds];
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int lrow = lane & 15, lk = lane >> 4;
+    const long bstep = (long)blksize * kK;  // bytes of one k step of all the block's columns in Rq
+
+    i32x4 acc[S][NT];
+    auto zero_acc = [&]() {
+#pragma unroll
+        for (int l = 0; l < S; ++l)
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) acc[l][nt] = i32x4{0, 0, 0, 0};
+    };
+
+    // k step t of item it: (tile, local step) -> A tile row and k step (the symmetric form visits
+    // the steps past the diagonal first, the diagonal last)
+    struct StepAddr {
+        const int8_t *a;  // this lane's 16 bytes of the wave's A tile, plane 0
+        const int8_t *b;  // this lane's 16 bytes of the B stage, plane 0, part 0
+        int dplane, rplane;
+    };
+    auto addr_of = [&](int it, int t) {
+        const ResItem &R = its[it];
+        const int q = t < R.n[0] ? 0 : 1;
+        const int l = q ? t - R.n[0] : t, n = R.n[q], T = R.T[q];
+        const int ks = R.sym ? (l < n - 1 ? T + 1 + l : T) : l;
+        const int ai = R.sym ? (l < n - 1 ? l + 1 : 0) : l;
+        StepAddr x;
+        x.a = reinterpret_cast<const int8_t *>(base + R.a[q]) + (long)wave * R.aw[q] + (long)ai * 1024 + lane * 16;
+        x.b = reinterpret_cast<const int8_t *>(base + R.rq) + ks * bstep + lane * 16;
+        x.dplane = R.dplane;
+        x.rplane = R.rplane;
+        return x;
+    };
+    auto load_a = [&](const StepAddr &x, i32x4 *dst) {
+#pragma unroll
+        for (int p = 0; p < S; ++p)  // each A tile is read once per evaluation: stream it past the caches
+            dst[p] = __builtin_nontemporal_load(reinterpret_cast<const i32x4 *>(x.a + (long)p * x.dplane));
+    };
+    constexpr int kBParts = S * NT, kBPerWave = (kBParts + 3) / 4;
+    auto issue_b = [&](const StepAddr &x, int8_t (*dst)[W * kK]) {
+#pragma unroll
+        for (int i = 0; i < kBPerWave; ++i) {
+            const int j = min(wave + 4 * i, kBParts - 1);
+            const int p = j / NT, part = j % NT;
+            copy16_to_lds(x.b + (long)p * x.rplane + part * 1024, &dst[p][part * 1024]);
+        }
+    };
+    auto compute = [&](const int8_t (*bs)[W * kK], const i32x4 *fa) {
+#pragma unroll
+        for (int sb = 0; sb < S; ++sb) {
+            i32x4 fb[NT];
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt)
+                fb[nt] = *reinterpret_cast<const i32x4 *>(&bs[sb][swz(nt * 16 + lrow, lk)]);
+#pragma unroll
+            for (int sa = 0; sa + sb < S; ++sa)
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt)
+                    acc[sa + sb][nt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[sa], fb[nt], acc[sa + sb][nt], 0, 0, 0);
+        }
+    };
+    // Epilogue of row tile T of item it (the same arithmetic as resident_tile's), in two parts: its
+    // loads, issued at the top of the tile's last k step ahead of that step's prefetch (so that they
+    // arrive during its MFMAs and the wait for them leaves the prefetch in flight), and the rest.
+    // No load sits under a condition (a load inside a divergent branch is waited for there, one
+    // after another): rows i < kpad and columns c < blksize are inside ed and Lc (zero padded),
+    // the grid point is clamped.
+    struct EpiLoads {
+        int eu[4], eg[NT];
+        double lv[NT][4];
+    };
+    auto epi_load = [&](int it, int T, EpiLoads &e) {
+        const ResItem &R = its[it];
+        const int *__restrict__ ed = reinterpret_cast<const int *>(base + R.ed);
+        const double *__restrict__ lc = reinterpret_cast<const double *>(base + R.lc);
+        const int gblk = R.b * blksize, i0 = T * kRows + wave * 16 + 4 * lk;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) e.eu[r] = ed[i0 + r];
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+            const int c = R.c0 + nt * 16 + lrow;
+            e.eg[nt] = fg[min(gblk + c, ngrid - 1)];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) e.lv[nt][r] = lc[(long)(i0 + r) * blksize + c];
+        }
+    };
+    auto epilogue = [&](int it, int T, const EpiLoads &e) {
+        const ResItem &R = its[it];
+        const int gblk = R.b * blksize, c0 = R.c0, nrows = R.nrows, s = R.s;
+        const int i0 = T * kRows + wave * 16 + 4 * lk;
+        const double nan = __builtin_nan("");
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+            const int c = c0 + nt * 16 + lrow;
+            const bool g_ok = gblk + c < ngrid;
+            double part = 0.0;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                double t = 0.0;
+#pragma unroll
+                for (int l = S - 1; l >= 0; --l) t = fma((double)acc[l][nt][r], level_weight(l), t);
+                t = (e.eu[r] == kExpNonFinite || e.eg[nt] == kExpNonFinite) ? nan : ldexp(t, e.eu[r] + e.eg[nt]);
+                part = i0 + r < nrows && g_ok ? fma(e.lv[nt][r], t, part) : part;
+            }
+            part += __shfl_xor(part, 16);
+            part += __shfl_xor(part, 32);
+            if (lk == 0) red[wave][nt * 16 + lrow] = part;
+        }
+        asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");  // LDS only: prefetches stay in flight
+        for (int c = tid; c < W; c += kThreads) {
+            const int g = gblk + c0 + c;
+            if (g < ngrid) {
+                double v = red[0][c];
+#pragma unroll
+                for (int w = 1; w < kThreads / 64; ++w) v += red[w][c];
+                partial[((long)T * nset + s) * ngrid + g] = v;
+            }
+        }
+    };
+
+    const int i_begin = slot_begin[blockIdx.x], i_end = slot_begin[blockIdx.x + 1];
+    for (int c_begin = i_begin; c_begin < i_end; c_begin += kItemsLds) {
+        const int cnt = min(kItemsLds, i_end - c_begin);
+        __syncthreads();  // the previous chunk's readers of its, Bs and red are done, its loads drained
+        for (int u = tid; u < cnt * 6; u += kThreads)
+            reinterpret_cast<int4 *>(its)[u] = reinterpret_cast<const int4 *>(items + c_begin)[u];
+        __syncthreads();
+        int total = 0;
+        for (int it = 0; it < cnt; ++it) total += its[it].n[0] + its[it].n[1];
+
+        // producer (the step whose loads are issued next) and consumer (the step computed next)
+        int pi = 0, pt = 0, ci = 0, ct = 0;
+        auto advance = [&](int &i, int &t) {
+            if (++t == its[i].n[0] + its[i].n[1] && i + 1 < cnt) {
+                ++i;
+                t = 0;
+            }
+        };
+        i32x4 fa0[S], fa1[S];
+        zero_acc();
+        {
+            const StepAddr x = addr_of(0, 0);
+            issue_b(x, Bs0);
+            load_a(x, fa0);
+        }
+        // Step j: wait for its operands, prefetch step j+1 (past the last step the last again, into
+        // the idle buffer, unused), compute, then the epilogue of a tile that ends here.
+        auto step = [&](int j, const i32x4 *fa_cur, i32x4 *fa_new, const int8_t (*bcur)[W * kK],
+                        int8_t (*bnew)[W * kK]) {
+            const ResItem &C = its[ci];
+            const int n0 = C.n[0], n1 = C.n[1];
+            if (C.sym && (ct == n0 - 1 || (n1 > 0 && ct == n0 + n1 - 1))) {  // the diagonal step: past it x2
+#pragma unroll
+                for (int l = 0; l < S; ++l)
+#pragma unroll
+                    for (int nt = 0; nt < NT; ++nt) acc[l][nt] <<= 1;
+            }
+            __builtin_amdgcn_s_waitcnt((0x7 << 4) | (0xF << 8));  // vmcnt(0): this step's A and B are in
+            asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+            // a tile ends with this step: its epilogue's loads first
+            const bool epi = ct == n0 - 1 || (n1 > 0 && ct == n0 + n1 - 1);
+            const int Te = ct == n0 - 1 ? C.T[0] : C.T[1];
+            EpiLoads el;
+            if (epi) epi_load(ci, Te, el);
+            if (j + 1 < total) advance(pi, pt);
+            const StepAddr x = addr_of(pi, pt);
+            issue_b(x, bnew);
+            load_a(x, fa_new);
+            __builtin_amdgcn_sched_barrier(0);  // keep the prefetch ahead of the MFMAs
+            compute(bcur, fa_cur);
+            if (epi) {
+                epilogue(ci, Te, el);
+                zero_acc();
+            }
+            advance(ci, ct);
+        };
+        // a counted loop over pairs of steps: with `j += 2` the compiler peeled a step and left one
+        // per iteration, whose register set then had to be copied (waiting for its prefetch)
+        const int npairs = total >> 1;
+        for (int q = 0; q < npairs; ++q) {
+            step(2 * q, fa0, fa1, Bs0, Bs1);
+            step(2 * q + 1, fa1, fa0, Bs1, Bs0);
+        }
+        if (total & 1) step(total - 1, fa0, fa1, Bs0, Bs1);
+        __builtin_amdgcn_s_waitcnt(0);  // no copy into LDS outlives the chunk
+    }
+}
+
 // One evaluation of a batch (device table).
 struct EvalDesc {
     const unsigned char *base;
@@ -561,6 +790,10 @@ struct ResidentRho {
     std::vector<int> task_prefix;  // tasks of blocks < b, for b = 0..nblk
     int2 *d_order = nullptr;        // [8][order_len] (task, density matrix) items per XCD, (-1, 0) padded
     int order_len = 0;
+    // persistent evaluation: the items of workgroup i are d_items[d_slot_begin[i] .. d_slot_begin[i+1])
+    int nslots = 0;
+    ResItem *d_items = nullptr;
+    int *d_slot_begin = nullptr;
 };
 
 template <int S>
@@ -687,6 +920,8 @@ int resident_create(hipStream_t stream, const double *dms, const double *ao_lhs,
     // ends on long tasks.  Whole blocks (their tasks share Rq and Lc in the XCD's L2) go to the
     // XCD with the least work so far, largest first, and each XCD's items run longest first.
     std::vector<int2> order;  // [8][order_len], copied to the device below
+    std::vector<ResItem> ritems;  // persistent evaluation: items of slot 0, then of slot 1, ...
+    std::vector<int> slot_begin;
     {
         std::vector<std::vector<int2>> xcd_items(8);
         auto steps_of = [&](const int4 &t) {  // the k steps resident_tile runs for task t
@@ -717,6 +952,57 @@ int resident_create(hipStream_t stream, const double *dms, const double *ao_lhs,
         }
         order.assign((size_t)8 * h->order_len, int2{-1, 0});
         for (int x = 0; x < 8; ++x) std::copy(xcd_items[x].begin(), xcd_items[x].end(), order.begin() + (size_t)x * h->order_len);
+
+        // Persistent evaluation: two workgroups per CU, workgroup i on XCD i % 8; each XCD's items,
+        // longest first, go to its least loaded workgroup.
+        const int per_xcd = std::max(1, 2 * device_cu_count() / 8);
+        h->nslots = 8 * per_xcd;
+        std::vector<std::vector<int2>> slot_items((size_t)h->nslots);
+        for (int x = 0; x < 8; ++x) {
+            std::vector<long> wl((size_t)per_xcd, 0);
+            for (const int2 &it : xcd_items[x]) {
+                const int k = (int)(std::min_element(wl.begin(), wl.end()) - wl.begin());
+                wl[(size_t)k] += steps[(size_t)it.x];
+                slot_items[(size_t)x + 8 * (size_t)k].push_back(it);
+            }
+        }
+        const int W = 16 * h->nt;
+        ritems.reserve(tasks.size() * nset);
+        slot_begin.assign((size_t)h->nslots + 1, 0);
+        for (int i = 0; i < h->nslots; ++i) {
+            for (const int2 &it : slot_items[(size_t)i]) {
+                const int4 &t = tasks[(size_t)it.x];
+                const BlockDesc &bd = blocks[(size_t)t.x];
+                const int kp = bd.kpad, nks = kp / kK, ntl = kp / kRows, s = it.y;
+                const long dplane = dq_plane_bytes(kp, bd.sym);
+                // first 1 KiB tile of wave 0's row of tile T, and the bytes between the waves' rows
+                auto arow = [&](int T) { return bd.sym ? sym_tiles_before(T, nks) : (long)T * (kRows / 16) * nks; };
+                auto awave = [&](int T) { return (bd.sym ? nks - T : nks) * 1024; };
+                const int T1 = t.y, T2 = bd.sym && t.w ? ntl - 1 - t.y : -1;
+                const int n1 = bd.sym ? nks - T1 : nks, n2 = T2 > T1 ? nks - T2 : 0;
+                const int Tq[2] = {T1, n2 > 0 ? T2 : T1};
+                ResItem r{};
+                for (int q = 0; q < 2; ++q) {
+                    r.a[q] = bd.d_off + (long)s * S * dplane + arow(Tq[q]) * 1024;
+                    r.aw[q] = awave(Tq[q]);
+                    r.T[q] = Tq[q];
+                }
+                r.n[0] = n1;
+                r.n[1] = n2;
+                r.rq = bd.r_off + (long)t.z * W * kK;
+                r.ed = bd.e_off + (long)s * kp * (long)sizeof(int);
+                r.lc = bd.l_off;
+                r.dplane = (int)dplane;
+                r.rplane = (int)(blksize * (size_t)kp);
+                r.c0 = t.z * W;
+                r.b = t.x;
+                r.s = s;
+                r.nrows = bd.n;
+                r.sym = bd.sym;
+                ritems.push_back(r);
+            }
+            slot_begin[(size_t)i + 1] = (int)ritems.size();
+        }
     }
     if ((size_t)8 * h->order_len > INT32_MAX) {
         delete h;
@@ -724,6 +1010,10 @@ int resident_create(hipStream_t stream, const double *dms, const double *ao_lhs,
     }
     const size_t o_order = off;
     off += al(sizeof(int2) * 8 * std::max(h->order_len, 1));
+    const size_t o_items = off;
+    off += al(sizeof(ResItem) * std::max<size_t>(ritems.size(), 1));
+    const size_t o_slots = off;
+    off += al(sizeof(int) * slot_begin.size());
     h->bytes = off;
 
     // 3. allocate, upload the tables, slice
@@ -738,6 +1028,8 @@ int resident_create(hipStream_t stream, const double *dms, const double *ao_lhs,
     h->d_fg = reinterpret_cast<int *>(h->base + o_fg);
     h->d_tasks = reinterpret_cast<int4 *>(h->base + o_tasks);
     h->d_order = reinterpret_cast<int2 *>(h->base + o_order);
+    h->d_items = reinterpret_cast<ResItem *>(h->base + o_items);
+    h->d_slot_begin = reinterpret_cast<int *>(h->base + o_slots);
     st = hip_status(hipMemcpyAsync(h->d_blocks, blocks.data(), sizeof(BlockDesc) * nblk, hipMemcpyHostToDevice, stream),
                     "resident rho: table H2D");
     if (st == CFGMM_OK)
@@ -751,6 +1043,14 @@ int resident_create(hipStream_t stream, const double *dms, const double *ao_lhs,
         st = hip_status(hipMemcpyAsync(h->d_order, order.data(), sizeof(int2) * order.size(), hipMemcpyHostToDevice,
                                        stream),
                         "resident rho: order H2D");
+    if (st == CFGMM_OK && !ritems.empty())
+        st = hip_status(hipMemcpyAsync(h->d_items, ritems.data(), sizeof(ResItem) * ritems.size(),
+                                       hipMemcpyHostToDevice, stream),
+                        "resident rho: items H2D");
+    if (st == CFGMM_OK)
+        st = hip_status(hipMemcpyAsync(h->d_slot_begin, slot_begin.data(), sizeof(int) * slot_begin.size(),
+                                       hipMemcpyHostToDevice, stream),
+                        "resident rho: slots H2D");
     if (st == CFGMM_OK && max_kpad > 0) {  // an all-empty mask has no tiles to fill
         switch (S) {
             case 2: st = prep_launch<2>(stream, h, dms, ao_lhs, ao_rhs, stride_ao, stride_grid, lists, max_kpad); break;
@@ -815,6 +1115,32 @@ static int eval_launch(hipStream_t stream, const ResidentRho *h, double *partial
     hipEvent_t t0 = trace_start(stream);
     const int depth = tune_get(CFGMM_TUNE_RESIDENT_DEPTH);
     const bool all = ntasks == h->task_prefix[h->nblk];
+    if (all && nset == (int)h->nset && depth < 2 && tune_get(CFGMM_TUNE_RESIDENT_PERSISTENT) == 0) {
+        switch (h->nt) {
+            case 1:
+                hipLaunchKernelGGL((rho_resident_persistent_kernel<S, 1>), dim3((unsigned)h->nslots), dim3(kThreads), 0,
+                                   stream, partial, h->base, h->d_items, h->d_slot_begin, h->d_fg, (int)h->blksize,
+                                   ngrid, nset);
+                break;
+            case 2:
+                hipLaunchKernelGGL((rho_resident_persistent_kernel<S, 2>), dim3((unsigned)h->nslots), dim3(kThreads), 0,
+                                   stream, partial, h->base, h->d_items, h->d_slot_begin, h->d_fg, (int)h->blksize,
+                                   ngrid, nset);
+                break;
+            case 3:
+                hipLaunchKernelGGL((rho_resident_persistent_kernel<S, 3>), dim3((unsigned)h->nslots), dim3(kThreads), 0,
+                                   stream, partial, h->base, h->d_items, h->d_slot_begin, h->d_fg, (int)h->blksize,
+                                   ngrid, nset);
+                break;
+            default:
+                hipLaunchKernelGGL((rho_resident_persistent_kernel<S, 4>), dim3((unsigned)h->nslots), dim3(kThreads), 0,
+                                   stream, partial, h->base, h->d_items, h->d_slot_begin, h->d_fg, (int)h->blksize,
+                                   ngrid, nset);
+                break;
+        }
+        trace_stop(stream, "rho_resident_kernel", t0);
+        return hip_status(hipGetLastError(), "rho_resident_persistent_kernel launch");
+    }
     const dim3 grid((unsigned)(all ? 8 * h->order_len : ntasks * nset));
     const int2 *order = all ? h->d_order : nullptr;
 #define CFGMM_RES_LAUNCH(NT_, D_)                                                                                    \

@@ -400,7 +400,10 @@ enum cfgmm_tunable {
                                         LDS, 128-row tiles (default); 2 = the same with 256-row
                                         tiles (blksize not a multiple of 64); 1 = register-staged
                                         slices, as for f32, transposed B and odd n */
-    CFGMM_TUNE_COUNT = 12
+    CFGMM_TUNE_RESIDENT_PERSISTENT = 12, /* resident rho, whole evaluations: 0 = two workgroups per CU
+                                        each walking a fixed list of items as one pipeline
+                                        (default), 1 = one workgroup per item */
+    CFGMM_TUNE_COUNT = 13
 };
 int cfgmm_tune_set(int param, int value);
 int cfgmm_tune_get(int param, int *value);
