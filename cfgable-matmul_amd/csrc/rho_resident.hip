@@ -593,6 +593,8 @@ __global__ __launch_bounds__(kThreads, 2) void rho_resident_persistent_kernel(
         int eu[4], eg[NT];
         double lv[NT][4];
     };
+    double pend_v = 0.0;           // a partial value of the last epilogue, not yet stored
+    double *pend_dst = nullptr;    // ... and where it goes (threads < W)
     auto epi_load = [&](int it, int T, EpiLoads &e) {
         const ResItem &R = its[it];
         const int *__restrict__ ed = reinterpret_cast<const int *>(base + R.ed);
@@ -631,14 +633,16 @@ __global__ __launch_bounds__(kThreads, 2) void rho_resident_persistent_kernel(
             if (lk == 0) red[wave][nt * 16 + lrow] = part;
         }
         asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");  // LDS only: prefetches stay in flight
-        for (int c = tid; c < W; c += kThreads) {
-            const int g = gblk + c0 + c;
-            if (g < ngrid) {
-                double v = red[0][c];
+        // thread c < W sums column c; its store is issued by the next step after that step's prefetch
+        // (stores count in vmcnt too: stored here, the next step's wait for its operands would also
+        // wait for the store)
+        if (tid < W) {
+            const int g = gblk + c0 + tid;
+            double v = red[0][tid];
 #pragma unroll
-                for (int w = 1; w < kThreads / 64; ++w) v += red[w][c];
-                partial[((long)T * nset + s) * ngrid + g] = v;
-            }
+            for (int w = 1; w < kThreads / 64; ++w) v += red[w][tid];
+            pend_v = v;
+            pend_dst = g < ngrid ? partial + ((long)T * nset + s) * ngrid + g : nullptr;
         }
     };
 
@@ -662,13 +666,14 @@ __global__ __launch_bounds__(kThreads, 2) void rho_resident_persistent_kernel(
         };
         i32x4 fa0[S], fa1[S];
         zero_acc();
-        {
-            const StepAddr x = addr_of(0, 0);
-            issue_b(x, Bs0);
-            load_a(x, fa0);
-        }
+        StepAddr nx = addr_of(0, 0);
+        issue_b(nx, Bs0);
+        load_a(nx, fa0);
+        if (total > 1) advance(pi, pt);
+        nx = addr_of(pi, pt);  // what step 0 prefetches: step 1 (or step 0 again)
         // Step j: wait for its operands, prefetch step j+1 (past the last step the last again, into
-        // the idle buffer, unused), compute, then the epilogue of a tile that ends here.
+        // the idle buffer, unused; its addresses were worked out at the end of step j-1, off the
+        // path from the barrier to the loads), compute, then the epilogue of a tile that ends here.
         auto step = [&](int j, const i32x4 *fa_cur, i32x4 *fa_new, const int8_t (*bcur)[W * kK],
                         int8_t (*bnew)[W * kK]) {
             const ResItem &C = its[ci];
@@ -686,17 +691,24 @@ __global__ __launch_bounds__(kThreads, 2) void rho_resident_persistent_kernel(
             const int Te = ct == n0 - 1 ? C.T[0] : C.T[1];
             EpiLoads el;
             if (epi) epi_load(ci, Te, el);
-            if (j + 1 < total) advance(pi, pt);
-            const StepAddr x = addr_of(pi, pt);
-            issue_b(x, bnew);
-            load_a(x, fa_new);
+            issue_b(nx, bnew);
+            load_a(nx, fa_new);
+            if (pend_dst) {  // the previous epilogue's value, behind this step's prefetch
+                *pend_dst = pend_v;
+                pend_dst = nullptr;
+            }
             __builtin_amdgcn_sched_barrier(0);  // keep the prefetch ahead of the MFMAs
             compute(bcur, fa_cur);
             if (epi) {
+                // the epilogue's loads were issued before this step's prefetch (kBPerWave + S
+                // operations, and a pending store): wait for them only, the prefetch stays in flight
+                __builtin_amdgcn_s_waitcnt((kBPerWave + S) | (0x7 << 4) | (0xF << 8));
                 epilogue(ci, Te, el);
                 zero_acc();
             }
             advance(ci, ct);
+            if (j + 2 < total) advance(pi, pt);
+            nx = addr_of(pi, pt);
         };
         // a counted loop over pairs of steps: with `j += 2` the compiler peeled a step and left one
         // per iteration, whose register set then had to be copied (waiting for its prefetch)
@@ -706,6 +718,10 @@ __global__ __launch_bounds__(kThreads, 2) void rho_resident_persistent_kernel(
             step(2 * q + 1, fa1, fa0, Bs1, Bs0);
         }
         if (total & 1) step(total - 1, fa0, fa1, Bs0, Bs1);
+        if (pend_dst) {  // the last epilogue's value
+            *pend_dst = pend_v;
+            pend_dst = nullptr;
+        }
         __builtin_amdgcn_s_waitcnt(0);  // no copy into LDS outlives the chunk
     }
 }
