@@ -513,13 +513,17 @@ static_assert(sizeof(ResItem) == 96, "ResItem is six 16-byte words");
 
 constexpr int kItemsLds = 32;  // item records staged in LDS at a time
 
-template <int S, int NT>
+// DEPTH: k steps prefetched (1 or 2); the B stages are DEPTH + 1 LDS buffers, the A fragments
+// DEPTH + 1 register sets, their roles static in a loop unrolled by DEPTH + 1.
+template <int S, int NT, int DEPTH>
 __global__ __launch_bounds__(kThreads, 2) void rho_resident_persistent_kernel(
     double *__restrict__ partial, const unsigned char *__restrict__ base, const ResItem *__restrict__ items,
     const int *__restrict__ slot_begin, const int *__restrict__ fg, int blksize, int ngrid, int nset) {
+    static_assert(DEPTH == 1 || DEPTH == 2, "one or two k steps of prefetch");
     constexpr int W = NT * 16;
     __shared__ __attribute__((aligned(16))) int8_t Bs0[S][W * kK];
     __shared__ __attribute__((aligned(16))) int8_t Bs1[S][W * kK];
+    __shared__ __attribute__((aligned(16))) int8_t Bs2[DEPTH >= 2 ? S : 1][W * kK];
     __shared__ double red[kThreads / 64][W];
     __shared__ __attribute__((aligned(16))) ResItem its[kItemsLds];
 
@@ -542,17 +546,61 @@ __global__ __launch_bounds__(kThreads, 2) void rho_resident_persistent_kernel(
         const int8_t *b;  // this lane's 16 bytes of the B stage, plane 0, part 0
         int dplane, rplane;
     };
-    auto addr_of = [&](int it, int t) {
+    // The producer keeps its item's fields in registers and reads the item records in LDS only
+    // when it moves on to the next item: a read of LDS while copies into LDS are in flight may
+    // make the compiler wait for all of them.
+    struct ProdItem {
+        long a[2], rq;
+        int aw[2], n[2], T[2], dplane, rplane, sym;
+    };
+    auto prod_item = [&](int it) {
         const ResItem &R = its[it];
-        const int q = t < R.n[0] ? 0 : 1;
-        const int l = q ? t - R.n[0] : t, n = R.n[q], T = R.T[q];
-        const int ks = R.sym ? (l < n - 1 ? T + 1 + l : T) : l;
-        const int ai = R.sym ? (l < n - 1 ? l + 1 : 0) : l;
+        ProdItem P;
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            P.a[q] = R.a[q];
+            P.aw[q] = R.aw[q];
+            P.n[q] = R.n[q];
+            P.T[q] = R.T[q];
+        }
+        P.rq = R.rq;
+        P.dplane = R.dplane;
+        P.rplane = R.rplane;
+        P.sym = R.sym;
+        return P;
+    };
+    // the consumer's item: what the diagonal doubling, the tile ends and the epilogues need
+    struct ConsItem {
+        long ed, lc;
+        int n[2], T[2], sym, b, c0, s, nrows;
+    };
+    auto cons_item = [&](int it) {
+        const ResItem &R = its[it];
+        ConsItem c;
+        c.ed = R.ed;
+        c.lc = R.lc;
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            c.n[q] = R.n[q];
+            c.T[q] = R.T[q];
+        }
+        c.sym = R.sym;
+        c.b = R.b;
+        c.c0 = R.c0;
+        c.s = R.s;
+        c.nrows = R.nrows;
+        return c;
+    };
+    auto addr_of = [&](const ProdItem &P, int t) {
+        const int q = t < P.n[0] ? 0 : 1;
+        const int l = q ? t - P.n[0] : t, n = P.n[q], T = P.T[q];
+        const int ks = P.sym ? (l < n - 1 ? T + 1 + l : T) : l;
+        const int ai = P.sym ? (l < n - 1 ? l + 1 : 0) : l;
         StepAddr x;
-        x.a = reinterpret_cast<const int8_t *>(base + R.a[q]) + (long)wave * R.aw[q] + (long)ai * 1024 + lane * 16;
-        x.b = reinterpret_cast<const int8_t *>(base + R.rq) + ks * bstep + lane * 16;
-        x.dplane = R.dplane;
-        x.rplane = R.rplane;
+        x.a = reinterpret_cast<const int8_t *>(base + P.a[q]) + (long)wave * P.aw[q] + (long)ai * 1024 + lane * 16;
+        x.b = reinterpret_cast<const int8_t *>(base + P.rq) + ks * bstep + lane * 16;
+        x.dplane = P.dplane;
+        x.rplane = P.rplane;
         return x;
     };
     auto load_a = [&](const StepAddr &x, i32x4 *dst) {
@@ -595,8 +643,7 @@ __global__ __launch_bounds__(kThreads, 2) void rho_resident_persistent_kernel(
     };
     double pend_v = 0.0;           // a partial value of the last epilogue, not yet stored
     double *pend_dst = nullptr;    // ... and where it goes (threads < W)
-    auto epi_load = [&](int it, int T, EpiLoads &e) {
-        const ResItem &R = its[it];
+    auto epi_load = [&](const ConsItem &R, int T, EpiLoads &e) {
         const int *__restrict__ ed = reinterpret_cast<const int *>(base + R.ed);
         const double *__restrict__ lc = reinterpret_cast<const double *>(base + R.lc);
         const int gblk = R.b * blksize, i0 = T * kRows + wave * 16 + 4 * lk;
@@ -610,8 +657,7 @@ __global__ __launch_bounds__(kThreads, 2) void rho_resident_persistent_kernel(
             for (int r = 0; r < 4; ++r) e.lv[nt][r] = lc[(long)(i0 + r) * blksize + c];
         }
     };
-    auto epilogue = [&](int it, int T, const EpiLoads &e) {
-        const ResItem &R = its[it];
+    auto epilogue = [&](const ConsItem &R, int T, const EpiLoads &e) {
         const int gblk = R.b * blksize, c0 = R.c0, nrows = R.nrows, s = R.s;
         const int i0 = T * kRows + wave * 16 + 4 * lk;
         const double nan = __builtin_nan("");
@@ -658,25 +704,47 @@ __global__ __launch_bounds__(kThreads, 2) void rho_resident_persistent_kernel(
 
         // producer (the step whose loads are issued next) and consumer (the step computed next)
         int pi = 0, pt = 0, ci = 0, ct = 0;
-        auto advance = [&](int &i, int &t) {
-            if (++t == its[i].n[0] + its[i].n[1] && i + 1 < cnt) {
-                ++i;
-                t = 0;
+        ProdItem P = prod_item(0);
+        auto advance_p = [&]() {
+            if (++pt == P.n[0] + P.n[1] && pi + 1 < cnt) {
+                ++pi;
+                pt = 0;
+                P = prod_item(pi);
             }
         };
-        i32x4 fa0[S], fa1[S];
+        ConsItem C = cons_item(0);
+        auto advance_c = [&]() {
+            if (++ct == C.n[0] + C.n[1] && ci + 1 < cnt) {
+                ++ci;
+                ct = 0;
+                C = cons_item(ci);
+            }
+        };
+        i32x4 fa0[S], fa1[S], fa2[DEPTH >= 2 ? S : 1];
         zero_acc();
-        StepAddr nx = addr_of(0, 0);
+        StepAddr nx = addr_of(P, 0);
         issue_b(nx, Bs0);
         load_a(nx, fa0);
-        if (total > 1) advance(pi, pt);
-        nx = addr_of(pi, pt);  // what step 0 prefetches: step 1 (or step 0 again)
-        // Step j: wait for its operands, prefetch step j+1 (past the last step the last again, into
-        // the idle buffer, unused; its addresses were worked out at the end of step j-1, off the
-        // path from the barrier to the loads), compute, then the epilogue of a tile that ends here.
+        if (total > 1) advance_p();
+        nx = addr_of(P, pt);
+        if constexpr (DEPTH >= 2) {
+            issue_b(nx, Bs1);
+            load_a(nx, fa1);
+            if (total > 2) advance_p();
+            nx = addr_of(P, pt);
+        }
+        // nx: what step 0 prefetches, step DEPTH (or the last step again)
+        // Step j: wait for its operands, prefetch step j+DEPTH (past the last step the last again,
+        // into an idle buffer, unused; its addresses were worked out at the end of step j-1, off
+        // the path from the barrier to the loads), compute, then the epilogue of a tile that ends
+        // here.  The loads of step j+1 may stay in flight at the wait: kBPerWave + S operations
+        // per step, and the epilogue loads and the deferred store of a step are issued before
+        // and after its prefetch, so that waiting for all but the newest kBPerWave + S leaves at
+        // most one operation more than needed.
+        constexpr int kVmAhead = (DEPTH - 1) * (kBPerWave + S);
+        static_assert(kVmAhead < 64, "vmcnt is a 6-bit count");
         auto step = [&](int j, const i32x4 *fa_cur, i32x4 *fa_new, const int8_t (*bcur)[W * kK],
                         int8_t (*bnew)[W * kK]) {
-            const ResItem &C = its[ci];
             const int n0 = C.n[0], n1 = C.n[1];
             if (C.sym && (ct == n0 - 1 || (n1 > 0 && ct == n0 + n1 - 1))) {  // the diagonal step: past it x2
 #pragma unroll
@@ -684,13 +752,13 @@ __global__ __launch_bounds__(kThreads, 2) void rho_resident_persistent_kernel(
 #pragma unroll
                     for (int nt = 0; nt < NT; ++nt) acc[l][nt] <<= 1;
             }
-            __builtin_amdgcn_s_waitcnt((0x7 << 4) | (0xF << 8));  // vmcnt(0): this step's A and B are in
+            __builtin_amdgcn_s_waitcnt((kVmAhead & 15) | ((kVmAhead >> 4) << 14) | (0x7 << 4) | (0xF << 8));
             asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
             // a tile ends with this step: its epilogue's loads first
             const bool epi = ct == n0 - 1 || (n1 > 0 && ct == n0 + n1 - 1);
             const int Te = ct == n0 - 1 ? C.T[0] : C.T[1];
             EpiLoads el;
-            if (epi) epi_load(ci, Te, el);
+            if (epi) epi_load(C, Te, el);
             issue_b(nx, bnew);
             load_a(nx, fa_new);
             if (pend_dst) {  // the previous epilogue's value, behind this step's prefetch
@@ -703,21 +771,34 @@ __global__ __launch_bounds__(kThreads, 2) void rho_resident_persistent_kernel(
                 // the epilogue's loads were issued before this step's prefetch (kBPerWave + S
                 // operations, and a pending store): wait for them only, the prefetch stays in flight
                 __builtin_amdgcn_s_waitcnt((kBPerWave + S) | (0x7 << 4) | (0xF << 8));
-                epilogue(ci, Te, el);
+                epilogue(C, Te, el);
                 zero_acc();
             }
-            advance(ci, ct);
-            if (j + 2 < total) advance(pi, pt);
-            nx = addr_of(pi, pt);
+            advance_c();
+            if (j + DEPTH + 1 < total) advance_p();
+            nx = addr_of(P, pt);
         };
-        // a counted loop over pairs of steps: with `j += 2` the compiler peeled a step and left one
-        // per iteration, whose register set then had to be copied (waiting for its prefetch)
-        const int npairs = total >> 1;
-        for (int q = 0; q < npairs; ++q) {
-            step(2 * q, fa0, fa1, Bs0, Bs1);
-            step(2 * q + 1, fa1, fa0, Bs1, Bs0);
+        // a counted loop over groups of DEPTH + 1 steps: with `j += 2` the compiler peeled a step
+        // and left one per iteration, whose register set then had to be copied (waiting for its
+        // prefetch)
+        if constexpr (DEPTH == 1) {
+            const int npairs = total >> 1;
+            for (int q = 0; q < npairs; ++q) {
+                step(2 * q, fa0, fa1, Bs0, Bs1);
+                step(2 * q + 1, fa1, fa0, Bs1, Bs0);
+            }
+            if (total & 1) step(total - 1, fa0, fa1, Bs0, Bs1);
+        } else {
+            const int ntri = total / 3;
+            for (int q = 0; q < ntri; ++q) {
+                step(3 * q, fa0, fa2, Bs0, Bs2);
+                step(3 * q + 1, fa1, fa0, Bs1, Bs0);
+                step(3 * q + 2, fa2, fa1, Bs2, Bs1);
+            }
+            const int r = total - 3 * ntri;
+            if (r >= 1) step(3 * ntri, fa0, fa2, Bs0, Bs2);
+            if (r >= 2) step(3 * ntri + 1, fa1, fa0, Bs1, Bs0);
         }
-        if (total & 1) step(total - 1, fa0, fa1, Bs0, Bs1);
         if (pend_dst) {  // the last epilogue's value
             *pend_dst = pend_v;
             pend_dst = nullptr;
@@ -1131,29 +1212,26 @@ static int eval_launch(hipStream_t stream, const ResidentRho *h, double *partial
     hipEvent_t t0 = trace_start(stream);
     const int depth = tune_get(CFGMM_TUNE_RESIDENT_DEPTH);
     const bool all = ntasks == h->task_prefix[h->nblk];
-    if (all && nset == (int)h->nset && depth < 2 && tune_get(CFGMM_TUNE_RESIDENT_PERSISTENT) == 0) {
-        switch (h->nt) {
-            case 1:
-                hipLaunchKernelGGL((rho_resident_persistent_kernel<S, 1>), dim3((unsigned)h->nslots), dim3(kThreads), 0,
-                                   stream, partial, h->base, h->d_items, h->d_slot_begin, h->d_fg, (int)h->blksize,
-                                   ngrid, nset);
-                break;
-            case 2:
-                hipLaunchKernelGGL((rho_resident_persistent_kernel<S, 2>), dim3((unsigned)h->nslots), dim3(kThreads), 0,
-                                   stream, partial, h->base, h->d_items, h->d_slot_begin, h->d_fg, (int)h->blksize,
-                                   ngrid, nset);
-                break;
-            case 3:
-                hipLaunchKernelGGL((rho_resident_persistent_kernel<S, 3>), dim3((unsigned)h->nslots), dim3(kThreads), 0,
-                                   stream, partial, h->base, h->d_items, h->d_slot_begin, h->d_fg, (int)h->blksize,
-                                   ngrid, nset);
-                break;
-            default:
-                hipLaunchKernelGGL((rho_resident_persistent_kernel<S, 4>), dim3((unsigned)h->nslots), dim3(kThreads), 0,
-                                   stream, partial, h->base, h->d_items, h->d_slot_begin, h->d_fg, (int)h->blksize,
-                                   ngrid, nset);
-                break;
+    if (all && nset == (int)h->nset && tune_get(CFGMM_TUNE_RESIDENT_PERSISTENT) == 0) {
+#define CFGMM_RESP_LAUNCH(NT_, D_)                                                                                 \
+    hipLaunchKernelGGL((rho_resident_persistent_kernel<S, NT_, D_>), dim3((unsigned)h->nslots), dim3(kThreads), 0, \
+                       stream, partial, h->base, h->d_items, h->d_slot_begin, h->d_fg, (int)h->blksize, ngrid, nset)
+        if (depth >= 2 && S <= 5) {  // with six slices the third register set spills
+            switch (h->nt) {
+                case 1: CFGMM_RESP_LAUNCH(1, 2); break;
+                case 2: CFGMM_RESP_LAUNCH(2, 2); break;
+                case 3: CFGMM_RESP_LAUNCH(3, 2); break;
+                default: CFGMM_RESP_LAUNCH(4, 2); break;
+            }
+        } else {
+            switch (h->nt) {
+                case 1: CFGMM_RESP_LAUNCH(1, 1); break;
+                case 2: CFGMM_RESP_LAUNCH(2, 1); break;
+                case 3: CFGMM_RESP_LAUNCH(3, 1); break;
+                default: CFGMM_RESP_LAUNCH(4, 1); break;
+            }
         }
+#undef CFGMM_RESP_LAUNCH
         trace_stop(stream, "rho_resident_kernel", t0);
         return hip_status(hipGetLastError(), "rho_resident_persistent_kernel launch");
     }

@@ -200,6 +200,32 @@ def test_resident_launch_tunables_bitwise(depth, cols):
     assert np.array_equal(got, ref)
 
 
+@pytest.mark.parametrize("same", [True, False])
+def test_resident_persistent_many_items(same):
+    """2048 blocks of 16 points and five density matrices: 40 (symmetric form) or 60 (general form)
+    items per workgroup of the persistent evaluation, more than its 32 item records staged in LDS at
+    a time.  Held to the oracle, and bitwise equal to one workgroup per item, at both prefetch depths."""
+    p = W.rho_problem(seed=12, nshell=200, nao=600, ngrid=32768, nset=5, blksize=16, density=0.3)
+    ref = _oracle(p)
+    r = Resident(p, ns=5, same=same)
+    old = (cfgmm.tune_get(cfgmm.TUNE_RESIDENT_PERSISTENT), cfgmm.tune_get(cfgmm.TUNE_RESIDENT_DEPTH))
+    outs = {}
+    try:
+        for pers, depth in ((0, 1), (0, 2), (1, 1)):
+            cfgmm.tune_set(cfgmm.TUNE_RESIDENT_PERSISTENT, pers)
+            cfgmm.tune_set(cfgmm.TUNE_RESIDENT_DEPTH, depth)
+            outs[(pers, depth)] = r.eval().reshape(p.nset, p.ngrid)
+    finally:
+        cfgmm.tune_set(cfgmm.TUNE_RESIDENT_PERSISTENT, old[0])
+        cfgmm.tune_set(cfgmm.TUNE_RESIDENT_DEPTH, old[1])
+        r.close()
+    got = outs[(0, 1)]
+    err = rel_fro(got, ref)
+    print(f"persistent resident, same={same}: rel_fro {err:.3e}")
+    assert err <= 1e-10
+    assert np.array_equal(outs[(0, 2)], got) and np.array_equal(outs[(1, 1)], got)
+
+
 def test_resident_eval_batched_bitwise():
     """One batched launch over several handles (different sizes, leading ranges, strides, slices)
     equals separate evaluations bitwise; empty entries are no-ops."""
