@@ -518,7 +518,7 @@ constexpr int kItemsLds = 32;  // item records staged in LDS at a time
 template <int S, int NT, int DEPTH>
 __global__ __launch_bounds__(kThreads, 2) void rho_resident_persistent_kernel(
     double *__restrict__ partial, const unsigned char *__restrict__ base, const ResItem *__restrict__ items,
-    const int *__restrict__ slot_begin, const int *__restrict__ fg, int blksize, int ngrid, int nset) {
+    const int *__restrict__ slot_begin, const int *__restrict__ fg, int blksize, int ngrid, int nset, int nblk_used) {
     static_assert(DEPTH == 1 || DEPTH == 2, "one or two k steps of prefetch");
     constexpr int W = NT * 16;
     __shared__ __attribute__((aligned(16))) int8_t Bs0[S][W * kK];
@@ -526,6 +526,7 @@ __global__ __launch_bounds__(kThreads, 2) void rho_resident_persistent_kernel(
     __shared__ __attribute__((aligned(16))) int8_t Bs2[DEPTH >= 2 ? S : 1][W * kK];
     __shared__ double red[kThreads / 64][W];
     __shared__ __attribute__((aligned(16))) ResItem its[kItemsLds];
+    __shared__ int its_cnt;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int lrow = lane & 15, lk = lane >> 4;
@@ -694,11 +695,25 @@ __global__ __launch_bounds__(kThreads, 2) void rho_resident_persistent_kernel(
 
     const int i_begin = slot_begin[blockIdx.x], i_end = slot_begin[blockIdx.x + 1];
     for (int c_begin = i_begin; c_begin < i_end; c_begin += kItemsLds) {
-        const int cnt = min(kItemsLds, i_end - c_begin);
+        const int raw = min(kItemsLds, i_end - c_begin);
         __syncthreads();  // the previous chunk's readers of its, Bs and red are done, its loads drained
-        for (int u = tid; u < cnt * 6; u += kThreads)
-            reinterpret_cast<int4 *>(its)[u] = reinterpret_cast<const int4 *>(items + c_begin)[u];
+        // the chunk's items inside the evaluated range (grid blocks < nblk_used, density matrices
+        // < nset), compacted in order by wave 0: lane l keeps item l
+        if (wave == 0) {
+            const ResItem *__restrict__ src = items + c_begin + min(lane, raw - 1);
+            const bool keep = lane < raw && src->b < nblk_used && src->s < nset;
+            const unsigned long long m = __ballot(keep);
+            if (keep) {
+                const int pos = __popcll(m & ((1ull << lane) - 1ull));
+#pragma unroll
+                for (int u = 0; u < 6; ++u)
+                    reinterpret_cast<int4 *>(its + pos)[u] = reinterpret_cast<const int4 *>(src)[u];
+            }
+            if (lane == 0) its_cnt = __popcll(m);
+        }
         __syncthreads();
+        const int cnt = its_cnt;
+        if (cnt == 0) continue;
         int total = 0;
         for (int it = 0; it < cnt; ++it) total += its[it].n[0] + its[it].n[1];
 
@@ -1212,10 +1227,17 @@ static int eval_launch(hipStream_t stream, const ResidentRho *h, double *partial
     hipEvent_t t0 = trace_start(stream);
     const int depth = tune_get(CFGMM_TUNE_RESIDENT_DEPTH);
     const bool all = ntasks == h->task_prefix[h->nblk];
-    if (all && nset == (int)h->nset && tune_get(CFGMM_TUNE_RESIDENT_PERSISTENT) == 0) {
+    // The persistent kernel walks the items dealt at creation, skipping those outside a leading
+    // range: taken when at least half of them are evaluated (each workgroup's list then keeps
+    // roughly its share; a short leading range is better spread by one workgroup per item).
+    const long nitems_all = (long)h->task_prefix[h->nblk] * (long)h->nset;
+    const bool persistent = tune_get(CFGMM_TUNE_RESIDENT_PERSISTENT) == 0 && 2L * ntasks * nset >= nitems_all;
+    if (persistent) {
+        const int nblk_used = (int)div_up((size_t)ngrid, h->blksize);
 #define CFGMM_RESP_LAUNCH(NT_, D_)                                                                                 \
     hipLaunchKernelGGL((rho_resident_persistent_kernel<S, NT_, D_>), dim3((unsigned)h->nslots), dim3(kThreads), 0, \
-                       stream, partial, h->base, h->d_items, h->d_slot_begin, h->d_fg, (int)h->blksize, ngrid, nset)
+                       stream, partial, h->base, h->d_items, h->d_slot_begin, h->d_fg, (int)h->blksize, ngrid, nset, \
+                       nblk_used)
         if (depth >= 2 && S <= 5) {  // with six slices the third register set spills
             switch (h->nt) {
                 case 1: CFGMM_RESP_LAUNCH(1, 2); break;

@@ -204,17 +204,22 @@ def test_resident_launch_tunables_bitwise(depth, cols):
 def test_resident_persistent_many_items(same):
     """2048 blocks of 16 points and five density matrices: 40 (symmetric form) or 60 (general form)
     items per workgroup of the persistent evaluation, more than its 32 item records staged in LDS at
-    a time.  Held to the oracle, and bitwise equal to one workgroup per item, at both prefetch depths."""
+    a time.  Held to the oracle, and bitwise equal to one workgroup per item, at both prefetch depths.
+    A leading range of 3/4 of the grid and 4 of the 5 density matrices
+    (60 % of the items: the persistent kernel, skipping the others) against the per-item kernel
+    bitwise, the rest of rho untouched."""
     p = W.rho_problem(seed=12, nshell=200, nao=600, ngrid=32768, nset=5, blksize=16, density=0.3)
     ref = _oracle(p)
     r = Resident(p, ns=5, same=same)
+    ng, ns = 3 * p.ngrid // 4, 4
     old = (cfgmm.tune_get(cfgmm.TUNE_RESIDENT_PERSISTENT), cfgmm.tune_get(cfgmm.TUNE_RESIDENT_DEPTH))
-    outs = {}
+    outs, parts = {}, {}
     try:
         for pers, depth in ((0, 1), (0, 2), (1, 1)):
             cfgmm.tune_set(cfgmm.TUNE_RESIDENT_PERSISTENT, pers)
             cfgmm.tune_set(cfgmm.TUNE_RESIDENT_DEPTH, depth)
             outs[(pers, depth)] = r.eval().reshape(p.nset, p.ngrid)
+            parts[(pers, depth)] = r.eval(ngrid=ng, nset=ns).reshape(p.nset, p.ngrid)
     finally:
         cfgmm.tune_set(cfgmm.TUNE_RESIDENT_PERSISTENT, old[0])
         cfgmm.tune_set(cfgmm.TUNE_RESIDENT_DEPTH, old[1])
@@ -224,6 +229,10 @@ def test_resident_persistent_many_items(same):
     print(f"persistent resident, same={same}: rel_fro {err:.3e}")
     assert err <= 1e-10
     assert np.array_equal(outs[(0, 2)], got) and np.array_equal(outs[(1, 1)], got)
+    part = parts[(0, 1)]
+    assert rel_fro(part[:ns, :ng], ref[:ns, :ng]) <= 1e-10
+    assert np.all(part[:, ng:] == 0.0) and np.all(part[ns:] == 0.0)
+    assert np.array_equal(parts[(0, 2)], part) and np.array_equal(parts[(1, 1)], part)
 
 
 def test_resident_eval_batched_bitwise():
