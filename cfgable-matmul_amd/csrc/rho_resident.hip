@@ -45,6 +45,9 @@
 #include "internal.hpp"
 
 #include <algorithm>
+#include <mutex>
+#include <atomic>
+#include <type_traits>
 #include <vector>
 
 namespace cfgmm {
@@ -511,26 +514,45 @@ struct ResItem {
 };
 static_assert(sizeof(ResItem) == 96, "ResItem is six 16-byte words");
 
+// An item of a batched persistent evaluation (items of several handles and evaluations in one
+// launch): self-contained, with absolute addresses, its evaluation's partial rows and sizes.
+struct ResItemX {
+    long a[2], rq, ed, lc;  // as in ResItem, absolute addresses
+    long part;              // absolute: the evaluation's partial + s * ngrid
+    long fg;                // absolute: the handle's per-point exponents
+    int aw[2], n[2], T[2];
+    int dplane, rplane, c0, b, s, nrows, sym;
+    int tstride;            // partial elements between row tiles: nset * ngrid of the evaluation
+    int blksize, ngrid;     // the handle's block size, the evaluation's grid points
+    int pad[2];
+};
+static_assert(sizeof(ResItemX) == 128, "ResItemX is eight 16-byte words");
+
 constexpr int kItemsLds = 32;  // item records staged in LDS at a time
 
 // DEPTH: k steps prefetched (1 or 2); the B stages are DEPTH + 1 LDS buffers, the A fragments
 // DEPTH + 1 register sets, their roles static in a loop unrolled by DEPTH + 1.
-template <int S, int NT, int DEPTH>
+// XR = false: the items of one handle (ResItem, offsets from base), evaluated over its leading
+// nblk_used blocks and nset density matrices; XR = true: a batch (ResItemX; the other arguments
+// unused).
+template <int S, int NT, int DEPTH, bool XR>
 __global__ __launch_bounds__(kThreads, 2) void rho_resident_persistent_kernel(
-    double *__restrict__ partial, const unsigned char *__restrict__ base, const ResItem *__restrict__ items,
+    double *__restrict__ partial, const unsigned char *__restrict__ base, const void *__restrict__ items_v,
     const int *__restrict__ slot_begin, const int *__restrict__ fg, int blksize, int ngrid, int nset, int nblk_used) {
     static_assert(DEPTH == 1 || DEPTH == 2, "one or two k steps of prefetch");
+    using Rec = typename std::conditional<XR, ResItemX, ResItem>::type;
+    constexpr int kWords = (int)sizeof(Rec) / 16;
+    const Rec *__restrict__ items = static_cast<const Rec *>(items_v);
     constexpr int W = NT * 16;
     __shared__ __attribute__((aligned(16))) int8_t Bs0[S][W * kK];
     __shared__ __attribute__((aligned(16))) int8_t Bs1[S][W * kK];
     __shared__ __attribute__((aligned(16))) int8_t Bs2[DEPTH >= 2 ? S : 1][W * kK];
     __shared__ double red[kThreads / 64][W];
-    __shared__ __attribute__((aligned(16))) ResItem its[kItemsLds];
+    __shared__ __attribute__((aligned(16))) Rec its[kItemsLds];
     __shared__ int its_cnt;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int lrow = lane & 15, lk = lane >> 4;
-    const long bstep = (long)blksize * kK;  // bytes of one k step of all the block's columns in Rq
 
     i32x4 acc[S][NT];
     auto zero_acc = [&]() {
@@ -550,12 +572,14 @@ __global__ __launch_bounds__(kThreads, 2) void rho_resident_persistent_kernel(
     // The producer keeps its item's fields in registers and reads the item records in LDS only
     // when it moves on to the next item: a read of LDS while copies into LDS are in flight may
     // make the compiler wait for all of them.
+    // (a and rq: offsets from base, or with XR absolute addresses)
     struct ProdItem {
-        long a[2], rq;
+        long a[2], rq, bstep;  // bstep (XR): bytes of one k step of all the block's columns in Rq
         int aw[2], n[2], T[2], dplane, rplane, sym;
     };
+    const long bstep = (long)blksize * kK;  // the same for a single handle
     auto prod_item = [&](int it) {
-        const ResItem &R = its[it];
+        const Rec &R = its[it];
         ProdItem P;
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
@@ -565,18 +589,22 @@ __global__ __launch_bounds__(kThreads, 2) void rho_resident_persistent_kernel(
             P.T[q] = R.T[q];
         }
         P.rq = R.rq;
+        if constexpr (XR) P.bstep = (long)R.blksize * kK;
         P.dplane = R.dplane;
         P.rplane = R.rplane;
         P.sym = R.sym;
         return P;
     };
-    // the consumer's item: what the diagonal doubling, the tile ends and the epilogues need
+    // the consumer's item: what the diagonal doubling, the tile ends and the epilogues need (ed
+    // and lc as prod_item's a; with XR also the evaluation's partial rows, exponents and sizes)
     struct ConsItem {
         long ed, lc;
-        int n[2], T[2], sym, b, c0, s, nrows;
+        double *part;
+        const int *fgp;
+        int n[2], T[2], sym, b, c0, s, nrows, tstride, blksize, ngrid;
     };
     auto cons_item = [&](int it) {
-        const ResItem &R = its[it];
+        const Rec &R = its[it];
         ConsItem c;
         c.ed = R.ed;
         c.lc = R.lc;
@@ -590,6 +618,13 @@ __global__ __launch_bounds__(kThreads, 2) void rho_resident_persistent_kernel(
         c.c0 = R.c0;
         c.s = R.s;
         c.nrows = R.nrows;
+        if constexpr (XR) {
+            c.part = reinterpret_cast<double *>(R.part);
+            c.fgp = reinterpret_cast<const int *>(R.fg);
+            c.tstride = R.tstride;
+            c.blksize = R.blksize;
+            c.ngrid = R.ngrid;
+        }
         return c;
     };
     auto addr_of = [&](const ProdItem &P, int t) {
@@ -598,8 +633,13 @@ __global__ __launch_bounds__(kThreads, 2) void rho_resident_persistent_kernel(
         const int ks = P.sym ? (l < n - 1 ? T + 1 + l : T) : l;
         const int ai = P.sym ? (l < n - 1 ? l + 1 : 0) : l;
         StepAddr x;
-        x.a = reinterpret_cast<const int8_t *>(base + P.a[q]) + (long)wave * P.aw[q] + (long)ai * 1024 + lane * 16;
-        x.b = reinterpret_cast<const int8_t *>(base + P.rq) + ks * bstep + lane * 16;
+        if constexpr (XR) {
+            x.a = reinterpret_cast<const int8_t *>(P.a[q]) + (long)wave * P.aw[q] + (long)ai * 1024 + lane * 16;
+            x.b = reinterpret_cast<const int8_t *>(P.rq) + ks * P.bstep + lane * 16;
+        } else {
+            x.a = reinterpret_cast<const int8_t *>(base + P.a[q]) + (long)wave * P.aw[q] + (long)ai * 1024 + lane * 16;
+            x.b = reinterpret_cast<const int8_t *>(base + P.rq) + ks * bstep + lane * 16;
+        }
         x.dplane = P.dplane;
         x.rplane = P.rplane;
         return x;
@@ -644,28 +684,40 @@ __global__ __launch_bounds__(kThreads, 2) void rho_resident_persistent_kernel(
     };
     double pend_v = 0.0;           // a partial value of the last epilogue, not yet stored
     double *pend_dst = nullptr;    // ... and where it goes (threads < W)
+    // the item's block size, grid points, exponents and partial row (tile T, grid point 0)
+    auto c_blksize = [&](const ConsItem &R) { if constexpr (XR) return R.blksize; else return blksize; };
+    auto c_ngrid = [&](const ConsItem &R) { if constexpr (XR) return R.ngrid; else return ngrid; };
+    auto c_fg = [&](const ConsItem &R) { if constexpr (XR) return R.fgp; else return fg; };
+    auto c_part = [&](const ConsItem &R, int T) {
+        if constexpr (XR) return R.part + (long)T * R.tstride;
+        else return partial + ((long)T * nset + R.s) * ngrid;
+    };
     auto epi_load = [&](const ConsItem &R, int T, EpiLoads &e) {
-        const int *__restrict__ ed = reinterpret_cast<const int *>(base + R.ed);
-        const double *__restrict__ lc = reinterpret_cast<const double *>(base + R.lc);
-        const int gblk = R.b * blksize, i0 = T * kRows + wave * 16 + 4 * lk;
+        const unsigned char *b0 = XR ? nullptr : base;
+        const int *__restrict__ ed = reinterpret_cast<const int *>(b0 + R.ed);
+        const double *__restrict__ lc = reinterpret_cast<const double *>(b0 + R.lc);
+        const int bs = c_blksize(R), ng = c_ngrid(R);
+        const int *__restrict__ fgp = c_fg(R);
+        const int gblk = R.b * bs, i0 = T * kRows + wave * 16 + 4 * lk;
 #pragma unroll
         for (int r = 0; r < 4; ++r) e.eu[r] = ed[i0 + r];
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) {
             const int c = R.c0 + nt * 16 + lrow;
-            e.eg[nt] = fg[min(gblk + c, ngrid - 1)];
+            e.eg[nt] = fgp[min(gblk + c, ng - 1)];
 #pragma unroll
-            for (int r = 0; r < 4; ++r) e.lv[nt][r] = lc[(long)(i0 + r) * blksize + c];
+            for (int r = 0; r < 4; ++r) e.lv[nt][r] = lc[(long)(i0 + r) * bs + c];
         }
     };
     auto epilogue = [&](const ConsItem &R, int T, const EpiLoads &e) {
-        const int gblk = R.b * blksize, c0 = R.c0, nrows = R.nrows, s = R.s;
+        const int ng = c_ngrid(R);
+        const int gblk = R.b * c_blksize(R), c0 = R.c0, nrows = R.nrows;
         const int i0 = T * kRows + wave * 16 + 4 * lk;
         const double nan = __builtin_nan("");
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) {
             const int c = c0 + nt * 16 + lrow;
-            const bool g_ok = gblk + c < ngrid;
+            const bool g_ok = gblk + c < ng;
             double part = 0.0;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
@@ -689,7 +741,7 @@ __global__ __launch_bounds__(kThreads, 2) void rho_resident_persistent_kernel(
 #pragma unroll
             for (int w = 1; w < kThreads / 64; ++w) v += red[w][tid];
             pend_v = v;
-            pend_dst = g < ngrid ? partial + ((long)T * nset + s) * ngrid + g : nullptr;
+            pend_dst = g < ng ? c_part(R, T) + g : nullptr;
         }
     };
 
@@ -700,13 +752,13 @@ __global__ __launch_bounds__(kThreads, 2) void rho_resident_persistent_kernel(
         // the chunk's items inside the evaluated range (grid blocks < nblk_used, density matrices
         // < nset), compacted in order by wave 0: lane l keeps item l
         if (wave == 0) {
-            const ResItem *__restrict__ src = items + c_begin + min(lane, raw - 1);
-            const bool keep = lane < raw && src->b < nblk_used && src->s < nset;
+            const Rec *__restrict__ src = items + c_begin + min(lane, raw - 1);
+            const bool keep = lane < raw && (XR || (src->b < nblk_used && src->s < nset));
             const unsigned long long m = __ballot(keep);
             if (keep) {
                 const int pos = __popcll(m & ((1ull << lane) - 1ull));
 #pragma unroll
-                for (int u = 0; u < 6; ++u)
+                for (int u = 0; u < kWords; ++u)
                     reinterpret_cast<int4 *>(its + pos)[u] = reinterpret_cast<const int4 *>(src)[u];
             }
             if (lane == 0) its_cnt = __popcll(m);
@@ -905,6 +957,8 @@ struct ResidentRho {
     // persistent evaluation: the items of workgroup i are d_items[d_slot_begin[i] .. d_slot_begin[i+1])
     int nslots = 0;
     ResItem *d_items = nullptr;
+    std::vector<ResItem> h_items;  // every item, task-major (block order), for batched evaluations
+    unsigned long long uid = 0;    // unique per handle created in this process (batch plan cache keys)
     int *d_slot_begin = nullptr;
 };
 
@@ -930,6 +984,8 @@ int resident_create(hipStream_t stream, const double *dms, const double *ao_lhs,
     if (nao > INT32_MAX || ngrid > INT32_MAX || nset > 65535 || blksize > 4096)
         return fail(CFGMM_EINVAL, "resident rho: size out of range");
     auto h = new ResidentRho();
+    static std::atomic<unsigned long long> next_uid{1};
+    h->uid = next_uid++;
     CFGMM_HIP_TRY(hipGetDevice(&h->device));
     h->nslices = nslices;
     h->nao = nao;
@@ -1079,42 +1135,48 @@ int resident_create(hipStream_t stream, const double *dms, const double *ao_lhs,
             }
         }
         const int W = 16 * h->nt;
+        // the record of (task i, density matrix s): offsets from the handle's base
+        auto make_item = [&](int i, int s) {
+            const int4 &t = tasks[(size_t)i];
+            const BlockDesc &bd = blocks[(size_t)t.x];
+            const int kp = bd.kpad, nks = kp / kK, ntl = kp / kRows;
+            const long dplane = dq_plane_bytes(kp, bd.sym);
+            // first 1 KiB tile of wave 0's row of tile T, and the bytes between the waves' rows
+            auto arow = [&](int T) { return bd.sym ? sym_tiles_before(T, nks) : (long)T * (kRows / 16) * nks; };
+            auto awave = [&](int T) { return (bd.sym ? nks - T : nks) * 1024; };
+            const int T1 = t.y, T2 = bd.sym && t.w ? ntl - 1 - t.y : -1;
+            const int n1 = bd.sym ? nks - T1 : nks, n2 = T2 > T1 ? nks - T2 : 0;
+            const int Tq[2] = {T1, n2 > 0 ? T2 : T1};
+            ResItem r{};
+            for (int q = 0; q < 2; ++q) {
+                r.a[q] = bd.d_off + (long)s * S * dplane + arow(Tq[q]) * 1024;
+                r.aw[q] = awave(Tq[q]);
+                r.T[q] = Tq[q];
+            }
+            r.n[0] = n1;
+            r.n[1] = n2;
+            r.rq = bd.r_off + (long)t.z * W * kK;
+            r.ed = bd.e_off + (long)s * kp * (long)sizeof(int);
+            r.lc = bd.l_off;
+            r.dplane = (int)dplane;
+            r.rplane = (int)(blksize * (size_t)kp);
+            r.c0 = t.z * W;
+            r.b = t.x;
+            r.s = s;
+            r.nrows = bd.n;
+            r.sym = bd.sym;
+            return r;
+        };
         ritems.reserve(tasks.size() * nset);
         slot_begin.assign((size_t)h->nslots + 1, 0);
         for (int i = 0; i < h->nslots; ++i) {
-            for (const int2 &it : slot_items[(size_t)i]) {
-                const int4 &t = tasks[(size_t)it.x];
-                const BlockDesc &bd = blocks[(size_t)t.x];
-                const int kp = bd.kpad, nks = kp / kK, ntl = kp / kRows, s = it.y;
-                const long dplane = dq_plane_bytes(kp, bd.sym);
-                // first 1 KiB tile of wave 0's row of tile T, and the bytes between the waves' rows
-                auto arow = [&](int T) { return bd.sym ? sym_tiles_before(T, nks) : (long)T * (kRows / 16) * nks; };
-                auto awave = [&](int T) { return (bd.sym ? nks - T : nks) * 1024; };
-                const int T1 = t.y, T2 = bd.sym && t.w ? ntl - 1 - t.y : -1;
-                const int n1 = bd.sym ? nks - T1 : nks, n2 = T2 > T1 ? nks - T2 : 0;
-                const int Tq[2] = {T1, n2 > 0 ? T2 : T1};
-                ResItem r{};
-                for (int q = 0; q < 2; ++q) {
-                    r.a[q] = bd.d_off + (long)s * S * dplane + arow(Tq[q]) * 1024;
-                    r.aw[q] = awave(Tq[q]);
-                    r.T[q] = Tq[q];
-                }
-                r.n[0] = n1;
-                r.n[1] = n2;
-                r.rq = bd.r_off + (long)t.z * W * kK;
-                r.ed = bd.e_off + (long)s * kp * (long)sizeof(int);
-                r.lc = bd.l_off;
-                r.dplane = (int)dplane;
-                r.rplane = (int)(blksize * (size_t)kp);
-                r.c0 = t.z * W;
-                r.b = t.x;
-                r.s = s;
-                r.nrows = bd.n;
-                r.sym = bd.sym;
-                ritems.push_back(r);
-            }
+            for (const int2 &it : slot_items[(size_t)i]) ritems.push_back(make_item(it.x, it.y));
             slot_begin[(size_t)i + 1] = (int)ritems.size();
         }
+        // batched evaluations deal the items of several handles per call: task-major, blocks in order
+        h->h_items.reserve(tasks.size() * nset);
+        for (size_t i = 0; i < tasks.size(); ++i)
+            for (size_t sset = 0; sset < nset; ++sset) h->h_items.push_back(make_item((int)i, (int)sset));
     }
     if ((size_t)8 * h->order_len > INT32_MAX) {
         delete h;
@@ -1235,7 +1297,7 @@ static int eval_launch(hipStream_t stream, const ResidentRho *h, double *partial
     if (persistent) {
         const int nblk_used = (int)div_up((size_t)ngrid, h->blksize);
 #define CFGMM_RESP_LAUNCH(NT_, D_)                                                                                 \
-    hipLaunchKernelGGL((rho_resident_persistent_kernel<S, NT_, D_>), dim3((unsigned)h->nslots), dim3(kThreads), 0, \
+    hipLaunchKernelGGL((rho_resident_persistent_kernel<S, NT_, D_, false>), dim3((unsigned)h->nslots), dim3(kThreads), 0, \
                        stream, partial, h->base, h->d_items, h->d_slot_begin, h->d_fg, (int)h->blksize, ngrid, nset, \
                        nblk_used)
         if (depth >= 2 && S <= 5) {  // with six slices the third register set spills
@@ -1307,6 +1369,134 @@ int resident_eval(hipStream_t stream, const ResidentRho *h, double *rhos, size_t
     return launch_rho_reduce(stream, rhos, partial, h->d_counts, stride_rho, ngrid_used, nset_used, h->blksize);
 }
 
+// Batched persistent evaluations: the items of every evaluation of a group, as self-contained
+// records, dealt to two workgroups per CU (workgroup i on XCD i % 8): each (evaluation, grid block)
+// -- its items share Rq and Lc -- to the XCD with the least work so far, its items round-robin over
+// that XCD's workgroups.  The records depend on the handles, their leading ranges and the partial
+// buffers (in the stream's workspace), not on rho: a plan is built on the host and uploaded once
+// per distinct batch, and kept for the next calls with the same batch (the last few batches).
+namespace {
+struct BatchPlan {
+    int device = -1;
+    std::vector<unsigned long long> key;
+    unsigned char *dev = nullptr;  // [slot_begin: nslots + 1 ints, padded to 256 bytes][ResItemX records]
+    int nslots = 0;
+};
+std::mutex g_plan_mutex;
+std::vector<BatchPlan> g_plans;  // least recently used first
+constexpr size_t kPlanCache = 4;
+}  // namespace
+
+struct BatchEvalRef {
+    const ResidentRho *h;
+    double *partial;
+    int ngrid, nset;
+};
+
+// The device plan of `evals` (records and slot ranges), from the cache or built and uploaded now.
+static int batch_plan(hipStream_t stream, const std::vector<BatchEvalRef> &evals, const unsigned char **dev,
+                      int *nslots) {
+    int device = 0;
+    CFGMM_HIP_TRY(hipGetDevice(&device));
+    std::vector<unsigned long long> key;
+    for (const BatchEvalRef &e : evals) {
+        key.push_back(e.h->uid);
+        key.push_back(reinterpret_cast<unsigned long long>(e.partial));
+        key.push_back(((unsigned long long)(unsigned)e.ngrid << 32) | (unsigned)e.nset);
+    }
+    std::lock_guard<std::mutex> lock(g_plan_mutex);
+    for (size_t i = 0; i < g_plans.size(); ++i) {
+        if (g_plans[i].device == device && g_plans[i].key == key) {
+            std::rotate(g_plans.begin() + (long)i, g_plans.begin() + (long)i + 1, g_plans.end());
+            *dev = g_plans.back().dev;
+            *nslots = g_plans.back().nslots;
+            return CFGMM_OK;
+        }
+    }
+    const int per_xcd = std::max(1, 2 * device_cu_count() / 8), ns_all = 8 * per_xcd;
+    std::vector<std::vector<ResItemX>> slots((size_t)ns_all);
+    long xload[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int rr[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    size_t nitems = 0;
+    for (const BatchEvalRef &e : evals) {
+        const ResidentRho *h = e.h;
+        const int nblk_used = (int)div_up((size_t)e.ngrid, h->blksize);
+        const long hb = reinterpret_cast<long>(h->base);
+        size_t i = 0;
+        while (i < h->h_items.size()) {
+            const int b = h->h_items[i].b;
+            size_t j = i;
+            while (j < h->h_items.size() && h->h_items[j].b == b) ++j;  // the items of block b
+            if (b < nblk_used) {
+                const int x = (int)(std::min_element(xload, xload + 8) - xload);
+                for (size_t k = i; k < j; ++k) {
+                    const ResItem &r = h->h_items[k];
+                    if (r.s >= e.nset) continue;
+                    ResItemX q{};
+                    for (int t = 0; t < 2; ++t) {
+                        q.a[t] = hb + r.a[t];
+                        q.aw[t] = r.aw[t];
+                        q.n[t] = r.n[t];
+                        q.T[t] = r.T[t];
+                    }
+                    q.rq = hb + r.rq;
+                    q.ed = hb + r.ed;
+                    q.lc = hb + r.lc;
+                    q.part = reinterpret_cast<long>(e.partial + (long)r.s * e.ngrid);
+                    q.fg = reinterpret_cast<long>(h->d_fg);
+                    q.dplane = r.dplane;
+                    q.rplane = r.rplane;
+                    q.c0 = r.c0;
+                    q.b = r.b;
+                    q.s = r.s;
+                    q.nrows = r.nrows;
+                    q.sym = r.sym;
+                    q.tstride = e.nset * e.ngrid;
+                    q.blksize = (int)h->blksize;
+                    q.ngrid = e.ngrid;
+                    slots[(size_t)x + 8 * (size_t)(rr[x]++ % per_xcd)].push_back(q);
+                    xload[x] += r.n[0] + r.n[1];
+                    ++nitems;
+                }
+            }
+            i = j;
+        }
+    }
+    if (nitems > (size_t)INT32_MAX) return fail(CFGMM_EINVAL, "resident rho: batch too large");
+    const size_t head = div_up(sizeof(int) * (size_t)(ns_all + 1), 256) * 256;
+    std::vector<unsigned char> host(head + sizeof(ResItemX) * std::max<size_t>(nitems, 1));
+    int *sb = reinterpret_cast<int *>(host.data());
+    ResItemX *rec = reinterpret_cast<ResItemX *>(host.data() + head);
+    size_t pos = 0;
+    sb[0] = 0;
+    for (int i = 0; i < ns_all; ++i) {
+        for (const ResItemX &q : slots[(size_t)i]) rec[pos++] = q;
+        sb[i + 1] = (int)pos;
+    }
+    if (g_plans.size() == kPlanCache) {  // the least recently used plan goes
+        (void)hipStreamSynchronize(stream);  // no launch of this stream still reads it
+        (void)hipDeviceSynchronize();        // nor of another
+        (void)hipFree(g_plans.front().dev);
+        g_plans.erase(g_plans.begin());
+    }
+    BatchPlan pl;
+    pl.device = device;
+    pl.key = key;
+    pl.nslots = ns_all;
+    CFGMM_HIP_TRY(hipMalloc(&pl.dev, host.size()));
+    // pageable source: the copy is complete when hipMemcpyAsync returns
+    const int st = hip_status(hipMemcpyAsync(pl.dev, host.data(), host.size(), hipMemcpyHostToDevice, stream),
+                              "resident rho: batch plan H2D");
+    if (st != CFGMM_OK) {
+        (void)hipFree(pl.dev);
+        return st;
+    }
+    g_plans.push_back(std::move(pl));
+    *dev = g_plans.back().dev;
+    *nslots = ns_all;
+    return CFGMM_OK;
+}
+
 // Evaluations of several handles in one launch per (slices, tile width) group; each exactly as
 // resident_eval would compute it (the same tile code and the same fixed reduction order).
 int resident_eval_batched(hipStream_t stream, const ResidentEvalReq *reqs, size_t n) {
@@ -1362,8 +1552,48 @@ int resident_eval_batched(hipStream_t stream, const ResidentEvalReq *reqs, size_
         // pageable source: the copy is complete when hipMemcpyAsync returns
         CFGMM_HIP_TRY(hipMemcpyAsync(base, d.data(), sizeof(EvalDesc) * d.size(), hipMemcpyHostToDevice, stream));
         const EvalDesc *dd = reinterpret_cast<const EvalDesc *>(base);
-        hipEvent_t t0 = trace_start(stream);
         const bool deep = tune_get(CFGMM_TUNE_RESIDENT_DEPTH) >= 2;
+        // the persistent batch for up to five slices (with six its register sets spill)
+        if (tune_get(CFGMM_TUNE_RESIDENT_PERSISTENT) == 0 && S <= 5) {
+            std::vector<BatchEvalRef> refs(group.size());
+            for (size_t j = 0; j < group.size(); ++j)
+                refs[j] = BatchEvalRef{reqs[group[j]].h, d[j].partial, d[j].ngrid, d[j].nset};
+            const unsigned char *plan = nullptr;
+            int nslots = 0;
+            st = batch_plan(stream, refs, &plan, &nslots);
+            if (st != CFGMM_OK) return st;
+            const size_t head = div_up(sizeof(int) * (size_t)(nslots + 1), 256) * 256;
+            const void *recs = plan + head;
+            const int *sb = reinterpret_cast<const int *>(plan);
+            hipEvent_t t0 = trace_start(stream);
+#define CFGMM_RESBP(S_, NT_, D_)                                                                              \
+    hipLaunchKernelGGL((rho_resident_persistent_kernel<S_, NT_, D_, true>), dim3((unsigned)nslots),            \
+                       dim3(kThreads), 0, stream, nullptr, nullptr, recs, sb, nullptr, 0, 0, 0, 0)
+#define CFGMM_RESBP_NT(S_, D_)               \
+    switch (nt) {                            \
+        case 1: CFGMM_RESBP(S_, 1, D_); break; \
+        case 2: CFGMM_RESBP(S_, 2, D_); break; \
+        case 3: CFGMM_RESBP(S_, 3, D_); break; \
+        default: CFGMM_RESBP(S_, 4, D_); break; \
+    }
+            switch (S) {  // one k step of prefetch: with the batch records a second register set spills
+                case 2: CFGMM_RESBP_NT(2, 1) break;
+                case 3: CFGMM_RESBP_NT(3, 1) break;
+                case 4: CFGMM_RESBP_NT(4, 1) break;
+                default: CFGMM_RESBP_NT(5, 1) break;
+            }
+#undef CFGMM_RESBP_NT
+#undef CFGMM_RESBP
+            trace_stop(stream, "rho_resident_kernel", t0);
+            st = hip_status(hipGetLastError(), "rho_resident_persistent_kernel (batch) launch");
+            if (st != CFGMM_OK) return st;
+            hipLaunchKernelGGL(resident_batched_reduce, dim3((unsigned)div_up(max_elems, 256), (unsigned)d.size()),
+                               dim3(256), 0, stream, dd);
+            st = hip_status(hipGetLastError(), "resident_batched_reduce launch");
+            if (st != CFGMM_OK) return st;
+            continue;
+        }
+        hipEvent_t t0 = trace_start(stream);
 #define CFGMM_RESB(S_, NT_)                                                                                     \
     do {                                                                                                         \
         if (deep)                                                                                                \

@@ -266,6 +266,55 @@ def test_resident_eval_batched_bitwise():
         assert np.array_equal(o, ref)
 
 
+def test_resident_eval_batched_persistent_plans():
+    """Five-slice handles in both forms in one batched launch (the persistent batch: records of
+    every evaluation dealt at the call, the plan kept for the same batch): bitwise equal to
+    separate evaluations and to the batch of one workgroup per item, on the first call and on
+    repeats (the kept plan), over full and leading ranges with padded strides, and for more
+    distinct batches than plans are kept."""
+    probs = [W.rho_problem(seed=50 + i, nshell=100 + 50 * i, nao=300 + 150 * i, ngrid=64 * (10 + 9 * i), nset=1 + i % 3,
+                           density=0.2 + 0.1 * i) for i in range(4)]
+    rs = [Resident(p, ns=5, same=(i % 2 == 0)) for i, p in enumerate(probs)]
+    jobs = [(0, probs[0].ngrid, probs[0].nset, probs[0].ngrid), (1, 64 * 7, 1, 64 * 7 + 5),
+            (2, probs[2].ngrid, probs[2].nset, probs[2].ngrid + 3), (3, 64 * 20, 1, 64 * 21)]
+    refs = [rs[i].eval(rhos0=np.zeros(rs[i].p.nset * st) + k, stride=st, ngrid=ng, nset=ns)
+            for k, (i, ng, ns, st) in enumerate(jobs)]
+    s = cfgmm.Stream()
+
+    def run(sel):
+        bufs = [cfgmm.DeviceBuffer.from_array(np.zeros(rs[jobs[k][0]].p.nset * jobs[k][3]) + k) for k in sel]
+        cfgmm.rho_resident_eval_batched(s, [(rs[jobs[k][0]].h, b, jobs[k][3], jobs[k][1], jobs[k][2])
+                                            for k, b in zip(sel, bufs)])
+        s.synchronize()
+        outs = [b.to_array(np.float64, refs[k].shape) for k, b in zip(sel, bufs)]
+        for b in bufs:
+            b.free()
+        return outs
+
+    old = cfgmm.tune_get(cfgmm.TUNE_RESIDENT_PERSISTENT)
+    try:
+        everything = list(range(len(jobs)))
+        for outs in (run(everything), run(everything)):  # built, then kept
+            for k, o in zip(everything, outs):
+                assert np.array_equal(o, refs[k])
+        for sel in ([0, 2], [1, 3], [3], [2, 1, 0], [0, 1, 3], everything):  # past the kept plans
+            for k, o in zip(sel, run(sel)):
+                assert np.array_equal(o, refs[k])
+        cfgmm.tune_set(cfgmm.TUNE_RESIDENT_PERSISTENT, 1)
+        for k, o in zip(everything, run(everything)):
+            assert np.array_equal(o, refs[k])
+    finally:
+        cfgmm.tune_set(cfgmm.TUNE_RESIDENT_PERSISTENT, old)
+        for r in rs:
+            r.close()
+        s.destroy()
+    for k, (i, ng, ns, st) in enumerate(jobs):
+        p = probs[i]
+        ref = _oracle(p).reshape(p.nset, p.ngrid)
+        got = refs[k].reshape(p.nset, st)
+        assert rel_fro(got[:ns, :ng] - k, ref[:ns, :ng]) <= 1e-10
+
+
 def test_resident_symmetric_form():
     """Symmetric D with lhs and rhs one buffer: the upper-triangle tiles only (about half the bytes),
     the oracle to 1e-10 over a partial last block, a leading range, both prefetch depths (bitwise
