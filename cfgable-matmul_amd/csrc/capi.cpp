@@ -136,6 +136,9 @@ static Workspace &workspace_of(int dev, hipStream_t stream) {
     return g_ws[std::make_pair(dev, stream)];
 }
 
+// The allocation holds kFlagBytes of flag words, then the workspace proper.
+constexpr size_t kFlagBytes = 256;
+
 int workspace_get(hipStream_t stream, size_t bytes, void **out) {
     int dev = 0;
     CFGMM_HIP_TRY(hipGetDevice(&dev));
@@ -151,20 +154,35 @@ int workspace_get(hipStream_t stream, size_t bytes, void **out) {
         }
         const size_t want = std::max(bytes + bytes / 4, (size_t)1 << 20);
         void *p = nullptr;
-        CFGMM_HIP_TRY(hipMalloc(&p, want));
+        CFGMM_HIP_TRY(hipMalloc(&p, want + kFlagBytes));
+        // fresh memory may hold anything, an earlier call's epoch included: the flag words start at 0
+        const hipError_t e = hipMemsetAsync(p, 0, kFlagBytes, stream);
+        if (e != hipSuccess) {
+            (void)hipFree(p);
+            return hip_status(e, "workspace flags");
+        }
         ws.ptr = p;
         ws.bytes = want;
     }
-    *out = ws.ptr;
+    *out = static_cast<char *>(ws.ptr) + kFlagBytes;
     return CFGMM_OK;
 }
 
-unsigned workspace_epoch(hipStream_t stream) {
+int *workspace_flag(hipStream_t stream) {
     int dev = 0;
     (void)hipGetDevice(&dev);
     Workspace &ws = workspace_of(dev, stream);
     std::lock_guard<std::mutex> lock(ws.mu);
-    return ++ws.epoch;
+    return static_cast<int *>(ws.ptr);
+}
+
+static std::atomic<unsigned> g_epoch{0};
+
+unsigned workspace_epoch() {
+    unsigned e;
+    do e = ++g_epoch;
+    while (e == 0);
+    return e;
 }
 
 int host_staging_get(hipStream_t stream, size_t bytes, void **out) {

@@ -39,12 +39,15 @@ struct Workspace {
     void *host = nullptr;  // pinned host staging for tables copied to the device
     size_t host_bytes = 0;
     hipEvent_t host_done = nullptr;  // recorded after the last copy out of `host`
-    unsigned epoch = 0;
 };
 int workspace_get(hipStream_t stream, size_t bytes, void **out);
-// A new value per call on the stream (wraps after 2^32): a flag in the workspace written with
-// the current value during a call was written by that call.
-unsigned workspace_epoch(hipStream_t stream);
+// The stream's flag word: beside the workspace (not in it), zeroed whenever the workspace is
+// allocated, written only with epochs.  Valid after workspace_get on the stream, until the next
+// workspace_get that grows it.
+int *workspace_flag(hipStream_t stream);
+// A new value per call, process-wide, never 0 (wraps after 2^32 calls): a flag word holding the
+// current value was written during this call; any other value (0 or an earlier call's) was not.
+unsigned workspace_epoch();
 // Pinned host memory of the stream, for host tables that a call copies to the device.  It waits
 // until the copies enqueued before the last host_staging_release have read it; the caller fills
 // it, enqueues its copies, then calls host_staging_release.
@@ -103,6 +106,7 @@ struct RhoScratch {
     void *ptr;
     size_t bytes;
     unsigned epoch;
+    int *flag;  // a word beside ptr .. ptr + bytes holding 0 or earlier calls' epochs
 };
 int launch_rho_multi(hipStream_t stream, double *rhos, size_t rho_cs, const double *dms, const double *ao_lhs,
                      size_t lhs_cs, size_t ncomp, const double *ao_rhs, size_t stride_ao, size_t stride_grid,

@@ -410,8 +410,10 @@ int launch_rho_multi(hipStream_t stream, double *rhos, size_t rho_cs, const doub
     int *counts = lists + nblk * nao;
     int *perm = counts + nblk;
     int *ntask = perm + nslot;
-    int *sym = ntask + 1;
-    int *bounds = sym + 1;
+    int *bounds = ntask + 2;
+    // the symmetry flag lives in a word of its own, beside the workspace (the scratch): inside it,
+    // where each problem size puts other data, it could hold a value that equals this call's epoch
+    int *sym = scratch ? scratch->flag : workspace_flag(stream);
     int *keys = bounds + 9;
     float *pwork = reinterpret_cast<float *>(keys + nslot);
     double *partial = reinterpret_cast<double *>(static_cast<char *>(ws) + part_off);
@@ -420,11 +422,12 @@ int launch_rho_multi(hipStream_t stream, double *rhos, size_t rho_cs, const doub
     st = launch_build_lists(stream, tab, ldtab, nao, nblk, lists, counts, ordered ? keys : nullptr, maxchunks);
     if (st != CFGMM_OK) return st;
     // the symmetric form: one set of orbitals on both sides and every D_s symmetric, tested on the
-    // device by the plan launch's other workgroups (a flag against this call's epoch: no clearing)
+    // device by the plan launch's other workgroups (a flag against this call's epoch, unique in the
+    // process: no clearing)
     const size_t nt64 = div_up(nao, 64), npairs = nt64 * (nt64 + 1) / 2;
     const bool try_sym = ordered && ncomp == 1 && ao_lhs == ao_rhs && tune_get(CFGMM_TUNE_RHO_SYMMETRIC) == 0 &&
                          npairs * nset + 1 <= (size_t)INT32_MAX;
-    const unsigned epoch = try_sym ? (scratch ? scratch->epoch : workspace_epoch(stream)) : 0u;
+    const unsigned epoch = try_sym ? (scratch ? scratch->epoch : workspace_epoch()) : 0u;
     if (ordered) {
         hipEvent_t t0 = trace_start(stream);
         hipLaunchKernelGGL(rho_plan_kernel, dim3((unsigned)(1 + (try_sym ? npairs * nset : 0))), dim3(kPlanThreads),
@@ -506,7 +509,11 @@ int launch_rho_noalloc(hipStream_t stream, double *rhos, const double *dms, cons
     const int st = scratch_address(&p);
     if (st != CFGMM_OK) return st;
     const TraceOff no_trace;  // library tracing would create events
-    const RhoScratch sc{p, kRhoScratchBytes, ++g_scratch_epoch};
+    // the first 256 bytes of `scratch` hold the symmetry flag (zero at load, then only epochs)
+    unsigned epoch;
+    do epoch = ++g_scratch_epoch;
+    while (epoch == 0);
+    const RhoScratch sc{static_cast<unsigned char *>(p) + 256, kRhoScratchBytes - 256, epoch, static_cast<int *>(p)};
     return launch_rho_multi(stream, rhos, 0, dms, ao_lhs, 0, 1, ao_rhs, stride_ao, stride_grid, nao, ngrid, nset,
                             tab, ldtab, blksize, &sc);
 }

@@ -175,7 +175,8 @@ int cfgmm_aodmao2rho_dev(void *stream, double *rhos, const double *dms, const do
  * loops, harnesses that forbid hipMalloc).  Same arguments and result as cfgmm_aodmao2rho_dev
  * (FP64, the same kernels); it calls no hipMalloc or hipFree and creates no events.  Its
  * scratch -- the compacted AO lists, the task plan and the chunk partials -- is a device array
- * of CFGMM_RHO_NOALLOC_SCRATCH_BYTES in the library, one per GPU, held until the process ends.
+ * of CFGMM_RHO_NOALLOC_SCRATCH_BYTES in the library (its first 256 bytes a flag word), one per
+ * GPU, held until the process ends.
  * A problem needs about 4*nblk*(nao + 3*ceil(nao/64)) + 8*ceil(nao/64)*nset*ngrid bytes
  * (nblk = ceil(ngrid/blksize); 16 MB for nao 2228, ngrid 16384, nset 3); a larger one returns
  * CFGMM_EINVAL with the needed size in cfgmm_last_error().  Every call on a GPU shares that
@@ -297,7 +298,10 @@ int cfgmm_rho_resident_eval(void *stream, const cfgmm_rho_resident *h, double *r
                             size_t ngrid_used, size_t nset_used);
 /* Several evaluations (of one or many handles) in one launch per (nslices, tile width) group;
  * each entry with the contract of cfgmm_rho_resident_eval and bitwise its result.  Entries
- * must not write overlapping rho ranges. */
+ * must not write overlapping rho ranges.  With up to five slices the launch is persistent: the
+ * first call of a batch deals every item to the device's workgroups on the host and uploads that
+ * plan (about 128 bytes per item, allocated with hipMalloc); calls with the same handles, ranges
+ * and stream workspace reuse it (the plans of the last four batches are kept, process-wide). */
 typedef struct cfgmm_rho_resident_job {
     const cfgmm_rho_resident *h;
     double *rhos;

@@ -348,6 +348,30 @@ def _oracle(p, dms, lhs, rhs):
     return ref
 
 
+def test_rho_symmetric_form_on_fresh_streams():
+    """Each call's symmetry flag is a word of its own, zeroed with the workspace, tested against an
+    epoch unique in the process: a new stream's workspace (possibly memory an earlier one used)
+    never carries a value that makes a symmetric D look asymmetric.  Per stream, an asymmetric call
+    (the flag written) then a symmetric one, which must take the symmetric form."""
+    p = W.rho_problem(seed=23, nshell=120, nao=380, ngrid=64 * 9 + 7, nset=2, density=0.4)
+    bad = p.dms.copy()
+    lst = np.flatnonzero(p.tab[:, 0])
+    bad[1, int(lst[2]), int(lst[-1])] += 10.0
+    old = cfgmm.tune_get(cfgmm.TUNE_RHO_SYMMETRIC)
+    try:
+        cfgmm.tune_set(cfgmm.TUNE_RHO_SYMMETRIC, 1)
+        gen = _rho_dev(p, p.dms, p.phi, p.phi)
+        cfgmm.tune_set(cfgmm.TUNE_RHO_SYMMETRIC, 0)
+        ref = _oracle(p, p.dms, p.phi, p.phi)
+        for _ in range(12):  # _rho_dev makes a stream, its workspace, and destroys the stream
+            _rho_dev(p, bad, p.phi, p.phi)
+            sym = _rho_dev(p, p.dms, p.phi, p.phi)
+            assert rel_fro(sym, ref) <= TOL
+            assert not np.array_equal(sym, gen)  # the symmetric form ran
+    finally:
+        cfgmm.tune_set(cfgmm.TUNE_RHO_SYMMETRIC, old)
+
+
 @pytest.mark.parametrize("blksize,nset", [(64, 3), (96, 5)])
 def test_rho_symmetric_form(blksize, nset):
     """Bitwise-symmetric D_s with lhs and rhs one buffer take the upper-triangle form: the oracle
