@@ -1062,7 +1062,7 @@ def main():
                    "mask_density": round(float(p.shell_tab.mean()), 4),
                    "skipped_tile_fraction": round(p.skipped_fraction(), 4),
                    "masked_gflop_per_eval": round(p.flops / 1e9, 3),
-                   "form": "symmetric (upper triangle per block; D == D^T checked on the device)" if symmetric
+                   "form": "symmetric (upper triangle per block; D == D^T to 2^-50 checked on the device)" if symmetric
                            else "general",
                    "form_gflop_per_eval": round(flops / 1e9, 3),
                    "wall_s_timed_region": round(r["wall_s"], 4)},

@@ -271,9 +271,13 @@ __device__ inline void rho_chunk_body(const RhoChunk &task, unsigned char *__res
 
 
 // Whether the 64x64 tiles (ti, tj) and (tj, ti) of the nao x nao matrix d (row stride ld) are
-// not each other's transpose, bitwise; the same answer on every thread of the workgroup (NT
-// threads, a multiple of 64 dividing 4096, all of which must call it).  t: LDS scratch of 64 x 65
-// words.
+// not each other's transpose; the same answer on every thread of the workgroup (NT threads, a
+// multiple of 64 dividing 4096, all of which must call it).  t: LDS scratch of 64 x 65 words.
+// Two entries d[i][j], d[j][i] count as equal when their bits are, or when they differ by at most
+// 2^-50 of the larger magnitude (4 units in the last place): a D formed as C C^T by a general
+// product is symmetric only to rounding.  The symmetric form then contracts d[min][max] for both,
+// which changes each term by at most that much; NaN (unless bitwise equal), infinities against
+// finite values and every larger difference keep the general form.
 template <int NT>
 __device__ inline bool tile_pair_differs(const double *__restrict__ d, long ld, int nao, int ti, int tj,
                                          unsigned long long (*t)[65]) {
@@ -305,7 +309,9 @@ __device__ inline bool tile_pair_differs(const double *__restrict__ d, long ld, 
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         const int r = r0 + q * DR, j = j0 + r, i = i0 + c;
-        differ |= i < nao && j < nao && v[q] != t[c][r];
+        const double x = __longlong_as_double((long long)v[q]), y = __longlong_as_double((long long)t[c][r]);
+        const bool close = fabs(x - y) <= 0x1p-50 * fmax(fabs(x), fabs(y));
+        differ |= i < nao && j < nao && v[q] != t[c][r] && !close;
     }
     return __syncthreads_or(differ ? 1 : 0) != 0;
 }

@@ -163,7 +163,8 @@ int cfgmm_aodmao2rho_anyway(double *rhos, size_t rhos_len, const double *dms, si
                             size_t tab_len, size_t ldtab);
 
 /* Device-pointer rho.  When ao_lhs == ao_rhs (one pointer) and every D_s equals its transpose
- * bitwise (tested on the device in the call, no host synchronisation), each block's upper
+ * (tested on the device in the call, no host synchronisation; entries equal bitwise or within
+ * 2^-50 of the larger magnitude, as a D formed by a general product C C^T is), each block's upper
  * triangle is contracted and the part past the diagonal doubled: the same sum within rounding,
  * half the work.  CFGMM_TUNE_RHO_SYMMETRIC = 1 turns this off. */
 int cfgmm_aodmao2rho_dev(void *stream, double *rhos, const double *dms, const double *ao_lhs, const double *ao_rhs,
@@ -282,8 +283,9 @@ int cfgmm_aodmao2rho_batched_dev(void *stream, const cfgmm_rho_problem *problems
  * sum_b (nset*nslices*kpad_b^2 + blksize*kpad_b*(nslices + 8)) bytes plus small
  * tables, kpad_b = |L_b| rounded up to 64 (cfgmm_rho_resident_bytes).  The slices are
  * balanced digits of about 8 bits: 5 keep rho within 1e-10 on the tested workloads.  With
- * ao_lhs == ao_rhs and every D_s symmetric (tested at creation) only the upper-triangle tiles
- * are kept (about half the bytes and time), for blocks of up to 8192 active AOs.
+ * ao_lhs == ao_rhs and every D_s symmetric (tested at creation, as for cfgmm_aodmao2rho_dev)
+ * only the upper-triangle tiles are kept (about half the bytes and time), for blocks of up to
+ * 8192 active AOs.
  * precision must be CFGMM_PREC_OZAKI_INT8. */
 typedef struct cfgmm_rho_resident cfgmm_rho_resident;
 int cfgmm_rho_resident_create(void *stream, const double *dms, const double *ao_lhs, const double *ao_rhs,

@@ -372,6 +372,59 @@ def test_rho_symmetric_form_on_fresh_streams():
         cfgmm.tune_set(cfgmm.TUNE_RHO_SYMMETRIC, old)
 
 
+def test_rho_symmetric_to_rounding():
+    """D symmetric only to rounding (entries of the upper triangle moved by one or two units in the
+    last place, as a general product C C^T leaves them) still takes the symmetric form, held to the
+    oracle on that D; a difference of 1e-13 relative (about 450 units) keeps the general form,
+    bitwise.  The resident handle's test at creation is the same."""
+    p = W.rho_problem(seed=29, nshell=200, nao=620, ngrid=64 * 12 + 9, nset=2, density=0.4)
+    rng = np.random.default_rng(3)
+    near, far = p.dms.copy(), p.dms.copy()
+    iu = np.triu_indices(p.nao, 1)
+    for s in range(p.nset):
+        m = rng.random(len(iu[0])) < 0.5
+        x = near[s][iu]
+        near[s][iu] = np.where(m, np.nextafter(np.nextafter(x, np.inf), np.inf), x)
+    far[1, iu[0][5], iu[1][5]] *= 1.0 + 1e-13
+    assert not np.array_equal(near[0], near[0].T)
+    old = cfgmm.tune_get(cfgmm.TUNE_RHO_SYMMETRIC)
+    try:
+        cfgmm.tune_set(cfgmm.TUNE_RHO_SYMMETRIC, 1)
+        gen_near = _rho_dev(p, near, p.phi, p.phi)
+        gen_far = _rho_dev(p, far, p.phi, p.phi)
+        cfgmm.tune_set(cfgmm.TUNE_RHO_SYMMETRIC, 0)
+        sym_near = _rho_dev(p, near, p.phi, p.phi)
+        got_far = _rho_dev(p, far, p.phi, p.phi)
+    finally:
+        cfgmm.tune_set(cfgmm.TUNE_RHO_SYMMETRIC, old)
+    assert not np.array_equal(sym_near, gen_near)  # the symmetric form ran
+    assert rel_fro(sym_near, _oracle(p, near, p.phi, p.phi)) <= TOL
+    assert np.array_equal(got_far, gen_far)  # the general form ran
+    # resident handles: the symmetric layout (about half the bytes) for `near`, the general one for `far`
+    s = cfgmm.Stream()
+    d_phi = cfgmm.DeviceBuffer.from_array(p.phi)
+    d_t = cfgmm.DeviceBuffer.from_array(np.ascontiguousarray(p.tab))
+    sizes = []
+    for dm in (near, far):
+        d_d = cfgmm.DeviceBuffer.from_array(np.ascontiguousarray(dm))
+        h = cfgmm.RhoResident(s, d_d, d_phi, d_phi, p.nao, p.ngrid, p.nao, p.ngrid, p.nset, d_t, p.nblk, p.blksize,
+                              cfgmm.PREC_OZAKI_INT8, 5)
+        d_r = cfgmm.DeviceBuffer(8 * p.nset * p.ngrid)
+        d_r.zero()
+        h.eval(s, d_r, p.ngrid)
+        s.synchronize()
+        out = d_r.to_array(np.float64, (p.nset * p.ngrid,))
+        sizes.append(h.bytes)
+        assert rel_fro(out, _oracle(p, dm, p.phi, p.phi)) <= 1e-10
+        h.destroy()
+        d_d.free()
+        d_r.free()
+    d_phi.free()
+    d_t.free()
+    s.destroy()
+    assert sizes[0] < 0.75 * sizes[1]
+
+
 @pytest.mark.parametrize("blksize,nset", [(64, 3), (96, 5)])
 def test_rho_symmetric_form(blksize, nset):
     """Bitwise-symmetric D_s with lhs and rhs one buffer take the upper-triangle form: the oracle
