@@ -1394,8 +1394,11 @@ struct BatchEvalRef {
 };
 
 // The device plan of `evals` (records and slot ranges), from the cache or built and uploaded now.
+// Returns with `lock` holding the cache: the caller enqueues the launch that reads the plan before
+// releasing it, so that an eviction (which synchronizes the plan's device first) cannot free a plan
+// whose launch is not yet enqueued.
 static int batch_plan(hipStream_t stream, const std::vector<BatchEvalRef> &evals, const unsigned char **dev,
-                      int *nslots) {
+                      int *nslots, std::unique_lock<std::mutex> &lock) {
     int device = 0;
     CFGMM_HIP_TRY(hipGetDevice(&device));
     std::vector<unsigned long long> key;
@@ -1404,7 +1407,7 @@ static int batch_plan(hipStream_t stream, const std::vector<BatchEvalRef> &evals
         key.push_back(reinterpret_cast<unsigned long long>(e.partial));
         key.push_back(((unsigned long long)(unsigned)e.ngrid << 32) | (unsigned)e.nset);
     }
-    std::lock_guard<std::mutex> lock(g_plan_mutex);
+    lock = std::unique_lock<std::mutex>(g_plan_mutex);
     for (size_t i = 0; i < g_plans.size(); ++i) {
         if (g_plans[i].device == device && g_plans[i].key == key) {
             std::rotate(g_plans.begin() + (long)i, g_plans.begin() + (long)i + 1, g_plans.end());
@@ -1474,9 +1477,13 @@ static int batch_plan(hipStream_t stream, const std::vector<BatchEvalRef> &evals
         sb[i + 1] = (int)pos;
     }
     if (g_plans.size() == kPlanCache) {  // the least recently used plan goes
-        (void)hipStreamSynchronize(stream);  // no launch of this stream still reads it
-        (void)hipDeviceSynchronize();        // nor of another
+        // no launch on its device still reads it (every launch that read it was enqueued under the
+        // cache lock, before this point)
+        const int pdev = g_plans.front().device;
+        if (pdev != device) (void)hipSetDevice(pdev);
+        (void)hipDeviceSynchronize();
         (void)hipFree(g_plans.front().dev);
+        if (pdev != device) (void)hipSetDevice(device);
         g_plans.erase(g_plans.begin());
     }
     BatchPlan pl;
@@ -1560,7 +1567,8 @@ int resident_eval_batched(hipStream_t stream, const ResidentEvalReq *reqs, size_
                 refs[j] = BatchEvalRef{reqs[group[j]].h, d[j].partial, d[j].ngrid, d[j].nset};
             const unsigned char *plan = nullptr;
             int nslots = 0;
-            st = batch_plan(stream, refs, &plan, &nslots);
+            std::unique_lock<std::mutex> plan_lock;  // held until the launch that reads the plan is enqueued
+            st = batch_plan(stream, refs, &plan, &nslots, plan_lock);
             if (st != CFGMM_OK) return st;
             const size_t head = div_up(sizeof(int) * (size_t)(nslots + 1), 256) * 256;
             const void *recs = plan + head;
@@ -1585,6 +1593,7 @@ int resident_eval_batched(hipStream_t stream, const ResidentEvalReq *reqs, size_
 #undef CFGMM_RESBP_NT
 #undef CFGMM_RESBP
             trace_stop(stream, "rho_resident_kernel", t0);
+            plan_lock.unlock();
             st = hip_status(hipGetLastError(), "rho_resident_persistent_kernel (batch) launch");
             if (st != CFGMM_OK) return st;
             hipLaunchKernelGGL(resident_batched_reduce, dim3((unsigned)div_up(max_elems, 256), (unsigned)d.size()),

@@ -315,6 +315,48 @@ def test_resident_eval_batched_persistent_plans():
         assert rel_fro(got[:ns, :ng] - k, ref[:ns, :ng]) <= 1e-10
 
 
+def test_resident_eval_batched_plans_two_threads():
+    """Two host threads, each on its own stream, run batched evaluations over more distinct batches
+    than plans are kept (plans built, kept and evicted while the other thread launches): every
+    result bitwise equal to the separate evaluation."""
+    import threading
+    probs = [W.rho_problem(seed=70 + i, nshell=80 + 30 * i, nao=240 + 90 * i, ngrid=64 * (6 + 5 * i), nset=1 + i % 2,
+                           density=0.3) for i in range(4)]
+    rs = [Resident(p, ns=5, same=(i % 2 == 1)) for i, p in enumerate(probs)]
+    refs = [r.eval() for r in rs]
+    sels = [[0, 1], [2, 3], [1, 2], [0, 3], [3], [0, 1, 2, 3], [2], [1, 3, 0]]
+    errors = []
+
+    def worker(order):
+        s = cfgmm.Stream()
+        try:
+            for _ in range(3):
+                for sel in order:
+                    bufs = [cfgmm.DeviceBuffer(8 * rs[k].p.nset * rs[k].p.ngrid) for k in sel]
+                    for b in bufs:
+                        b.zero()
+                    cfgmm.rho_resident_eval_batched(s, [(rs[k].h, b, rs[k].p.ngrid, rs[k].p.ngrid, rs[k].p.nset)
+                                                        for k, b in zip(sel, bufs)])
+                    s.synchronize()
+                    for k, b in zip(sel, bufs):
+                        if not np.array_equal(b.to_array(np.float64, refs[k].shape), refs[k]):
+                            errors.append((sel, k))
+                        b.free()
+        except Exception as e:  # reported below, on the main thread
+            errors.append(repr(e))
+        finally:
+            s.destroy()
+
+    threads = [threading.Thread(target=worker, args=(order,)) for order in (sels, sels[::-1])]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    for r in rs:
+        r.close()
+    assert not errors, errors
+
+
 def test_resident_symmetric_form():
     """Symmetric D with lhs and rhs one buffer: the upper-triangle tiles only (about half the bytes),
     the oracle to 1e-10 over a partial last block, a leading range, both prefetch depths (bitwise
