@@ -148,7 +148,7 @@ def bench_rho(p: W.RhoProblem, steps: int, warmup: int, precision=cfgmm.PREC_FP6
                 kernel_avg_ms=kern["rho_chunk_kernel"])
 
 
-def bench_resident(p: W.RhoProblem, steps: int, nslices: int = RESIDENT_SLICES):
+def bench_resident(p: W.RhoProblem, steps: int, warmup: int, nslices: int = RESIDENT_SLICES):
     s = cfgmm.Stream()
     d_r = cfgmm.DeviceBuffer(8 * p.nset * p.ngrid)
     d_d = cfgmm.DeviceBuffer.from_array(p.dms)
@@ -164,7 +164,7 @@ def bench_resident(p: W.RhoProblem, steps: int, nslices: int = RESIDENT_SLICES):
     h.eval(s, d_r, p.ngrid)
     s.synchronize()
     rho = d_r.to_array(np.float64, (p.nset, p.ngrid))
-    for _ in range(5):
+    for _ in range(warmup):  # as many as the FP64 leg: the clocks ramp up over the first milliseconds
         h.eval(s, d_r, p.ngrid)
     e0, e1 = cfgmm.Event(), cfgmm.Event()
     e0.record(s)
@@ -1020,15 +1020,17 @@ def main():
     r = bench_rho(p, args.steps, args.warmup)
     evals_per_s = args.steps / (r["gpu_ms"] * 1e-3)
     ms_per_step = r["gpu_ms"] / args.steps
-    # one buffer on both sides and bitwise-symmetric D: the library takes the upper-triangle form,
+    # one buffer on both sides and D symmetric (entries equal bitwise or within 2^-50 of the larger
+    # magnitude, the library's test): the library takes the upper-triangle form,
     # whose algorithmic flops are the ones the roofline counts
-    symmetric = all(np.array_equal(d, d.T) for d in p.dms)
+    symmetric = all(np.all((d == d.T) | (np.abs(d - d.T) <= 2.0 ** -50 * np.maximum(np.abs(d), np.abs(d.T))))
+                    for d in p.dms)
     flops = p.flops_symmetric if symmetric else p.flops
     kern_tflops = flops / (r["kernel_avg_ms"] * 1e-3) / 1e12
     log("[bench] rho in the general form (CFGMM_TUNE_RHO_SYMMETRIC = 1), for reference")
     cfgmm.tune_set(cfgmm.TUNE_RHO_SYMMETRIC, 1)
     try:
-        rg = bench_rho(p, max(20, args.steps // 4), 5)
+        rg = bench_rho(p, max(20, args.steps // 4), args.warmup)
     finally:
         cfgmm.tune_set(cfgmm.TUNE_RHO_SYMMETRIC, 0)
     general = {"ms_per_step": round(rg["gpu_ms"] / max(20, args.steps // 4), 5),
@@ -1083,7 +1085,7 @@ def main():
     # split precision over resident operands (RESIDENT_SLICES int8 slices): repeated evaluations of the same
     # (D, phi, mask), as in serving; reported beside the FP64 line, never as `value`
     log(f"[bench] split precision, resident operands ({RESIDENT_SLICES} slices)")
-    rs = bench_resident(p, steps=max(20, args.steps // 2))
+    rs = bench_resident(p, steps=max(20, args.steps // 2), warmup=args.warmup)
     out_split = {"slices": RESIDENT_SLICES, "ms_per_eval": round(rs["ms"], 4), "evals_per_s": round(1e3 / rs["ms"], 2),
                  "kernel_avg_ms": round(rs["kernel_ms"], 4), "create_ms": round(rs["create_ms"], 2),
                  "resident_GiB": round(rs["bytes"] / 2 ** 30, 3), "speedup_vs_fp64": round(ms_per_step / rs["ms"], 2),
