@@ -50,21 +50,32 @@ __global__ __launch_bounds__(256) void build_ao_lists(const uint8_t *__restrict_
     if (b >= nblk) return;
     __shared__ int wave_cnt[256 / 64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // the column's bytes of kQ chunks of 256 rows loaded together (clamped addresses), then compacted
+    // chunk by chunk: one load latency per kQ chunks instead of one per chunk
+    constexpr int kQ = 16;
     int base = 0;
-    for (int x0 = 0; x0 < nao; x0 += 256) {
-        const int x = x0 + threadIdx.x;
-        const bool on = x < nao && tab[(long)x * ldtab + b] != 0;
-        const unsigned long long bal = __ballot(on);
-        const int prefix = __popcll(bal & ((1ull << lane) - 1ull));
-        if (lane == 0) wave_cnt[wave] = __popcll(bal);
-        __syncthreads();
-        int off = base;
-        for (int w = 0; w < wave; ++w) off += wave_cnt[w];
-        if (on) lists[(long)b * nao + off + prefix] = x;
-        int tot = 0;
-        for (int w = 0; w < 256 / 64; ++w) tot += wave_cnt[w];
-        base += tot;
-        __syncthreads();
+    for (int x00 = 0; x00 < nao; x00 += 256 * kQ) {
+        uint8_t v[kQ];
+#pragma unroll
+        for (int q = 0; q < kQ; ++q) v[q] = tab[(long)min(x00 + q * 256 + (int)threadIdx.x, nao - 1) * ldtab + b];
+#pragma unroll
+        for (int q = 0; q < kQ; ++q) {
+            const int x0 = x00 + q * 256;
+            if (x0 >= nao) break;  // uniform
+            const int x = x0 + threadIdx.x;
+            const bool on = x < nao && v[q] != 0;
+            const unsigned long long bal = __ballot(on);
+            const int prefix = __popcll(bal & ((1ull << lane) - 1ull));
+            if (lane == 0) wave_cnt[wave] = __popcll(bal);
+            __syncthreads();
+            int off = base;
+            for (int w = 0; w < wave; ++w) off += wave_cnt[w];
+            if (on) lists[(long)b * nao + off + prefix] = x;
+            int tot = 0;
+            for (int w = 0; w < 256 / 64; ++w) tot += wave_cnt[w];
+            base += tot;
+            __syncthreads();
+        }
     }
     if (threadIdx.x == 0) counts[b] = base;
     if (keys) {
