@@ -74,7 +74,8 @@ int device_cu_count();  // compute units of the current device (cached per devic
 __host__ __device__ inline size_t div_up(size_t a, size_t b) { return (a + b - 1) / b; }
 
 // ---- kernel launchers (one per .hip file) -----------------------------------
-// *flag = 0 unless every D_s (s < nset) equals its transpose bitwise on [0, nao)^2; the caller
+// *flag = 0 unless every D_s (s < nset) equals its transpose on [0, nao)^2 (each pair of entries
+// bitwise or within 2^-50 of the larger magnitude: rho::tile_pair_differs); the caller
 // sets *flag beforehand (nothing is written when all are symmetric)
 int launch_dm_symmetry(hipStream_t stream, const double *dms, size_t stride_ao, size_t nao, size_t nset, int *flag);
 // Compacts, for each column block b < nblk, the rows x < nrows with

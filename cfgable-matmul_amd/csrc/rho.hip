@@ -28,9 +28,9 @@
 //      increasing chunk order -- deterministic, no float atomics.
 //
 // When lhs and rhs are one array (the density of one set of orbitals), the other workgroups of
-// the plan launch test every D_s for D_s == D_s^T, bitwise, beside the plan; if all are, each
-// chunk contracts only the list positions from its own rows on, those past its rows twice: half
-// the gathers and MFMAs.  The flag stays on the device (no host synchronisation); the chunk
+// the plan launch test every D_s for D_s == D_s^T (entries equal bitwise or within 2^-50 of the
+// larger magnitude) beside the plan; if all are, each chunk contracts only the list positions
+// from its own rows on, those past its rows twice: half the gathers and MFMAs.  The flag stays on the device (no host synchronisation); the chunk
 // kernel reads it.
 #include "internal.hpp"
 #include "rho_body.hpp"
@@ -246,7 +246,7 @@ __global__ __launch_bounds__(kPlanThreads) void rho_plan_kernel(
     }
 }
 
-// *sym = 0 unless every D_s (s < nset) equals its transpose bitwise on [0, nao)^2.  Workgroup
+// *sym = 0 unless every D_s (s < nset) equals its transpose on [0, nao)^2 (tile_pair_differs's rule).  Workgroup
 // (tj, ti, s) with ti <= tj compares the 64x64 tile (ti, tj) with the transpose of (tj, ti)
 // through LDS; the others exit.
 __global__ __launch_bounds__(256) void dm_symmetry_kernel(const double *__restrict__ dms, long stride_ao, int nao,

@@ -34,7 +34,7 @@
 // digit weights and the power-of-two scales, multiplies by lhs and reduces the 64 rows; per-tile
 // partial rows are summed in a fixed order by launch_rho_reduce.
 //
-// Symmetric form (every D_s == D_s^T bitwise, lhs and rhs one array, kpad <= 8192): only the
+// Symmetric form (every D_s == D_s^T, bitwise or to 2^-50; lhs and rhs one array; kpad <= 8192): only the
 // tiles (row tile T of 64 rows, k step ks >= T) are stored, a plane being [T][4 row groups]
 // [ks - T] 1 KiB tiles; a workgroup on row tile T streams its steps past the diagonal, doubles
 // its int32 sums (exact: 2 * 6 * 8192 * 127^2 < 2^31), then adds the diagonal step -- the pairs

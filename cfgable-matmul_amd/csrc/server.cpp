@@ -17,6 +17,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -274,17 +275,22 @@ int server_add_molecule(Server *sv, int id, const double *dms, const double *phi
     m.blksize = blksize;
     m.nblk = div_up(ngrid, blksize);
     if (sv->cfg.rho_precision == CFGMM_PREC_OZAKI_INT8) {
-        // the tiles' size: the upper-triangle form when every D_s is symmetric, bitwise (phi is on
-        // both sides), as resident_create will find on the device
+        // the tiles' size: the upper-triangle form when every D_s is symmetric (phi is on both
+        // sides), by the rule resident_create applies on the device: each pair of entries equal
+        // bitwise or within 2^-50 of the larger magnitude
         bool sym = true;
         for (size_t s = 0; s < nset && sym; ++s) {
             const double *d = dms + s * nao * nao;
             for (size_t i = 0; i < nao && sym; ++i)
-                for (size_t j = i + 1; j < nao; ++j)
-                    if (std::memcmp(d + i * nao + j, d + j * nao + i, sizeof(double)) != 0) {
+                for (size_t j = i + 1; j < nao; ++j) {
+                    const double x = d[i * nao + j], y = d[j * nao + i];
+                    const bool same = std::memcmp(&x, &y, sizeof(double)) == 0 ||
+                                      std::fabs(x - y) <= 0x1p-50 * std::fmax(std::fabs(x), std::fabs(y));
+                    if (!same) {
                         sym = false;
                         break;
                     }
+                }
         }
         const size_t cap = sv->cfg.split_tile_cap_bytes ? sv->cfg.split_tile_cap_bytes : sv->cfg.hbm_budget_bytes / 4;
         m.split = resident_bytes_bound(tab, m.nblk, nao, m.nblk, nset, blksize, sv->cfg.rho_nslices, sym) <= cap;
