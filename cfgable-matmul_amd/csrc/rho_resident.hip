@@ -133,58 +133,96 @@ __device__ __attribute__((always_inline)) inline void copy16_to_lds(const void *
 #endif
 }
 
-// One wave per compacted row i of block b and set s: ed and the S slice planes of the row.
+// One workgroup per (16-row group g of the padded list, grid block b, density matrix s): the
+// exponents ed of its 16 rows (each row's largest magnitude over all its columns), then every
+// 1 KiB tile of the group (the k steps from the diagonal's on, in the symmetric form), each
+// plane's tile written as 256 consecutive 4-byte words, one per thread (whole cache lines: with
+// a row per wave the tiles' 16-byte row pieces were written by 16 different waves).  The loads of
+// a batch (list entries, then the D values they index) are issued together; rows and columns past
+// |L_b| are zero, their loads clamped.
 template <int S>
 __global__ __launch_bounds__(kThreads) void prep_dq_kernel(unsigned char *__restrict__ base,
                                                            const BlockDesc *__restrict__ blocks,
                                                            const double *__restrict__ dms, long stride_ao, int nao,
                                                            const int *__restrict__ lists) {
-    const int b = blockIdx.y, s = blockIdx.z;
+    const int g = blockIdx.x, b = blockIdx.y, s = blockIdx.z;
     const BlockDesc bd = blocks[b];
-    const int lane = threadIdx.x & 63;
-    const int i = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
-    if (i >= bd.kpad) return;
-    const long plane = dq_plane_bytes(bd.kpad, bd.sym);
-    const int nks = bd.kpad / kK;
-    // byte of (i, j) inside a plane: 1 KiB tile (i/16, j/64), chunk (j%64)/16 * 16 + i%16, byte j%16;
-    // the symmetric form keeps the tiles j/64 >= T = i/64 of row group i/16, [T][(i/16)%4][j/64 - T]
-    const int T = i >> 6, j0 = bd.sym ? T * kK : 0;
-    const long row_tile = bd.sym ? sym_tiles_before(T, nks) + (long)((i >> 4) & 3) * (nks - T) : (long)(i >> 4) * nks;
-    unsigned char *__restrict__ out = base + bd.d_off + (long)s * S * plane + row_tile * 1024 + (i & 15) * 16;
-    auto at = [&](int j) { return (long)((j - j0) >> 6) * 1024 + ((j & 63) >> 4) * 256 + (j & 15); };
-    int *__restrict__ ed = reinterpret_cast<int *>(base + bd.e_off) + (long)s * bd.kpad;
+    const int n = bd.n, kpad = bd.kpad;
+    if (g * 16 >= kpad || n == 0) return;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int nks = kpad / kK;
+    const long plane = dq_plane_bytes(kpad, bd.sym);
+    const int T = g >> 2, ks0 = bd.sym ? T : 0;  // the group's row tile, its first stored k step
+    const long row_tile = bd.sym ? sym_tiles_before(T, nks) + (long)(g & 3) * (nks - T) : (long)g * nks;
+    unsigned char *__restrict__ out = base + bd.d_off + (long)s * S * plane + row_tile * 1024 + 4 * tid;
+    int *__restrict__ ed = reinterpret_cast<int *>(base + bd.e_off) + (long)s * kpad;
     const int *__restrict__ list = lists + (long)b * nao;
-    if (i >= bd.n) {
-        for (int j = j0 + 4 * lane; j < bd.kpad; j += 256)
+    const double *__restrict__ dset = dms + (long)s * stride_ao * stride_ao;
+    __shared__ int e_s[16];
+
+    // exponents: wave w takes rows 4 w .. 4 w + 3 of the group, sixteen lanes per row
+    {
+        constexpr int kU = 8;
+        const int r = 4 * wave + (lane >> 4), i = g * 16 + r, l16 = lane & 15;
+        const double *__restrict__ drow = dset + (long)list[min(i, n - 1)] * stride_ao;
+        u64 m = 0;
+        for (int jb = 0; jb < n; jb += 16 * kU) {
+            int idx[kU];
 #pragma unroll
-            for (int p = 0; p < S; ++p) *reinterpret_cast<unsigned *>(out + p * plane + at(j)) = 0u;
-        if (lane == 0) ed[i] = 0;
-        return;
-    }
-    const double *__restrict__ row = dms + (long)s * stride_ao * stride_ao + (long)list[i] * stride_ao;
-    u64 m = 0;
-    for (int j = lane; j < bd.n; j += 64) {
-        const u64 bits = (u64)__double_as_longlong(fabs(row[list[j]]));
-        m = bits > m ? bits : m;
-    }
-    for (int off = 32; off > 0; off >>= 1) {
-        const u64 o = __shfl_xor(m, off);
-        m = o > m ? o : m;
-    }
-    const int e = exponent_of(m);
-    if (lane == 0) ed[i] = e;
-    for (int j = j0 + 4 * lane; j < bd.kpad; j += 256) {
-        double x[4];
-        int ee[4];
+            for (int u = 0; u < kU; ++u) idx[u] = list[min(jb + 16 * u + l16, n - 1)];
+            double v[kU];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            x[q] = j + q < bd.n ? row[list[j + q]] : 0.0;
-            ee[q] = e;
+            for (int u = 0; u < kU; ++u) v[u] = drow[idx[u]];
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                const u64 bits = (u64)__double_as_longlong(fabs(v[u]));
+                m = jb + 16 * u + l16 < n && bits > m ? bits : m;
+            }
         }
-        unsigned w[S];
-        slice4<S>(x, ee, w);
 #pragma unroll
-        for (int p = 0; p < S; ++p) *reinterpret_cast<unsigned *>(out + p * plane + at(j)) = w[p];
+        for (int off = 8; off > 0; off >>= 1) {
+            const u64 o = __shfl_xor(m, off);
+            m = o > m ? o : m;
+        }
+        if (l16 == 0) {
+            const int e = i < n ? exponent_of(m) : 0;
+            e_s[r] = e;
+            ed[i] = e;
+        }
+    }
+    __syncthreads();
+
+    // tiles: thread t holds row r = (t % 64) / 4 of the group and columns 16 (t / 64) + 4 (t % 4)
+    // .. + 3 of each k step: bytes 4 t .. 4 t + 3 of the tile
+    constexpr int kB = 4;  // tiles per batch
+    const int r = lane >> 2, i = g * 16 + r, jc = 16 * wave + 4 * (lane & 3);
+    const int e = e_s[r];
+    const int ee[4] = {e, e, e, e};
+    const double *__restrict__ drow = dset + (long)list[min(i, n - 1)] * stride_ao;
+    for (int ks = ks0; ks < nks; ks += kB) {
+        int idx[kB][4];
+#pragma unroll
+        for (int u = 0; u < kB; ++u)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) idx[u][q] = list[min((ks + u) * kK + jc + q, n - 1)];
+        double x[kB][4];
+#pragma unroll
+        for (int u = 0; u < kB; ++u)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const double v = drow[idx[u][q]];
+                x[u][q] = i < n && (ks + u) * kK + jc + q < n ? v : 0.0;
+            }
+#pragma unroll
+        for (int u = 0; u < kB; ++u) {
+            if (ks + u < nks) {
+                unsigned w[S];
+                slice4<S>(x[u], ee, w);
+#pragma unroll
+                for (int p = 0; p < S; ++p)
+                    *reinterpret_cast<unsigned *>(out + p * plane + (long)(ks + u - ks0) * 1024) = w[p];
+            }
+        }
     }
 }
 
@@ -204,10 +242,24 @@ __global__ __launch_bounds__(kThreads) void prep_phi_kernel(unsigned char *__res
     const int *__restrict__ list = lists + (long)b * nao;
     for (int c = threadIdx.x; c < blksize; c += kThreads) colmax[c] = 0;
     __syncthreads();
-    for (int idx = threadIdx.x; idx < bd.n * ncol; idx += kThreads) {
-        const int i = idx / ncol, c = idx % ncol;
-        const u64 bits = (u64)__double_as_longlong(fabs(ao_rhs[(long)list[i] * stride_grid + g0 + c]));
-        atomicMax(&colmax[c], bits);
+    // every pass below loads a batch of kU elements per thread together (list entries, then values;
+    // clamped addresses, out-of-range elements masked): two load latencies per batch
+    constexpr int kU = 8;
+    const int n = bd.n;
+    const int nel = n * ncol;
+    for (int i0 = threadIdx.x; i0 < nel; i0 += kThreads * kU) {
+        int li[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) li[u] = list[min(i0 + u * kThreads, nel - 1) / ncol];
+        double v[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u)
+            v[u] = ao_rhs[(long)li[u] * stride_grid + g0 + min(i0 + u * kThreads, nel - 1) % ncol];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int idx = i0 + u * kThreads;
+            if (idx < nel) atomicMax(&colmax[idx % ncol], (u64)__double_as_longlong(fabs(v[u])));
+        }
     }
     __syncthreads();
     for (int c = threadIdx.x; c < ncol; c += kThreads) fg[g0 + c] = exponent_of(colmax[c]);
@@ -215,28 +267,62 @@ __global__ __launch_bounds__(kThreads) void prep_phi_kernel(unsigned char *__res
     const int quads = bd.kpad / 4;
     const long plane = (long)blksize * bd.kpad;
     unsigned char *__restrict__ rq = base + bd.r_off;
-    for (int idx = threadIdx.x; idx < blksize * quads; idx += kThreads) {
-        const int c = idx / quads, i = 4 * (idx % quads);
-        double x[4];
-        int ee[4];
-        const int e = c < ncol ? exponent_of(colmax[c]) : 0;
+    constexpr int kR = 2;  // iterations of four values each per batch
+    const int nrq = blksize * quads;
+    for (int i0 = threadIdx.x; i0 < nrq; i0 += kThreads * kR) {
+        int li[kR][4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            x[q] = (i + q < bd.n && c < ncol) ? ao_rhs[(long)list[i + q] * stride_grid + g0 + c] : 0.0;
-            ee[q] = e;
+        for (int u = 0; u < kR; ++u) {
+            const int idx = min(i0 + u * kThreads, nrq - 1), i = 4 * (idx % quads);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) li[u][q] = n > 0 ? list[min(i + q, n - 1)] : 0;  // row 0: masked
         }
-        unsigned w[S];
-        slice4<S>(x, ee, w);
+        double x[kR][4];
 #pragma unroll
-        for (int p = 0; p < S; ++p)
-            *reinterpret_cast<unsigned *>(rq + p * plane + ((long)(i >> 6) * blksize + c) * kK + swz_in_row(c, i & 63)) =
-                w[p];
+        for (int u = 0; u < kR; ++u) {
+            const int idx = min(i0 + u * kThreads, nrq - 1), c = idx / quads, i = 4 * (idx % quads);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const double v = ao_rhs[(long)li[u][q] * stride_grid + g0 + min(c, ncol - 1)];
+                x[u][q] = (i + q < n && c < ncol) ? v : 0.0;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kR; ++u) {
+            const int idx = i0 + u * kThreads;
+            if (idx < nrq) {
+                const int c = idx / quads, i = 4 * (idx % quads);
+                const int e = c < ncol ? exponent_of(colmax[c]) : 0;
+                const int ee[4] = {e, e, e, e};
+                unsigned w[S];
+                slice4<S>(x[u], ee, w);
+#pragma unroll
+                for (int p = 0; p < S; ++p)
+                    *reinterpret_cast<unsigned *>(rq + p * plane + ((long)(i >> 6) * blksize + c) * kK +
+                                                  swz_in_row(c, i & 63)) = w[p];
+            }
+        }
     }
     // Lc[i][c]
     double *__restrict__ lc = reinterpret_cast<double *>(base + bd.l_off);
-    for (int idx = threadIdx.x; idx < bd.kpad * blksize; idx += kThreads) {
-        const int i = idx / blksize, c = idx % blksize;
-        lc[idx] = (i < bd.n && c < ncol) ? ao_lhs[(long)list[i] * stride_grid + g0 + c] : 0.0;
+    const int nlc = bd.kpad * blksize;
+    for (int i0 = threadIdx.x; i0 < nlc; i0 += kThreads * kU) {
+        int li[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u)
+            li[u] = n > 0 ? list[min(min(i0 + u * kThreads, nlc - 1) / blksize, n - 1)] : 0;  // row 0: masked
+        double v[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u)
+            v[u] = ao_lhs[(long)li[u] * stride_grid + g0 + min(min(i0 + u * kThreads, nlc - 1) % blksize, ncol - 1)];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int idx = i0 + u * kThreads;
+            if (idx < nlc) {
+                const int i = idx / blksize, c = idx % blksize;
+                lc[idx] = (i < n && c < ncol) ? v[u] : 0.0;
+            }
+        }
     }
 }
 
@@ -966,7 +1052,7 @@ template <int S>
 static int prep_launch(hipStream_t stream, ResidentRho *h, const double *dms, const double *ao_lhs,
                        const double *ao_rhs, size_t stride_ao, size_t stride_grid, const int *lists, int max_kpad) {
     hipEvent_t t0 = trace_start(stream);
-    hipLaunchKernelGGL((prep_dq_kernel<S>), dim3((unsigned)div_up(max_kpad, kThreads / 64), (unsigned)h->nblk,
+    hipLaunchKernelGGL((prep_dq_kernel<S>), dim3((unsigned)div_up(max_kpad, 16), (unsigned)h->nblk,
                                                  (unsigned)h->nset),
                        dim3(kThreads), 0, stream, h->base, h->d_blocks, dms, (long)stride_ao, (int)h->nao, lists);
     hipLaunchKernelGGL((prep_phi_kernel<S>), dim3((unsigned)h->nblk), dim3(kThreads), sizeof(u64) * h->blksize, stream,
