@@ -185,6 +185,75 @@ def bench_resident(p: W.RhoProblem, steps: int, warmup: int, nslices: int = RESI
     return dict(ms=ms, kernel_ms=km, create_ms=create_ms, bytes=nbytes, rho=rho)
 
 
+def bench_served_compact(p: W.RhoProblem, n_req: int = 600, seed: int = 2024):
+    """The serving loop on the config-2 molecule, saturated: n_req rho requests, all arriving at
+    once, each over a leading range of the molecule (64 U{64..nblk} points, U{1..nset} density
+    matrices; default_rng(seed)).  Served in split precision (resident tiles) and in FP64 (the
+    better of batched and one-launch-per-request dispatch), and as plain FP64 per-call launches on
+    one stream; a sample of the served split results against the FP64 per-call ones."""
+    rng = np.random.default_rng(seed)
+    reqs = [(64 * int(rng.integers(64, p.nblk + 1)), int(rng.integers(1, p.nset + 1))) for _ in range(n_req)]
+    sample = set(range(0, n_req, max(1, n_req // 8)))
+    free, _ = cfgmm.mem_get_info()
+    budget = int(min(free * 0.85, 220 * 2 ** 30))
+
+    def served(prec, max_batch):
+        sv = cfgmm.Server(budget, slots=4, rho_precision=prec, rho_nslices=RESIDENT_SLICES, max_batch=max_batch)
+        sv.add_molecule(0, p.dms, p.phi, np.ascontiguousarray(p.tab), p.blksize)
+        sv.warm([0])
+        outs = [np.zeros(ns * ng) if i in sample else None for i, (ng, ns) in enumerate(reqs)]
+        creq = [cfgmm.Request(cfgmm.REQ_RHO, 0, ng, ns, 0, 0, 0, 0, 0.0,
+                              None if outs[i] is None else outs[i].ctypes.data) for i, (ng, ns) in enumerate(reqs)]
+        sv.run(creq)  # untimed pass: the clocks ramp up, the server's buffers and plans are made
+        res = sv.run(creq)
+        sv.destroy()
+        done = np.array([r.done_s for r in res]) * 1e3
+        return {"rho_evals_per_s": round(n_req / (done.max() * 1e-3), 1),
+                "p50_ms": round(float(np.percentile(done, 50)), 3), "p99_ms": round(float(np.percentile(done, 99)), 3),
+                "mean_batch": round(float(np.mean([r.batch_size for r in res])), 2)}, outs
+
+    split, outs = served(cfgmm.PREC_OZAKI_INT8, 16)
+    fp64_b, _ = served(cfgmm.PREC_FP64, 16)
+    fp64_1, _ = served(cfgmm.PREC_FP64, 1)
+    fp64_b["dispatch"], fp64_1["dispatch"] = "batched, max_batch 16", "one launch per request, max_batch 1"
+    fp64 = fp64_b if fp64_b["rho_evals_per_s"] >= fp64_1["rho_evals_per_s"] else fp64_1
+    # plain FP64 per-call launches of the same requests on one stream, and the sampled references
+    s = cfgmm.Stream()
+    d_d, d_phi = cfgmm.DeviceBuffer.from_array(p.dms), cfgmm.DeviceBuffer.from_array(p.phi)
+    d_tab = cfgmm.DeviceBuffer.from_array(np.ascontiguousarray(p.tab))
+    d_r = cfgmm.DeviceBuffer(8 * p.nset * p.ngrid)
+    call = lambda ng, ns: cfgmm.aodmao2rho_dev(s, d_r, d_d, d_phi, d_phi, p.nao, p.ngrid, p.nao, ng, ns, d_tab,
+                                               p.nblk, p.blksize)
+    for ng, ns in reqs[:20]:
+        call(ng, ns)
+    e0, e1 = cfgmm.Event(), cfgmm.Event()
+    e0.record(s)
+    for ng, ns in reqs:
+        call(ng, ns)
+    e1.record(s)
+    s.synchronize()
+    plain = n_req / (e0.elapsed_ms(e1) * 1e-3)
+    errs = []
+    for i in sorted(sample):
+        ng, ns = reqs[i]
+        d_r.zero()
+        call(ng, ns)
+        s.synchronize()
+        ref = d_r.to_array(np.float64, (p.nset, p.ngrid))[:ns, :ng].ravel()
+        errs.append(rel_fro(outs[i], ref))
+    for b in (d_d, d_phi, d_tab, d_r):
+        b.free()
+    e0.destroy()
+    e1.destroy()
+    s.destroy()
+    return {"requests": n_req, "generator": f"config-2 molecule; leading ranges 64 U{{64..{p.nblk}}} points, "
+                                            f"U{{1..{p.nset}}} DMs, default_rng({seed}); all arriving at t = 0",
+            "served_split": split, "served_fp64": fp64, "plain_fp64_per_call_rho_evals_per_s": round(plain, 1),
+            "speedup_served_split_vs_plain_fp64": round(split["rho_evals_per_s"] / plain, 2),
+            "speedup_served_split_vs_served_fp64": round(split["rho_evals_per_s"] / fp64["rho_evals_per_s"], 2),
+            "split_rel_err_vs_fp64_max": max(errs)}
+
+
 def bench_multi(p: W.RhoProblem, ncomp: int = 4, reps: int = 20):
     """ncomp left-hand components over one D.rhs (cfgmm_aodmao2rho_multi_dev) vs ncomp separate evaluations."""
     rng = np.random.default_rng(5)
@@ -1130,6 +1199,10 @@ def main():
         out["rel_err"] = None
 
     out["split_resident"] = out_split
+
+    # the same molecule served: continuous batching over resident tiles vs FP64, saturated
+    log("[bench] serving loop on this molecule (saturated rho requests over leading ranges)")
+    out["served_rho"] = bench_served_compact(p)
     out["rho_4_components"] = out_gga
     out["aogridao2mat"] = out_vxc
     out["gto_tabulation"] = out_gto
