@@ -114,6 +114,40 @@ def test_batched_split_precision_equals_single(batch):
         assert np.array_equal(a, b)
 
 
+def test_batched_form_choice_matches_single_calls():
+    """D symmetric bitwise, symmetric only to rounding (two units in the last place on half the
+    upper triangle), and asymmetric by 1e-13 relative in one entry, in one FP64 batch: each result
+    bitwise that of its separate call, so the batch chose each problem's form as the per-call path
+    does; the rounding-symmetric problem took the symmetric form (it differs from the general form)."""
+    import dataclasses
+    # lists longer than one 64-row chunk (with one chunk per block both forms compute the same)
+    base = [W.rho_problem(seed=31 + i, nshell=150 + 20 * i, nao=460 + 60 * i, ngrid=64 * (5 + i) + 7 * i,
+                          nset=2, blksize=64, density=0.4) for i in range(3)]
+    rng = np.random.default_rng(5)
+    near = base[1].dms.copy()
+    iu = np.triu_indices(base[1].nao, 1)
+    for s in range(base[1].nset):
+        m = rng.random(len(iu[0])) < 0.5
+        x = near[s][iu]
+        near[s][iu] = np.where(m, np.nextafter(np.nextafter(x, np.inf), np.inf), x)
+    far = base[2].dms.copy()
+    far[0, 0, 1] *= 1.0 + 1e-13
+    probs = [base[0], dataclasses.replace(base[1], dms=near), dataclasses.replace(base[2], dms=far)]
+    rho0s = [np.zeros((p.nset, p.ngrid)) for p in probs]
+    single = _run_single(probs, rho0s)
+    bat = _run_batched(probs, rho0s)
+    for a, b in zip(single, bat):
+        assert np.array_equal(a, b)
+    old = cfgmm.tune_get(cfgmm.TUNE_RHO_SYMMETRIC)
+    try:
+        cfgmm.tune_set(cfgmm.TUNE_RHO_SYMMETRIC, 1)
+        general = _run_single(probs[1:2], rho0s[1:2])[0]
+    finally:
+        cfgmm.tune_set(cfgmm.TUNE_RHO_SYMMETRIC, old)
+    assert not np.array_equal(single[1], general)
+    assert rel_fro(single[1], general) <= 1e-12
+
+
 def test_batched_empty_and_bad_problem():
     s = cfgmm.Stream()
     cfgmm.aodmao2rho_batched_dev(s, [])
