@@ -259,7 +259,8 @@ __global__ __launch_bounds__(256) void dm_symmetry_kernel(const double *__restri
         __hip_atomic_store(sym, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-template <int NS, int NT>
+// OFF32: 32-bit byte offsets into ao_rhs (see rho_chunk_body)
+template <int NS, int NT, bool OFF32>
 __global__ __launch_bounds__(kThreads, 2) void rho_chunk_kernel(
     double *__restrict__ partial, const double *__restrict__ dms, const double *__restrict__ ao_lhs,
     const double *__restrict__ ao_rhs, long stride_ao, long stride_grid, int nao, int ngrid, int nset, int set0,
@@ -292,9 +293,9 @@ __global__ __launch_bounds__(kThreads, 2) void rho_chunk_kernel(
         t.lhs_comp_stride = lhs_comp_stride;
         t.part_comp_stride = part_comp_stride;
         if (symmetric)
-            rho::rho_chunk_body<NS, NT, true>(t, smem);
+            rho::rho_chunk_body<NS, NT, true, OFF32>(t, smem);
         else
-            rho::rho_chunk_body<NS, NT, false>(t, smem);
+            rho::rho_chunk_body<NS, NT, false, OFF32>(t, smem);
     };
     const int x = blockIdx.x % 8, k = blockIdx.x / 8;
     if (perm && symmetric) {
@@ -333,7 +334,7 @@ __global__ __launch_bounds__(kThreads, 2) void rho_chunk_kernel(
     }
 }
 
-template <int NS>
+template <int NS, bool OFF32>
 int launch_ns(int nt, dim3 grid, hipStream_t stream, double *partial, const double *dms, const double *ao_lhs,
               const double *ao_rhs, long stride_ao, long stride_grid, int nao, int ngrid, int nset, int set0,
               const int *lists, const int *counts, int blksize, int nsub, int maxchunks, int ngroups, int ncomp,
@@ -341,7 +342,7 @@ int launch_ns(int nt, dim3 grid, hipStream_t stream, double *partial, const doub
               const int *bounds) {
     hipEvent_t t0 = trace_start(stream);
 #define CFGMM_RHO_LAUNCH(NT_)                                                                                      \
-    hipLaunchKernelGGL((rho_chunk_kernel<NS, NT_>), grid, dim3(kThreads), 0, stream, partial, dms, ao_lhs, ao_rhs, \
+    hipLaunchKernelGGL((rho_chunk_kernel<NS, NT_, OFF32>), grid, dim3(kThreads), 0, stream, partial, dms, ao_lhs, ao_rhs, \
                        stride_ao, stride_grid, nao, ngrid, nset, set0, lists, counts, blksize, nsub, maxchunks, \
                        ngroups, ncomp, lhs_cs, part_cs, perm, ntask, sym, epoch, bounds)
     switch (nt) {
@@ -455,17 +456,23 @@ int launch_rho_multi(hipStream_t stream, double *rhos, size_t rho_cs, const doub
     const int NS = (int)std::min<size_t>((size_t)tune_get(CFGMM_TUNE_RHO_MAX_SETS), nset);
     const size_t full = nset / NS, rem = nset % NS;
     const long sa = (long)stride_ao, sg = (long)stride_grid;
+    // every byte offset into ao_rhs below 2^32: its slice loads take 32-bit offsets
+    const bool off32 = (uint64_t)nao * stride_grid * 8 <= (uint64_t)UINT32_MAX;
     auto launch_groups = [&](int ns, size_t set0, size_t ngroups) -> int {
         if (nblk * nsub * maxchunks * ngroups + 8 > (size_t)INT32_MAX)
             return fail(CFGMM_EINVAL, "rho: too many workgroups for one launch");
         const size_t slots = nblk * nsub * ngroups * maxchunks;
         const dim3 grid((unsigned)(ordered ? div_up(slots, 8) * 8 : slots), 1, 1);
         const int *pp = ordered ? perm : nullptr;
+#define CFGMM_RHO_NS(NS_)                                                                          \
+    (off32 ? launch_ns<NS_, true>(nt, grid, stream, partial, dms, ao_lhs, ao_rhs, sa, sg, (int)nao, (int)ngrid, (int)nset, (int)set0, lists, counts, (int)blksize, nsub, (int)maxchunks, (int)ngroups, (int)ncomp, (long)lhs_cs, (long)part_cs, pp, ntask, symp, epoch, bounds) \
+           : launch_ns<NS_, false>(nt, grid, stream, partial, dms, ao_lhs, ao_rhs, sa, sg, (int)nao, (int)ngrid, (int)nset, (int)set0, lists, counts, (int)blksize, nsub, (int)maxchunks, (int)ngroups, (int)ncomp, (long)lhs_cs, (long)part_cs, pp, ntask, symp, epoch, bounds))
         switch (ns) {
-            case 1: return launch_ns<1>(nt, grid, stream, partial, dms, ao_lhs, ao_rhs, sa, sg, (int)nao, (int)ngrid, (int)nset, (int)set0, lists, counts, (int)blksize, nsub, (int)maxchunks, (int)ngroups, (int)ncomp, (long)lhs_cs, (long)part_cs, pp, ntask, symp, epoch, bounds);
-            case 2: return launch_ns<2>(nt, grid, stream, partial, dms, ao_lhs, ao_rhs, sa, sg, (int)nao, (int)ngrid, (int)nset, (int)set0, lists, counts, (int)blksize, nsub, (int)maxchunks, (int)ngroups, (int)ncomp, (long)lhs_cs, (long)part_cs, pp, ntask, symp, epoch, bounds);
-            case 3: return launch_ns<3>(nt, grid, stream, partial, dms, ao_lhs, ao_rhs, sa, sg, (int)nao, (int)ngrid, (int)nset, (int)set0, lists, counts, (int)blksize, nsub, (int)maxchunks, (int)ngroups, (int)ncomp, (long)lhs_cs, (long)part_cs, pp, ntask, symp, epoch, bounds);
-            default: return launch_ns<4>(nt, grid, stream, partial, dms, ao_lhs, ao_rhs, sa, sg, (int)nao, (int)ngrid, (int)nset, (int)set0, lists, counts, (int)blksize, nsub, (int)maxchunks, (int)ngroups, (int)ncomp, (long)lhs_cs, (long)part_cs, pp, ntask, symp, epoch, bounds);
+            case 1: return CFGMM_RHO_NS(1);
+            case 2: return CFGMM_RHO_NS(2);
+            case 3: return CFGMM_RHO_NS(3);
+            default: return CFGMM_RHO_NS(4);
+#undef CFGMM_RHO_NS
         }
     };
     st = launch_groups(NS, 0, full);

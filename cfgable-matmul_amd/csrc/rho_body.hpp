@@ -37,7 +37,10 @@ constexpr size_t chunk_lds_bytes() {
     return sizeof(double) * (2 * NS * kKV * (kUC + 1) + 2 * kKV * NT * 16) + sizeof(int) * (kSeg + kUC);
 }
 
-template <int NS, int NT, bool SYM = false>
+// OFF32: every byte offset into ao_rhs (row x stride_grid + point, times 8) is below 2^32, so the
+// slice loads of phi_rhs take 32-bit offsets from its base (fewer VALU instructions in the slice
+// loop than 64-bit row addresses); the values loaded are the same.
+template <int NS, int NT, bool SYM = false, bool OFF32 = false>
 __device__ inline void rho_chunk_body(const RhoChunk &task, unsigned char *__restrict__ smem) {
     constexpr int W = NT * 16;                          // grid points per workgroup
     constexpr int PHI_PER_T = kKV * W / kThreads;       // phi_rhs elements per thread per slice
@@ -129,7 +132,12 @@ __device__ inline void rho_chunk_body(const RhoChunk &task, unsigned char *__res
                 for (int e = 0; e < PHI_PER_T; ++e) {
                     const int idx = tid + kThreads * e;
                     const int g = min(g0 + idx % W, ngrid - 1);
-                    rphi[set][e] = ao_rhs[(long)vls[k0 + idx / W] * stride_grid + g];
+                    if constexpr (OFF32)
+                        rphi[set][e] = *reinterpret_cast<const double *>(
+                            reinterpret_cast<const char *>(ao_rhs) +
+                            ((unsigned)vls[k0 + idx / W] * (unsigned)(stride_grid * 8) + (unsigned)g * 8u));
+                    else
+                        rphi[set][e] = ao_rhs[(long)vls[k0 + idx / W] * stride_grid + g];
                 }
             };
             auto load_tail = [&](int k0) {

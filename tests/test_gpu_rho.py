@@ -348,6 +348,35 @@ def _oracle(p, dms, lhs, rhs):
     return ref
 
 
+def test_rho_phi_offsets_beyond_32_bits():
+    """phi rows 2^21 + 9 points apart (byte offsets into phi past 2^32; only the used points of each
+    row uploaded), so the slice loads take 64-bit offsets: equal to the oracle on the compact arrays
+    and bitwise to the compact evaluation, whose offsets fit in 32 bits."""
+    p = W.rho_problem(seed=17, nshell=100, nao=300, ngrid=64 * 9 + 5, nset=2, density=0.4)
+    big = (1 << 21) + 9
+    assert (p.nao - 1) * big * 8 > 2 ** 32
+    lib = cfgmm.lib()
+    s = cfgmm.Stream()
+    d_phi = cfgmm.DeviceBuffer(8 * ((p.nao - 1) * big + p.ngrid))
+    phi = np.ascontiguousarray(p.phi)
+    for x in range(p.nao):
+        cfgmm.check(lib.cfgmm_memcpy_htod(d_phi.ptr + 8 * x * big, phi[x].ctypes.data, 8 * p.ngrid))
+    d_d = cfgmm.DeviceBuffer.from_array(p.dms)
+    d_t = cfgmm.DeviceBuffer.from_array(np.ascontiguousarray(p.tab))
+    d_r = cfgmm.DeviceBuffer(8 * ((p.nset - 1) * big + p.ngrid))
+    d_r.zero()
+    cfgmm.aodmao2rho_dev(s, d_r, d_d, d_phi, d_phi, p.nao, big, p.nao, p.ngrid, p.nset, d_t, p.nblk, p.blksize)
+    s.synchronize()
+    got = np.empty((p.nset, p.ngrid))
+    for q in range(p.nset):
+        cfgmm.check(lib.cfgmm_memcpy_dtoh(got[q].ctypes.data, d_r.ptr + 8 * q * big, 8 * p.ngrid))
+    for b in (d_phi, d_d, d_t, d_r):
+        b.free()
+    s.destroy()
+    assert rel_fro(got, _oracle_rho(p).reshape(p.nset, p.ngrid)) <= TOL
+    assert np.array_equal(got, _rho_dev(p, p.dms, p.phi, p.phi).reshape(p.nset, p.ngrid))
+
+
 def test_rho_symmetric_form_on_fresh_streams():
     """Each call's symmetry flag is a word of its own, zeroed with the workspace, tested against an
     epoch unique in the process: a new stream's workspace (possibly memory an earlier one used)
