@@ -127,6 +127,13 @@ __device__ inline int swz_in_row(int c, int k) { return (((k >> 4) ^ ((c >> 2) &
 // 16 bytes per lane from global memory straight into LDS at dst + 16 * lane (dst uniform across
 // the wave; global_load_lds_dwordx4).  Device-only: a kernel template that names the builtin in
 // its host pass loses its launch stub.
+// a value the same in every lane of the wave, moved to a scalar register (64 bits as two halves)
+__device__ __attribute__((always_inline)) inline long uniform64(long x) {
+    const unsigned lo = __builtin_amdgcn_readfirstlane((int)(unsigned)x);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((int)(x >> 32));
+    return (long)(((unsigned long)hi << 32) | lo);
+}
+
 __device__ __attribute__((always_inline)) inline void copy16_to_lds(const void *src, void *dst) {
 #if defined(__HIP_DEVICE_COMPILE__)
     __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void *)dst, 16, 0, 0);
@@ -637,7 +644,7 @@ __global__ __launch_bounds__(kThreads, 2) void rho_resident_persistent_kernel(
     __shared__ __attribute__((aligned(16))) Rec its[kItemsLds];
     __shared__ int its_cnt;
 
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int lrow = lane & 15, lk = lane >> 4;
 
     i32x4 acc[S][NT];
@@ -650,11 +657,14 @@ __global__ __launch_bounds__(kThreads, 2) void rho_resident_persistent_kernel(
 
     // k step t of item it: (tile, local step) -> A tile row and k step (the symmetric form visits
     // the steps past the diagonal first, the diagonal last)
+    // (the same for every lane: the lane's 16 bytes are added at the load, as a 32-bit offset
+    // beside a scalar base)
     struct StepAddr {
-        const int8_t *a;  // this lane's 16 bytes of the wave's A tile, plane 0
-        const int8_t *b;  // this lane's 16 bytes of the B stage, plane 0, part 0
+        const int8_t *a;  // the wave's A tile, plane 0
+        const int8_t *b;  // the B stage, plane 0, part 0
         int dplane, rplane;
     };
+    const unsigned lane16 = lane * 16u;
     // The producer keeps its item's fields in registers and reads the item records in LDS only
     // when it moves on to the next item: a read of LDS while copies into LDS are in flight may
     // make the compiler wait for all of them.
@@ -664,21 +674,23 @@ __global__ __launch_bounds__(kThreads, 2) void rho_resident_persistent_kernel(
         int aw[2], n[2], T[2], dplane, rplane, sym;
     };
     const long bstep = (long)blksize * kK;  // the same for a single handle
+    // (every field read once per lane from LDS and made scalar: addresses and branches on them
+    // are then worked out by the scalar unit)
     auto prod_item = [&](int it) {
         const Rec &R = its[it];
         ProdItem P;
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
-            P.a[q] = R.a[q];
-            P.aw[q] = R.aw[q];
-            P.n[q] = R.n[q];
-            P.T[q] = R.T[q];
+            P.a[q] = uniform64(R.a[q]);
+            P.aw[q] = __builtin_amdgcn_readfirstlane(R.aw[q]);
+            P.n[q] = __builtin_amdgcn_readfirstlane(R.n[q]);
+            P.T[q] = __builtin_amdgcn_readfirstlane(R.T[q]);
         }
-        P.rq = R.rq;
-        if constexpr (XR) P.bstep = (long)R.blksize * kK;
-        P.dplane = R.dplane;
-        P.rplane = R.rplane;
-        P.sym = R.sym;
+        P.rq = uniform64(R.rq);
+        if constexpr (XR) P.bstep = (long)__builtin_amdgcn_readfirstlane(R.blksize) * kK;
+        P.dplane = __builtin_amdgcn_readfirstlane(R.dplane);
+        P.rplane = __builtin_amdgcn_readfirstlane(R.rplane);
+        P.sym = __builtin_amdgcn_readfirstlane(R.sym);
         return P;
     };
     // the consumer's item: what the diagonal doubling, the tile ends and the epilogues need (ed
@@ -692,24 +704,24 @@ __global__ __launch_bounds__(kThreads, 2) void rho_resident_persistent_kernel(
     auto cons_item = [&](int it) {
         const Rec &R = its[it];
         ConsItem c;
-        c.ed = R.ed;
-        c.lc = R.lc;
+        c.ed = uniform64(R.ed);
+        c.lc = uniform64(R.lc);
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
-            c.n[q] = R.n[q];
-            c.T[q] = R.T[q];
+            c.n[q] = __builtin_amdgcn_readfirstlane(R.n[q]);
+            c.T[q] = __builtin_amdgcn_readfirstlane(R.T[q]);
         }
-        c.sym = R.sym;
-        c.b = R.b;
-        c.c0 = R.c0;
-        c.s = R.s;
-        c.nrows = R.nrows;
+        c.sym = __builtin_amdgcn_readfirstlane(R.sym);
+        c.b = __builtin_amdgcn_readfirstlane(R.b);
+        c.c0 = __builtin_amdgcn_readfirstlane(R.c0);
+        c.s = __builtin_amdgcn_readfirstlane(R.s);
+        c.nrows = __builtin_amdgcn_readfirstlane(R.nrows);
         if constexpr (XR) {
-            c.part = reinterpret_cast<double *>(R.part);
-            c.fgp = reinterpret_cast<const int *>(R.fg);
-            c.tstride = R.tstride;
-            c.blksize = R.blksize;
-            c.ngrid = R.ngrid;
+            c.part = reinterpret_cast<double *>(uniform64((long)R.part));
+            c.fgp = reinterpret_cast<const int *>(uniform64((long)R.fg));
+            c.tstride = __builtin_amdgcn_readfirstlane(R.tstride);
+            c.blksize = __builtin_amdgcn_readfirstlane(R.blksize);
+            c.ngrid = __builtin_amdgcn_readfirstlane(R.ngrid);
         }
         return c;
     };
@@ -720,11 +732,11 @@ __global__ __launch_bounds__(kThreads, 2) void rho_resident_persistent_kernel(
         const int ai = P.sym ? (l < n - 1 ? l + 1 : 0) : l;
         StepAddr x;
         if constexpr (XR) {
-            x.a = reinterpret_cast<const int8_t *>(P.a[q]) + (long)wave * P.aw[q] + (long)ai * 1024 + lane * 16;
-            x.b = reinterpret_cast<const int8_t *>(P.rq) + ks * P.bstep + lane * 16;
+            x.a = reinterpret_cast<const int8_t *>(P.a[q]) + (long)wave * P.aw[q] + (long)ai * 1024;
+            x.b = reinterpret_cast<const int8_t *>(P.rq) + ks * P.bstep;
         } else {
-            x.a = reinterpret_cast<const int8_t *>(base + P.a[q]) + (long)wave * P.aw[q] + (long)ai * 1024 + lane * 16;
-            x.b = reinterpret_cast<const int8_t *>(base + P.rq) + ks * bstep + lane * 16;
+            x.a = reinterpret_cast<const int8_t *>(base + P.a[q]) + (long)wave * P.aw[q] + (long)ai * 1024;
+            x.b = reinterpret_cast<const int8_t *>(base + P.rq) + ks * bstep;
         }
         x.dplane = P.dplane;
         x.rplane = P.rplane;
@@ -733,7 +745,7 @@ __global__ __launch_bounds__(kThreads, 2) void rho_resident_persistent_kernel(
     auto load_a = [&](const StepAddr &x, i32x4 *dst) {
 #pragma unroll
         for (int p = 0; p < S; ++p)  // each A tile is read once per evaluation: stream it past the caches
-            dst[p] = __builtin_nontemporal_load(reinterpret_cast<const i32x4 *>(x.a + (long)p * x.dplane));
+            dst[p] = __builtin_nontemporal_load(reinterpret_cast<const i32x4 *>(x.a + (long)p * x.dplane + lane16));
     };
     constexpr int kBParts = S * NT, kBPerWave = (kBParts + 3) / 4;
     auto issue_b = [&](const StepAddr &x, int8_t (*dst)[W * kK]) {
@@ -741,7 +753,7 @@ __global__ __launch_bounds__(kThreads, 2) void rho_resident_persistent_kernel(
         for (int i = 0; i < kBPerWave; ++i) {
             const int j = min(wave + 4 * i, kBParts - 1);
             const int p = j / NT, part = j % NT;
-            copy16_to_lds(x.b + (long)p * x.rplane + part * 1024, &dst[p][part * 1024]);
+            copy16_to_lds(x.b + (long)p * x.rplane + part * 1024 + lane16, &dst[p][part * 1024]);
         }
     };
     auto compute = [&](const int8_t (*bs)[W * kK], const i32x4 *fa) {
@@ -850,10 +862,11 @@ __global__ __launch_bounds__(kThreads, 2) void rho_resident_persistent_kernel(
             if (lane == 0) its_cnt = __popcll(m);
         }
         __syncthreads();
-        const int cnt = its_cnt;
+        const int cnt = __builtin_amdgcn_readfirstlane(its_cnt);
         if (cnt == 0) continue;
         int total = 0;
         for (int it = 0; it < cnt; ++it) total += its[it].n[0] + its[it].n[1];
+        total = __builtin_amdgcn_readfirstlane(total);
 
         // producer (the step whose loads are issued next) and consumer (the step computed next)
         int pi = 0, pt = 0, ci = 0, ct = 0;
