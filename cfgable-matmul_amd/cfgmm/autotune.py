@@ -76,7 +76,7 @@ def tune_dgemm_small(run_gemm, stream, tiles=(64, 128), splits=(1, 2, 3, 4, 5, 6
     it applies), or the tile edge and K ranges of the split over workgroups."""
     params = (cfgmm.TUNE_DGEMM_KGROUPS, cfgmm.TUNE_DGEMM_SMALL_TILE, cfgmm.TUNE_DGEMM_SMALL_SPLITS)
     old = {p: cfgmm.tune_get(p) for p in params}
-    cands = [{"kgroups": 0, "small_tile": 0, "small_splits": 0}] + \
+    cands = [{"kgroups": 0, "small_tile": 0, "small_splits": 0}, {"kgroups": 2, "small_tile": 0, "small_splits": 0}] + \
         [{"kgroups": 1, "small_tile": t, "small_splits": sp} for t, sp in itertools.product(tiles, splits)]
     results = []
     try:

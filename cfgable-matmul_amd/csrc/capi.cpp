@@ -86,7 +86,7 @@ void trace_stop(hipStream_t stream, const char *name, hipEvent_t start) {
 // ---------------------------------------------------------------------------
 static std::atomic<int> g_tune[CFGMM_TUNE_COUNT] = {{2}, {1}, {4}, {1}, {0}, {0}, {1}, {0}, {0}, {0}, {0}, {0}, {0}};
 static const int g_tune_lo[CFGMM_TUNE_COUNT] = {1, 1, 1, 1, 0, 0, 1, 0, 0, 0, 0, 0, 0};
-static const int g_tune_hi[CFGMM_TUNE_COUNT] = {8, 4, 64, 4, 128, 32, 2, 64, 1, 1, 1, 2, 1};
+static const int g_tune_hi[CFGMM_TUNE_COUNT] = {8, 4, 64, 4, 128, 32, 2, 64, 1, 1, 2, 2, 1};
 
 int tune_get(int param) { return g_tune[param].load(std::memory_order_relaxed); }
 

@@ -553,32 +553,48 @@ DgemmPlan plan_dgemm(int m, int n, int k, int cus) {
             p.splits = split_for(rem, slots128);
         }
     } else {
-        const long t64 = (long)div_up(m, 64) * div_up(n, 64);
-        const long slots64 = (long)kSlots64 * cus;
-        // Few tiles: 128x128 tiles split along K fill the chip with fewer,
-        // larger pieces when that is possible; otherwise 64x64 tiles.  The
-        // autotuner may fix the tile edge and the number of K ranges.
+        // Few tiles: the autotuner may fix the tile edge and the number of K ranges; otherwise the
+        // (edge, ranges) of least modelled time.  Model (fitted to 18 shapes of 256..2000 rows and
+        // columns and k 256..6000 on MI355X, within 2 % of the best plan on average): the largest
+        // per-CU share of workgroups times one workgroup's multiply-adds at the rate a CU sustains
+        // with that many resident (128 x 128: 128 G/s alone, 137 G/s with two; 64 x 64: 91, 123,
+        // 131, 128 G/s with one to four), plus, with more than one range, the partial tiles read
+        // again by tail_reduce_kernel at 4.5 TB/s and its launch (1.4 us).
         const int force_tile = tune_get(CFGMM_TUNE_DGEMM_SMALL_TILE);
         const int force_splits = tune_get(CFGMM_TUNE_DGEMM_SMALL_SPLITS);
-        const int s128 = split_for(t128, slots128);
-        if (force_tile == 128 || (force_tile == 0 && t128 * s128 >= slots128 * 3 / 4)) {
-            p.bm = 128;
-            p.tiles = (int)t128;
-        } else {
-            p.bm = 64;
-            p.tiles = (int)t64;
+        auto model_us = [&](int bm, int s) {
+            const long tiles = (long)div_up(m, bm) * div_up(n, bm);
+            const long kc = div_up(div_up(k, s), 16) * 16;
+            const long sp = div_up(k, kc);
+            const long per_cu = div_up(tiles * sp, (long)cus);
+            static const double us_per_kmac128[2] = {0.0078, 0.0073};
+            static const double us_per_kmac64[4] = {0.0110, 0.0081, 0.0077, 0.0078};
+            const double rate = bm == 128 ? us_per_kmac128[std::min(per_cu, 2L) - 1]
+                                          : us_per_kmac64[std::min(per_cu, 4L) - 1];
+            double t = rate * (double)per_cu * bm * bm * kc / 1e3;
+            if (sp > 1) t += 0.2224 * (double)tiles * sp * bm * bm * 8 / 1e6 + 1.4;
+            return t;
+        };
+        const int s_cap = std::max(1, k / 16);  // ranges at least 16 deep
+        const int s_lo = force_splits > 0 ? std::min(force_splits, s_cap) : 1;
+        const int s_hi = force_splits > 0 ? s_lo : std::min(16, s_cap);
+        int best_bm = 0, best_s = 1;
+        double best_t = 0.0;
+        for (int bm : {128, 64}) {
+            if (force_tile != 0 && bm != force_tile) continue;
+            for (int s = s_lo; s <= s_hi; ++s) {
+                const double t = model_us(bm, s);
+                if (best_bm == 0 || t < best_t) {
+                    best_t = t;
+                    best_bm = bm;
+                    best_s = s;
+                }
+            }
         }
-        const long slots = p.bm == 128 ? slots128 : slots64;
-        if (force_splits > 0) {
-            p.dp_tiles = force_splits == 1 ? p.tiles : 0;
-            p.splits = std::min(force_splits, std::max(1, k / 16));
-        } else if (p.tiles >= slots) {
-            p.dp_tiles = p.tiles;
-            p.splits = 1;
-        } else {
-            p.dp_tiles = 0;
-            p.splits = split_for(p.tiles, slots);
-        }
+        p.bm = best_bm;
+        p.tiles = (int)((long)div_up(m, best_bm) * div_up(n, best_bm));
+        p.splits = best_s;
+        p.dp_tiles = best_s == 1 ? p.tiles : 0;
     }
     if (p.splits <= 1) {
         p.splits = 1;
@@ -644,8 +660,11 @@ int launch_dgemm(hipStream_t stream, double *c, const double *a, const double *b
     const int cus = device_cu_count();
     // too few 128x128 tiles to fill the chip, both operands K-contiguous: the in-workgroup K split
     // with the narrowest tile width that gives every tile a CU of its own, if the tiles then cover
-    // at least 3/4 of the CUs
-    if (!transa && transb && tune_get(CFGMM_TUNE_DGEMM_KGROUPS) == 0 &&
+    // at least 3/4 of the CUs.  By default only up to k = 1024: deeper, the K split over
+    // workgroups is 8 - 25 % faster on such shapes (it keeps more waves per CU busy), shallower
+    // its partial tiles and second launch cost more than the lower rate here (measured on MI355X).
+    const int kgroups = tune_get(CFGMM_TUNE_DGEMM_KGROUPS);
+    if (!transa && transb && (kgroups == 2 || (kgroups == 0 && k <= 1024)) &&
         (long)div_up(m, 128) * div_up(n, 128) < (long)kSlots128 * cus) {
         const long tm = (long)div_up(m, 64);
         for (int bn : {64, 80, 96}) {

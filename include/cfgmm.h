@@ -399,8 +399,9 @@ enum cfgmm_tunable {
     CFGMM_TUNE_DGEMM_KGROUPS = 10,   /* DGEMM too small for 128x128 tiles, A not transposed and B
                                         transposed: 0 = one workgroup per 64-row tile splitting K
                                         among four groups of waves when the tiles then cover 3/4 or
-                                        more of the CUs (default), 1 = never (K split over
-                                        workgroups) */
+                                        more of the CUs and k <= 1024 (default), 1 = never (K split
+                                        over workgroups), 2 = whenever the tiles cover 3/4 of the
+                                        CUs, any k */
     CFGMM_TUNE_MASKED_LDS = 11,      /* FP64 masked GEMM with B not transposed and n even: 0 = op(A)
                                         transposed into the workspace, slices copied straight into
                                         LDS, 128-row tiles (default); 2 = the same with 256-row

@@ -183,20 +183,31 @@ def test_dgemm_kgroup_plan(m, n, k):
             return _run_both(m, n, k, False, True, seed=m + n + k, pad=(2, 3, 1), api="dev")
         finally:
             cfgmm.tune_set(cfgmm.TUNE_DGEMM_KGROUPS, old)
-    c0, c_ref, c_kg, ldc = run(0)
+    c0, c_ref, c_kg, ldc = run(2)
     _, _, c_split, _ = run(1)
     G, R, S = (x.reshape(m, ldc) for x in (c_kg, c_ref, c_split))
     assert rel_fro(G[:, :n], R[:, :n]) <= TOL
     assert rel_fro(S[:, :n], R[:, :n]) <= TOL
     assert np.array_equal(G[:, n:], c0.reshape(m, ldc)[:, n:])
     # the fixed group order makes the result bitwise repeatable
-    _, _, c_kg2, _ = run(0)
+    _, _, c_kg2, _ = run(2)
     assert np.array_equal(c_kg, c_kg2)
 
 
-def test_dgemm_kgroup_integer_exact():
-    """Small-integer operands through the in-workgroup K split: every partial sum and the group
-    reduction are exact, so the result is bitwise the integer product."""
+@pytest.mark.parametrize("kgroups", [2, 0], ids=["kgroup", "default"])
+def test_dgemm_kgroup_integer_exact(kgroups):
+    """Small-integer operands through the in-workgroup K split, and through the default plan of
+    this shape (the K split over workgroups): every partial sum and the reduction of the partial
+    tiles are exact, so the result is bitwise the integer product."""
+    old = cfgmm.tune_get(cfgmm.TUNE_DGEMM_KGROUPS)
+    cfgmm.tune_set(cfgmm.TUNE_DGEMM_KGROUPS, kgroups)
+    try:
+        _kgroup_integer_exact()
+    finally:
+        cfgmm.tune_set(cfgmm.TUNE_DGEMM_KGROUPS, old)
+
+
+def _kgroup_integer_exact():
     rng = np.random.default_rng(9)
     m, n, k = 1031, 977, 1543
     a = rng.integers(-2, 3, size=m * k).astype(np.float64)
