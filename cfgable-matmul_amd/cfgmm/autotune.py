@@ -9,9 +9,10 @@ Tunables:
   batched rho  -- persistent workgroups per CU (1..4) and density matrices per
                   task (1, 2, 4);
   DGEMM        -- tile rows per raster group (1, 2, 4, 8, 16); for shapes with
-                  fewer 128x128 tiles than workgroup slots, the in-workgroup K
-                  split or the tile edge (64, 128) and the number of K ranges
-                  per tile (1..12);
+                  fewer 128x128 tiles than workgroup slots, the built-in plan
+                  (the cost model's pick, or the in-workgroup K split up to
+                  k = 1024), the in-workgroup K split at any k, or the tile
+                  edge (64, 128) and the number of K ranges per tile (1..12);
   FP64 rho     -- density matrices per workgroup (1..4).
 """
 from __future__ import annotations
